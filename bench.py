@@ -1,0 +1,185 @@
+"""Benchmark of the SIMP finite-element solve path on one MI355X.
+
+Workload (default): 3D cantilever, 128 x 64 x 64 hex8 (1,635,075 DOF, FP64),
+x = 0 face clamped, unit downward load along the bottom edge of the x = L
+face, modified SIMP (E0 = 1, Emin = 1e-9, p = 3), volume fraction 0.3,
+density filter r = 1.5, OC (move 0.2).  Solver: GMG-PCG to 1e-8 relative
+residual, 4 coarsenings, 2 pre + 2 post Jacobi sweeps with omega = 0.6,
+warm start.  A step is one SIMP iteration: filter -> interpolation -> solve
+-> compliance -> sensitivity -> chain rule -> filter adjoint -> OC update.
+
+Also measured: the isolated K.u of the dominant kernel (seeded density
+x ~ U[0.01, 1], default_rng(0)) timed with HIP events on the stream it runs
+on, its roofline fraction, and the FP64 CPU oracle on the host cores.
+
+Prints one JSON line.  --gpus N > 1 is refused: this build runs on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "s/SIMP iter to 1e-8 rel res, mixed+learned vs FP64 GMG-PCG; GDOF/s K·u"
+
+CONFIGS = {
+    "c2": dict(nel=(128, 64, 64), n_level=4, name="cantilever_128x64x64"),
+    "c3": dict(nel=(256, 128, 128), n_level=5, name="cantilever_256x128x128"),
+    "small": dict(nel=(32, 16, 16), n_level=3, name="cantilever_32x16x16"),
+}
+
+
+def build_problem(pf, nel, n_level, tol=1e-8):
+    mesh = pf.StructuredMesh3D(*nel, *nel, physics=pf.LinearElasticity(E=1.0, nu=0.3))
+    K = pf.StructuredStiffnessKernel(mesh)
+    solver = pf.MultiGrid(mesh, K, maxiter=1000, tol=tol, n_smooth=2, omega=0.6, n_level=n_level, cycle="V")
+    FE = pf.FiniteElement(mesh, K, solver)
+    nx, ny, nz = (n + 1 for n in nel)
+    j, k = np.meshgrid(np.arange(ny), np.arange(nz), indexing="ij")
+    FE.add_dirichlet_boundary_condition(node_ids=(k + nz * nx * j).ravel(), rhs=0)
+    FE.add_point_forces(forces=np.array([[0.0, -1.0 / nz, 0.0]]), node_ids=np.arange(nz) + nz * (nx - 1))
+    filt = pf.StructuredFilter3D(mesh, 1.5)
+    prob = pf.MinimumCompliance(FE, filt, E_mul=[1.0], void=1e-9, penalty=3.0, volume_fraction=[0.3],
+                                heavyside=False, law="modified")
+    return mesh, K, solver, FE, filt, prob
+
+
+def time_ku(torch, pf, K, mesh, reps=50):
+    """Average duration of the K.u kernel from HIP events on its stream."""
+    from pyfantom_amd import _lib
+    x = np.random.default_rng(0).uniform(0.01, 1.0, mesh.n_elements)
+    E = torch.from_numpy(1e-9 + x ** 3 * (1 - 1e-9)).cuda()
+    K.set_rho(E)
+    u = torch.from_numpy(np.random.default_rng(1).standard_normal(K.shape[0])).cuda()
+    u[K.constraints] = 0
+    y = torch.empty_like(u)
+    kid, k0 = K.op_args()
+    args = (kid, *K.grid, K.k0_ptr(k0), E.data_ptr(), K.mask.data_ptr(), u.data_ptr(), y.data_ptr())
+    stream = torch.cuda.current_stream()
+    for _ in range(5):
+        _lib.call("fantom_ku_f64", *args)
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    for _ in range(reps):
+        _lib.call("fantom_ku_f64", *args)
+    ev1.record(stream)
+    ev1.synchronize()
+    t = ev0.elapsed_time(ev1) / 1e3 / reps
+    n_dof, n_el = K.shape[0], mesh.n_elements
+    bytes_alg = 16.0 * n_dof + 8.0 * n_el
+    return dict(seconds=t, gdof_s=n_dof / t / 1e9, achieved_gbs=bytes_alg / t / 1e9, bytes=bytes_alg,
+                flops_dense=1152.0 * n_el, kernel="ku3d_tile_f64" if kid == _lib.KU_TILE else "ku3d_node_f64")
+
+
+def cpu_baseline(nel, n_level, pcg_iters_per_step, max_iter=4):
+    """FP64 oracle (C/OpenMP) on the host: K.u and a bounded GMG-PCG sample."""
+    from oracle import oracle as O
+    O.build()
+    n_el = int(np.prod(nel))
+    x = np.random.default_rng(0).uniform(0.01, 1.0, n_el)
+    E = 1e-9 + x ** 3 * (1 - 1e-9)
+    K0 = O.k0_exact(3, 0.3)
+    el = O.elements(nel)
+    nx, ny, nz = (n + 1 for n in nel)
+    n_dof = nx * ny * nz * 3
+    con = np.zeros(n_dof, bool)
+    j, k = np.meshgrid(np.arange(ny), np.arange(nz), indexing="ij")
+    clamp = (k + nz * nx * j).ravel()
+    for d in range(3):
+        con[clamp * 3 + d] = True
+    f = np.zeros(n_dof)
+    f[(np.arange(nz) + nz * (nx - 1)) * 3 + 1] = -1.0 / nz
+    u = np.random.default_rng(1).standard_normal(n_dof)
+    O.matvec(K0, el, E, u, con)
+    t0 = time.perf_counter()
+    O.matvec(K0, el, E, u, con)
+    t_ku = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    _, it, _ = O.mgpcg_solve(nel, K0, E, con, f, n_level=n_level, n_smooth=2, omega=0.6, tol=1e-30,
+                             maxiter=max_iter)
+    t_solve = time.perf_counter() - t0
+    per_it = t_solve / max(it, 1)
+    return dict(value=per_it * pcg_iters_per_step, unit="s/SIMP-iter", cores=O.num_threads(), kind="port",
+                sample=(f"oracle GMG-PCG on the same cantilever ({nel[0]}x{nel[1]}x{nel[2]}, seeded E), {it} PCG "
+                        f"iterations incl. hierarchy setup ({t_solve:.2f} s) scaled to the GPU run's mean "
+                        f"{pcg_iters_per_step:.1f} PCG iterations per SIMP step; filter/OC excluded"),
+                ku_seconds=t_ku, ku_gdof_s=n_dof / t_ku / 1e9)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--ku-reps", type=int, default=50)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import pyfantom_amd as pf
+
+    cfg = CONFIGS[args.config]
+    nel, n_level = cfg["nel"], cfg["n_level"]
+    mesh, K, solver, FE, filt, prob = build_problem(pf, nel, n_level)
+    print(f"# setup done: {cfg['name']} dofs={K.shape[0]}", file=sys.stderr, flush=True)
+    opt = pf.OC(prob, move=0.2)
+    for w in range(args.warmup):
+        opt.iter()
+        print(f"# warmup {w}: c={prob.f():.6e} iters={solver.last_iterations}", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    iters = []
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        opt.iter()
+        iters.append(solver.last_iterations)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    ms = t / args.steps * 1e3
+    print(f"# timed {args.steps} steps: {ms:.1f} ms/step, pcg iters {iters}", file=sys.stderr, flush=True)
+    ku = time_ku(torch, pf, K, mesh, args.ku_reps)
+    print(f"# K.u: {ku['seconds']*1e6:.1f} us, {ku['achieved_gbs']:.0f} GB/s", file=sys.stderr, flush=True)
+    out = {
+        "metric": METRIC,
+        "value": t / args.steps,
+        "unit": "s/SIMP-iter",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": cfg["name"] + "_simp_oc_gmgpcg_1e-8", "elements": list(nel), "dofs": K.shape[0],
+                   "coarsenings": len(solver.levels) - 1, "smoother": "jacobi 2+2 omega 0.6",
+                   "precision": "fp64 plain GMG-PCG"},
+        "pcg_iterations_mean": float(np.mean(iters)),
+        "compliance": prob.f(),
+        "solve_rel_residual": float(prob._residual),
+        "ku": {"gdof_s": ku["gdof_s"], "seconds": ku["seconds"], "kernel": ku["kernel"]},
+        "roofline": {"bound": "hbm", "achieved": ku["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": ku["achieved_gbs"] / HBM_PEAK_GBS, "traffic": None, "kernel": ku["kernel"],
+                     "algorithmic_bytes": ku["bytes"]},
+    }
+    if not args.no_cpu:
+        print("# cpu baseline ...", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(nel, n_level, float(np.mean(iters)))
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,180 @@
+/*
+ * fantom.h -- C ABI of libfantom_hip.so, the gfx950 (MI355X) kernels behind
+ * the structured SIMP finite-element solve path.
+ *
+ * Conventions
+ *   - Grids are structured: nelx x nely x nelz elements; nelz == 0 means 2D
+ *     (4-node quads, 2 DOF per node), nelz > 0 means 3D (8-node hexes, 3 DOF).
+ *   - Layouts are the reference's (pyFANTOM core/CPU/_geom.py:4-98):
+ *       3D node  n(i,j,k) = k + nz*i + nz*nx*j       (nx = nelx+1, nz = nelz+1)
+ *       3D elem  e(i,j,k) = (i*nely + j)*nelz + k
+ *       2D node  n(i,j)   = j*nx + i;  2D elem e(i,j) = i*nely + j
+ *       DOF      node*dof + component (interleaved)
+ *   - Every pointer except K0/taps metadata is a device pointer owned by the
+ *     caller; no call allocates device memory.  Calls are asynchronous on the
+ *     given stream (NULL = default stream) unless stated otherwise.
+ *   - mask: optional (NULL = no constraints) one byte per NODE, bit c set when
+ *     DOF c of that node is Dirichlet-constrained.  Constrained rows act as
+ *     identity and constrained columns are skipped, the reference's convention
+ *     (core/CUDA/Kernels/matvec.cu:45-98).
+ *   - Return value: 0 on success; 1 = invalid argument (ValueError in the
+ *     reference's Python API), 2 = HIP runtime error (RuntimeError),
+ *     3 = unsupported.  fantom_last_error() returns a thread-local message.
+ *
+ * The reference launches CuPy RawModule kernels with raw pointers and no
+ * status codes (pyFANTOM core/CUDA/_ops.py:59-100, core/CUDA/_mgm.py:173-264,
+ * core/CUDA/_filter.py:11-63); each entry point below names the one it
+ * replaces.
+ */
+#ifndef FANTOM_H
+#define FANTOM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FANTOM_ABI_VERSION 1
+
+/* Number of double slots of the PCG device-scalar array. */
+#define FANTOM_SCALARS 16
+/* Slots of that array. */
+#define FANTOM_S_RZ 0
+#define FANTOM_S_PQ 1
+#define FANTOM_S_ALPHA 2
+#define FANTOM_S_RR 3
+#define FANTOM_S_BETA 4
+/* finalize ops */
+#define FANTOM_FIN_STORE 0
+#define FANTOM_FIN_ALPHA 1
+#define FANTOM_FIN_BETA 2
+#define FANTOM_FIN_RZ0 3
+/* matvec kernels */
+#define FANTOM_KU_NODE 0 /* node-centric gather, dense K0 rows (2D and 3D) */
+#define FANTOM_KU_TILE 1 /* element-tiled Walsh-Hadamard block form (3D only) */
+
+int fantom_abi_version(void);
+const char *fantom_last_error(void);
+/* Blocks until all work on the stream is complete. */
+int fantom_stream_sync(void *stream);
+
+/* ---- A5: y = K(E) u -------------------------------------------------------
+ * Replaces mat_vec_node_basis_parallel(_wcon)_cuda (core/CUDA/_ops.py:59-72,
+ * kernel core/CUDA/Kernels/matvec.cu:1-98) as used by
+ * StructuredStiffnessKernel._matvec (stiffness/CUDA/_FEA.py:518-549).
+ * K0: for FANTOM_KU_NODE a DEVICE pointer to the (npe*dof)^2 row-major
+ * element matrix for E = 1; for FANTOM_KU_TILE a HOST pointer to the 72-double
+ * Walsh-Hadamard block form from fantom_hex8_blocks (passed by value to the
+ * kernel).
+ */
+int fantom_ku_f64(int kernel, int nelx, int nely, int nelz, const double *K0, const double *E,
+                  const uint8_t *mask, const double *u, double *y, void *stream);
+/* Jacobi sweep fused into the operator: x_out = x_in + omega*dinv*(b - K x_in)
+ * (solvers/CUDA/_solvers.py:437-440). x_out must not alias x_in. */
+int fantom_ku_jacobi_f64(int kernel, int nelx, int nely, int nelz, const double *K0, const double *E,
+                         const uint8_t *mask, const double *b, const double *dinv, double omega,
+                         const double *x_in, double *x_out, void *stream);
+/* Residual fused into the operator: r = b - K x (solvers/CUDA/_solvers.py:538). */
+int fantom_ku_residual_f64(int kernel, int nelx, int nely, int nelz, const double *K0, const double *E,
+                           const uint8_t *mask, const double *b, const double *x, double *r, void *stream);
+/* y = K u plus per-block partial sums of u.y (PCG's p.Kp,
+ * solvers/CUDA/_solvers.py:636-637).  partials needs
+ * fantom_ku_partials(kernel, ...) doubles; reduce with fantom_finalize_f64. */
+int fantom_ku_dot_f64(int kernel, int nelx, int nely, int nelz, const double *K0, const double *E,
+                      const uint8_t *mask, const double *u, double *y, double *partials, void *stream);
+long fantom_ku_partials(int kernel, int nelx, int nely, int nelz);
+/* Host: derive the Walsh-Hadamard block form (8 blocks of 3x3, 72 doubles)
+ * of a box hex8 K0 (24x24, host pointer).  Returns 1 if K0 does not have the
+ * box-element reflection symmetry that the form needs. */
+int fantom_hex8_blocks(const double *K0_host, double *blocks_host, double *max_offblock);
+
+/* ---- A6: diag(K(E)), constrained -> 1; invert != 0 stores 1/diag ----------
+ * Replaces get_diagonal_node_basis_cuda (core/CUDA/_ops.py:430-447,
+ * core/CUDA/Kernels/diag.cu:1-47). */
+int fantom_diag_f64(int nelx, int nely, int nelz, const double *K0, const double *E, const uint8_t *mask,
+                    double *d, int invert, void *stream);
+
+/* ---- A14: dc_e = -u_e^T K0 u_e ---------------------------------------------
+ * Replaces process_dk_kernel_cuda (core/CUDA/_ops.py:74-81,
+ * core/CUDA/Kernels/grad.cu:1-16). */
+int fantom_dc_f64(int nelx, int nely, int nelz, const double *K0, const double *u, double *dc, void *stream);
+
+/* ---- A7: grid transfers (fine grid nelx x nely x nelz) ---------------------
+ * Replace apply_restriction_cuda / apply_prolongation_cuda
+ * (core/CUDA/_mgm.py:173-264, core/CUDA/Kernels/mgm.cu:1-244).
+ * normalize = 1: the reference's TW-normalised operators; 0: plain trilinear
+ * P_lin^T / P_lin.  mask_c / mask_f (optional, per node) zero constrained
+ * DOFs of the output.  accumulate: xf += P ec instead of xf = P ec. */
+int fantom_restrict_f64(int nelx, int nely, int nelz, int normalize, const uint8_t *mask_c, const double *rf,
+                        double *rc, void *stream);
+int fantom_prolong_f64(int nelx, int nely, int nelz, int normalize, const uint8_t *mask_f, const double *ec,
+                       double *xf, int accumulate, void *stream);
+/* Rediscretised hierarchy (no reference counterpart: the reference builds
+ * Galerkin CSR operators, core/CUDA/_mgm.py:19-171): coarse E = mean of the
+ * 2^d children; coarse mask = mask of the coincident fine node. */
+int fantom_coarsen_E_f64(int nelx, int nely, int nelz, const double *Ef, double *Ec, void *stream);
+int fantom_coarsen_mask(int nelx, int nely, int nelz, const uint8_t *mf, uint8_t *mc, void *stream);
+
+/* ---- A10: coarsest direct solve ---------------------------------------------
+ * Replaces the host CHOLMOD factor/solve of solvers/CUDA/_solvers.py:498-516.
+ * dense_assemble writes the dense n_dof x n_dof K(E) with the Dirichlet
+ * convention; spd_inverse inverts an SPD matrix in place (Gauss-Jordan,
+ * work = 2n doubles); gemv computes y = A x. */
+int fantom_dense_assemble_f64(int nelx, int nely, int nelz, const double *K0, const double *E,
+                              const uint8_t *mask, double *A, void *stream);
+int fantom_spd_inverse_f64(int n, double *A, double *work, void *stream);
+int fantom_gemv_f64(int n, const double *A, const double *x, double *y, void *stream);
+
+/* ---- A11: PCG vector kernels (solvers/CUDA/_solvers.py:629-650) -----------
+ * dot_partials: per-block partial sums of a.b into partials
+ * (fantom_vec_partials(n) doubles).  finalize reduces np partials in a fixed
+ * order and applies op to the device scalars (FANTOM_FIN_*; STORE writes
+ * scal[slot]).  pcg_xr: x += alpha p; r -= alpha q; partials of r.r.
+ * pcg_p: p = z + beta p (first != 0: p = z).  scale: x = omega*dinv*b.
+ * axpby: y = a x + b y. */
+long fantom_vec_partials(long n);
+int fantom_dot_partials_f64(long n, const double *a, const double *b, double *partials, void *stream);
+int fantom_finalize_f64(const double *partials, long np, double *scal, int op, int slot, void *stream);
+int fantom_pcg_xr_f64(long n, double *x, double *r, const double *p, const double *q, const double *scal,
+                      double *partials, void *stream);
+int fantom_pcg_p_f64(long n, double *p, const double *z, const double *scal, int first, void *stream);
+int fantom_scale_f64(long n, double omega, const double *dinv, const double *b, double *x, void *stream);
+int fantom_axpby_f64(long n, double a, const double *x, double b, double *y, void *stream);
+
+/* ---- A15: structured density filter ----------------------------------------
+ * Replace apply_filter_2D/3D_cuda, get_filter_2D/3D_weights_cuda and the
+ * transpose kernels (core/CUDA/_filter.py:11-63, core/CUDA/Kernels/filters.cu).
+ * offsets: device int32[ntap*dim]; weights: device double[ntap]; ntap <= 2048.
+ * Elements in the reference's order (x slowest). */
+int fantom_filter_normalizer_f64(int nelx, int nely, int nelz, int ntap, const int32_t *offsets,
+                                 const double *weights, double *norm, void *stream);
+int fantom_filter_fwd_f64(int nelx, int nely, int nelz, int ntap, const int32_t *offsets, const double *weights,
+                          const double *x, double *y, void *stream);
+int fantom_filter_adj_f64(int nelx, int nely, int nelz, int ntap, const int32_t *offsets, const double *weights,
+                          const double *norm, const double *y, double *x, void *stream);
+
+/* ---- A16: OC update (Optimizers/CUDA/OC.py:67-140) -------------------------
+ * state: device double[16] (slot 2 = final Lagrange multiplier); partials:
+ * device double[16*1024]; ocp: device double[n].  x_new gets the update and
+ * partials[0:fantom_oc_partials(n)] the partial sums of (x - x_new)^2. */
+long fantom_oc_partials(long n);
+int fantom_oc_update_f64(long n, const double *x, const double *df, const double *dg, double vf, double move,
+                         double *ocp, double *state, double *partials, double *x_new, void *stream);
+
+/* ---- A4: SIMP interpolation ------------------------------------------------
+ * Replaces MinimumCompliance.penalize / penalize_grad
+ * (Problem/CUDA/MinimumCompliance.py:305-351, CPU :358-464), single material.
+ * law 0: E = max(clip(h^p, lo, 1)*hi, lo) with lo = void, hi = E_mul (the
+ * reference's law); law 1: E = lo + h^p (hi - lo) with lo = Emin, hi = E0.
+ * h is the Heaviside projection of rho when heaviside != 0, else rho.
+ * E or dE may be NULL.  mul: out = a*b elementwise (the chain rule). */
+int fantom_simp_f64(long n, const double *rho, int law, double p, double lo, double hi, int heaviside,
+                    double beta, double eta, double *E, double *dE, void *stream);
+int fantom_mul_f64(long n, const double *a, const double *b, double *out, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FANTOM_H */

@@ -1,0 +1,783 @@
+/*
+ * FP64 CPU oracle for the structured SIMP finite-element solve path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is a plain-C restatement of the
+ * reference algorithms (pyFANTOM, CPU/Numba backend).  It is used by the test
+ * suite as the checker, by __graft_entry__.smoke() as the checker and by
+ * bench.py as the timed CPU baseline.  The product path (pyfantom_amd) never
+ * links, loads or calls it.
+ *
+ * Every function cites the reference file:line it follows.  Layouts follow
+ * the reference exactly:
+ *   3D node id   n(i,j,k) = k + nz*i + nz*nx*j        (y slowest, z fastest)
+ *   3D element   e(i,j,k) = i*nely*nelz + j*nelz + k  (x slowest, z fastest)
+ *   2D node id   n(i,j)   = j*nx + i
+ *   2D element   e(i,j)   = i*nely + j
+ *   DOF          node*dof + component
+ * Constraint masks are one byte per DOF (nonzero = constrained).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* A1: connectivity.  core/CPU/_geom.py:29-59 (3D), :4-27 (2D).              */
+/* ------------------------------------------------------------------------ */
+void or_elements_3d(int nelx, int nely, int nelz, int32_t *out)
+{
+    const int nx = nelx + 1, nz = nelz + 1;
+    const long n_el = (long)nelx * nely * nelz;
+#pragma omp parallel for schedule(static)
+    for (long c = 0; c < n_el; ++c) {
+        const int i = (int)(c / ((long)nely * nelz));
+        const int t = (int)(c % ((long)nely * nelz));
+        const int j = t / nelz;
+        const int k = t % nelz;
+        int32_t *o = out + 8 * c;
+        o[0] = nz * i + nz * nx * j + k;
+        o[1] = nz * (i + 1) + nz * nx * j + k;
+        o[2] = nz * (i + 1) + nz * nx * (j + 1) + k;
+        o[3] = nz * i + nz * nx * (j + 1) + k;
+        o[4] = nz * i + nz * nx * j + k + 1;
+        o[5] = nz * (i + 1) + nz * nx * j + k + 1;
+        o[6] = nz * (i + 1) + nz * nx * (j + 1) + k + 1;
+        o[7] = nz * i + nz * nx * (j + 1) + k + 1;
+    }
+}
+
+void or_elements_2d(int nelx, int nely, int32_t *out)
+{
+    const int nx = nelx + 1;
+    const long n_el = (long)nelx * nely;
+#pragma omp parallel for schedule(static)
+    for (long c = 0; c < n_el; ++c) {
+        const int i = (int)(c / nely);
+        const int j = (int)(c % nely);
+        int32_t *o = out + 4 * c;
+        o[0] = j * nx + i;
+        o[1] = j * nx + i + 1;
+        o[2] = (j + 1) * nx + i + 1;
+        o[3] = (j + 1) * nx + i;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* A2: node -> element incidence (stiffness/CPU/_FEA.py:351-354).            */
+/* The reference sorts elements_flat (argsort) and searches node starts; a   */
+/* counting sort gives the same grouping with element ids ascending inside   */
+/* each node group, which is the order a stable argsort produces.            */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    int n_nodes;
+    int npe;
+    int32_t *start; /* n_nodes + 1 */
+    int32_t *slot;  /* flat index e*npe + a, grouped by node */
+} or_incidence;
+
+static or_incidence or_make_incidence(const int32_t *elements, long n_el, int npe, int n_nodes)
+{
+    or_incidence inc;
+    inc.n_nodes = n_nodes;
+    inc.npe = npe;
+    inc.start = (int32_t *)calloc((size_t)n_nodes + 1, sizeof(int32_t));
+    inc.slot = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n_el * npe));
+    for (long f = 0; f < n_el * npe; ++f)
+        inc.start[elements[f] + 1]++;
+    for (int n = 0; n < n_nodes; ++n)
+        inc.start[n + 1] += inc.start[n];
+    int32_t *fill = (int32_t *)malloc(sizeof(int32_t) * (size_t)n_nodes);
+    memcpy(fill, inc.start, sizeof(int32_t) * (size_t)n_nodes);
+    for (long f = 0; f < n_el * npe; ++f)
+        inc.slot[fill[elements[f]]++] = (int32_t)f;
+    free(fill);
+    return inc;
+}
+
+static void or_free_incidence(or_incidence *inc)
+{
+    free(inc->start);
+    free(inc->slot);
+}
+
+/* ------------------------------------------------------------------------ */
+/* A5: node-basis K(E)u with Dirichlet rows/columns.                         */
+/* core/CPU/_ops.py:330-368 (mat_vec_node_basis_parallel_wcon) and           */
+/* :212-245 (no constraints, con == NULL).                                   */
+/* Constrained row i: out_i = u_i.  Constrained column: skipped.             */
+/* ------------------------------------------------------------------------ */
+static void or_matvec_inc(const double *K0, const int32_t *elements, const or_incidence *inc, int dof,
+                          const double *E, const double *u, const uint8_t *con, double *out)
+{
+    const int npe = inc->npe, nl = npe * dof;
+#pragma omp parallel for schedule(static)
+    for (int n = 0; n < inc->n_nodes; ++n) {
+        for (int c = 0; c < dof; ++c) {
+            const long r = (long)n * dof + c;
+            out[r] = (con && con[r]) ? u[r] : 0.0;
+        }
+        for (int s = inc->start[n]; s < inc->start[n + 1]; ++s) {
+            const int f = inc->slot[s];
+            const int e = f / npe, a = f - e * npe;
+            const int32_t *em = elements + (long)e * npe;
+            for (int c = 0; c < dof; ++c) {
+                const long r = (long)n * dof + c;
+                if (con && con[r])
+                    continue;
+                double val = 0.0;
+                for (int l = 0; l < nl; ++l) {
+                    const long kk = (long)em[l / dof] * dof + l % dof;
+                    if (con && con[kk])
+                        continue;
+                    val += K0[(a * dof + c) * nl + l] * u[kk];
+                }
+                out[r] += val * E[e];
+            }
+        }
+    }
+}
+
+void or_matvec(const double *K0, const int32_t *elements, long n_el, int npe, int dof, int n_nodes,
+               const double *E, const double *u, const uint8_t *con, double *out)
+{
+    or_incidence inc = or_make_incidence(elements, n_el, npe, n_nodes);
+    or_matvec_inc(K0, elements, &inc, dof, E, u, con, out);
+    or_free_incidence(&inc);
+}
+
+/* A6: diag(K(E)); constrained DOF -> 1.  core/CPU/_ops.py:45-75. */
+void or_diag(const double *K0, const int32_t *elements, long n_el, int npe, int dof, int n_nodes,
+             const double *E, const uint8_t *con, double *out)
+{
+    or_incidence inc = or_make_incidence(elements, n_el, npe, n_nodes);
+    const int nl = npe * dof;
+#pragma omp parallel for schedule(static)
+    for (int n = 0; n < n_nodes; ++n) {
+        for (int c = 0; c < dof; ++c)
+            out[(long)n * dof + c] = 0.0;
+        for (int s = inc.start[n]; s < inc.start[n + 1]; ++s) {
+            const int f = inc.slot[s];
+            const int e = f / npe, a = f - e * npe;
+            for (int c = 0; c < dof; ++c) {
+                const long r = (long)n * dof + c;
+                if (con && con[r]) {
+                    out[r] = 1.0;
+                    continue;
+                }
+                out[r] += K0[(a * dof + c) * nl + a * dof + c] * E[e];
+            }
+        }
+    }
+    or_free_incidence(&inc);
+}
+
+/* A14: dc_e = -u_e^T K0 u_e (unscaled by E).  core/CPU/_ops.py:5-13. */
+void or_process_dk(const double *K0, const int32_t *elements, long n_el, int npe, int dof,
+                   const double *u, double *out)
+{
+    const int nl = npe * dof;
+#pragma omp parallel for schedule(static)
+    for (long e = 0; e < n_el; ++e) {
+        double ue[24];
+        for (int l = 0; l < nl; ++l)
+            ue[l] = u[(long)elements[e * npe + l / dof] * dof + l % dof];
+        double acc = 0.0;
+        for (int r = 0; r < nl; ++r) {
+            double t = 0.0;
+            for (int l = 0; l < nl; ++l)
+                t += ue[l] * K0[l * nl + r];
+            acc += t * ue[r];
+        }
+        out[e] = -acc;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* A7: restriction / prolongation.  core/CPU/_mgm.py:10-84 (3D restriction), */
+/* :86-148 (2D), :159-238 (3D prolongation), :240-297 (2D).                  */
+/* normalize = 1 reproduces the reference (division by the boundary-         */
+/* dependent total weight TW); normalize = 0 gives the plain trilinear       */
+/* transfer pair P_lin^T / P_lin used by the rediscretised hierarchy.        */
+/* ------------------------------------------------------------------------ */
+static double or_tw3(int b) { return b == 3 ? 3.375 : (b == 2 ? 4.5 : (b == 1 ? 6.0 : 8.0)); }
+static double or_tw2(int b) { return b == 2 ? 2.25 : (b == 1 ? 3.0 : 4.0); }
+
+void or_restrict_3d(int nelx, int nely, int nelz, int dof, int normalize, const double *v, double *out)
+{
+    const int nxf = nelx + 1, nyf = nely + 1, nzf = nelz + 1;
+    const int nxc = nelx / 2 + 1, nyc = nely / 2 + 1, nzc = nelz / 2 + 1;
+    const long ncn = (long)nxc * nyc * nzc;
+#pragma omp parallel for schedule(static)
+    for (long ic = 0; ic < ncn; ++ic) {
+        const int kc = (int)(ic % nzc);
+        const long t = ic / nzc;
+        const int icx = (int)(t % nxc), jc = (int)(t / nxc);
+        int b = (icx == 0 || icx == nxc - 1) + (jc == 0 || jc == nyc - 1) + (kc == 0 || kc == nzc - 1);
+        const double tw = normalize ? or_tw3(b) : 1.0;
+        for (int d = 0; d < dof; ++d) {
+            double val = 0.0;
+            for (int di = -1; di <= 1; ++di) {
+                const int ii = 2 * icx + di;
+                if (ii < 0 || ii >= nxf)
+                    continue;
+                for (int dj = -1; dj <= 1; ++dj) {
+                    const int jj = 2 * jc + dj;
+                    if (jj < 0 || jj >= nyf)
+                        continue;
+                    for (int dk = -1; dk <= 1; ++dk) {
+                        const int kk = 2 * kc + dk;
+                        if (kk < 0 || kk >= nzf)
+                            continue;
+                        const int dist = abs(di) + abs(dj) + abs(dk);
+                        const double w = dist == 0 ? 1.0 : (dist == 1 ? 0.5 : (dist == 2 ? 0.25 : 0.125));
+                        const long f = kk + (long)nzf * ii + (long)nzf * nxf * jj;
+                        val += w * v[f * dof + d];
+                    }
+                }
+            }
+            out[ic * dof + d] = val / tw;
+        }
+    }
+}
+
+void or_restrict_2d(int nelx, int nely, int dof, int normalize, const double *v, double *out)
+{
+    const int nxf = nelx + 1, nyf = nely + 1;
+    const int nxc = nelx / 2 + 1, nyc = nely / 2 + 1;
+    const long ncn = (long)nxc * nyc;
+#pragma omp parallel for schedule(static)
+    for (long ic = 0; ic < ncn; ++ic) {
+        const int icx = (int)(ic % nxc), jc = (int)(ic / nxc);
+        int b = (icx == 0 || icx == nxc - 1) + (jc == 0 || jc == nyc - 1);
+        const double tw = normalize ? or_tw2(b) : 1.0;
+        for (int d = 0; d < dof; ++d) {
+            double val = 0.0;
+            for (int di = -1; di <= 1; ++di) {
+                const int ii = 2 * icx + di;
+                if (ii < 0 || ii >= nxf)
+                    continue;
+                for (int dj = -1; dj <= 1; ++dj) {
+                    const int jj = 2 * jc + dj;
+                    if (jj < 0 || jj >= nyf)
+                        continue;
+                    const int dist = abs(di) + abs(dj);
+                    const double w = dist == 0 ? 1.0 : (dist == 1 ? 0.5 : 0.25);
+                    val += w * v[((long)jj * nxf + ii) * dof + d];
+                }
+            }
+            out[ic * dof + d] = val / tw;
+        }
+    }
+}
+
+void or_prolong_3d(int nelx, int nely, int nelz, int dof, int normalize, const double *v, double *out)
+{
+    const int nxf = nelx + 1, nyf = nely + 1, nzf = nelz + 1;
+    const int nxc = nelx / 2 + 1, nyc = nely / 2 + 1, nzc = nelz / 2 + 1;
+    const long nfn = (long)nxf * nyf * nzf;
+#pragma omp parallel for schedule(static)
+    for (long f = 0; f < nfn; ++f) {
+        const int k = (int)(f % nzf);
+        const long t = f / nzf;
+        const int i = (int)(t % nxf), j = (int)(t / nxf);
+        for (int d = 0; d < dof; ++d) {
+            double val = 0.0;
+            int count = 0;
+            for (int ci = i / 2; ci <= i / 2 + i % 2; ++ci) {
+                if (ci >= nxc)
+                    continue;
+                for (int cj = j / 2; cj <= j / 2 + j % 2; ++cj) {
+                    if (cj >= nyc)
+                        continue;
+                    for (int ck = k / 2; ck <= k / 2 + k % 2; ++ck) {
+                        if (ck >= nzc)
+                            continue;
+                        int b = (ci == 0 || ci == nxc - 1) + (cj == 0 || cj == nyc - 1) + (ck == 0 || ck == nzc - 1);
+                        const double tw = normalize ? or_tw3(b) : 1.0;
+                        const long c = ck + (long)nzc * ci + (long)nzc * nxc * cj;
+                        val += v[c * dof + d] / tw;
+                        count++;
+                    }
+                }
+            }
+            out[f * dof + d] = count > 0 ? val / count : 0.0;
+        }
+    }
+}
+
+void or_prolong_2d(int nelx, int nely, int dof, int normalize, const double *v, double *out)
+{
+    const int nxf = nelx + 1, nyf = nely + 1;
+    const int nxc = nelx / 2 + 1, nyc = nely / 2 + 1;
+    const long nfn = (long)nxf * nyf;
+#pragma omp parallel for schedule(static)
+    for (long f = 0; f < nfn; ++f) {
+        const int i = (int)(f % nxf), j = (int)(f / nxf);
+        for (int d = 0; d < dof; ++d) {
+            double val = 0.0;
+            int count = 0;
+            for (int ci = i / 2; ci <= i / 2 + i % 2; ++ci) {
+                if (ci >= nxc)
+                    continue;
+                for (int cj = j / 2; cj <= j / 2 + j % 2; ++cj) {
+                    if (cj >= nyc)
+                        continue;
+                    int b = (ci == 0 || ci == nxc - 1) + (cj == 0 || cj == nyc - 1);
+                    const double tw = normalize ? or_tw2(b) : 1.0;
+                    val += v[((long)cj * nxc + ci) * dof + d] / tw;
+                    count++;
+                }
+            }
+            out[f * dof + d] = count > 0 ? val / count : 0.0;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* A15: structured density filter.  core/CPU/_filter.py:6-124.               */
+/* offsets: int32[ntap][dim], weights: float64[ntap] (already normalised by  */
+/* their sum, as geom/CPU/_filters.py:83-88 does).                           */
+/* ------------------------------------------------------------------------ */
+static inline int or_el_in(int dim, const int *nel, const int *p)
+{
+    for (int a = 0; a < dim; ++a)
+        if (p[a] < 0 || p[a] >= nel[a])
+            return 0;
+    return 1;
+}
+
+static inline long or_el_id(int dim, const int *nel, const int *p)
+{
+    return dim == 3 ? ((long)p[0] * nel[1] + p[1]) * nel[2] + p[2] : (long)p[0] * nel[1] + p[1];
+}
+
+static inline void or_el_pos(int dim, const int *nel, long i, int *p)
+{
+    if (dim == 3) {
+        p[0] = (int)(i / ((long)nel[1] * nel[2]));
+        p[1] = (int)((i / nel[2]) % nel[1]);
+        p[2] = (int)(i % nel[2]);
+    } else {
+        p[0] = (int)(i / nel[1]);
+        p[1] = (int)(i % nel[1]);
+    }
+}
+
+void or_filter_normalizer(int dim, const int *nel, int ntap, const int32_t *offsets, const double *weights,
+                          double *norm)
+{
+    long n = (long)nel[0] * nel[1] * (dim == 3 ? nel[2] : 1);
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n; ++i) {
+        int p[3], q[3];
+        or_el_pos(dim, nel, i, p);
+        double s = 0.0;
+        for (int t = 0; t < ntap; ++t) {
+            for (int a = 0; a < dim; ++a)
+                q[a] = p[a] + offsets[t * dim + a];
+            if (or_el_in(dim, nel, q))
+                s += weights[t];
+        }
+        norm[i] = s;
+    }
+}
+
+void or_filter_fwd(int dim, const int *nel, int ntap, const int32_t *offsets, const double *weights,
+                   const double *x, double *out)
+{
+    long n = (long)nel[0] * nel[1] * (dim == 3 ? nel[2] : 1);
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n; ++i) {
+        int p[3], q[3];
+        or_el_pos(dim, nel, i, p);
+        double sv = 0.0, sw = 0.0;
+        for (int t = 0; t < ntap; ++t) {
+            for (int a = 0; a < dim; ++a)
+                q[a] = p[a] + offsets[t * dim + a];
+            if (or_el_in(dim, nel, q)) {
+                sv += weights[t] * x[or_el_id(dim, nel, q)];
+                sw += weights[t];
+            }
+        }
+        out[i] = sw > 0 ? sv / sw : 0.0;
+    }
+}
+
+void or_filter_adj(int dim, const int *nel, int ntap, const int32_t *offsets, const double *weights,
+                   const double *norm, const double *y, double *out)
+{
+    long n = (long)nel[0] * nel[1] * (dim == 3 ? nel[2] : 1);
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n; ++i) {
+        int p[3], q[3];
+        or_el_pos(dim, nel, i, p);
+        double sv = 0.0;
+        for (int t = 0; t < ntap; ++t) {
+            for (int a = 0; a < dim; ++a)
+                q[a] = p[a] + offsets[t * dim + a];
+            if (or_el_in(dim, nel, q)) {
+                const long j = or_el_id(dim, nel, q);
+                sv += (weights[t] / norm[j]) * y[j];
+            }
+        }
+        out[i] = sv;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* A16: OC update with bisection on the raw-variable volume constraint.      */
+/* Optimizers/CPU/OC.py:82-140 (single material), constraint                  */
+/* Problem/CPU/MinimumCompliance.py:547-575 with nabla_g = As/volume.        */
+/* Writes x_new; returns the final Lagrange multiplier midpoint.             */
+/* ------------------------------------------------------------------------ */
+double or_oc_update(long n, const double *x, const double *df, const double *dg, double vf, double move,
+                    double *x_new)
+{
+    double *ocp = (double *)malloc(sizeof(double) * (size_t)n);
+    double asum = 0.0;
+    for (long i = 0; i < n; ++i) {
+        double q = -df[i] / dg[i];
+        double s = q > 0 ? sqrt(q) : 0.0;
+        if (isnan(s))
+            s = 0.0;
+        ocp[i] = x[i] * s;
+        asum += fabs(ocp[i]);
+    }
+    if (asum == 0.0)
+        for (long i = 0; i < n; ++i)
+            ocp[i] = 1e-3;
+    double l1 = 1e-9, l2 = 1e9, lm = 0.0;
+    while ((l2 - l1) / (l2 + l1) > 1e-8) {
+        lm = (l1 + l2) / 2;
+        double g = 0.0;
+        for (long i = 0; i < n; ++i) {
+            double xu = fmin(fmax(x[i] + move, 0.0), 1.0), xl = fmin(fmax(x[i] - move, 0.0), 1.0);
+            double v = fmax(0.0, fmax(xl, fmin(1.0, fmin(xu, ocp[i] / lm))));
+            x_new[i] = v;
+            g += dg[i] * v;
+        }
+        if (g - vf <= 0.0)
+            l2 = lm;
+        else
+            l1 = lm;
+    }
+    free(ocp);
+    return lm;
+}
+
+/* ------------------------------------------------------------------------ */
+/* GMG-PCG (FP64), the baseline solver: Jacobi-smoothed V-cycle with          */
+/* rediscretised matrix-free coarse operators and a dense Cholesky coarsest   */
+/* solve, inside PCG (solvers/CPU/_solvers.py:1036-1147 for the cycle and     */
+/* PCG structure; fixed omega; zero initial guesses).                         */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    int nel[3];
+    long n_el, n_nodes, n_dof;
+    int32_t *elements;
+    or_incidence inc;
+    double *E;
+    uint8_t *con;
+    double *K0;
+    double *dinv;
+    double *x, *b, *r, *t;
+} or_level;
+
+static void or_level_matvec(const or_level *L, int dof, const double *u, double *out)
+{
+    or_matvec_inc(L->K0, L->elements, &L->inc, dof, L->E, u, L->con, out);
+}
+
+static void or_coarse_E(int dim, const int *nelf, const double *Ef, double *Ec)
+{
+    int nelc[3] = {nelf[0] / 2, nelf[1] / 2, dim == 3 ? nelf[2] / 2 : 1};
+    long nc = (long)nelc[0] * nelc[1] * nelc[2];
+    for (long c = 0; c < nc; ++c) {
+        int p[3];
+        or_el_pos(dim, nelc, c, p);
+        double s = 0.0;
+        int m = 0;
+        for (int a = 0; a < 2; ++a)
+            for (int bb = 0; bb < 2; ++bb)
+                for (int cc = 0; cc < (dim == 3 ? 2 : 1); ++cc) {
+                    int q[3] = {2 * p[0] + a, 2 * p[1] + bb, 2 * p[2] + cc};
+                    s += Ef[or_el_id(dim, nelf, q)];
+                    m++;
+                }
+        Ec[c] = s / m;
+    }
+}
+
+/* coarse constraint: a coarse DOF is constrained iff the coincident fine DOF is */
+static void or_coarse_con(int dim, const int *nelf, int dof, const uint8_t *cf, uint8_t *cc)
+{
+    int nxf = nelf[0] + 1, nzf = dim == 3 ? nelf[2] + 1 : 1;
+    int nxc = nelf[0] / 2 + 1, nyc = nelf[1] / 2 + 1, nzc = dim == 3 ? nelf[2] / 2 + 1 : 1;
+    for (int j = 0; j < nyc; ++j)
+        for (int i = 0; i < nxc; ++i)
+            for (int k = 0; k < nzc; ++k) {
+                long nc = dim == 3 ? k + (long)nzc * i + (long)nzc * nxc * j : (long)j * nxc + i;
+                long nf = dim == 3 ? 2 * k + (long)nzf * 2 * i + (long)nzf * nxf * 2 * j : (long)2 * j * nxf + 2 * i;
+                for (int d = 0; d < dof; ++d)
+                    cc[nc * dof + d] = cf[nf * dof + d];
+            }
+}
+
+/* dense Cholesky (lower, in place); returns 0 on success */
+static int or_cholesky(double *A, int n)
+{
+    for (int j = 0; j < n; ++j) {
+        double d = A[(long)j * n + j];
+        for (int k = 0; k < j; ++k)
+            d -= A[(long)j * n + k] * A[(long)j * n + k];
+        if (!(d > 0))
+            return -1;
+        d = sqrt(d);
+        A[(long)j * n + j] = d;
+#pragma omp parallel for schedule(static)
+        for (int i = j + 1; i < n; ++i) {
+            double s = A[(long)i * n + j];
+            for (int k = 0; k < j; ++k)
+                s -= A[(long)i * n + k] * A[(long)j * n + k];
+            A[(long)i * n + j] = s / d;
+        }
+    }
+    return 0;
+}
+
+static void or_chol_solve(const double *L, int n, const double *b, double *x)
+{
+    for (int i = 0; i < n; ++i) {
+        double s = b[i];
+        for (int k = 0; k < i; ++k)
+            s -= L[(long)i * n + k] * x[k];
+        x[i] = s / L[(long)i * n + i];
+    }
+    for (int i = n - 1; i >= 0; --i) {
+        double s = x[i];
+        for (int k = i + 1; k < n; ++k)
+            s -= L[(long)k * n + i] * x[k];
+        x[i] = s / L[(long)i * n + i];
+    }
+}
+
+typedef struct {
+    int dim, dof, nlev;
+    double omega;
+    int n_smooth;
+    or_level lev[12];
+    double *Lc; /* dense Cholesky factor of the coarsest operator */
+    int nc;
+} or_mg;
+
+static void or_vcycle(or_mg *mg, int l, const double *b, double *x)
+{
+    or_level *L = &mg->lev[l];
+    const int dof = mg->dof;
+    if (l == mg->nlev - 1) {
+        or_chol_solve(mg->Lc, mg->nc, b, x);
+        return;
+    }
+    const long n = L->n_dof;
+    /* pre-smoothing from zero */
+    for (long i = 0; i < n; ++i)
+        x[i] = 0.0;
+    for (int s = 0; s < mg->n_smooth; ++s) {
+        or_level_matvec(L, dof, x, L->t);
+#pragma omp parallel for schedule(static)
+        for (long i = 0; i < n; ++i)
+            x[i] += mg->omega * L->dinv[i] * (b[i] - L->t[i]);
+    }
+    or_level_matvec(L, dof, x, L->t);
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n; ++i)
+        L->r[i] = b[i] - L->t[i];
+    or_level *C = &mg->lev[l + 1];
+    if (mg->dim == 3)
+        or_restrict_3d(L->nel[0], L->nel[1], L->nel[2], dof, 0, L->r, C->b);
+    else
+        or_restrict_2d(L->nel[0], L->nel[1], dof, 0, L->r, C->b);
+    for (long i = 0; i < C->n_dof; ++i)
+        if (C->con[i])
+            C->b[i] = 0.0;
+    or_vcycle(mg, l + 1, C->b, C->x);
+    if (mg->dim == 3)
+        or_prolong_3d(L->nel[0], L->nel[1], L->nel[2], dof, 0, C->x, L->t);
+    else
+        or_prolong_2d(L->nel[0], L->nel[1], dof, 0, C->x, L->t);
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n; ++i)
+        x[i] += L->con[i] ? 0.0 : L->t[i];
+    for (int s = 0; s < mg->n_smooth; ++s) {
+        or_level_matvec(L, dof, x, L->t);
+#pragma omp parallel for schedule(static)
+        for (long i = 0; i < n; ++i)
+            x[i] += mg->omega * L->dinv[i] * (b[i] - L->t[i]);
+    }
+}
+
+static double or_dot(const double *a, const double *b, long n)
+{
+    double s = 0.0;
+#pragma omp parallel for reduction(+ : s) schedule(static)
+    for (long i = 0; i < n; ++i)
+        s += a[i] * b[i];
+    return s;
+}
+
+/*
+ * Solve K(E) u = f.  K0f is the fine element matrix; coarse element matrices
+ * are K0f * 2^level in 3D (stiffness of a linear-elastic hex scales with its
+ * edge length) and K0f in 2D (plane stress, unit thickness).
+ * nlev_coarse coarsenings, n_smooth pre/post Jacobi sweeps with weight omega.
+ * Returns the PCG iteration count (negative if the coarsest factorisation
+ * failed); *rel_res gets the final recursive ||r||/||f||.
+ * max_iter limits the PCG loop (a bounded sample for timing).
+ */
+int or_mgpcg_solve(int dim, const int *nel, const double *K0f, const double *E, const uint8_t *con,
+                   const double *f, double *u, int nlev_coarse, int n_smooth, double omega, double tol,
+                   int max_iter, double *rel_res)
+{
+    const int npe = dim == 3 ? 8 : 4, dof = dim == 3 ? 3 : 2, nl = npe * dof;
+    or_mg mg;
+    memset(&mg, 0, sizeof(mg));
+    mg.dim = dim;
+    mg.dof = dof;
+    mg.nlev = nlev_coarse + 1;
+    mg.omega = omega;
+    mg.n_smooth = n_smooth;
+    for (int l = 0; l < mg.nlev; ++l) {
+        or_level *L = &mg.lev[l];
+        for (int a = 0; a < 3; ++a)
+            L->nel[a] = a < dim ? (nel[a] >> l) : 1;
+        L->n_el = (long)L->nel[0] * L->nel[1] * L->nel[2];
+        L->n_nodes = (long)(L->nel[0] + 1) * (L->nel[1] + 1) * (dim == 3 ? L->nel[2] + 1 : 1);
+        L->n_dof = L->n_nodes * dof;
+        L->elements = (int32_t *)malloc(sizeof(int32_t) * (size_t)(L->n_el * npe));
+        if (dim == 3)
+            or_elements_3d(L->nel[0], L->nel[1], L->nel[2], L->elements);
+        else
+            or_elements_2d(L->nel[0], L->nel[1], L->elements);
+        L->inc = or_make_incidence(L->elements, L->n_el, npe, (int)L->n_nodes);
+        L->E = (double *)malloc(sizeof(double) * (size_t)L->n_el);
+        L->con = (uint8_t *)malloc((size_t)L->n_dof);
+        if (l == 0) {
+            memcpy(L->E, E, sizeof(double) * (size_t)L->n_el);
+            for (long i = 0; i < L->n_dof; ++i)
+                L->con[i] = con ? con[i] : 0;
+        } else {
+            or_coarse_E(dim, mg.lev[l - 1].nel, mg.lev[l - 1].E, L->E);
+            or_coarse_con(dim, mg.lev[l - 1].nel, dof, mg.lev[l - 1].con, L->con);
+        }
+        L->K0 = (double *)malloc(sizeof(double) * (size_t)(nl * nl));
+        const double sc = dim == 3 ? (double)(1L << l) : 1.0;
+        for (int q = 0; q < nl * nl; ++q)
+            L->K0[q] = K0f[q] * sc;
+        L->dinv = (double *)malloc(sizeof(double) * (size_t)L->n_dof);
+        L->x = (double *)calloc((size_t)L->n_dof, sizeof(double));
+        L->b = (double *)calloc((size_t)L->n_dof, sizeof(double));
+        L->r = (double *)calloc((size_t)L->n_dof, sizeof(double));
+        L->t = (double *)calloc((size_t)L->n_dof, sizeof(double));
+        or_diag(L->K0, L->elements, L->n_el, npe, dof, (int)L->n_nodes, L->E, L->con, L->dinv);
+        for (long i = 0; i < L->n_dof; ++i)
+            L->dinv[i] = 1.0 / L->dinv[i];
+    }
+    /* dense coarsest operator by applying K to unit vectors */
+    or_level *Lc = &mg.lev[mg.nlev - 1];
+    int nc = (int)Lc->n_dof;
+    mg.nc = nc;
+    mg.Lc = (double *)malloc(sizeof(double) * (size_t)nc * nc);
+    {
+        double *ev = (double *)calloc((size_t)nc, sizeof(double));
+        double *col = (double *)malloc(sizeof(double) * (size_t)nc);
+        for (int j = 0; j < nc; ++j) {
+            ev[j] = 1.0;
+            or_level_matvec(Lc, dof, ev, col);
+            ev[j] = 0.0;
+            for (int i = 0; i < nc; ++i)
+                mg.Lc[(long)i * nc + j] = col[i];
+        }
+        free(ev);
+        free(col);
+    }
+    int it = 0;
+    int ok = or_cholesky(mg.Lc, nc);
+    if (ok == 0) {
+        or_level *F = &mg.lev[0];
+        const long n = F->n_dof;
+        double *r = (double *)malloc(sizeof(double) * (size_t)n);
+        double *z = (double *)malloc(sizeof(double) * (size_t)n);
+        double *p = (double *)malloc(sizeof(double) * (size_t)n);
+        double *q = (double *)malloc(sizeof(double) * (size_t)n);
+        for (long i = 0; i < n; ++i)
+            u[i] = 0.0;
+        or_level_matvec(F, dof, u, q);
+        for (long i = 0; i < n; ++i)
+            r[i] = f[i] - q[i];
+        double nb = sqrt(or_dot(f, f, n));
+        if (nb == 0.0)
+            nb = 1.0;
+        double R = sqrt(or_dot(r, r, n)) / nb;
+        if (R >= tol) {
+            or_vcycle(&mg, 0, r, z);
+            memcpy(p, z, sizeof(double) * (size_t)n);
+            double rz = or_dot(r, z, n);
+            for (it = 1; it <= max_iter; ++it) {
+                or_level_matvec(F, dof, p, q);
+                const double alpha = rz / or_dot(p, q, n);
+#pragma omp parallel for schedule(static)
+                for (long i = 0; i < n; ++i) {
+                    u[i] += alpha * p[i];
+                    r[i] -= alpha * q[i];
+                }
+                R = sqrt(or_dot(r, r, n)) / nb;
+                if (R < tol)
+                    break;
+                or_vcycle(&mg, 0, r, z);
+                const double rz_new = or_dot(r, z, n);
+                const double beta = rz_new / rz;
+                rz = rz_new;
+#pragma omp parallel for schedule(static)
+                for (long i = 0; i < n; ++i)
+                    p[i] = z[i] + beta * p[i];
+            }
+            if (it > max_iter)
+                it = max_iter;
+        }
+        *rel_res = R;
+        free(r);
+        free(z);
+        free(p);
+        free(q);
+    } else {
+        it = -1;
+    }
+    for (int l = 0; l < mg.nlev; ++l) {
+        or_level *L = &mg.lev[l];
+        free(L->elements);
+        or_free_incidence(&L->inc);
+        free(L->E);
+        free(L->con);
+        free(L->K0);
+        free(L->dinv);
+        free(L->x);
+        free(L->b);
+        free(L->r);
+        free(L->t);
+    }
+    free(mg.Lc);
+    return it;
+}
+
+int or_num_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}

@@ -1,0 +1,93 @@
+// Shared helpers for the gfx950 kernels behind the fantom C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <cstdio>
+#include <cstring>
+
+namespace fantom {
+
+// Thread-local message of the last failed call, returned by fantom_last_error().
+void set_error(const char *fmt, ...);
+const char *get_error();
+
+constexpr int kStatusOk = 0;
+constexpr int kStatusBadArg = 1;     // maps to ValueError on the Python side
+constexpr int kStatusHipError = 2;   // maps to RuntimeError
+constexpr int kStatusNotSupported = 3;
+
+inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+#define FANTOM_CHECK_ARG(cond, ...)                                                                        \
+    do {                                                                                                   \
+        if (!(cond)) {                                                                                     \
+            ::fantom::set_error(__VA_ARGS__);                                                              \
+            return ::fantom::kStatusBadArg;                                                                \
+        }                                                                                                  \
+    } while (0)
+
+#define FANTOM_LAUNCH_CHECK(where)                                                                         \
+    do {                                                                                                   \
+        hipError_t _e = hipGetLastError();                                                                 \
+        if (_e != hipSuccess) {                                                                            \
+            ::fantom::set_error("%s: %s", where, hipGetErrorString(_e));                                  \
+            return ::fantom::kStatusHipError;                                                              \
+        }                                                                                                  \
+    } while (0)
+
+// Structured grid of nelx x nely x nelz elements (nelz = 0 marks 2D).
+struct Grid {
+    int nelx, nely, nelz;
+    int nx, ny, nz;  // nodes per axis (nz = 1 in 2D)
+    long n_nodes, n_el;
+};
+
+inline Grid make_grid(int nelx, int nely, int nelz)
+{
+    Grid g;
+    g.nelx = nelx;
+    g.nely = nely;
+    g.nelz = nelz;
+    g.nx = nelx + 1;
+    g.ny = nely + 1;
+    g.nz = nelz > 0 ? nelz + 1 : 1;
+    g.n_nodes = (long)g.nx * g.ny * g.nz;
+    g.n_el = (long)nelx * nely * (nelz > 0 ? nelz : 1);
+    return g;
+}
+
+inline bool grid_ok(int nelx, int nely, int nelz)
+{
+    if (nelx <= 0 || nely <= 0 || nelz < 0)
+        return false;
+    // 32-bit DOF indices are used on the device
+    long nn = (long)(nelx + 1) * (nely + 1) * (nelz > 0 ? nelz + 1 : 1);
+    return nn * 3 < (1L << 31);
+}
+
+// Deterministic block sum: wave64 butterfly, then the first wave sums the
+// per-wave results in a fixed order.  Returns the sum in thread 0.
+template <int BLOCK>
+__device__ inline double block_sum(double v)
+{
+    static_assert(BLOCK % 64 == 0 && BLOCK <= 1024, "block must be a multiple of 64");
+    __shared__ double wsum[BLOCK / 64];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        v += __shfl_xor(v, off, 64);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0)
+        wsum[wid] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int w = 0; w < BLOCK / 64; ++w)
+            s += wsum[w];
+    }
+    __syncthreads();
+    return s;
+}
+
+}  // namespace fantom

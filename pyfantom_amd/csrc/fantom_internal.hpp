@@ -1,0 +1,36 @@
+// Launchers shared between the kernel files and the extern "C" layer.
+#pragma once
+#include "common.hpp"
+
+namespace fantom {
+
+enum { MODE_KU = 0, MODE_JAC = 1, MODE_RES = 2, MODE_KUDOT = 3 };
+
+// ku_node.hip
+int launch_ku_node(int mode, const Grid &g, int dim, const double *K0, const double *E, const uint8_t *mask,
+                   const double *u, double *y, const double *b, const double *dinv, double omega,
+                   double *partials, hipStream_t s);
+long ku_node_partials(const Grid &g);
+int launch_diag(const Grid &g, int dim, const double *K0, const double *E, const uint8_t *mask, double *d,
+                int invert, hipStream_t s);
+int launch_dc(const Grid &g, int dim, const double *K0, const double *u, double *dc, hipStream_t s);
+
+// ku_tile.hip (hex8, element-tiled Walsh-Hadamard form)
+int launch_ku_tile(int mode, const Grid &g, const double *Kb, const double *E, const uint8_t *mask,
+                   const double *u, double *y, const double *b, const double *dinv, double omega,
+                   double *partials, hipStream_t s);
+long ku_tile_partials(const Grid &g);
+
+// mg.hip
+int launch_restrict(const Grid &gf, int dim, int dof, int normalize, const uint8_t *mask_c, const double *rf,
+                    double *rc, hipStream_t s);
+int launch_prolong(const Grid &gf, int dim, int dof, int normalize, const uint8_t *mask_f, const double *ec,
+                   double *xf, int accumulate, hipStream_t s);
+int launch_coarsen_E(const Grid &gf, int dim, const double *Ef, double *Ec, hipStream_t s);
+int launch_coarsen_mask(const Grid &gf, int dim, const uint8_t *mf, uint8_t *mc, hipStream_t s);
+int launch_dense_assemble(const Grid &g, int dim, const double *K0, const double *E, const uint8_t *mask,
+                          double *A, hipStream_t s);
+int launch_spd_inverse(int n, double *A, double *work, hipStream_t s);
+int launch_gemv(int n, const double *A, const double *x, double *y, hipStream_t s);
+
+}  // namespace fantom

@@ -1,0 +1,426 @@
+// Geometric multigrid transfers and the coarsest-level direct solve (FP64).
+//
+// Restriction / prolongation follow the reference's structured transfer
+// (core/CUDA/Kernels/mgm.cu:1-244, core/CPU/_mgm.py:10-297): coarse node c sits
+// on fine node 2c; full weighting with weights 2^-dist over the 3^d fine
+// neighbourhood.  normalize=1 divides by the boundary-dependent total weight TW
+// exactly as the reference does; normalize=0 gives the plain trilinear pair
+// P_lin^T / P_lin used by the rediscretised hierarchy (the V-cycle with an
+// exact coarse solve is invariant to that diagonal scaling).
+#include "common.hpp"
+#include "fantom_internal.hpp"
+
+namespace fantom {
+
+constexpr int kBlock = 256;
+
+__device__ __forceinline__ double tw3(int b) { return b == 3 ? 3.375 : (b == 2 ? 4.5 : (b == 1 ? 6.0 : 8.0)); }
+__device__ __forceinline__ double tw2(int b) { return b == 2 ? 2.25 : (b == 1 ? 3.0 : 4.0); }
+
+// One thread per coarse node.  mask_c (optional): constrained coarse DOFs get 0.
+__global__ __launch_bounds__(kBlock) void restrict3d_f64(Grid gf, int dof, int normalize,
+                                                       const uint8_t *__restrict__ mask_c,
+                                                       const double *__restrict__ rf, double *__restrict__ rc)
+{
+    const int nxc = gf.nelx / 2 + 1, nyc = gf.nely / 2 + 1, nzc = gf.nelz / 2 + 1;
+    const long ncn = (long)nxc * nyc * nzc;
+    const long ic = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (ic >= ncn)
+        return;
+    const int kc = (int)(ic % nzc);
+    const long t = ic / nzc;
+    const int icx = (int)(t % nxc), jc = (int)(t / nxc);
+    const int bnd = (icx == 0 || icx == nxc - 1) + (jc == 0 || jc == nyc - 1) + (kc == 0 || kc == nzc - 1);
+    double acc[3] = {0.0, 0.0, 0.0};
+    // x outer, y, z inner: the reference's summation order (mgm.cu:28-60)
+    for (int di = -1; di <= 1; ++di) {
+        const int ii = 2 * icx + di;
+        if (ii < 0 || ii >= gf.nx)
+            continue;
+        for (int dj = -1; dj <= 1; ++dj) {
+            const int jj = 2 * jc + dj;
+            if (jj < 0 || jj >= gf.ny)
+                continue;
+            for (int dk = -1; dk <= 1; ++dk) {
+                const int kk = 2 * kc + dk;
+                if (kk < 0 || kk >= gf.nz)
+                    continue;
+                const int dist = abs(di) + abs(dj) + abs(dk);
+                const double w = dist == 0 ? 1.0 : (dist == 1 ? 0.5 : (dist == 2 ? 0.25 : 0.125));
+                const long f = kk + (long)gf.nz * ii + (long)gf.nz * gf.nx * jj;
+                for (int d = 0; d < dof; ++d)
+                    acc[d] += w * rf[f * dof + d];
+            }
+        }
+    }
+    const int mc = mask_c ? mask_c[ic] : 0;
+    for (int d = 0; d < dof; ++d) {
+        // divide (not multiply by 1/TW) to round like the reference (mgm.cu:66)
+        const double v = normalize ? acc[d] / tw3(bnd) : acc[d];
+        rc[ic * dof + d] = ((mc >> d) & 1) ? 0.0 : v;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void restrict2d_f64(Grid gf, int dof, int normalize,
+                                                       const uint8_t *__restrict__ mask_c,
+                                                       const double *__restrict__ rf, double *__restrict__ rc)
+{
+    const int nxc = gf.nelx / 2 + 1, nyc = gf.nely / 2 + 1;
+    const long ncn = (long)nxc * nyc;
+    const long ic = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (ic >= ncn)
+        return;
+    const int icx = (int)(ic % nxc), jc = (int)(ic / nxc);
+    const int bnd = (icx == 0 || icx == nxc - 1) + (jc == 0 || jc == nyc - 1);
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int di = -1; di <= 1; ++di) {
+        const int ii = 2 * icx + di;
+        if (ii < 0 || ii >= gf.nx)
+            continue;
+        for (int dj = -1; dj <= 1; ++dj) {
+            const int jj = 2 * jc + dj;
+            if (jj < 0 || jj >= gf.ny)
+                continue;
+            const int dist = abs(di) + abs(dj);
+            const double w = dist == 0 ? 1.0 : (dist == 1 ? 0.5 : 0.25);
+            const long f = (long)jj * gf.nx + ii;
+            for (int d = 0; d < dof; ++d)
+                acc[d] += w * rf[f * dof + d];
+        }
+    }
+    const int mc = mask_c ? mask_c[ic] : 0;
+    for (int d = 0; d < dof; ++d) {
+        const double v = normalize ? acc[d] / tw2(bnd) : acc[d];
+        rc[ic * dof + d] = ((mc >> d) & 1) ? 0.0 : v;
+    }
+}
+
+// One thread per fine node: average of parent coarse values (each divided by
+// its TW when normalize).  accumulate: xf += P ec, else xf = P ec.
+// mask_f (optional): constrained fine DOFs receive 0.
+__global__ __launch_bounds__(kBlock) void prolong3d_f64(Grid gf, int dof, int normalize,
+                                                      const uint8_t *__restrict__ mask_f,
+                                                      const double *__restrict__ ec, double *__restrict__ xf,
+                                                      int accumulate)
+{
+    const int nxc = gf.nelx / 2 + 1, nyc = gf.nely / 2 + 1, nzc = gf.nelz / 2 + 1;
+    const long f = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (f >= gf.n_nodes)
+        return;
+    const int k = (int)(f % gf.nz);
+    const long t = f / gf.nz;
+    const int i = (int)(t % gf.nx), j = (int)(t / gf.nx);
+    double acc[3] = {0.0, 0.0, 0.0};
+    int count = 0;
+    for (int ci = i / 2; ci <= i / 2 + (i & 1); ++ci) {
+        if (ci >= nxc)
+            continue;
+        for (int cj = j / 2; cj <= j / 2 + (j & 1); ++cj) {
+            if (cj >= nyc)
+                continue;
+            for (int ck = k / 2; ck <= k / 2 + (k & 1); ++ck) {
+                if (ck >= nzc)
+                    continue;
+                const int bnd = (ci == 0 || ci == nxc - 1) + (cj == 0 || cj == nyc - 1) + (ck == 0 || ck == nzc - 1);
+                const long c = ck + (long)nzc * ci + (long)nzc * nxc * cj;
+                for (int d = 0; d < dof; ++d)
+                    acc[d] += normalize ? ec[c * dof + d] / tw3(bnd) : ec[c * dof + d];
+                count++;
+            }
+        }
+    }
+    const int mf = mask_f ? mask_f[f] : 0;
+    for (int d = 0; d < dof; ++d) {
+        double v = count > 0 ? acc[d] / count : 0.0;
+        if ((mf >> d) & 1)
+            v = 0.0;
+        const long r = f * dof + d;
+        xf[r] = accumulate ? xf[r] + v : v;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void prolong2d_f64(Grid gf, int dof, int normalize,
+                                                      const uint8_t *__restrict__ mask_f,
+                                                      const double *__restrict__ ec, double *__restrict__ xf,
+                                                      int accumulate)
+{
+    const int nxc = gf.nelx / 2 + 1, nyc = gf.nely / 2 + 1;
+    const long f = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (f >= gf.n_nodes)
+        return;
+    const int i = (int)(f % gf.nx), j = (int)(f / gf.nx);
+    double acc[3] = {0.0, 0.0, 0.0};
+    int count = 0;
+    for (int ci = i / 2; ci <= i / 2 + (i & 1); ++ci) {
+        if (ci >= nxc)
+            continue;
+        for (int cj = j / 2; cj <= j / 2 + (j & 1); ++cj) {
+            if (cj >= nyc)
+                continue;
+            const int bnd = (ci == 0 || ci == nxc - 1) + (cj == 0 || cj == nyc - 1);
+            const long c = (long)cj * nxc + ci;
+            for (int d = 0; d < dof; ++d)
+                acc[d] += normalize ? ec[c * dof + d] / tw2(bnd) : ec[c * dof + d];
+            count++;
+        }
+    }
+    const int mf = mask_f ? mask_f[f] : 0;
+    for (int d = 0; d < dof; ++d) {
+        double v = count > 0 ? acc[d] / count : 0.0;
+        if ((mf >> d) & 1)
+            v = 0.0;
+        const long r = f * dof + d;
+        xf[r] = accumulate ? xf[r] + v : v;
+    }
+}
+
+// Coarse element modulus = mean of its 2^d children.
+__global__ __launch_bounds__(kBlock) void coarsen_E_f64(Grid gf, int dim, const double *__restrict__ Ef,
+                                                      double *__restrict__ Ec)
+{
+    const int ncx = gf.nelx / 2, ncy = gf.nely / 2, ncz = dim == 3 ? gf.nelz / 2 : 1;
+    const long n = (long)ncx * ncy * ncz;
+    const long c = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (c >= n)
+        return;
+    if (dim == 3) {
+        const int ci = (int)(c / ((long)ncy * ncz));
+        const int r = (int)(c % ((long)ncy * ncz));
+        const int cj = r / ncz, ck = r % ncz;
+        double s = 0.0;
+        for (int a = 0; a < 2; ++a)
+            for (int b = 0; b < 2; ++b)
+                for (int d = 0; d < 2; ++d)
+                    s += Ef[((long)(2 * ci + a) * gf.nely + (2 * cj + b)) * gf.nelz + (2 * ck + d)];
+        Ec[c] = s * 0.125;
+    } else {
+        const int ci = (int)(c / ncy), cj = (int)(c % ncy);
+        double s = 0.0;
+        for (int a = 0; a < 2; ++a)
+            for (int b = 0; b < 2; ++b)
+                s += Ef[(long)(2 * ci + a) * gf.nely + (2 * cj + b)];
+        Ec[c] = s * 0.25;
+    }
+}
+
+// A coarse DOF is constrained iff its coincident fine DOF is.
+__global__ __launch_bounds__(kBlock) void coarsen_mask_u8(Grid gf, int dim, const uint8_t *__restrict__ mf,
+                                                        uint8_t *__restrict__ mc)
+{
+    const int nxc = gf.nelx / 2 + 1, nyc = gf.nely / 2 + 1, nzc = dim == 3 ? gf.nelz / 2 + 1 : 1;
+    const long n = (long)nxc * nyc * nzc;
+    const long c = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (c >= n)
+        return;
+    long f;
+    if (dim == 3) {
+        const int k = (int)(c % nzc);
+        const long t = c / nzc;
+        const int i = (int)(t % nxc), j = (int)(t / nxc);
+        f = 2 * k + (long)gf.nz * 2 * i + (long)gf.nz * gf.nx * 2 * j;
+    } else {
+        const int i = (int)(c % nxc), j = (int)(c / nxc);
+        f = (long)2 * j * gf.nx + 2 * i;
+    }
+    mc[c] = mf[f];
+}
+
+// Dense K(E) with the Dirichlet convention, one thread per row.  A must be
+// zero on entry (row-major n_dof x n_dof).
+__global__ __launch_bounds__(kBlock) void dense_assemble_f64(Grid g, int dim, const double *__restrict__ K0,
+                                                           const double *__restrict__ E,
+                                                           const uint8_t *__restrict__ mask, double *__restrict__ A)
+{
+    const int dof = dim == 3 ? 3 : 2, nl = dim == 3 ? 24 : 8;
+    const long ndof = g.n_nodes * dof;
+    const long row = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (row >= ndof)
+        return;
+    const long n = row / dof;
+    const int c = (int)(row % dof);
+    double *Ar = A + row * ndof;
+    const int mn = mask ? mask[n] : 0;
+    if ((mn >> c) & 1) {
+        Ar[row] = 1.0;
+        return;
+    }
+    if (dim == 3) {
+        const int k = (int)(n % g.nz);
+        const long t = n / g.nz;
+        const int i = (int)(t % g.nx), j = (int)(t / g.nx);
+        const long plane = (long)g.nx * g.nz;
+        for (int oy = 0; oy < 2; ++oy)
+            for (int ox = 0; ox < 2; ++ox)
+                for (int oz = 0; oz < 2; ++oz) {
+                    const int ei = i - ox, ej = j - oy, ek = k - oz;
+                    if (ei < 0 || ei >= g.nelx || ej < 0 || ej >= g.nely || ek < 0 || ek >= g.nelz)
+                        continue;
+                    const double Ee = E[((long)ei * g.nely + ej) * g.nelz + ek];
+                    const int a = oz * 4 + (oy ? (ox ? 2 : 3) : (ox ? 1 : 0));
+                    for (int cc = 0; cc < 8; ++cc) {
+                        const int cx = (cc == 1 || cc == 2 || cc == 5 || cc == 6);
+                        const int cy = (cc == 2 || cc == 3 || cc == 6 || cc == 7);
+                        const int cz = cc >= 4;
+                        const long m = (long)(ej + cy) * plane + (long)(ei + cx) * g.nz + (ek + cz);
+                        const int mm = mask ? mask[m] : 0;
+                        for (int d = 0; d < 3; ++d) {
+                            if ((mm >> d) & 1)
+                                continue;
+                            Ar[m * 3 + d] += Ee * K0[(a * 3 + c) * nl + cc * 3 + d];
+                        }
+                    }
+                }
+    } else {
+        const int i = (int)(n % g.nx), j = (int)(n / g.nx);
+        for (int oy = 0; oy < 2; ++oy)
+            for (int ox = 0; ox < 2; ++ox) {
+                const int ei = i - ox, ej = j - oy;
+                if (ei < 0 || ei >= g.nelx || ej < 0 || ej >= g.nely)
+                    continue;
+                const double Ee = E[(long)ei * g.nely + ej];
+                const int a = oy ? (ox ? 2 : 3) : (ox ? 1 : 0);
+                for (int cc = 0; cc < 4; ++cc) {
+                    const int cx = (cc == 1 || cc == 2), cy = cc >= 2;
+                    const long m = (long)(ej + cy) * g.nx + (ei + cx);
+                    const int mm = mask ? mask[m] : 0;
+                    for (int d = 0; d < 2; ++d) {
+                        if ((mm >> d) & 1)
+                            continue;
+                        Ar[m * 2 + d] += Ee * K0[(a * 2 + c) * nl + cc * 2 + d];
+                    }
+                }
+            }
+    }
+}
+
+// In-place Gauss-Jordan inversion of an SPD matrix (no pivoting needed), one
+// pivot per pair of launches.  work holds 2n doubles: the old pivot column and
+// the scaled pivot row.
+__global__ __launch_bounds__(kBlock) void gj_stage(int n, int k, const double *__restrict__ A,
+                                                 double *__restrict__ work)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n)
+        return;
+    const double p = A[(long)k * n + k];
+    work[i] = A[(long)i * n + k];
+    work[n + i] = A[(long)k * n + i] / p;
+}
+
+__global__ __launch_bounds__(kBlock) void gj_update(int n, int k, double *__restrict__ A,
+                                                  const double *__restrict__ work)
+{
+    const long idx = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (idx >= (long)n * n)
+        return;
+    const int i = (int)(idx / n), j = (int)(idx % n);
+    const double p = work[k];  // old A[k][k]
+    if (i == k && j == k)
+        A[idx] = 1.0 / p;
+    else if (i == k)
+        A[idx] = work[n + j];
+    else if (j == k)
+        A[idx] = -work[i] / p;
+    else
+        A[idx] -= work[i] * work[n + j];
+}
+
+// y = A x for a dense row-major n x n matrix: one wave per row.
+__global__ __launch_bounds__(kBlock) void gemv_f64(int n, const double *__restrict__ A,
+                                                 const double *__restrict__ x, double *__restrict__ y)
+{
+    const int row = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= n)
+        return;
+    const double *Ar = A + (long)row * n;
+    double s = 0.0;
+    for (int j = lane; j < n; j += 64)
+        s += Ar[j] * x[j];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        s += __shfl_xor(s, off, 64);
+    if (lane == 0)
+        y[row] = s;
+}
+
+static unsigned nblk(long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+int launch_restrict(const Grid &gf, int dim, int dof, int normalize, const uint8_t *mask_c, const double *rf,
+                    double *rc, hipStream_t s)
+{
+    if (dim == 3) {
+        long ncn = (long)(gf.nelx / 2 + 1) * (gf.nely / 2 + 1) * (gf.nelz / 2 + 1);
+        hipLaunchKernelGGL(restrict3d_f64, dim3(nblk(ncn)), dim3(kBlock), 0, s, gf, dof, normalize, mask_c, rf, rc);
+    } else {
+        long ncn = (long)(gf.nelx / 2 + 1) * (gf.nely / 2 + 1);
+        hipLaunchKernelGGL(restrict2d_f64, dim3(nblk(ncn)), dim3(kBlock), 0, s, gf, dof, normalize, mask_c, rf, rc);
+    }
+    FANTOM_LAUNCH_CHECK("restrict");
+    return kStatusOk;
+}
+
+int launch_prolong(const Grid &gf, int dim, int dof, int normalize, const uint8_t *mask_f, const double *ec,
+                   double *xf, int accumulate, hipStream_t s)
+{
+    if (dim == 3)
+        hipLaunchKernelGGL(prolong3d_f64, dim3(nblk(gf.n_nodes)), dim3(kBlock), 0, s, gf, dof, normalize, mask_f, ec,
+                           xf, accumulate);
+    else
+        hipLaunchKernelGGL(prolong2d_f64, dim3(nblk(gf.n_nodes)), dim3(kBlock), 0, s, gf, dof, normalize, mask_f, ec,
+                           xf, accumulate);
+    FANTOM_LAUNCH_CHECK("prolong");
+    return kStatusOk;
+}
+
+int launch_coarsen_E(const Grid &gf, int dim, const double *Ef, double *Ec, hipStream_t s)
+{
+    long n = (long)(gf.nelx / 2) * (gf.nely / 2) * (dim == 3 ? gf.nelz / 2 : 1);
+    hipLaunchKernelGGL(coarsen_E_f64, dim3(nblk(n)), dim3(kBlock), 0, s, gf, dim, Ef, Ec);
+    FANTOM_LAUNCH_CHECK("coarsen_E");
+    return kStatusOk;
+}
+
+int launch_coarsen_mask(const Grid &gf, int dim, const uint8_t *mf, uint8_t *mc, hipStream_t s)
+{
+    long n = (long)(gf.nelx / 2 + 1) * (gf.nely / 2 + 1) * (dim == 3 ? gf.nelz / 2 + 1 : 1);
+    hipLaunchKernelGGL(coarsen_mask_u8, dim3(nblk(n)), dim3(kBlock), 0, s, gf, dim, mf, mc);
+    FANTOM_LAUNCH_CHECK("coarsen_mask");
+    return kStatusOk;
+}
+
+int launch_dense_assemble(const Grid &g, int dim, const double *K0, const double *E, const uint8_t *mask,
+                          double *A, hipStream_t s)
+{
+    const int dof = dim == 3 ? 3 : 2;
+    const long ndof = g.n_nodes * dof;
+    hipError_t e = hipMemsetAsync(A, 0, sizeof(double) * ndof * ndof, s);
+    if (e != hipSuccess) {
+        set_error("dense_assemble memset: %s", hipGetErrorString(e));
+        return kStatusHipError;
+    }
+    hipLaunchKernelGGL(dense_assemble_f64, dim3(nblk(ndof)), dim3(kBlock), 0, s, g, dim, K0, E, mask, A);
+    FANTOM_LAUNCH_CHECK("dense_assemble");
+    return kStatusOk;
+}
+
+int launch_spd_inverse(int n, double *A, double *work, hipStream_t s)
+{
+    const long nn = (long)n * n;
+    for (int k = 0; k < n; ++k) {
+        hipLaunchKernelGGL(gj_stage, dim3(nblk(n)), dim3(kBlock), 0, s, n, k, A, work);
+        hipLaunchKernelGGL(gj_update, dim3(nblk(nn)), dim3(kBlock), 0, s, n, k, A, work);
+    }
+    FANTOM_LAUNCH_CHECK("spd_inverse");
+    return kStatusOk;
+}
+
+int launch_gemv(int n, const double *A, const double *x, double *y, hipStream_t s)
+{
+    const unsigned blocks = (unsigned)((n + (kBlock / 64) - 1) / (kBlock / 64));
+    hipLaunchKernelGGL(gemv_f64, dim3(blocks), dim3(kBlock), 0, s, n, A, x, y);
+    FANTOM_LAUNCH_CHECK("gemv");
+    return kStatusOk;
+}
+
+}  // namespace fantom

@@ -1,0 +1,230 @@
+"""Structured meshes and structured density filters.
+
+``StructuredMesh2D/3D`` mirror the reference's structured meshes
+(geom/CPU/_mesh.py:10-210, geom/CUDA/_mesh.py:10-170): same constructor,
+attributes and node/element numbering (core/CPU/_geom.py:4-98).  The kernels
+never read the connectivity -- they index the grid implicitly -- so
+``elements`` / ``nodes`` are built on first access only, on the host, as
+vectorised index arithmetic, then moved to the device.
+
+``StructuredFilter2D/3D`` mirror geom/CPU/_filters.py:14-194 and
+geom/commons/_filters.py:19-157: cone weights over integer offsets,
+normalised by their sum, forward ``dot`` and adjoint ``_rmatvec`` / ``T``.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import _lib
+from .physics import LinearElasticity
+
+
+def generate_elements(nel) -> np.ndarray:
+    """Connectivity int32[n_el, 2^d] in the reference's order (core/CPU/_geom.py:4-59)."""
+    nel = [int(v) for v in nel]
+    if len(nel) == 3:
+        nelx, nely, nelz = nel
+        nx, nz = nelx + 1, nelz + 1
+        i, j, k = np.meshgrid(np.arange(nelx), np.arange(nely), np.arange(nelz), indexing="ij")
+        i, j, k = (a.reshape(-1).astype(np.int64) for a in (i, j, k))
+        base = nz * i + nz * nx * j + k
+        d = [0, nz, nz + nz * nx, nz * nx, 1, nz + 1, nz + nz * nx + 1, nz * nx + 1]
+        return np.stack([base + o for o in d], axis=1).astype(np.int32)
+    nelx, nely = nel
+    nx = nelx + 1
+    i, j = np.meshgrid(np.arange(nelx), np.arange(nely), indexing="ij")
+    i, j = i.reshape(-1).astype(np.int64), j.reshape(-1).astype(np.int64)
+    base = j * nx + i
+    return np.stack([base, base + 1, base + nx + 1, base + nx], axis=1).astype(np.int32)
+
+
+def generate_nodes(nel, lengths, dtype=np.float64) -> np.ndarray:
+    """Node coordinates in meshgrid('xy') flatten order (core/CPU/_geom.py:74-91)."""
+    axes = [np.linspace(0, L, n + 1, dtype=dtype) for L, n in zip(lengths, nel)]
+    grids = np.meshgrid(*axes)
+    return np.stack([g.flatten() for g in grids], axis=-1)
+
+
+_HEX_BITS = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]],
+                     dtype=np.float64)
+_QUAD_BITS = np.array([[0, 0], [1, 0], [1, 1], [0, 1]], dtype=np.float64)
+
+
+class _StructuredMesh:
+    """Common parts of the 2D / 3D structured meshes."""
+
+    def _finish(self, physics, dtype):
+        self.dtype = dtype
+        self.physics = physics
+        # first element's corners (the reference evaluates physics on
+        # nodes[elements[0]]): local corner pattern times the mesh spacing
+        h = np.array(self.spacing, dtype=np.float64)
+        bits = _HEX_BITS if len(self.nel) == 3 else _QUAD_BITS
+        ref = bits * h[None, :]
+        self.K_single = np.asarray(physics.K(ref), dtype=dtype)
+        self.locals = [np.asarray(a, dtype=dtype) for a in physics.locals(ref)]
+        self.A_single = np.array([physics.volume(ref)], dtype=dtype)
+        self.As = self.A_single
+        self.volume = self.A_single[0] * int(np.prod(self.nel))
+        self.elements_size = 2 ** len(self.nel)
+        self.dof = int(self.K_single.shape[0] // self.elements_size)
+        self._elements = None
+        self._nodes = None
+
+    @property
+    def n_elements(self) -> int:
+        return int(np.prod(self.nel))
+
+    @property
+    def n_nodes(self) -> int:
+        return int(np.prod(self.nel + 1))
+
+    @property
+    def elements(self) -> torch.Tensor:
+        if self._elements is None:
+            self._elements = torch.from_numpy(generate_elements(self.nel)).to(_lib.device())
+        return self._elements
+
+    @property
+    def nodes(self) -> torch.Tensor:
+        if self._nodes is None:
+            self._nodes = torch.from_numpy(generate_nodes(self.nel, self.dim, self.dtype)).to(_lib.device())
+        return self._nodes
+
+    @property
+    def grid(self):
+        """(nelx, nely, nelz) as the kernels take it (nelz = 0 in 2D)."""
+        return (int(self.nel[0]), int(self.nel[1]), int(self.nel[2]) if len(self.nel) == 3 else 0)
+
+
+class StructuredMesh3D(_StructuredMesh):
+    """Box of nx x ny x nz hex8 elements (geom/CUDA/_mesh.py:89-170)."""
+
+    def __init__(self, nx, ny, nz, lx, ly, lz, dtype=np.float64, physics=None):
+        physics = physics if physics is not None else LinearElasticity(E=1.0, nu=1 / 3)
+        self.nelx, self.nely, self.nelz = int(nx), int(ny), int(nz)
+        self.lx, self.ly, self.lz = lx, ly, lz
+        self.nel = np.array([nx, ny, nz], dtype=np.int32)
+        self.dim = np.array([lx, ly, lz], dtype=dtype)
+        self.dx, self.dy, self.dz = lx / nx, ly / ny, lz / nz
+        self.spacing = (self.dx, self.dy, self.dz)
+        self._finish(physics, dtype)
+
+    @property
+    def centroids(self) -> torch.Tensor:
+        c = np.meshgrid(np.linspace(self.dx / 2, self.lx - self.dx / 2, self.nelx),
+                        np.linspace(self.dy / 2, self.ly - self.dy / 2, self.nely),
+                        np.linspace(self.dz / 2, self.lz - self.dz / 2, self.nelz), indexing="ij")
+        return torch.from_numpy(np.stack(c, -1).reshape(-1, 3).astype(self.dtype)).to(_lib.device())
+
+
+class StructuredMesh2D(_StructuredMesh):
+    """Rectangle of nx x ny bilinear quads (geom/CPU/_mesh.py:10-104)."""
+
+    def __init__(self, nx, ny, lx, ly, dtype=np.float64, physics=None):
+        physics = physics if physics is not None else LinearElasticity(E=1.0, nu=1 / 3)
+        self.nelx, self.nely = int(nx), int(ny)
+        self.lx, self.ly = lx, ly
+        self.nel = np.array([nx, ny], dtype=np.int32)
+        self.dim = np.array([lx, ly], dtype=dtype)
+        self.dx, self.dy = lx / nx, ly / ny
+        self.spacing = (self.dx, self.dy)
+        self._finish(physics, dtype)
+
+    @property
+    def centroids(self) -> torch.Tensor:
+        c = np.meshgrid(np.linspace(self.dx / 2, self.lx - self.dx / 2, self.nelx),
+                        np.linspace(self.dy / 2, self.ly - self.dy / 2, self.nely), indexing="ij")
+        return torch.from_numpy(np.stack(c, -1).reshape(-1, 2).astype(self.dtype)).to(_lib.device())
+
+
+def filter_taps(dim: int, r_min: float, spacing):
+    """Offsets and normalised cone weights (geom/CPU/_filters.py:74-88)."""
+    scales = np.asarray(spacing, dtype=np.float64)
+    scales = scales / scales.min()
+    n = int(np.ceil(r_min))
+    rng = np.arange(-n, n + 1, dtype=np.int32)
+    grids = np.meshgrid(*([rng] * dim), indexing="ij")
+    offsets = np.vstack([g.ravel() for g in grids]).T.astype(np.int32)
+    dist = np.linalg.norm(offsets * scales[None], axis=1)
+    w = (r_min - dist) / r_min
+    keep = w > 0
+    offsets, w = offsets[keep], w[keep]
+    w = w / w.sum()
+    return np.ascontiguousarray(offsets), np.ascontiguousarray(w)
+
+
+class _TransposeView:
+    def __init__(self, f):
+        self._f = f
+
+    def dot(self, y):
+        return self._f._rmatvec(y)
+
+    def __matmul__(self, y):
+        return self._f._rmatvec(y)
+
+
+class _StructuredFilter:
+    def _setup(self, mesh, r_min):
+        self.mesh = mesh
+        self.r_min = r_min
+        self.dtype = mesh.dtype
+        self.grid = mesh.grid
+        n = mesh.n_elements
+        self.shape = (n, n)
+        offsets, w = filter_taps(len(mesh.nel), r_min, mesh.spacing)
+        dev = _lib.device()
+        self.offsets = offsets
+        self.weights = torch.from_numpy(w).to(dev)
+        self._offsets_d = torch.from_numpy(offsets.reshape(-1)).to(dev)
+        self.normalizer = torch.empty(n, dtype=torch.float64, device=dev)
+        _lib.call("fantom_filter_normalizer_f64", *self.grid, len(w), _lib.ptr(self._offsets_d),
+                  _lib.ptr(self.weights), _lib.ptr(self.normalizer))
+
+    def _matvec(self, rho):
+        out = torch.empty_like(rho)
+        _lib.call("fantom_filter_fwd_f64", *self.grid, self.weights.numel(), _lib.ptr(self._offsets_d),
+                  _lib.ptr(self.weights), _lib.ptr(rho), _lib.ptr(out))
+        return out
+
+    def _rmatvec(self, y):
+        y = y.contiguous()
+        out = torch.empty_like(y)
+        _lib.call("fantom_filter_adj_f64", *self.grid, self.weights.numel(), _lib.ptr(self._offsets_d),
+                  _lib.ptr(self.weights), _lib.ptr(self.normalizer), _lib.ptr(y), _lib.ptr(out))
+        return out
+
+    def dot(self, rho):
+        """Forward filter (geom/commons/_filters.py:106-143)."""
+        if isinstance(rho, torch.Tensor):
+            if rho.ndim == 1:
+                if rho.shape[0] == self.shape[1]:
+                    return self._matvec(rho.to(torch.float64).contiguous())
+                raise ValueError("Input vector size does not match the filter kernel size.")
+            raise NotImplementedError("Only vector inputs are supported.")
+        raise ValueError(f"Input must be a {torch.Tensor} array vector.")
+
+    def __matmul__(self, rhs):
+        return self.dot(rhs)
+
+    @property
+    def T(self):
+        return _TransposeView(self)
+
+
+class StructuredFilter3D(_StructuredFilter):
+    """Cone density filter on a 3D structured mesh (geom/CUDA/_filters.py:65-108)."""
+
+    def __init__(self, mesh: StructuredMesh3D, r_min):
+        self.nelx, self.nely, self.nelz = mesh.nelx, mesh.nely, mesh.nelz
+        self._setup(mesh, r_min)
+
+
+class StructuredFilter2D(_StructuredFilter):
+    """Cone density filter on a 2D structured mesh (geom/CPU/_filters.py:106-194)."""
+
+    def __init__(self, mesh: StructuredMesh2D, r_min):
+        self.nelx, self.nely = mesh.nelx, mesh.nely
+        self._setup(mesh, r_min)

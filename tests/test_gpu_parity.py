@@ -1,0 +1,305 @@
+"""GPU parity: every gfx950 kernel against the FP64 oracle on the same seeded
+inputs, through the C ABI (pyfantom_amd._lib).
+
+Tolerances: index/integer work is bit-exact; FP64 kernels that sum in the
+oracle's order are compared at 1e-15 relative, kernels that change the
+summation order (the Walsh-Hadamard tile kernel, tree reductions) at
+1e-13 relative to the largest entry, solves by the 1e-10 relative compliance
+bar of the north star.
+"""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    return torch
+
+
+def _dev(a, dtype=None):
+    torch = _torch()
+    t = torch.from_numpy(np.ascontiguousarray(a))
+    if dtype is not None:
+        t = t.to(dtype)
+    return t.cuda()
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+def _case(oracle, nel, seed=0, bc="random"):
+    dim = len(nel)
+    K0 = oracle.k0_exact(dim, 0.3)
+    el = oracle.elements(nel)
+    n_nodes = int(np.prod([n + 1 for n in nel]))
+    rng = np.random.default_rng(seed)
+    E = rng.uniform(0.01, 1.0, len(el)) ** 3
+    u = rng.standard_normal(n_nodes * dim)
+    con = np.zeros(n_nodes * dim, bool)
+    if bc == "random":
+        con[rng.choice(n_nodes * dim, size=max(1, n_nodes * dim // 7), replace=False)] = True
+    elif bc == "clamp":
+        nodes = oracle.nodes(nel, nel)
+        con[(np.repeat(nodes[:, 0] == 0, dim))] = True
+    return K0, el, E, u, con
+
+
+def _mask_from_con(con, dim):
+    c = con.reshape(-1, dim).astype(np.uint8)
+    m = np.zeros(c.shape[0], np.uint8)
+    for d in range(dim):
+        m |= c[:, d] << d
+    return m
+
+
+def _grid(nel):
+    return (nel[0], nel[1], nel[2] if len(nel) == 3 else 0)
+
+
+def _ku(kernel, nel, K0, E, u, con, mode="ku", b=None, dinv=None, omega=0.6):
+    from pyfantom_amd import _lib
+    from pyfantom_amd.stiffness import hex8_blocks
+    torch = _torch()
+    dim = len(nel)
+    kid = _lib.KU_TILE if kernel == "tile" else _lib.KU_NODE
+    if kid == _lib.KU_TILE:
+        kb = hex8_blocks(K0)
+        kptr = kb.ctypes.data
+    else:
+        k0d = _dev(K0)
+        kptr = k0d.data_ptr()
+    Ed, ud = _dev(E), _dev(u)
+    md = _dev(_mask_from_con(con, dim)) if con is not None else None
+    y = torch.empty_like(ud)
+    g = _grid(nel)
+    mp = md.data_ptr() if md is not None else None
+    if mode == "ku":
+        _lib.call("fantom_ku_f64", kid, *g, kptr, Ed.data_ptr(), mp, ud.data_ptr(), y.data_ptr())
+        return _np(y)
+    if mode == "jac":
+        bd, dd = _dev(b), _dev(dinv)
+        _lib.call("fantom_ku_jacobi_f64", kid, *g, kptr, Ed.data_ptr(), mp, bd.data_ptr(), dd.data_ptr(), omega,
+                  ud.data_ptr(), y.data_ptr())
+        return _np(y)
+    if mode == "res":
+        bd = _dev(b)
+        _lib.call("fantom_ku_residual_f64", kid, *g, kptr, Ed.data_ptr(), mp, bd.data_ptr(), ud.data_ptr(),
+                  y.data_ptr())
+        return _np(y)
+    if mode == "dot":
+        npart = _lib.load().fantom_ku_partials(kid, *g)
+        part = torch.zeros(npart, dtype=torch.float64, device="cuda")
+        scal = torch.zeros(16, dtype=torch.float64, device="cuda")
+        _lib.call("fantom_ku_dot_f64", kid, *g, kptr, Ed.data_ptr(), mp, ud.data_ptr(), y.data_ptr(),
+                  part.data_ptr())
+        _lib.call("fantom_finalize_f64", part.data_ptr(), npart, scal.data_ptr(), _lib.FIN_STORE, 5)
+        return _np(y), float(scal[5].item())
+    raise ValueError(mode)
+
+
+def _rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+SIZES_3D = [(1, 1, 1), (3, 2, 5), (16, 4, 15), (17, 9, 31), (33, 17, 20), (30, 14, 45)]
+
+
+@pytest.mark.parametrize("nel", SIZES_3D)
+@pytest.mark.parametrize("bc", ["none", "random", "clamp"])
+@pytest.mark.parametrize("kernel", ["node", "tile"])
+def test_ku_3d(oracle, nel, bc, kernel):
+    print(f"ku_3d {kernel} {nel} {bc}", flush=True)
+    K0, el, E, u, con = _case(oracle, nel, bc=bc)
+    ref = oracle.matvec(K0, el, E, u, con if bc != "none" else None)
+    y = _ku(kernel, nel, K0, E, u, con if bc != "none" else None)
+    tol = 1e-15 if kernel == "node" else 1e-13
+    assert _rel(y, ref) <= tol
+    if bc != "none":
+        assert np.array_equal(y[con], u[con])
+
+
+@pytest.mark.parametrize("nel", [(1, 1), (5, 3), (120, 40)])
+@pytest.mark.parametrize("bc", ["none", "random"])
+def test_ku_2d(oracle, nel, bc):
+    print(f"ku_2d {nel} {bc}", flush=True)
+    K0, el, E, u, con = _case(oracle, nel, bc=bc)
+    ref = oracle.matvec(K0, el, E, u, con if bc != "none" else None)
+    y = _ku("node", nel, K0, E, u, con if bc != "none" else None)
+    assert _rel(y, ref) <= 1e-15
+
+
+@pytest.mark.parametrize("kernel", ["node", "tile"])
+def test_ku_epilogues(oracle, kernel):
+    nel = (17, 9, 31)
+    print(f"ku_epilogues {kernel}", flush=True)
+    K0, el, E, u, con = _case(oracle, nel, bc="random", seed=5)
+    rng = np.random.default_rng(9)
+    b = rng.standard_normal(len(u))
+    dinv = 1.0 / oracle.diag(K0, el, E, len(u), con)
+    ku = oracle.matvec(K0, el, E, u, con)
+    jac = _ku(kernel, nel, K0, E, u, con, "jac", b, dinv, 0.6)
+    assert _rel(jac, u + 0.6 * dinv * (b - ku)) <= 1e-12
+    res = _ku(kernel, nel, K0, E, u, con, "res", b)
+    assert _rel(res, b - ku) <= 1e-13
+    y, s = _ku(kernel, nel, K0, E, u, con, "dot")
+    assert _rel(y, ku) <= 1e-13
+    assert abs(s - np.dot(u, ku)) <= 1e-12 * abs(np.dot(u, ku))
+
+
+@pytest.mark.parametrize("nel", [(3, 2, 5), (17, 9, 31), (5, 3), (120, 40)])
+def test_diag_dc(oracle, nel):
+    from pyfantom_amd import _lib
+    torch = _torch()
+    print(f"diag_dc {nel}", flush=True)
+    K0, el, E, u, con = _case(oracle, nel, bc="random", seed=2)
+    dim = len(nel)
+    k0d, Ed, ud, md = _dev(K0), _dev(E), _dev(u), _dev(_mask_from_con(con, dim))
+    d = torch.empty_like(ud)
+    _lib.call("fantom_diag_f64", *_grid(nel), k0d.data_ptr(), Ed.data_ptr(), md.data_ptr(), d.data_ptr(), 0)
+    assert _rel(_np(d), oracle.diag(K0, el, E, len(u), con)) <= 1e-15
+    _lib.call("fantom_diag_f64", *_grid(nel), k0d.data_ptr(), Ed.data_ptr(), md.data_ptr(), d.data_ptr(), 1)
+    assert _rel(_np(d), 1 / oracle.diag(K0, el, E, len(u), con)) <= 1e-15
+    dc = torch.empty_like(Ed)
+    _lib.call("fantom_dc_f64", *_grid(nel), k0d.data_ptr(), ud.data_ptr(), dc.data_ptr())
+    assert _rel(_np(dc), oracle.process_dk(K0, el, u)) <= 1e-14
+
+
+@pytest.mark.parametrize("nel", [(2, 2, 2), (8, 4, 6), (32, 16, 16), (6, 4), (120, 40)])
+@pytest.mark.parametrize("normalize", [0, 1])
+def test_transfers(oracle, nel, normalize):
+    from pyfantom_amd import _lib
+    torch = _torch()
+    print(f"transfers {nel} {normalize}", flush=True)
+    dim = len(nel)
+    rng = np.random.default_rng(3)
+    nf = int(np.prod([n + 1 for n in nel])) * dim
+    nc = int(np.prod([n // 2 + 1 for n in nel])) * dim
+    v, w = rng.standard_normal(nf), rng.standard_normal(nc)
+    vd, wd = _dev(v), _dev(w)
+    rc = torch.empty(nc, dtype=torch.float64, device="cuda")
+    _lib.call("fantom_restrict_f64", *_grid(nel), normalize, None, vd.data_ptr(), rc.data_ptr())
+    assert _rel(_np(rc), oracle.restrict(v, nel, dim, bool(normalize))) <= 1e-15
+    xf = torch.empty(nf, dtype=torch.float64, device="cuda")
+    _lib.call("fantom_prolong_f64", *_grid(nel), normalize, None, wd.data_ptr(), xf.data_ptr(), 0)
+    assert _rel(_np(xf), oracle.prolong(w, nel, dim, bool(normalize))) <= 1e-15
+    # accumulate into an existing vector
+    _lib.call("fantom_prolong_f64", *_grid(nel), normalize, None, wd.data_ptr(), vd.data_ptr(), 1)
+    assert _rel(_np(vd), v + oracle.prolong(w, nel, dim, bool(normalize))) <= 1e-15
+
+
+def test_coarsen(oracle):
+    from pyfantom_amd import _lib
+    torch = _torch()
+    for nel in ((8, 4, 6), (6, 4)):
+        dim = len(nel)
+        E = np.random.default_rng(0).uniform(0, 1, int(np.prod(nel)))
+        Ec = torch.empty(int(np.prod([n // 2 for n in nel])), dtype=torch.float64, device="cuda")
+        _lib.call("fantom_coarsen_E_f64", *_grid(nel), _dev(E).data_ptr(), Ec.data_ptr())
+        ref = E.reshape(*sum(([n // 2, 2] for n in nel), [])).mean(axis=tuple(range(1, 2 * dim, 2))).ravel()
+        assert _rel(_np(Ec), ref) <= 1e-15
+        nn = int(np.prod([n + 1 for n in nel]))
+        m = np.random.default_rng(1).integers(0, 8 if dim == 3 else 4, nn).astype(np.uint8)
+        mc = torch.empty(int(np.prod([n // 2 + 1 for n in nel])), dtype=torch.uint8, device="cuda")
+        _lib.call("fantom_coarsen_mask", *_grid(nel), _dev(m).data_ptr(), mc.data_ptr())
+        X = oracle.nodes(nel, nel)
+        Xc = oracle.nodes([n // 2 for n in nel], [n // 2 for n in nel]) * 2
+        lut = {tuple(p.astype(int)): i for i, p in enumerate(X)}
+        ref = np.array([m[lut[tuple(p.astype(int))]] for p in Xc], np.uint8)
+        assert np.array_equal(_np(mc), ref)
+
+
+@pytest.mark.parametrize("nel", [(4, 2, 2), (8, 4, 4), (8, 4)])
+def test_dense_coarse_solve(oracle, nel):
+    from pyfantom_amd import _lib
+    torch = _torch()
+    print(f"dense {nel}", flush=True)
+    K0, el, E, u, con = _case(oracle, nel, bc="clamp", seed=4)
+    dim = len(nel)
+    n = len(u)
+    A = torch.empty((n, n), dtype=torch.float64, device="cuda")
+    _lib.call("fantom_dense_assemble_f64", *_grid(nel), _dev(K0).data_ptr(), _dev(E).data_ptr(),
+              _dev(_mask_from_con(con, dim)).data_ptr(), A.data_ptr())
+    Aref = oracle.assemble(K0, el, E, n, con).toarray()
+    assert _rel(_np(A), Aref) <= 1e-15
+    work = torch.empty(2 * n, dtype=torch.float64, device="cuda")
+    _lib.call("fantom_spd_inverse_f64", n, A.data_ptr(), work.data_ptr())
+    Ainv = np.linalg.inv(Aref)
+    assert _rel(_np(A), Ainv) <= 1e-9
+    b = np.random.default_rng(0).standard_normal(n)
+    y = torch.empty(n, dtype=torch.float64, device="cuda")
+    _lib.call("fantom_gemv_f64", n, A.data_ptr(), _dev(b).data_ptr(), y.data_ptr())
+    assert np.abs(Aref @ _np(y) - b).max() <= 1e-9 * np.abs(b).max()
+
+
+@pytest.mark.parametrize("nel,r", [((5, 4, 3), 1.5), ((40, 20, 20), 1.5), ((16, 8, 8), 2.0), ((120, 40), 1.5),
+                                   ((30, 20), 3.3)])
+def test_filter(oracle, nel, r):
+    from pyfantom_amd import _lib
+    torch = _torch()
+    print(f"filter {nel} {r}", flush=True)
+    offs, w = oracle.filter_taps(len(nel), r)
+    n = int(np.prod(nel))
+    x = np.random.default_rng(0).uniform(0, 1, n)
+    od, wd, xd = _dev(offs.reshape(-1)), _dev(w), _dev(x)
+    norm = torch.empty(n, dtype=torch.float64, device="cuda")
+    y = torch.empty(n, dtype=torch.float64, device="cuda")
+    g = _grid(nel)
+    _lib.call("fantom_filter_normalizer_f64", *g, len(w), od.data_ptr(), wd.data_ptr(), norm.data_ptr())
+    assert _rel(_np(norm), oracle.filter_normalizer(nel, offs, w)) <= 1e-15
+    _lib.call("fantom_filter_fwd_f64", *g, len(w), od.data_ptr(), wd.data_ptr(), xd.data_ptr(), y.data_ptr())
+    assert _rel(_np(y), oracle.filter_fwd(nel, offs, w, x)) <= 1e-15
+    _lib.call("fantom_filter_adj_f64", *g, len(w), od.data_ptr(), wd.data_ptr(), norm.data_ptr(), xd.data_ptr(),
+              y.data_ptr())
+    assert _rel(_np(y), oracle.filter_adj(nel, offs, w, x)) <= 1e-15
+
+
+@pytest.mark.parametrize("n,seed", [(4, 0), (1000, 1), (524288, 2)])
+def test_oc_update(oracle, n, seed):
+    from pyfantom_amd import _lib
+    torch = _torch()
+    print(f"oc {n}", flush=True)
+    rng = np.random.default_rng(seed)
+    if n == 4:
+        x = np.full(4, 0.5)
+        df = -np.array([4.0, 1.0, 0.25, 1.0])
+    else:
+        x = rng.uniform(0, 1, n)
+        df = -rng.uniform(0.0, 5, n)
+    dg = np.full(n, 1.0 / n)
+    ref, lm = oracle.oc_update(x, df, dg, 0.4 if n > 4 else 0.5, 0.2)
+    xd, dfd, dgd = _dev(x), _dev(df), _dev(dg)
+    ocp = torch.empty_like(xd)
+    state = torch.zeros(16, dtype=torch.float64, device="cuda")
+    part = torch.zeros(16 * 1024, dtype=torch.float64, device="cuda")
+    xn = torch.empty_like(xd)
+    _lib.call("fantom_oc_update_f64", n, xd.data_ptr(), dfd.data_ptr(), dgd.data_ptr(), 0.4 if n > 4 else 0.5, 0.2,
+              ocp.data_ptr(), state.data_ptr(), part.data_ptr(), xn.data_ptr())
+    lam = float(state[2].item())
+    # same sequence of midpoints unless a constraint sum lands within rounding
+    # of the target; either way the multiplier agrees to the bisection width
+    assert abs(lam - lm) <= 2e-8 * lm
+    assert np.abs(_np(xn) - ref).max() <= 1e-7
+
+
+def test_simp(oracle):
+    from pyfantom_amd import _lib
+    torch = _torch()
+    rho = np.random.default_rng(0).uniform(0, 1, 10000)
+    rd = _dev(rho)
+    E = torch.empty_like(rd)
+    dE = torch.empty_like(rd)
+    for hv in (0, 1):
+        _lib.call("fantom_simp_f64", len(rho), rd.data_ptr(), 0, 3.0, 1e-6, 1.0, hv, 2.0, 0.5, E.data_ptr(),
+                  dE.data_ptr())
+        assert _rel(_np(E), oracle.penalize_reference(rho, 3.0, 1e-6, 1.0, bool(hv))) <= 1e-14
+        assert _rel(_np(dE), oracle.penalize_grad_reference(rho, 3.0, 1.0, bool(hv))) <= 1e-14
+    _lib.call("fantom_simp_f64", len(rho), rd.data_ptr(), 1, 3.0, 1e-9, 1.0, 0, 2.0, 0.5, E.data_ptr(), dE.data_ptr())
+    assert _rel(_np(E), oracle.penalize_modified(rho, 3.0, 1.0, 1e-9)) <= 1e-14
+    assert _rel(_np(dE), oracle.penalize_grad_modified(rho, 3.0, 1.0, 1e-9)) <= 1e-14

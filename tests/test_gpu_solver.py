@@ -1,0 +1,185 @@
+"""GPU solve-level parity through the reference-shaped Python API.
+
+* MultiGrid / CG solutions vs a scipy direct solve (small) and vs the FP64
+  oracle's GMG-PCG (cantilever), compliance to 1e-10 relative;
+* size-independent properties at the benchmark size (compliance identity
+  c = f.u = u.Ku, true residual, symmetry);
+* the 2D MBB pipeline (filter -> SIMP -> solve -> dc -> filter^T -> OC) for
+  several iterations against the oracle's pipeline on the same inputs.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    return torch
+
+
+def cantilever(nel, nu=0.3, n_level=4, tol=1e-8, n_smooth=2, omega=0.6, kernel=None, maxiter=1000):
+    """Cantilever of the benchmark: x = 0 face clamped, unit total downward
+    load spread over the nodes of the bottom edge of the x = L face."""
+    import pyfantom_amd as pf
+    mesh = pf.StructuredMesh3D(*nel, *nel, physics=pf.LinearElasticity(E=1.0, nu=nu))
+    K = pf.StructuredStiffnessKernel(mesh, kernel=kernel)
+    solver = pf.MultiGrid(mesh, K, maxiter=maxiter, tol=tol, n_smooth=n_smooth, omega=omega, n_level=n_level,
+                          cycle="V")
+    FE = pf.FiniteElement(mesh, K, solver)
+    nx, ny, nz = (n + 1 for n in nel)
+    j, k = np.meshgrid(np.arange(ny), np.arange(nz), indexing="ij")
+    clamp = (k + nz * 0 + nz * nx * j).ravel()
+    FE.add_dirichlet_boundary_condition(node_ids=clamp, rhs=0)
+    kk = np.arange(nz)
+    load = kk + nz * (nx - 1) + nz * nx * 0
+    FE.add_point_forces(forces=np.array([[0.0, -1.0 / nz, 0.0]]), node_ids=load)
+    return mesh, K, solver, FE
+
+
+def oracle_cantilever_inputs(nel):
+    nx, ny, nz = (n + 1 for n in nel)
+    n_nodes = nx * ny * nz
+    con = np.zeros(n_nodes * 3, bool)
+    j, k = np.meshgrid(np.arange(ny), np.arange(nz), indexing="ij")
+    clamp = (k + nz * nx * j).ravel()
+    for d in range(3):
+        con[clamp * 3 + d] = True
+    f = np.zeros(n_nodes * 3)
+    load = np.arange(nz) + nz * (nx - 1)
+    f[load * 3 + 1] += -1.0 / nz
+    return con, f
+
+
+@pytest.mark.parametrize("kernel", ["node", "tile"])
+def test_multigrid_vs_direct(oracle, kernel):
+    """Small cantilever solved to 1e-12: displacement vs scipy spsolve."""
+    import scipy.sparse.linalg as spla
+    torch = _torch()
+    nel = (16, 8, 8)
+    print(f"mg_vs_direct {kernel}", flush=True)
+    mesh, K, solver, FE = cantilever(nel, tol=1e-12, n_level=3, kernel=kernel)
+    E = np.random.default_rng(0).uniform(0.01, 1, mesh.n_elements) ** 3
+    U, res = FE.solve(torch.from_numpy(E).cuda())
+    assert res < 1e-11
+    con, f = oracle_cantilever_inputs(nel)
+    assert np.array_equal(FE.rhs.cpu().numpy(), f)
+    assert np.array_equal(K.constraints.cpu().numpy(), con)
+    A = oracle.assemble(oracle.k0_exact(3, 0.3), oracle.elements(nel), E, len(f), con)
+    ud = spla.spsolve(A.tocsc(), f)
+    u = U.cpu().numpy()
+    assert np.abs(u - ud).max() <= 1e-9 * np.abs(ud).max()
+    assert abs(f @ u - f @ ud) <= 1e-10 * abs(f @ ud)
+
+
+def test_cg_vs_direct(oracle):
+    import scipy.sparse.linalg as spla
+    import pyfantom_amd as pf
+    torch = _torch()
+    nel = (8, 4, 4)
+    mesh, K, _, FE = cantilever(nel)
+    FE.solver = pf.CG(K, maxiter=5000, tol=1e-12)
+    E = np.random.default_rng(1).uniform(0.2, 1, mesh.n_elements)
+    U, res = FE.solve(torch.from_numpy(E).cuda())
+    con, f = oracle_cantilever_inputs(nel)
+    A = oracle.assemble(oracle.k0_exact(3, 0.3), oracle.elements(nel), E, len(f), con)
+    ud = spla.spsolve(A.tocsc(), f)
+    assert abs(f @ U.cpu().numpy() - f @ ud) <= 1e-10 * abs(f @ ud)
+
+
+def test_cantilever_vs_oracle_solver(oracle):
+    """64 x 32 x 32 cantilever, seeded E, both sides to 1e-12: compliance to 1e-10."""
+    torch = _torch()
+    nel = (64, 32, 32)
+    print("cantilever vs oracle", flush=True)
+    mesh, K, solver, FE = cantilever(nel, tol=1e-12, n_level=3)
+    x = np.random.default_rng(0).uniform(0.01, 1.0, mesh.n_elements)
+    E = 1e-9 + x ** 3 * (1 - 1e-9)
+    U, res = FE.solve(torch.from_numpy(E).cuda())
+    con, f = oracle_cantilever_inputs(nel)
+    uo, it, ro = oracle.mgpcg_solve(nel, oracle.k0_exact(3, 0.3), E, con, f, n_level=3, n_smooth=2, omega=0.6,
+                                    tol=1e-12)
+    c_gpu, c_cpu = float(f @ U.cpu().numpy()), float(f @ uo)
+    print(f"c_gpu={c_gpu!r} c_cpu={c_cpu!r} it_gpu={solver.last_iterations} it_cpu={it}", flush=True)
+    assert abs(c_gpu - c_cpu) <= 1e-10 * abs(c_cpu)
+    assert np.abs(U.cpu().numpy() - uo).max() <= 1e-8 * np.abs(uo).max()
+
+
+def test_cantilever_full_size_properties(oracle):
+    """128 x 64 x 64 (1.6M DOF), seeded E, 1e-8 stop: true residual, compliance
+    identity c = f.u = u.Ku, and the tile/node operators agree on U."""
+    torch = _torch()
+    nel = (128, 64, 64)
+    print("cantilever full size", flush=True)
+    mesh, K, solver, FE = cantilever(nel, tol=1e-8, n_level=4)
+    x = np.random.default_rng(0).uniform(0.01, 1.0, mesh.n_elements)
+    E = torch.from_numpy(1e-9 + x ** 3 * (1 - 1e-9)).cuda()
+    U, res = FE.solve(E)
+    assert res < 1e-8 * 1.5
+    c1 = float(torch.dot(FE.rhs, U))
+    KU = K.dot(U).clone()
+    c2 = float(torch.dot(U, KU))
+    assert abs(c1 - c2) <= 1e-7 * abs(c1)
+    K.kernel_kind = "node"
+    KU2 = K.dot(U).clone()
+    K.kernel_kind = "tile"
+    assert float((KU - KU2).abs().max()) <= 1e-13 * float(KU.abs().max())
+    print(f"iterations={solver.last_iterations} residual={res:.3e} c={c1!r}", flush=True)
+
+
+def _mbb(pf, nelx=120, nely=40, n_level=3):
+    mesh = pf.StructuredMesh2D(nelx, nely, nelx, nely, physics=pf.LinearElasticity(E=1.0, nu=0.3))
+    K = pf.StructuredStiffnessKernel(mesh)
+    solver = pf.MultiGrid(mesh, K, maxiter=2000, tol=1e-12, n_smooth=2, omega=0.6, n_level=n_level, cycle="V")
+    FE = pf.FiniteElement(mesh, K, solver)
+    nx = nelx + 1
+    left = np.arange(nely + 1) * nx
+    FE.add_dirichlet_boundary_condition(node_ids=left, dofs=np.array([[1, 0]]), rhs=0)
+    FE.add_dirichlet_boundary_condition(node_ids=np.array([nelx]), dofs=np.array([[0, 1]]), rhs=0)
+    FE.add_point_forces(forces=np.array([[0.0, -1.0]]), node_ids=np.array([nely * nx]))
+    filt = pf.StructuredFilter2D(mesh, 1.5)
+    prob = pf.MinimumCompliance(FE, filt, E_mul=[1.0], void=1e-9, penalty=3.0, volume_fraction=[0.5],
+                                heavyside=False, law="modified")
+    return mesh, K, FE, filt, prob
+
+
+def test_mbb_pipeline_vs_oracle(oracle):
+    """MBB 120 x 40: 5 OC iterations on the device vs the oracle pipeline fed
+    the device's design each step; compliance and sensitivities per step."""
+    import scipy.sparse.linalg as spla
+    import pyfantom_amd as pf
+    torch = _torch()
+    print("mbb pipeline", flush=True)
+    nelx, nely = 120, 40
+    mesh, K, FE, filt, prob = _mbb(pf, nelx, nely)
+    opt = pf.OC(prob, move=0.2)
+    K0 = oracle.k0_exact(2, 0.3)
+    el = oracle.elements((nelx, nely))
+    offs, w = oracle.filter_taps(2, 1.5)
+    con = K.constraints.cpu().numpy()
+    f = FE.rhs.cpu().numpy()
+    # the boundary conditions of the configuration
+    nx = nelx + 1
+    exp = np.zeros_like(con)
+    exp[np.arange(nely + 1) * nx * 2] = True
+    exp[nelx * 2 + 1] = True
+    assert np.array_equal(con, exp)
+    assert f[nely * nx * 2 + 1] == -1.0 and np.count_nonzero(f) == 1
+    for it in range(5):
+        x = prob.get_desvars().cpu().numpy()
+        rho = oracle.filter_fwd((nelx, nely), offs, w, x)
+        E = oracle.penalize_modified(rho, 3.0, 1.0, 1e-9)
+        A = oracle.assemble(K0, el, E, len(f), con)
+        u = spla.spsolve(A.tocsc(), f)
+        c = f @ u
+        dc = oracle.process_dk(K0, el, u) * oracle.penalize_grad_modified(rho, 3.0, 1.0, 1e-9)
+        dx = oracle.filter_adj((nelx, nely), offs, w, dc)
+        assert abs(prob.f() - c) <= 1e-10 * abs(c), (it, prob.f(), c)
+        assert np.abs(prob.nabla_f().cpu().numpy() - dx).max() <= 1e-8 * np.abs(dx).max()
+        xn_ref, _ = oracle.oc_update(x, dx, np.full(len(x), 1.0 / len(x)), 0.5, 0.2)
+        opt.iter()
+        xn = prob.get_desvars().cpu().numpy()
+        assert np.abs(xn - xn_ref).max() <= 1e-6
+    print(f"compliance after 5 iterations {prob.f()!r}", flush=True)

@@ -1,0 +1,314 @@
+"""CPU checks that pin the FP64 oracle (oracle/oracle.c + oracle/oracle.py).
+
+Every oracle function is compared with something independent of its own
+formula: brute-force enumeration, exact rational integration, explicit sparse
+matrices, dense products, finite differences or a direct solve.  Each test
+would fail for the plausible mistakes noted in its docstring.
+"""
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _rng(s=0):
+    return np.random.default_rng(s)
+
+
+# --------------------------------------------------------------------------- A1
+@pytest.mark.parametrize("nel", [(1, 1, 1), (2, 1, 3), (3, 2, 2), (4, 3, 5), (1, 1), (3, 2), (5, 4)])
+def test_connectivity_bruteforce(oracle, nel):
+    """Index formulas vs coordinate lookup (catches swapped axes / wrong strides)."""
+    assert np.array_equal(oracle.elements(nel), oracle.elements_bruteforce(nel))
+
+
+def test_connectivity_golden_hash(oracle):
+    """Bit-exact connectivity at the benchmark sizes against committed hashes."""
+    with open(os.path.join(GOLDEN, "connectivity_sha256.json")) as f:
+        ref = json.load(f)
+    for key, h in ref.items():
+        nel = tuple(int(v) for v in key.split("x"))
+        el = oracle.elements(nel)
+        assert hashlib.sha256(el.tobytes()).hexdigest() == h, key
+
+
+def test_element_volumes_positive(oracle):
+    """Every hex has positive Jacobian volume with the reference node order."""
+    nel = (3, 2, 2)
+    X = oracle.nodes(nel, nel)
+    el = oracle.elements(nel)
+    for e in el:
+        P = X[e]
+        # signed volume of the tetra (0,1,3,4) is positive for a right-handed hex
+        v = np.linalg.det(np.stack([P[1] - P[0], P[3] - P[0], P[4] - P[0]]))
+        assert v > 0
+
+
+# --------------------------------------------------------------------------- A3
+def test_k0_exact_properties(oracle):
+    """Symmetric, PSD, rigid-body nullspace of dim 6 (3D) / 3 (2D)."""
+    for dim, nrb in ((3, 6), (2, 3)):
+        K = oracle.k0_exact(dim, 0.3)
+        assert np.allclose(K, K.T, atol=0)
+        w = np.linalg.eigvalsh(K)
+        assert np.sum(np.abs(w) < 1e-12 * w.max()) == nrb
+        assert w.min() > -1e-13
+
+
+def test_k0_exact_rigid_modes(oracle):
+    """Translations and rotations are in the nullspace (catches a sign slip in B)."""
+    K = oracle.k0_exact(3, 0.3)
+    X = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]], float)
+    modes = []
+    for c in range(3):
+        t = np.zeros((8, 3))
+        t[:, c] = 1
+        modes.append(t.ravel())
+    for axis in range(3):
+        w = np.zeros(3)
+        w[axis] = 1
+        modes.append(np.cross(w[None, :], X).ravel())
+    for m in modes:
+        assert np.abs(K @ m).max() < 1e-14
+
+
+def test_k0_2d_matches_top88(oracle):
+    """2D K0 equals the published top88 KE under the node permutation that maps
+    top88's (y-down) node order onto the reference order."""
+    K = oracle.k0_exact(2, 0.3)
+    T = oracle.k0_top88(0.3)
+    import itertools
+    found = False
+    for perm in itertools.permutations(range(4)):
+        # flips of the y displacement component for the y-down grid
+        for sy in (1.0, -1.0):
+            P = np.zeros((8, 8))
+            for a, b in enumerate(perm):
+                P[2 * a, 2 * b] = 1.0
+                P[2 * a + 1, 2 * b + 1] = sy
+            if np.allclose(P @ T @ P.T, K, atol=1e-14):
+                found = True
+                break
+        if found:
+            break
+    assert found
+
+
+def test_product_k0_matches_exact(oracle):
+    """Product host K0 (Gauss quadrature) equals exact integration, 3D and 2D,
+    unit and anisotropic spacing."""
+    from pyfantom_amd.physics import LinearElasticity
+    for nu in (0.3, 1 / 3):
+        phys = LinearElasticity(E=1.0, nu=nu)
+        for h in ((1, 1, 1), (0.5, 1, 2)):
+            X = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]],
+                         float) * np.array(h)[None, :]
+            Kp = phys.K(X)
+            Ke = oracle.k0_exact(3, nu, *h)
+            assert np.abs(Kp - Ke).max() <= 1e-14 * np.abs(Ke).max()
+        X2 = np.array([[0, 0], [1, 0], [1, 1], [0, 1]], float)
+        assert np.abs(phys.K(X2) - oracle.k0_exact(2, nu)).max() <= 1e-14
+
+
+def test_k0_golden(oracle):
+    """Committed K0 fixtures (hex8 and quad4, nu = 0.3)."""
+    d = np.load(os.path.join(GOLDEN, "k0.npz"))
+    assert np.abs(d["hex8"] - oracle.k0_exact(3, 0.3)).max() < 1e-15
+    assert np.abs(d["quad4"] - oracle.k0_exact(2, 0.3)).max() < 1e-15
+
+
+# --------------------------------------------------------------------------- A5 A6 A14
+def _mesh_case(oracle, nel, seed=0, bc=True):
+    dim = len(nel)
+    K0 = oracle.k0_exact(dim, 0.3)
+    el = oracle.elements(nel)
+    n_nodes = int(np.prod([n + 1 for n in nel]))
+    dof = dim
+    rng = _rng(seed)
+    E = rng.uniform(0.01, 1.0, len(el)) ** 3
+    u = rng.standard_normal(n_nodes * dof)
+    con = np.zeros(n_nodes * dof, bool)
+    if bc:
+        con[rng.choice(n_nodes * dof, size=max(1, n_nodes * dof // 7), replace=False)] = True
+    return K0, el, E, u, con
+
+
+@pytest.mark.parametrize("nel", [(2, 3, 2), (4, 2, 3), (3, 4), (5, 2)])
+@pytest.mark.parametrize("bc", [False, True])
+def test_matvec_vs_assembly(oracle, nel, bc):
+    """Node-basis gather vs scipy COO assembly with the Dirichlet convention."""
+    K0, el, E, u, con = _mesh_case(oracle, nel, bc=bc)
+    K = oracle.assemble(K0, el, E, len(u), con if bc else None)
+    y = oracle.matvec(K0, el, E, u, con if bc else None)
+    assert np.abs(y - K @ u).max() <= 1e-13 * np.abs(K @ u).max()
+    if bc:
+        assert np.array_equal(y[con], u[con])
+
+
+def test_matvec_symmetry(oracle):
+    """<Ku, v> = <u, Kv> on a mid-size grid with BCs."""
+    K0, el, E, u, con = _mesh_case(oracle, (6, 5, 4))
+    v = _rng(3).standard_normal(len(u))
+    a = np.dot(oracle.matvec(K0, el, E, u, con), v)
+    b = np.dot(u, oracle.matvec(K0, el, E, v, con))
+    assert abs(a - b) <= 1e-12 * abs(a)
+
+
+@pytest.mark.parametrize("nel", [(2, 3, 2), (3, 4)])
+def test_diag_vs_assembly(oracle, nel):
+    K0, el, E, u, con = _mesh_case(oracle, nel)
+    K = oracle.assemble(K0, el, E, len(u), con)
+    d = oracle.diag(K0, el, E, len(u), con)
+    assert np.abs(d - K.diagonal()).max() <= 1e-14 * np.abs(d).max()
+
+
+@pytest.mark.parametrize("nel", [(2, 3, 2), (3, 4)])
+def test_dc_vs_dense_and_fd(oracle, nel):
+    """-u_e^T K0 u_e vs einsum, and vs a finite difference of u^T K(E) u in E_e."""
+    K0, el, E, u, _ = _mesh_case(oracle, nel)
+    dof = len(nel)
+    dc = oracle.process_dk(K0, el, u)
+    ue = u[(el[:, :, None] * dof + np.arange(dof)).reshape(len(el), -1)]
+    assert np.abs(dc + np.einsum("ei,ij,ej->e", ue, K0, ue)).max() <= 1e-13 * np.abs(dc).max()
+    e0, h = 3, 1e-3
+    K1 = oracle.assemble(K0, el, E, len(u))
+    E2 = E.copy()
+    E2[e0] += h
+    K2 = oracle.assemble(K0, el, E2, len(u))
+    fd = (u @ (K2 @ u) - u @ (K1 @ u)) / h
+    assert abs(fd + dc[e0]) <= 1e-9 * abs(dc[e0])
+
+
+# --------------------------------------------------------------------------- A7 A8
+@pytest.mark.parametrize("nel", [(4, 2, 6), (6, 4), (2, 2, 2)])
+@pytest.mark.parametrize("normalize", [True, False])
+def test_transfers_vs_explicit(oracle, nel, normalize):
+    """Restriction / prolongation kernels vs the explicit sparse R; P == R^T."""
+    dof = len(nel)
+    R = oracle.transfer_matrix(nel, dof, normalize)
+    rng = _rng(1)
+    v = rng.standard_normal(R.shape[1])
+    w = rng.standard_normal(R.shape[0])
+    assert np.abs(oracle.restrict(v, nel, dof, normalize) - R @ v).max() < 1e-13
+    assert np.abs(oracle.prolong(w, nel, dof, normalize) - R.T @ w).max() < 1e-13
+
+
+def test_transfer_constants(oracle):
+    """Normalised R maps constants to constants (rows sum to 1); unnormalised
+    P_lin interpolates constants exactly."""
+    nel, dof = (4, 2, 6), 3
+    R = oracle.transfer_matrix(nel, dof, True)
+    assert np.allclose(np.asarray(R.sum(axis=1)).ravel(), 1.0)
+    P = oracle.transfer_matrix(nel, dof, False).T
+    nc = R.shape[0]
+    assert np.allclose(P @ np.ones(nc), 1.0)
+
+
+def test_galerkin_product(oracle):
+    """R A R^T (sparse) vs a dense product."""
+    nel, dof = (2, 2, 2), 3
+    K0, el, E, u, con = _mesh_case(oracle, nel)
+    A = oracle.assemble(K0, el, E, len(u), con)
+    Ac = oracle.galerkin(A, nel, dof).toarray()
+    R = oracle.transfer_matrix(nel, dof, True).toarray()
+    assert np.abs(Ac - R @ A.toarray() @ R.T).max() < 1e-13
+
+
+# --------------------------------------------------------------------------- A15
+@pytest.mark.parametrize("nel,r", [((5, 4, 3), 1.5), ((6, 5), 1.5), ((4, 4, 4), 2.0), ((7, 3), 2.5)])
+def test_filter_vs_dense(oracle, nel, r):
+    offs, w = oracle.filter_taps(len(nel), r)
+    H = oracle.filter_dense(nel, offs, w)
+    x = _rng(2).uniform(0, 1, H.shape[0])
+    y = _rng(3).standard_normal(H.shape[0])
+    assert np.abs(oracle.filter_fwd(nel, offs, w, x) - H @ x).max() < 1e-14
+    assert np.abs(oracle.filter_adj(nel, offs, w, y) - H.T @ y).max() < 1e-14
+    assert np.allclose(H @ np.ones(H.shape[0]), 1.0)
+
+
+def test_filter_tap_counts(oracle):
+    """r = 1.5 -> 19 taps (3D) / 9 (2D); r = 2.0 -> 27 (3D)."""
+    assert len(oracle.filter_taps(3, 1.5)[1]) == 19
+    assert len(oracle.filter_taps(2, 1.5)[1]) == 9
+    assert len(oracle.filter_taps(3, 2.0)[1]) == 27
+
+
+# --------------------------------------------------------------------------- A16
+def test_oc_hand_case(oracle):
+    """4 elements: the bisection finds the multiplier that meets the volume."""
+    x = np.array([0.5, 0.5, 0.5, 0.5])
+    df = -np.array([4.0, 1.0, 0.25, 1.0])
+    dg = np.full(4, 0.25)
+    xn, lm = oracle.oc_update(x, df, dg, 0.5, 0.2)
+    # ocP = x*sqrt(-df/dg) = [2, 1, 0.5, 1]; x_new = clip(ocP/l, 0.3, 0.7).
+    # The volume sum(x_new)/4 = 0.5 holds only at l = 2, where element 0 is
+    # capped at 0.7, element 2 floored at 0.3 and elements 1, 3 sit at 0.5.
+    assert np.allclose(xn, [0.7, 0.5, 0.3, 0.5], atol=1e-7)
+    assert abs(lm - 2.0) < 1e-7
+    assert abs(np.dot(dg, xn) - 0.5) < 1e-7
+
+
+def test_oc_bisection_invariants(oracle):
+    rng = _rng(4)
+    x = rng.uniform(0, 1, 1000)
+    df = -rng.uniform(0.01, 5, 1000)
+    dg = np.full(1000, 1e-3)
+    xn, lm = oracle.oc_update(x, df, dg, 0.4, 0.2)
+    assert np.all(xn >= np.maximum(0, x - 0.2) - 1e-15) and np.all(xn <= np.minimum(1, x + 0.2) + 1e-15)
+    assert abs(np.dot(dg, xn) - 0.4) < 1e-6
+    # monotone in the multiplier: a larger multiplier never increases a variable
+    ocp = x * np.sqrt(-df / dg)
+    lo = np.maximum(0, np.maximum(np.clip(x - 0.2, 0, 1), np.minimum(1, np.minimum(np.clip(x + 0.2, 0, 1), ocp / (lm * 1.01)))))
+    assert np.all(lo <= xn + 1e-15)
+
+
+# --------------------------------------------------------------------------- A9-A13
+@pytest.mark.parametrize("nel,nlev", [((8, 4, 4), 2), ((16, 8), 3)])
+def test_mgpcg_vs_direct(oracle, nel, nlev):
+    """GMG-PCG to 1e-12 equals a sparse direct solve; compliance identity."""
+    import scipy.sparse.linalg as spla
+    dim = len(nel)
+    K0 = oracle.k0_exact(dim, 0.3)
+    el = oracle.elements(nel)
+    n_nodes = int(np.prod([n + 1 for n in nel]))
+    E = _rng(0).uniform(0.01, 1, len(el)) ** 3
+    con = np.zeros(n_nodes * dim, bool)
+    # clamp the x = 0 face
+    if dim == 3:
+        nx, nz = nel[0] + 1, nel[2] + 1
+        for j in range(nel[1] + 1):
+            for k in range(nz):
+                n = k + nz * 0 + nz * nx * j
+                con[3 * n:3 * n + 3] = True
+    else:
+        nx = nel[0] + 1
+        for j in range(nel[1] + 1):
+            con[2 * (j * nx):2 * (j * nx) + 2] = True
+    f = np.zeros(n_nodes * dim)
+    f[-dim + 1] = -1.0
+    u, it, res = oracle.mgpcg_solve(nel, K0, E, con, f, n_level=nlev, n_smooth=2, omega=0.6, tol=1e-12)
+    assert it > 0 and res < 1e-12
+    K = oracle.assemble(K0, el, E, len(f), con)
+    ud = spla.spsolve(K.tocsc(), f)
+    assert np.abs(u - ud).max() <= 1e-9 * np.abs(ud).max()
+    c1, c2 = f @ u, u @ (K @ u)
+    assert abs(c1 - c2) <= 1e-10 * abs(c1)
+    assert abs(c1 - f @ ud) <= 1e-10 * abs(c1)
+
+
+# --------------------------------------------------------------------------- A4
+def test_penalize_grad_fd(oracle):
+    rho = np.linspace(0.05, 0.95, 19)
+    h = 1e-6
+    for hv in (True, False):
+        g = oracle.penalize_grad_reference(rho, heavyside=hv)
+        fd = (oracle.penalize_reference(rho + h, void=0, heavyside=hv)
+              - oracle.penalize_reference(rho - h, void=0, heavyside=hv)) / (2 * h)
+        assert np.abs(g - fd).max() < 1e-7
+    g = oracle.penalize_grad_modified(rho)
+    fd = (oracle.penalize_modified(rho + h) - oracle.penalize_modified(rho - h)) / (2 * h)
+    assert np.abs(g - fd).max() < 1e-7
