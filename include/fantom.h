@@ -162,6 +162,88 @@ long fantom_oc_partials(long n);
 int fantom_oc_update_f64(long n, const double *x, const double *df, const double *dg, double vf, double move,
                          double *ocp, double *state, double *partials, double *x_new, void *stream);
 
+/* ---- A9-A11: native GMG-PCG driver ------------------------------------------
+ * Replaces MultiGrid._setup / _multi_grid / solve
+ * (solvers/CUDA/_solvers.py:442-660): rediscretised matrix-free hierarchy of
+ * n_coarse coarsenings (the grid must halve evenly that many times), Jacobi
+ * smoothing (n_smooth sweeps, weight omega), exact dense coarsest solve, PCG
+ * with all scalars on the device and each iteration replayed from a HIP
+ * graph (use_graph != 0).
+ * workspace_size: bytes of device workspace the caller allocates and passes
+ * to create (the library allocates no device memory).  K0_host: host dense
+ * element matrix for E = 1 on the fine grid.  The handle owns one stream of
+ * default priority and joins the caller's stream on entry and exit.
+ * setup: copies E and mask (per-node bits, may be NULL) and builds the
+ * hierarchy and the coarsest inverse.  precondition: z = V(r).
+ * solve: x = K^-1 f to relative recursive residual tol (x is the initial
+ * guess when use_x0 != 0); *iters gets the PCG iteration count and *rel_res
+ * the true residual ||f - K x|| / ||f||. */
+long fantom_mg_workspace_size(int nelx, int nely, int nelz, int n_coarse);
+int fantom_mg_create(int nelx, int nely, int nelz, int n_coarse, int n_smooth, double omega, int kernel,
+                     const double *K0_host, void *workspace, long workspace_bytes, void **handle);
+int fantom_mg_destroy(void *handle);
+int fantom_mg_setup(void *handle, const double *E, const uint8_t *mask, void *stream);
+int fantom_mg_precondition(void *handle, const double *r, double *z, void *stream);
+int fantom_mg_solve(void *handle, const double *f, double *x, double tol, int maxiter, int use_x0, int use_graph,
+                    int *iters, double *rel_res, void *stream);
+
+/* ---- A19/A21: mixed-precision solve (no reference counterpart) --------------
+ * FP64 iterative refinement around an FP32 GMG-PCG on a hex8 grid: the true
+ * residual f - K x and the update x += ||r|| d stay FP64; the inner PCG on
+ * K d = r/||r|| (from d = 0, to relative residual inner_tol) stores vectors,
+ * moduli and transfers in FP32 and runs the Walsh-Hadamard element
+ * contraction in FP32, with FP64 dot products, step lengths and coarsest
+ * solve.  *outer_iters gets the refinement steps, *inner_iters the total
+ * inner PCG iterations, *rel_res the final FP64 ||f - K x||/||f||.
+ * Workspace / handle / stream rules as for the fantom_mg_* driver. */
+long fantom_mx_workspace_size(int nelx, int nely, int nelz, int n_coarse);
+int fantom_mx_create(int nelx, int nely, int nelz, int n_coarse, int n_smooth, double omega, const double *K0_host,
+                     void *workspace, long workspace_bytes, void **handle);
+int fantom_mx_destroy(void *handle);
+int fantom_mx_setup(void *handle, const double *E, const uint8_t *mask, void *stream);
+int fantom_mx_solve(void *handle, const double *f, double *x, double tol, double inner_tol, int max_outer,
+                    int max_inner, int use_x0, int use_graph, int *outer_iters, int *inner_iters, double *rel_res,
+                    void *stream);
+
+/* ---- A20: learned preconditioner kernels (no reference counterpart) ----------
+ * Channels-last BF16 volumes (D, H, W, C), C a multiple of 16; weights
+ * W[tap][c_out][c_in] BF16, tap = (dy+1)*9 + (dx+1)*3 + (dz+1); FP32
+ * accumulation on MFMA.  conv3d_fwd: out = act(conv(x, W) + bias + skip)
+ * (bias, skip, out_f32 optional; act = ReLU when relu != 0; Cout in
+ * {16, 32, 64}).  wgrad: dW = sum_v dy[v] x[v + off]^T through
+ * wgrad_parts() partial slices of 27*Cout*Cin floats (part), reduced in a
+ * fixed order into dw.  bias_grad: db = sum_v dy[v] (part: 256*C floats).
+ * flip_transpose: W'[t][ci][co] = W[26-t][co][ci] (input adjoint weights).
+ * relu_mask: g = dy where y > 0.  add: a += b.  vol_restrict / vol_prolong:
+ * vertex-centred full weighting / trilinear interpolation (odd sizes),
+ * times scale.  adam: Adam step on FP32 master weights with BF16 copy.
+ * net_input / net_output / mse: map a nodal vector (3 DOF per node) to the
+ * 16-channel network input normalised by the RMS stored in scal[slot] (a
+ * sum of squares), back, and the MSE loss gradient against a target.
+ * node_density: nodal mean of an element field (hex8). */
+int fantom_conv3d_fwd_bf16(int D, int H, int W, int Cin, int Cout, const uint16_t *x, const uint16_t *w,
+                           const float *bias, const uint16_t *skip, uint16_t *out, float *out_f32, int relu,
+                           void *stream);
+long fantom_conv3d_wgrad_parts(int D, int H, int W, int Cin, int Cout, long *chunks_per_wave);
+int fantom_conv3d_wgrad_bf16(int D, int H, int W, int Cin, int Cout, const uint16_t *x, const uint16_t *dy,
+                             float *part, float *dw, void *stream);
+int fantom_bias_grad_bf16(long N, int C, const uint16_t *dy, float *part, float *db, void *stream);
+int fantom_flip_transpose_bf16(int Cout, int Cin, const uint16_t *w, uint16_t *wt, void *stream);
+int fantom_relu_mask_bf16(long n, const uint16_t *y, const uint16_t *dy, uint16_t *g, void *stream);
+int fantom_add_bf16(long n, uint16_t *a, const uint16_t *b, void *stream);
+int fantom_vol_restrict_bf16(int D, int H, int W, int C, const uint16_t *x, uint16_t *out, float scale, void *stream);
+int fantom_vol_prolong_bf16(int D, int H, int W, int C, const uint16_t *xc, uint16_t *out, float scale, int accumulate,
+                            void *stream);
+int fantom_adam_f32(long n, float *p, const float *g, float *m, float *v, uint16_t *pbf, float lr, float b1, float b2,
+                    float eps, int step, void *stream);
+int fantom_f32_to_bf16(long n, const float *a, uint16_t *b, void *stream);
+int fantom_net_input(long nv, const double *r, const float *rho_node, const double *scal, int slot, uint16_t *x,
+                     void *stream);
+int fantom_net_output(long nv, const float *y, const double *scal, int slot, double *z, int accumulate, void *stream);
+int fantom_mse_bf16(long nv, const float *y, const double *t, const double *scal, int slot, uint16_t *dy,
+                    double *part, void *stream);
+int fantom_node_density(int nelx, int nely, int nelz, const double *x, float *rn, void *stream);
+
 /* ---- A4: SIMP interpolation ------------------------------------------------
  * Replaces MinimumCompliance.penalize / penalize_grad
  * (Problem/CUDA/MinimumCompliance.py:305-351, CPU :358-464), single material.

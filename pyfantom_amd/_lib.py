@@ -59,6 +59,38 @@ SIGNATURES = {
     "fantom_simp_f64": (_c_int, [_c_long, _vp, _c_int, _c_double, _c_double, _c_double, _c_int, _c_double,
                                  _c_double, _vp, _vp, _vp]),
     "fantom_mul_f64": (_c_int, [_c_long, _vp, _vp, _vp, _vp]),
+    "fantom_mg_workspace_size": (_c_long, [_c_int, _c_int, _c_int, _c_int]),
+    "fantom_mg_create": (_c_int, [_c_int, _c_int, _c_int, _c_int, _c_int, _c_double, _c_int, _vp, _vp, _c_long,
+                                  ctypes.POINTER(_vp)]),
+    "fantom_mg_destroy": (_c_int, [_vp]),
+    "fantom_mg_setup": (_c_int, [_vp, _vp, _vp, _vp]),
+    "fantom_mg_precondition": (_c_int, [_vp, _vp, _vp, _vp]),
+    "fantom_mg_solve": (_c_int, [_vp, _vp, _vp, _c_double, _c_int, _c_int, _c_int, ctypes.POINTER(_c_int),
+                                 ctypes.POINTER(_c_double), _vp]),
+    "fantom_mx_workspace_size": (_c_long, [_c_int, _c_int, _c_int, _c_int]),
+    "fantom_mx_create": (_c_int, [_c_int, _c_int, _c_int, _c_int, _c_int, _c_double, _vp, _vp, _c_long,
+                                  ctypes.POINTER(_vp)]),
+    "fantom_mx_destroy": (_c_int, [_vp]),
+    "fantom_mx_setup": (_c_int, [_vp, _vp, _vp, _vp]),
+    "fantom_mx_solve": (_c_int, [_vp, _vp, _vp, _c_double, _c_double, _c_int, _c_int, _c_int, _c_int,
+                                 ctypes.POINTER(_c_int), ctypes.POINTER(_c_int), ctypes.POINTER(_c_double), _vp]),
+    "fantom_conv3d_fwd_bf16": (_c_int, [_c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp,
+                                        _c_int, _vp]),
+    "fantom_conv3d_wgrad_parts": (_c_long, [_c_int, _c_int, _c_int, _c_int, _c_int, ctypes.POINTER(_c_long)]),
+    "fantom_conv3d_wgrad_bf16": (_c_int, [_c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp]),
+    "fantom_bias_grad_bf16": (_c_int, [_c_long, _c_int, _vp, _vp, _vp, _vp]),
+    "fantom_flip_transpose_bf16": (_c_int, [_c_int, _c_int, _vp, _vp, _vp]),
+    "fantom_relu_mask_bf16": (_c_int, [_c_long, _vp, _vp, _vp, _vp]),
+    "fantom_add_bf16": (_c_int, [_c_long, _vp, _vp, _vp]),
+    "fantom_vol_restrict_bf16": (_c_int, [_c_int, _c_int, _c_int, _c_int, _vp, _vp, ctypes.c_float, _vp]),
+    "fantom_vol_prolong_bf16": (_c_int, [_c_int, _c_int, _c_int, _c_int, _vp, _vp, ctypes.c_float, _c_int, _vp]),
+    "fantom_adam_f32": (_c_int, [_c_long, _vp, _vp, _vp, _vp, _vp, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                 ctypes.c_float, _c_int, _vp]),
+    "fantom_f32_to_bf16": (_c_int, [_c_long, _vp, _vp, _vp]),
+    "fantom_net_input": (_c_int, [_c_long, _vp, _vp, _vp, _c_int, _vp, _vp]),
+    "fantom_net_output": (_c_int, [_c_long, _vp, _vp, _c_int, _vp, _c_int, _vp]),
+    "fantom_mse_bf16": (_c_int, [_c_long, _vp, _vp, _vp, _c_int, _vp, _vp, _vp]),
+    "fantom_node_density": (_c_int, [_c_int, _c_int, _c_int, _vp, _vp, _vp]),
     "fantom_oc_update_f64": (_c_int, [_c_long, _vp, _vp, _vp, _c_double, _c_double, _vp, _vp, _vp, _vp, _vp]),
 }
 

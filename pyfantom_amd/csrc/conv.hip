@@ -1,0 +1,590 @@
+// BF16 3x3x3 convolutions on MFMA (implicit GEMM) and the volume kernels of
+// the learned preconditioner: forward, backward-data, backward-weight, bias
+// gradient, vertex-centred transfers, ReLU masks, Adam.
+//
+// Layout: channels-last volumes (D, H, W, C) with C a multiple of 16, stored
+// as BF16 bit patterns (uint16).  The node grid of the FE mesh maps to
+// (D, H, W) = (ny+1, nx+1, nz+1) in the mesh's node order, so a displacement
+// vector (node*3 + c) is already a channels-last volume with 3 channels.
+// Weights: W[tap][c_out][c_in] BF16, tap = (dy+1)*9 + (dx+1)*3 + (dz+1).
+// Accumulation is FP32 on v_mfma_f32_16x16x16_bf16:
+//   A (16 x 16, M x K): lane l holds A[l%16][4*(l/16) .. +3]
+//   B (16 x 16, K x N): lane l holds B[4*(l/16) .. +3][l%16]
+//   D (16 x 16):        lane l holds D[4*(l/16) + i][l%16], i = 0..3
+// Forward: M = voxels, K = c_in (per tap), N = c_out.  Backward-weight:
+// M = c_out, K = voxels, N = c_in.  Backward-data is the forward with the
+// tap-flipped, channel-transposed weights.
+#include "common.hpp"
+
+namespace fantom {
+
+typedef short s4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+__device__ __forceinline__ uint16_t f2bf(float f)
+{
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u)
+        return 0x7fc0;
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+constexpr int kConvBlock = 256;  // 4 waves
+constexpr int kRowTiles = 2;     // 16-voxel row tiles per wave
+
+// out = act(conv(x, w) + bias + skip); act = ReLU when relu != 0.
+// out_f32 (optional) receives the FP32 pre-rounding values.
+template <int CO_T>
+__global__ __launch_bounds__(kConvBlock) void conv3d_fwd_k(int D, int H, int W, int Cin,
+                                                         const uint16_t *__restrict__ x,
+                                                         const uint16_t *__restrict__ w,
+                                                         const float *__restrict__ bias,
+                                                         const uint16_t *__restrict__ skip,
+                                                         uint16_t *__restrict__ out, float *__restrict__ out_f32,
+                                                         int relu)
+{
+    constexpr int Cout = CO_T * 16;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const long N = (long)D * H * W;
+    const long v0 = ((long)blockIdx.x * 4 + wave) * (16 * kRowTiles);
+    if (v0 >= N)
+        return;  // uniform per wave; the kernel has no block-wide barrier
+    const int row = lane & 15, kg = lane >> 4;
+    int pj[kRowTiles], pi[kRowTiles], pk[kRowTiles];
+    bool pok[kRowTiles];
+#pragma unroll
+    for (int rt = 0; rt < kRowTiles; ++rt) {
+        const long v = v0 + rt * 16 + row;
+        pok[rt] = v < N;
+        const long vv = pok[rt] ? v : 0;
+        pk[rt] = (int)(vv % W);
+        pi[rt] = (int)((vv / W) % H);
+        pj[rt] = (int)(vv / ((long)W * H));
+    }
+    f4 acc[kRowTiles][CO_T];
+#pragma unroll
+    for (int rt = 0; rt < kRowTiles; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < CO_T; ++ct)
+            acc[rt][ct] = f4{0.f, 0.f, 0.f, 0.f};
+    const int cchunks = Cin / 16;
+    for (int t = 0; t < 27; ++t) {
+        const int dy = t / 9 - 1, dx = (t / 3) % 3 - 1, dz = t % 3 - 1;
+        long nb[kRowTiles];
+        bool ok[kRowTiles];
+#pragma unroll
+        for (int rt = 0; rt < kRowTiles; ++rt) {
+            const int j = pj[rt] + dy, i = pi[rt] + dx, k = pk[rt] + dz;
+            ok[rt] = pok[rt] && j >= 0 && j < D && i >= 0 && i < H && k >= 0 && k < W;
+            nb[rt] = ok[rt] ? ((long)j * H + i) * W + k : 0;
+        }
+        for (int c = 0; c < cchunks; ++c) {
+            s4 a[kRowTiles];
+#pragma unroll
+            for (int rt = 0; rt < kRowTiles; ++rt) {
+                a[rt] = s4{0, 0, 0, 0};
+                if (ok[rt])
+                    a[rt] = *reinterpret_cast<const s4 *>(x + nb[rt] * Cin + c * 16 + 4 * kg);
+            }
+#pragma unroll
+            for (int ct = 0; ct < CO_T; ++ct) {
+                const s4 b = *reinterpret_cast<const s4 *>(w + ((long)t * Cout + ct * 16 + row) * Cin + c * 16 + 4 * kg);
+#pragma unroll
+                for (int rt = 0; rt < kRowTiles; ++rt)
+                    acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[rt], b, acc[rt][ct], 0, 0, 0);
+            }
+        }
+    }
+#pragma unroll
+    for (int rt = 0; rt < kRowTiles; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < CO_T; ++ct) {
+            const int co = ct * 16 + row;
+            const float bco = bias ? bias[co] : 0.f;
+#pragma unroll
+            for (int ii = 0; ii < 4; ++ii) {
+                const long v = v0 + rt * 16 + 4 * kg + ii;
+                if (v >= N)
+                    continue;
+                float val = acc[rt][ct][ii] + bco;
+                if (skip)
+                    val += bf2f(skip[v * Cout + co]);
+                if (relu)
+                    val = fmaxf(val, 0.f);
+                out[v * Cout + co] = f2bf(val);
+                if (out_f32)
+                    out_f32[v * Cout + co] = val;
+            }
+        }
+}
+
+// Partial weight gradients: part[p][tap][co][ci] = sum over this wave's voxel
+// chunks of dy[v][co] * x[v + off(tap)][ci].  grid = (n_parts, 27).
+template <int CO_T, int CI_T>
+__global__ __launch_bounds__(kConvBlock) void conv3d_wgrad_k(int D, int H, int W, const uint16_t *__restrict__ x,
+                                                           const uint16_t *__restrict__ dy,
+                                                           float *__restrict__ part, long chunks_per_wave)
+{
+    constexpr int Cout = CO_T * 16, Cin = CI_T * 16;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int t = blockIdx.y;
+    const int ddy = t / 9 - 1, ddx = (t / 3) % 3 - 1, ddz = t % 3 - 1;
+    const long N = (long)D * H * W;
+    const long nchunks = (N + 15) / 16;
+    const long wid = (long)blockIdx.x * 4 + wave;
+    const int row = lane & 15, kg = lane >> 4;
+    f4 acc[CO_T][CI_T];
+#pragma unroll
+    for (int a = 0; a < CO_T; ++a)
+#pragma unroll
+        for (int b = 0; b < CI_T; ++b)
+            acc[a][b] = f4{0.f, 0.f, 0.f, 0.f};
+    const long c0 = wid * chunks_per_wave;
+    for (long ch = c0; ch < c0 + chunks_per_wave && ch < nchunks; ++ch) {
+        s4 av[CO_T], bv[CI_T];
+#pragma unroll
+        for (int a = 0; a < CO_T; ++a)
+            av[a] = s4{0, 0, 0, 0};
+#pragma unroll
+        for (int b = 0; b < CI_T; ++b)
+            bv[b] = s4{0, 0, 0, 0};
+#pragma unroll
+        for (int ii = 0; ii < 4; ++ii) {
+            const long v = ch * 16 + 4 * kg + ii;
+            if (v >= N)
+                continue;
+            const int k = (int)(v % W), i = (int)((v / W) % H), j = (int)(v / ((long)W * H));
+            const int jj = j + ddy, i2 = i + ddx, kk = k + ddz;
+            const bool ok = jj >= 0 && jj < D && i2 >= 0 && i2 < H && kk >= 0 && kk < W;
+            const long nb = ((long)jj * H + i2) * W + kk;
+#pragma unroll
+            for (int a = 0; a < CO_T; ++a)
+                av[a][ii] = (short)dy[v * Cout + a * 16 + row];
+            if (ok) {
+#pragma unroll
+                for (int b = 0; b < CI_T; ++b)
+                    bv[b][ii] = (short)x[nb * Cin + b * 16 + row];
+            }
+        }
+#pragma unroll
+        for (int a = 0; a < CO_T; ++a)
+#pragma unroll
+            for (int b = 0; b < CI_T; ++b)
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[a], bv[b], acc[a][b], 0, 0, 0);
+    }
+    float *P = part + (wid * 27 + t) * (long)(Cout * Cin);
+#pragma unroll
+    for (int a = 0; a < CO_T; ++a)
+#pragma unroll
+        for (int b = 0; b < CI_T; ++b)
+#pragma unroll
+            for (int ii = 0; ii < 4; ++ii)
+                P[(a * 16 + 4 * kg + ii) * Cin + b * 16 + row] = acc[a][b][ii];
+}
+
+// out[i] = sum_p part[p][i], fixed order (deterministic)
+__global__ void reduce_parts_k(long n, long nparts, const float *__restrict__ part, float *__restrict__ out)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    float s = 0.f;
+    for (long p = 0; p < nparts; ++p)
+        s += part[p * n + i];
+    out[i] = s;
+}
+
+// db[co] partials: grid-stride over voxels, per-block partial per channel
+__global__ __launch_bounds__(256) void bias_grad_k(long N, int C, const uint16_t *__restrict__ dy,
+                                                   float *__restrict__ part)
+{
+    // thread layout: channel = threadIdx.x % C (C divides 256 for C in {16, 32, 64})
+    const int c = threadIdx.x % C, lanes = 256 / C, sub = threadIdx.x / C;
+    float s = 0.f;
+    for (long v = (long)blockIdx.x * lanes + sub; v < N; v += (long)gridDim.x * lanes)
+        s += bf2f(dy[v * C + c]);
+    __shared__ float sh[256];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    if (sub == 0) {
+        float t = 0.f;
+        for (int q = 0; q < lanes; ++q)
+            t += sh[q * C + c];
+        part[(long)blockIdx.x * C + c] = t;
+    }
+}
+
+// wt[t][ci][co] = w[26 - t][co][ci]   (weights of the input adjoint)
+__global__ void flip_transpose_k(int Cout, int Cin, const uint16_t *__restrict__ w, uint16_t *__restrict__ wt)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long n = 27L * Cout * Cin;
+    if (i >= n)
+        return;
+    const int t = (int)(i / ((long)Cout * Cin));
+    const int r = (int)(i % ((long)Cout * Cin));
+    const int ci = r / Cout, co = r % Cout;
+    wt[i] = w[((long)(26 - t) * Cout + co) * Cin + ci];
+}
+
+// g = dy where y > 0 else 0 (ReLU backward through the stored activation)
+__global__ void relu_mask_k(long n, const uint16_t *__restrict__ y, const uint16_t *__restrict__ dy,
+                            uint16_t *__restrict__ g)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        g[i] = bf2f(y[i]) > 0.f ? dy[i] : (uint16_t)0;
+}
+
+// a += b (bf16, FP32 add)
+__global__ void add_bf16_k(long n, uint16_t *__restrict__ a, const uint16_t *__restrict__ b)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        a[i] = f2bf(bf2f(a[i]) + bf2f(b[i]));
+}
+
+// Vertex-centred transfers on channels-last BF16 volumes.
+// restrict: out[c] = scale * sum_{3^3} 2^-dist x[2c + d]
+__global__ void vol_restrict_k(int D, int H, int W, int C, const uint16_t *__restrict__ x,
+                               uint16_t *__restrict__ out, float scale)
+{
+    const int Dc = (D - 1) / 2 + 1, Hc = (H - 1) / 2 + 1, Wc = (W - 1) / 2 + 1;
+    const long n = (long)Dc * Hc * Wc * C;
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n)
+        return;
+    const int c = (int)(idx % C);
+    const long v = idx / C;
+    const int kc = (int)(v % Wc), ic = (int)((v / Wc) % Hc), jc = (int)(v / ((long)Wc * Hc));
+    float s = 0.f;
+    for (int dj = -1; dj <= 1; ++dj)
+        for (int di = -1; di <= 1; ++di)
+            for (int dk = -1; dk <= 1; ++dk) {
+                const int j = 2 * jc + dj, i = 2 * ic + di, k = 2 * kc + dk;
+                if (j < 0 || j >= D || i < 0 || i >= H || k < 0 || k >= W)
+                    continue;
+                const float wgt = 1.f / (float)(1 << (abs(dj) + abs(di) + abs(dk)));
+                s += wgt * bf2f(x[(((long)j * H + i) * W + k) * C + c]);
+            }
+    out[idx] = f2bf(s * scale);
+}
+
+// prolong: out[f] (+)= scale * trilinear interpolation of the coarse volume
+__global__ void vol_prolong_k(int D, int H, int W, int C, const uint16_t *__restrict__ xc, uint16_t *__restrict__ out,
+                              float scale, int accumulate)
+{
+    const int Hc = (H - 1) / 2 + 1, Wc = (W - 1) / 2 + 1;
+    const long n = (long)D * H * W * C;
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n)
+        return;
+    const int c = (int)(idx % C);
+    const long v = idx / C;
+    const int k = (int)(v % W), i = (int)((v / W) % H), j = (int)(v / ((long)W * H));
+    const float wgt = 1.f / (float)(1 << ((j & 1) + (i & 1) + (k & 1)));
+    float s = 0.f;
+    for (int cj = j / 2; cj <= (j + 1) / 2; ++cj)
+        for (int ci = i / 2; ci <= (i + 1) / 2; ++ci)
+            for (int ck = k / 2; ck <= (k + 1) / 2; ++ck)
+                s += wgt * bf2f(xc[(((long)cj * Hc + ci) * Wc + ck) * C + c]);
+    s *= scale;
+    out[idx] = f2bf(accumulate ? bf2f(out[idx]) + s : s);
+}
+
+// Adam on FP32 master weights; also refreshes the BF16 copy used by the kernels.
+__global__ void adam_k(long n, float *__restrict__ p, const float *__restrict__ g, float *__restrict__ m,
+                       float *__restrict__ v, uint16_t *__restrict__ pbf, float lr, float b1, float b2, float eps,
+                       float bc1, float bc2)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    const float upd = lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+    const float pi = p[i] - upd;
+    p[i] = pi;
+    if (pbf)
+        pbf[i] = f2bf(pi);
+}
+
+__global__ void f32_to_bf16_k(long n, const float *__restrict__ a, uint16_t *__restrict__ b)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        b[i] = f2bf(a[i]);
+}
+
+// Network input: x[v][0..2] = r[3v + c] * inv_s, x[v][3] = rho_node[v],
+// channels 4..15 zero (C = 16).
+__global__ void net_input_k(long nv, const double *__restrict__ r, const float *__restrict__ rho_node,
+                            const double *__restrict__ scal, int slot, uint16_t *__restrict__ x)
+{
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nv * 16)
+        return;
+    const long v = idx / 16;
+    const int c = (int)(idx % 16);
+    const double inv_s = 1.0 / sqrt(fmax(scal[slot], 1e-300) / (double)(3 * nv));
+    float val = 0.f;
+    if (c < 3)
+        val = (float)(r[3 * v + c] * inv_s);
+    else if (c == 3 && rho_node)
+        val = rho_node[v];
+    x[idx] = f2bf(val);
+}
+
+// z[3v + c] = s * y[v][c] (c < 3, y has C = 16 channels), s = RMS of r
+__global__ void net_output_k(long nv, const float *__restrict__ y, const double *__restrict__ scal, int slot,
+                             double *__restrict__ z, int accumulate)
+{
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nv * 3)
+        return;
+    const long v = idx / 3;
+    const int c = (int)(idx % 3);
+    const double s = sqrt(fmax(scal[slot], 0.0) / (double)(3 * nv));
+    const double val = s * (double)y[v * 16 + c];
+    z[idx] = accumulate ? z[idx] + val : val;
+}
+
+// MSE loss against the target t[3v + c] scaled by 1/s: dy[v][c] = 2 (y - t/s) / (3 nv)
+// (c < 3; channels 3..15 get 0); part[block] = sum of squared errors
+__global__ __launch_bounds__(256) void mse_k(long nv, const float *__restrict__ y, const double *__restrict__ t,
+                                            const double *__restrict__ scal, int slot, uint16_t *__restrict__ dy,
+                                            double *__restrict__ part)
+{
+    const double inv_s = 1.0 / sqrt(fmax(scal[slot], 1e-300) / (double)(3 * nv));
+    double acc = 0.0;
+    for (long idx = (long)blockIdx.x * 256 + threadIdx.x; idx < nv * 16; idx += (long)gridDim.x * 256) {
+        const long v = idx / 16;
+        const int c = (int)(idx % 16);
+        float g = 0.f;
+        if (c < 3) {
+            const double e = (double)y[idx] - t[3 * v + c] * inv_s;
+            acc += e * e;
+            g = (float)(2.0 * e / (double)(3 * nv));
+        }
+        dy[idx] = f2bf(g);
+    }
+    acc = block_sum<256>(acc);
+    if (threadIdx.x == 0)
+        part[blockIdx.x] = acc;
+}
+
+// nodal density: mean of the moduli-free design values of the elements
+// around each node (hex8 grid, node order of the mesh)
+__global__ void node_density_k(Grid g, const double *__restrict__ x, float *__restrict__ rn)
+{
+    const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= g.n_nodes)
+        return;
+    const int k = (int)(n % g.nz);
+    const long t = n / g.nz;
+    const int i = (int)(t % g.nx), j = (int)(t / g.nx);
+    double s = 0.0;
+    int m = 0;
+    for (int oy = 0; oy < 2; ++oy)
+        for (int ox = 0; ox < 2; ++ox)
+            for (int oz = 0; oz < 2; ++oz) {
+                const int ei = i - ox, ej = j - oy, ek = k - oz;
+                if (ei < 0 || ei >= g.nelx || ej < 0 || ej >= g.nely || ek < 0 || ek >= g.nelz)
+                    continue;
+                s += x[((long)ei * g.nely + ej) * g.nelz + ek];
+                m++;
+            }
+    rn[n] = (float)(s / m);
+}
+
+static unsigned blk(long n, int b = 256) { return (unsigned)((n + b - 1) / b); }
+
+}  // namespace fantom
+
+using namespace fantom;
+
+extern "C" {
+
+int fantom_conv3d_fwd_bf16(int D, int H, int W, int Cin, int Cout, const uint16_t *x, const uint16_t *w,
+                           const float *bias, const uint16_t *skip, uint16_t *out, float *out_f32, int relu,
+                           void *stream)
+{
+    FANTOM_CHECK_ARG(D > 0 && H > 0 && W > 0 && x && w && out, "bad operands");
+    FANTOM_CHECK_ARG(Cin % 16 == 0 && Cin >= 16 && Cin <= 256, "Cin must be a multiple of 16 in [16, 256]");
+    const long N = (long)D * H * W;
+    const unsigned blocks = (unsigned)((N + 4 * 16 * kRowTiles - 1) / (4 * 16 * kRowTiles));
+    hipStream_t s = as_stream(stream);
+    switch (Cout) {
+    case 16:
+        hipLaunchKernelGGL(conv3d_fwd_k<1>, dim3(blocks), dim3(kConvBlock), 0, s, D, H, W, Cin, x, w, bias, skip, out,
+                           out_f32, relu);
+        break;
+    case 32:
+        hipLaunchKernelGGL(conv3d_fwd_k<2>, dim3(blocks), dim3(kConvBlock), 0, s, D, H, W, Cin, x, w, bias, skip, out,
+                           out_f32, relu);
+        break;
+    case 64:
+        hipLaunchKernelGGL(conv3d_fwd_k<4>, dim3(blocks), dim3(kConvBlock), 0, s, D, H, W, Cin, x, w, bias, skip, out,
+                           out_f32, relu);
+        break;
+    default:
+        set_error("Cout must be 16, 32 or 64 (got %d)", Cout);
+        return kStatusBadArg;
+    }
+    FANTOM_LAUNCH_CHECK("conv3d_fwd");
+    return kStatusOk;
+}
+
+long fantom_conv3d_wgrad_parts(int D, int H, int W, int Cin, int Cout, long *chunks_per_wave)
+{
+    const long N = (long)D * H * W;
+    const long nchunks = (N + 15) / 16;
+    long waves = 256;  // 64 blocks of 4 waves per tap
+    long cpw = (nchunks + waves - 1) / waves;
+    if (chunks_per_wave)
+        *chunks_per_wave = cpw;
+    return waves;  // partial slices, each 27 * Cout * Cin floats
+}
+
+int fantom_conv3d_wgrad_bf16(int D, int H, int W, int Cin, int Cout, const uint16_t *x, const uint16_t *dy,
+                             float *part, float *dw, void *stream)
+{
+    FANTOM_CHECK_ARG(D > 0 && H > 0 && W > 0 && x && dy && part && dw, "bad operands");
+    long cpw = 0;
+    const long waves = fantom_conv3d_wgrad_parts(D, H, W, Cin, Cout, &cpw);
+    const dim3 grid((unsigned)(waves / 4), 27);
+    hipStream_t s = as_stream(stream);
+#define WG(CO, CI)                                                                                         \
+    if (Cout == CO * 16 && Cin == CI * 16) {                                                               \
+        hipLaunchKernelGGL((conv3d_wgrad_k<CO, CI>), grid, dim3(kConvBlock), 0, s, D, H, W, x, dy, part, cpw); \
+    } else
+    WG(1, 1) WG(2, 1) WG(1, 2) WG(2, 2) WG(4, 2) WG(2, 4) WG(4, 4) WG(1, 4) WG(4, 1)
+    {
+        set_error("unsupported channel pair %d -> %d", Cin, Cout);
+        return kStatusBadArg;
+    }
+#undef WG
+    const long n = 27L * Cout * Cin;
+    hipLaunchKernelGGL(reduce_parts_k, dim3(blk(n)), dim3(256), 0, s, n, waves, part, dw);
+    FANTOM_LAUNCH_CHECK("conv3d_wgrad");
+    return kStatusOk;
+}
+
+int fantom_bias_grad_bf16(long N, int C, const uint16_t *dy, float *part, float *db, void *stream)
+{
+    FANTOM_CHECK_ARG(N > 0 && (C == 16 || C == 32 || C == 64) && dy && part && db, "bad operands");
+    const int nb = 256;
+    hipStream_t s = as_stream(stream);
+    hipLaunchKernelGGL(bias_grad_k, dim3(nb), dim3(256), 0, s, N, C, dy, part);
+    hipLaunchKernelGGL(reduce_parts_k, dim3(1), dim3(64), 0, s, (long)C, (long)nb, part, db);
+    FANTOM_LAUNCH_CHECK("bias_grad");
+    return kStatusOk;
+}
+
+int fantom_flip_transpose_bf16(int Cout, int Cin, const uint16_t *w, uint16_t *wt, void *stream)
+{
+    FANTOM_CHECK_ARG(w && wt && w != wt, "bad operands");
+    const long n = 27L * Cout * Cin;
+    hipLaunchKernelGGL(flip_transpose_k, dim3(blk(n)), dim3(256), 0, as_stream(stream), Cout, Cin, w, wt);
+    FANTOM_LAUNCH_CHECK("flip_transpose");
+    return kStatusOk;
+}
+
+int fantom_relu_mask_bf16(long n, const uint16_t *y, const uint16_t *dy, uint16_t *g, void *stream)
+{
+    FANTOM_CHECK_ARG(n > 0 && y && dy && g, "bad operands");
+    hipLaunchKernelGGL(relu_mask_k, dim3(blk(n)), dim3(256), 0, as_stream(stream), n, y, dy, g);
+    FANTOM_LAUNCH_CHECK("relu_mask");
+    return kStatusOk;
+}
+
+int fantom_add_bf16(long n, uint16_t *a, const uint16_t *b, void *stream)
+{
+    FANTOM_CHECK_ARG(n > 0 && a && b, "bad operands");
+    hipLaunchKernelGGL(add_bf16_k, dim3(blk(n)), dim3(256), 0, as_stream(stream), n, a, b);
+    FANTOM_LAUNCH_CHECK("add_bf16");
+    return kStatusOk;
+}
+
+int fantom_vol_restrict_bf16(int D, int H, int W, int C, const uint16_t *x, uint16_t *out, float scale, void *stream)
+{
+    FANTOM_CHECK_ARG(D % 2 == 1 && H % 2 == 1 && W % 2 == 1 && x && out, "volume sizes must be odd");
+    const long n = (long)((D - 1) / 2 + 1) * ((H - 1) / 2 + 1) * ((W - 1) / 2 + 1) * C;
+    hipLaunchKernelGGL(vol_restrict_k, dim3(blk(n)), dim3(256), 0, as_stream(stream), D, H, W, C, x, out, scale);
+    FANTOM_LAUNCH_CHECK("vol_restrict");
+    return kStatusOk;
+}
+
+int fantom_vol_prolong_bf16(int D, int H, int W, int C, const uint16_t *xc, uint16_t *out, float scale, int accumulate,
+                            void *stream)
+{
+    FANTOM_CHECK_ARG(D % 2 == 1 && H % 2 == 1 && W % 2 == 1 && xc && out, "volume sizes must be odd");
+    const long n = (long)D * H * W * C;
+    hipLaunchKernelGGL(vol_prolong_k, dim3(blk(n)), dim3(256), 0, as_stream(stream), D, H, W, C, xc, out, scale,
+                       accumulate);
+    FANTOM_LAUNCH_CHECK("vol_prolong");
+    return kStatusOk;
+}
+
+int fantom_adam_f32(long n, float *p, const float *g, float *m, float *v, uint16_t *pbf, float lr, float b1, float b2,
+                    float eps, int step, void *stream)
+{
+    FANTOM_CHECK_ARG(n > 0 && p && g && m && v && step >= 1, "bad operands");
+    const float bc1 = 1.f - powf(b1, (float)step), bc2 = 1.f - powf(b2, (float)step);
+    hipLaunchKernelGGL(adam_k, dim3(blk(n)), dim3(256), 0, as_stream(stream), n, p, g, m, v, pbf, lr, b1, b2, eps, bc1,
+                       bc2);
+    FANTOM_LAUNCH_CHECK("adam");
+    return kStatusOk;
+}
+
+int fantom_f32_to_bf16(long n, const float *a, uint16_t *b, void *stream)
+{
+    FANTOM_CHECK_ARG(n > 0 && a && b, "bad operands");
+    hipLaunchKernelGGL(f32_to_bf16_k, dim3(blk(n)), dim3(256), 0, as_stream(stream), n, a, b);
+    FANTOM_LAUNCH_CHECK("f32_to_bf16");
+    return kStatusOk;
+}
+
+int fantom_net_input(long nv, const double *r, const float *rho_node, const double *scal, int slot, uint16_t *x,
+                     void *stream)
+{
+    FANTOM_CHECK_ARG(nv > 0 && r && scal && x && slot >= 0 && slot < 16, "bad operands");
+    hipLaunchKernelGGL(net_input_k, dim3(blk(nv * 16)), dim3(256), 0, as_stream(stream), nv, r, rho_node, scal, slot,
+                       x);
+    FANTOM_LAUNCH_CHECK("net_input");
+    return kStatusOk;
+}
+
+int fantom_net_output(long nv, const float *y, const double *scal, int slot, double *z, int accumulate, void *stream)
+{
+    FANTOM_CHECK_ARG(nv > 0 && y && scal && z && slot >= 0 && slot < 16, "bad operands");
+    hipLaunchKernelGGL(net_output_k, dim3(blk(nv * 3)), dim3(256), 0, as_stream(stream), nv, y, scal, slot, z,
+                       accumulate);
+    FANTOM_LAUNCH_CHECK("net_output");
+    return kStatusOk;
+}
+
+int fantom_mse_bf16(long nv, const float *y, const double *t, const double *scal, int slot, uint16_t *dy,
+                    double *part, void *stream)
+{
+    FANTOM_CHECK_ARG(nv > 0 && y && t && scal && dy && part, "bad operands");
+    hipLaunchKernelGGL(mse_k, dim3(256), dim3(256), 0, as_stream(stream), nv, y, t, scal, slot, dy, part);
+    FANTOM_LAUNCH_CHECK("mse");
+    return kStatusOk;
+}
+
+int fantom_node_density(int nelx, int nely, int nelz, const double *x, float *rn, void *stream)
+{
+    FANTOM_CHECK_ARG(grid_ok(nelx, nely, nelz) && nelz > 0 && x && rn, "bad operands");
+    const Grid g = make_grid(nelx, nely, nelz);
+    hipLaunchKernelGGL(node_density_k, dim3(blk(g.n_nodes)), dim3(256), 0, as_stream(stream), g, x, rn);
+    FANTOM_LAUNCH_CHECK("node_density");
+    return kStatusOk;
+}
+
+}  // extern "C"

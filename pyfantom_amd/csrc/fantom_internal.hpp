@@ -19,6 +19,9 @@ int launch_dc(const Grid &g, int dim, const double *K0, const double *u, double 
 int launch_ku_tile(int mode, const Grid &g, const double *Kb, const double *E, const uint8_t *mask,
                    const double *u, double *y, const double *b, const double *dinv, double omega,
                    double *partials, hipStream_t s);
+int launch_ku_tile_f32(int mode, const Grid &g, const double *Kb, const float *E, const uint8_t *mask,
+                       const float *u, float *y, const float *b, const float *dinv, double omega, double *partials,
+                       hipStream_t s);
 long ku_tile_partials(const Grid &g);
 
 // mg.hip
@@ -27,9 +30,18 @@ int launch_restrict(const Grid &gf, int dim, int dof, int normalize, const uint8
 int launch_prolong(const Grid &gf, int dim, int dof, int normalize, const uint8_t *mask_f, const double *ec,
                    double *xf, int accumulate, hipStream_t s);
 int launch_coarsen_E(const Grid &gf, int dim, const double *Ef, double *Ec, hipStream_t s);
+int launch_restrict_f32(const Grid &gf, int dim, int dof, int normalize, const uint8_t *mask_c, const float *rf,
+                        float *rc, hipStream_t s);
+int launch_prolong_f32(const Grid &gf, int dim, int dof, int normalize, const uint8_t *mask_f, const float *ec,
+                       float *xf, int accumulate, hipStream_t s);
+int launch_coarsen_E_f32(const Grid &gf, int dim, const float *Ef, float *Ec, hipStream_t s);
+int launch_gemv_f32(int n, const double *A, const float *x, float *y, hipStream_t s);
+int launch_diag_f32(const Grid &g, const double *K0, const float *E, const uint8_t *mask, float *d, hipStream_t s);
 int launch_coarsen_mask(const Grid &gf, int dim, const uint8_t *mf, uint8_t *mc, hipStream_t s);
 int launch_dense_assemble(const Grid &g, int dim, const double *K0, const double *E, const uint8_t *mask,
                           double *A, hipStream_t s);
+int launch_dense_assemble_f32E(const Grid &g, const double *K0, const float *E, const uint8_t *mask, double *A,
+                               hipStream_t s);
 int launch_spd_inverse(int n, double *A, double *work, hipStream_t s);
 int launch_gemv(int n, const double *A, const double *x, double *y, hipStream_t s);
 

@@ -284,6 +284,43 @@ __global__ __launch_bounds__(kNodeBlock) void dc_elem_f64(Grid g, const double *
     dc[e] = -acc;
 }
 
+// FP32 Jacobi weights dinv = 1 / diag(K(E)) for a hex8 grid (constrained -> 1),
+// accumulated in FP64 from the FP64 element matrix.
+__global__ __launch_bounds__(kNodeBlock) void dinv3d_f32(Grid g, const double *__restrict__ K0g,
+                                                       const float *__restrict__ E,
+                                                       const uint8_t *__restrict__ mask, float *__restrict__ d)
+{
+    const long n = (long)blockIdx.x * kNodeBlock + threadIdx.x;
+    if (n >= g.n_nodes)
+        return;
+    double acc[3] = {0.0, 0.0, 0.0};
+    const int k = (int)(n % g.nz);
+    const long t = n / g.nz;
+    const int i = (int)(t % g.nx), j = (int)(t / g.nx);
+    for (int oy = 0; oy < 2; ++oy)
+        for (int ox = 0; ox < 2; ++ox)
+            for (int oz = 0; oz < 2; ++oz) {
+                const int ei = i - ox, ej = j - oy, ek = k - oz;
+                if (ei < 0 || ei >= g.nelx || ej < 0 || ej >= g.nely || ek < 0 || ek >= g.nelz)
+                    continue;
+                const double Ee = E[((long)ei * g.nely + ej) * g.nelz + ek];
+                const int a = hex_corner(ox, oy, oz);
+                for (int c = 0; c < 3; ++c)
+                    acc[c] += K0g[(a * 3 + c) * 24 + a * 3 + c] * Ee;
+            }
+    const int mn = mask ? mask[n] : 0;
+    for (int c = 0; c < 3; ++c)
+        d[3 * n + c] = ((mn >> c) & 1) ? 1.0f : (float)(1.0 / acc[c]);
+}
+
+int launch_diag_f32(const Grid &g, const double *K0, const float *E, const uint8_t *mask, float *d, hipStream_t s)
+{
+    const unsigned blocks = (unsigned)((g.n_nodes + kNodeBlock - 1) / kNodeBlock);
+    hipLaunchKernelGGL(dinv3d_f32, dim3(blocks), dim3(kNodeBlock), 0, s, g, K0, E, mask, d);
+    FANTOM_LAUNCH_CHECK("dinv_f32");
+    return kStatusOk;
+}
+
 int launch_ku_node(int mode, const Grid &g, int dim, const double *K0, const double *E, const uint8_t *mask,
                    const double *u, double *y, const double *b, const double *dinv, double omega,
                    double *partials, hipStream_t s)

@@ -40,18 +40,20 @@ constexpr int kTileBlock = TB * TB;
 constexpr int kPrefetch = (3 * PLANE + kTileBlock - 1) / kTileBlock;   // doubles per thread
 constexpr int kPrefetchM = (PLANE + kTileBlock - 1) / kTileBlock;      // mask bytes per thread
 
+template <typename T>
 struct HexBlocks {
-    double b[72];  // B_s[c][d] at b[9*s + 3*c + d], scaled by 1/8
+    T b[72];  // B_s[c][d] at b[9*s + 3*c + d], scaled by 1/8
 };
 
-__device__ __forceinline__ void wht8(double *w)
+template <typename T>
+__device__ __forceinline__ void wht8(T *w)
 {
 #pragma unroll
     for (int h = 1; h < 8; h <<= 1) {
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
             if ((i & h) == 0) {
-                const double a = w[i], b = w[i | h];
+                const T a = w[i], b = w[i | h];
                 w[i] = a + b;
                 w[i | h] = a - b;
             }
@@ -59,18 +61,17 @@ __device__ __forceinline__ void wht8(double *w)
     }
 }
 
-template <int MODE>
-__global__ __launch_bounds__(kTileBlock, 2) void ku3d_tile_f64(Grid g, HexBlocks B, const double *__restrict__ E,
-                                                             const uint8_t *__restrict__ mask,
-                                                             const double *__restrict__ u,
-                                                             double *__restrict__ y, const double *__restrict__ bvec,
-                                                             const double *__restrict__ dinv, double omega,
-                                                             double *__restrict__ partials, int tiles_x,
-                                                             int tiles_z, int chunks_y)
+template <typename T, int MODE>
+__global__ __launch_bounds__(kTileBlock, 2) void ku3d_tile(Grid g, HexBlocks<T> B, const T *__restrict__ E,
+                                                         const uint8_t *__restrict__ mask, const T *__restrict__ u,
+                                                         T *__restrict__ y, const T *__restrict__ bvec,
+                                                         const T *__restrict__ dinv, T omega,
+                                                         double *__restrict__ partials, int tiles_x, int tiles_z,
+                                                         int chunks_y)
 {
-    __shared__ double us[2][3][PLANE];   // u node planes, SoA, ring of 2
+    __shared__ T us[2][3][PLANE];        // u node planes, SoA, ring of 2
     __shared__ uint8_t ms[2][PLANE];     // constraint bits of those planes
-    __shared__ double outs[24][kTileBlock];  // element outputs, SoA
+    __shared__ T outs[24][kTileBlock];   // element outputs, SoA
 
     // XCD-aware tile order: the hardware deals workgroups to the 8 XCDs
     // round-robin, so give each XCD a contiguous range of logical tiles.
@@ -92,11 +93,11 @@ __global__ __launch_bounds__(kTileBlock, 2) void ku3d_tile_f64(Grid g, HexBlocks
     const int jc1 = min(jc0 + CHUNK, g.ny);               // output node planes [jc0, jc1)
     const long nplane = (long)g.nx * g.nz;
 
-    auto load_plane = [&](int j, double *vals, uint8_t *mv) {
+    auto load_plane = [&](int j, T *vals, uint8_t *mv) {
 #pragma unroll
         for (int q = 0; q < kPrefetch; ++q) {
             const int idx = tid + q * kTileBlock;
-            double v = 0.0;
+            T v = T(0);
             if (idx < 3 * PLANE && j >= 0 && j < g.ny) {
                 const int a = idx / (3 * TN), rem = idx % (3 * TN);
                 const int gx = ex0 + a, gz = ez0 + rem / 3;
@@ -118,7 +119,7 @@ __global__ __launch_bounds__(kTileBlock, 2) void ku3d_tile_f64(Grid g, HexBlocks
             mv[q] = m;
         }
     };
-    auto store_plane = [&](int slot, const double *vals, const uint8_t *mv) {
+    auto store_plane = [&](int slot, const T *vals, const uint8_t *mv) {
 #pragma unroll
         for (int q = 0; q < kPrefetch; ++q) {
             const int idx = tid + q * kTileBlock;
@@ -135,7 +136,7 @@ __global__ __launch_bounds__(kTileBlock, 2) void ku3d_tile_f64(Grid g, HexBlocks
         }
     };
 
-    double pre[kPrefetch];
+    T pre[kPrefetch];
     uint8_t prem[kPrefetchM];
     if (valid_tile) {
         load_plane(jc0 - 1, pre, prem);
@@ -145,7 +146,7 @@ __global__ __launch_bounds__(kTileBlock, 2) void ku3d_tile_f64(Grid g, HexBlocks
     }
     __syncthreads();
 
-    double hi[3] = {0.0, 0.0, 0.0};    // upper-plane contributions of the previous layer
+    T hi[3] = {T(0), T(0), T(0)};    // upper-plane contributions of the previous layer
     const int gxe = ex0 + lx, gze = ez0 + lz;  // this thread's element (x, z)
     const bool el_in = gxe >= 0 && gxe < g.nelx && gze >= 0 && gze < g.nelz;
 
@@ -156,7 +157,7 @@ __global__ __launch_bounds__(kTileBlock, 2) void ku3d_tile_f64(Grid g, HexBlocks
             load_plane(L + 2, pre, prem);
 
         // own node (lx, lz) of plane L, unmasked, for the epilogue
-        double uown[3];
+        T uown[3] = {T(0), T(0), T(0)};
         int mown = 0;
         if (L >= jc0) {
 #pragma unroll
@@ -166,10 +167,10 @@ __global__ __launch_bounds__(kTileBlock, 2) void ku3d_tile_f64(Grid g, HexBlocks
         }
 
         // ---- 1. element outputs
-        double w[3][8];
+        T w[3][8];
         const bool active = el_in && L >= 0 && L < g.nely;
         if (active) {
-            const double Ee = E[((long)gxe * g.nely + L) * g.nelz + gze];
+            const T Ee = E[((long)gxe * g.nely + L) * g.nelz + gze];
 #pragma unroll
             for (int b = 0; b < 8; ++b) {
                 const int bx = b & 1, by = (b >> 1) & 1, bz = b >> 2;
@@ -178,16 +179,16 @@ __global__ __launch_bounds__(kTileBlock, 2) void ku3d_tile_f64(Grid g, HexBlocks
                 const int m = ms[sidx][p];
 #pragma unroll
                 for (int c = 0; c < 3; ++c)
-                    w[c][b] = ((m >> c) & 1) ? 0.0 : us[sidx][c][p];
+                    w[c][b] = ((m >> c) & 1) ? T(0) : us[sidx][c][p];
             }
 #pragma unroll
             for (int c = 0; c < 3; ++c)
                 wht8(w[c]);
-            double z[3][8];
+            T z[3][8];
 #pragma unroll
             for (int s = 0; s < 8; ++s) {
-                const double h0 = w[0][s ^ 1], h1 = w[1][s ^ 2], h2 = w[2][s ^ 4];
-                const double *Bs = B.b + 9 * s;
+                const T h0 = w[0][s ^ 1], h1 = w[1][s ^ 2], h2 = w[2][s ^ 4];
+                const T *Bs = B.b + 9 * s;
                 z[0][s ^ 1] = Ee * (Bs[0] * h0 + Bs[1] * h1 + Bs[2] * h2);
                 z[1][s ^ 2] = Ee * (Bs[3] * h0 + Bs[4] * h1 + Bs[5] * h2);
                 z[2][s ^ 4] = Ee * (Bs[6] * h0 + Bs[7] * h1 + Bs[8] * h2);
@@ -202,14 +203,14 @@ __global__ __launch_bounds__(kTileBlock, 2) void ku3d_tile_f64(Grid g, HexBlocks
         } else {
 #pragma unroll
             for (int v = 0; v < 24; ++v)
-                outs[v][tid] = 0.0;
+                outs[v][tid] = T(0);
         }
         __syncthreads();
 
         // ---- 2. node gather for plane L (lower corners) and L+1 (upper)
         const bool node_thread = lx >= 1 && lz >= 1;
         if (node_thread) {
-            double lo[3] = {0.0, 0.0, 0.0}, up[3] = {0.0, 0.0, 0.0};
+            T lo[3] = {T(0), T(0), T(0)}, up[3] = {T(0), T(0), T(0)};
 #pragma unroll
             for (int bx = 0; bx < 2; ++bx)
 #pragma unroll
@@ -229,8 +230,8 @@ __global__ __launch_bounds__(kTileBlock, 2) void ku3d_tile_f64(Grid g, HexBlocks
 #pragma unroll
                     for (int c = 0; c < 3; ++c) {
                         const long r = 3 * n + c;
-                        const double ku = ((mown >> c) & 1) ? uown[c] : hi[c] + lo[c];
-                        double o;
+                        const T ku = ((mown >> c) & 1) ? uown[c] : hi[c] + lo[c];
+                        T o;
                         if (MODE == MODE_JAC)
                             o = uown[c] + omega * dinv[r] * (bvec[r] - ku);
                         else if (MODE == MODE_RES)
@@ -239,7 +240,7 @@ __global__ __launch_bounds__(kTileBlock, 2) void ku3d_tile_f64(Grid g, HexBlocks
                             o = ku;
                         y[r] = o;
                         if (MODE == MODE_KUDOT)
-                            part += uown[c] * o;
+                            part += (double)uown[c] * (double)o;
                     }
                 }
             }
@@ -273,20 +274,22 @@ long ku_tile_partials(const Grid &g)
     return (long)tx * tz * cy;
 }
 
-int launch_ku_tile(int mode, const Grid &g, const double *Kb, const double *E, const uint8_t *mask,
-                   const double *u, double *y, const double *b, const double *dinv, double omega,
-                   double *partials, hipStream_t s)
+template <typename T>
+int launch_ku_tile_t(int mode, const Grid &g, const double *Kb, const T *E, const uint8_t *mask, const T *u, T *y,
+                     const T *b, const T *dinv, double omega, double *partials, hipStream_t s)
 {
-    HexBlocks B;
-    // Kb is a host pointer here: the 72 block entries travel as kernel arguments
-    memcpy(B.b, Kb, sizeof(B.b));
+    HexBlocks<T> B;
+    // Kb is a host pointer: the 72 block entries travel as kernel arguments
+    for (int i = 0; i < 72; ++i)
+        B.b[i] = (T)Kb[i];
     int tx, tz, cy;
     tile_grid(g, tx, tz, cy);
     const unsigned blocks = (unsigned)(tx * tz * cy);
+    const T om = (T)omega;
 #define FANTOM_TILE_CASE(M)                                                                                \
     case M:                                                                                                \
-        hipLaunchKernelGGL(ku3d_tile_f64<M>, dim3(blocks), dim3(kTileBlock), 0, s, g, B, E, mask, u, y, b,   \
-                           dinv, omega, partials, tx, tz, cy);                                             \
+        hipLaunchKernelGGL((ku3d_tile<T, M>), dim3(blocks), dim3(kTileBlock), 0, s, g, B, E, mask, u, y, b, dinv, \
+                           om, partials, tx, tz, cy);                                                      \
         break;
     switch (mode) {
         FANTOM_TILE_CASE(MODE_KU)
@@ -300,6 +303,20 @@ int launch_ku_tile(int mode, const Grid &g, const double *Kb, const double *E, c
 #undef FANTOM_TILE_CASE
     FANTOM_LAUNCH_CHECK("ku_tile");
     return kStatusOk;
+}
+
+int launch_ku_tile(int mode, const Grid &g, const double *Kb, const double *E, const uint8_t *mask,
+                   const double *u, double *y, const double *b, const double *dinv, double omega,
+                   double *partials, hipStream_t s)
+{
+    return launch_ku_tile_t<double>(mode, g, Kb, E, mask, u, y, b, dinv, omega, partials, s);
+}
+
+int launch_ku_tile_f32(int mode, const Grid &g, const double *Kb, const float *E, const uint8_t *mask,
+                       const float *u, float *y, const float *b, const float *dinv, double omega, double *partials,
+                       hipStream_t s)
+{
+    return launch_ku_tile_t<float>(mode, g, Kb, E, mask, u, y, b, dinv, omega, partials, s);
 }
 
 }  // namespace fantom

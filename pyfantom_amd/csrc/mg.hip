@@ -18,9 +18,10 @@ __device__ __forceinline__ double tw3(int b) { return b == 3 ? 3.375 : (b == 2 ?
 __device__ __forceinline__ double tw2(int b) { return b == 2 ? 2.25 : (b == 1 ? 3.0 : 4.0); }
 
 // One thread per coarse node.  mask_c (optional): constrained coarse DOFs get 0.
-__global__ __launch_bounds__(kBlock) void restrict3d_f64(Grid gf, int dof, int normalize,
-                                                       const uint8_t *__restrict__ mask_c,
-                                                       const double *__restrict__ rf, double *__restrict__ rc)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void restrict3d(Grid gf, int dof, int normalize,
+                                                   const uint8_t *__restrict__ mask_c, const T *__restrict__ rf,
+                                                   T *__restrict__ rc)
 {
     const int nxc = gf.nelx / 2 + 1, nyc = gf.nely / 2 + 1, nzc = gf.nelz / 2 + 1;
     const long ncn = (long)nxc * nyc * nzc;
@@ -57,13 +58,14 @@ __global__ __launch_bounds__(kBlock) void restrict3d_f64(Grid gf, int dof, int n
     for (int d = 0; d < dof; ++d) {
         // divide (not multiply by 1/TW) to round like the reference (mgm.cu:66)
         const double v = normalize ? acc[d] / tw3(bnd) : acc[d];
-        rc[ic * dof + d] = ((mc >> d) & 1) ? 0.0 : v;
+        rc[ic * dof + d] = ((mc >> d) & 1) ? T(0) : (T)v;
     }
 }
 
-__global__ __launch_bounds__(kBlock) void restrict2d_f64(Grid gf, int dof, int normalize,
-                                                       const uint8_t *__restrict__ mask_c,
-                                                       const double *__restrict__ rf, double *__restrict__ rc)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void restrict2d(Grid gf, int dof, int normalize,
+                                                   const uint8_t *__restrict__ mask_c, const T *__restrict__ rf,
+                                                   T *__restrict__ rc)
 {
     const int nxc = gf.nelx / 2 + 1, nyc = gf.nely / 2 + 1;
     const long ncn = (long)nxc * nyc;
@@ -91,17 +93,17 @@ __global__ __launch_bounds__(kBlock) void restrict2d_f64(Grid gf, int dof, int n
     const int mc = mask_c ? mask_c[ic] : 0;
     for (int d = 0; d < dof; ++d) {
         const double v = normalize ? acc[d] / tw2(bnd) : acc[d];
-        rc[ic * dof + d] = ((mc >> d) & 1) ? 0.0 : v;
+        rc[ic * dof + d] = ((mc >> d) & 1) ? T(0) : (T)v;
     }
 }
 
 // One thread per fine node: average of parent coarse values (each divided by
 // its TW when normalize).  accumulate: xf += P ec, else xf = P ec.
 // mask_f (optional): constrained fine DOFs receive 0.
-__global__ __launch_bounds__(kBlock) void prolong3d_f64(Grid gf, int dof, int normalize,
-                                                      const uint8_t *__restrict__ mask_f,
-                                                      const double *__restrict__ ec, double *__restrict__ xf,
-                                                      int accumulate)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void prolong3d(Grid gf, int dof, int normalize,
+                                                  const uint8_t *__restrict__ mask_f, const T *__restrict__ ec,
+                                                  T *__restrict__ xf, int accumulate)
 {
     const int nxc = gf.nelx / 2 + 1, nyc = gf.nely / 2 + 1, nzc = gf.nelz / 2 + 1;
     const long f = (long)blockIdx.x * kBlock + threadIdx.x;
@@ -124,7 +126,7 @@ __global__ __launch_bounds__(kBlock) void prolong3d_f64(Grid gf, int dof, int no
                 const int bnd = (ci == 0 || ci == nxc - 1) + (cj == 0 || cj == nyc - 1) + (ck == 0 || ck == nzc - 1);
                 const long c = ck + (long)nzc * ci + (long)nzc * nxc * cj;
                 for (int d = 0; d < dof; ++d)
-                    acc[d] += normalize ? ec[c * dof + d] / tw3(bnd) : ec[c * dof + d];
+                    acc[d] += normalize ? (double)ec[c * dof + d] / tw3(bnd) : (double)ec[c * dof + d];
                 count++;
             }
         }
@@ -135,14 +137,14 @@ __global__ __launch_bounds__(kBlock) void prolong3d_f64(Grid gf, int dof, int no
         if ((mf >> d) & 1)
             v = 0.0;
         const long r = f * dof + d;
-        xf[r] = accumulate ? xf[r] + v : v;
+        xf[r] = accumulate ? (T)((double)xf[r] + v) : (T)v;
     }
 }
 
-__global__ __launch_bounds__(kBlock) void prolong2d_f64(Grid gf, int dof, int normalize,
-                                                      const uint8_t *__restrict__ mask_f,
-                                                      const double *__restrict__ ec, double *__restrict__ xf,
-                                                      int accumulate)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void prolong2d(Grid gf, int dof, int normalize,
+                                                  const uint8_t *__restrict__ mask_f, const T *__restrict__ ec,
+                                                  T *__restrict__ xf, int accumulate)
 {
     const int nxc = gf.nelx / 2 + 1, nyc = gf.nely / 2 + 1;
     const long f = (long)blockIdx.x * kBlock + threadIdx.x;
@@ -160,7 +162,7 @@ __global__ __launch_bounds__(kBlock) void prolong2d_f64(Grid gf, int dof, int no
             const int bnd = (ci == 0 || ci == nxc - 1) + (cj == 0 || cj == nyc - 1);
             const long c = (long)cj * nxc + ci;
             for (int d = 0; d < dof; ++d)
-                acc[d] += normalize ? ec[c * dof + d] / tw2(bnd) : ec[c * dof + d];
+                acc[d] += normalize ? (double)ec[c * dof + d] / tw2(bnd) : (double)ec[c * dof + d];
             count++;
         }
     }
@@ -170,13 +172,13 @@ __global__ __launch_bounds__(kBlock) void prolong2d_f64(Grid gf, int dof, int no
         if ((mf >> d) & 1)
             v = 0.0;
         const long r = f * dof + d;
-        xf[r] = accumulate ? xf[r] + v : v;
+        xf[r] = accumulate ? (T)((double)xf[r] + v) : (T)v;
     }
 }
 
 // Coarse element modulus = mean of its 2^d children.
-__global__ __launch_bounds__(kBlock) void coarsen_E_f64(Grid gf, int dim, const double *__restrict__ Ef,
-                                                      double *__restrict__ Ec)
+template <typename TI, typename TO>
+__global__ __launch_bounds__(kBlock) void coarsen_E(Grid gf, int dim, const TI *__restrict__ Ef, TO *__restrict__ Ec)
 {
     const int ncx = gf.nelx / 2, ncy = gf.nely / 2, ncz = dim == 3 ? gf.nelz / 2 : 1;
     const long n = (long)ncx * ncy * ncz;
@@ -191,15 +193,15 @@ __global__ __launch_bounds__(kBlock) void coarsen_E_f64(Grid gf, int dim, const 
         for (int a = 0; a < 2; ++a)
             for (int b = 0; b < 2; ++b)
                 for (int d = 0; d < 2; ++d)
-                    s += Ef[((long)(2 * ci + a) * gf.nely + (2 * cj + b)) * gf.nelz + (2 * ck + d)];
-        Ec[c] = s * 0.125;
+                    s += (double)Ef[((long)(2 * ci + a) * gf.nely + (2 * cj + b)) * gf.nelz + (2 * ck + d)];
+        Ec[c] = (TO)(s * 0.125);
     } else {
         const int ci = (int)(c / ncy), cj = (int)(c % ncy);
         double s = 0.0;
         for (int a = 0; a < 2; ++a)
             for (int b = 0; b < 2; ++b)
-                s += Ef[(long)(2 * ci + a) * gf.nely + (2 * cj + b)];
-        Ec[c] = s * 0.25;
+                s += (double)Ef[(long)(2 * ci + a) * gf.nely + (2 * cj + b)];
+        Ec[c] = (TO)(s * 0.25);
     }
 }
 
@@ -227,9 +229,10 @@ __global__ __launch_bounds__(kBlock) void coarsen_mask_u8(Grid gf, int dim, cons
 
 // Dense K(E) with the Dirichlet convention, one thread per row.  A must be
 // zero on entry (row-major n_dof x n_dof).
-__global__ __launch_bounds__(kBlock) void dense_assemble_f64(Grid g, int dim, const double *__restrict__ K0,
-                                                           const double *__restrict__ E,
-                                                           const uint8_t *__restrict__ mask, double *__restrict__ A)
+template <typename TE>
+__global__ __launch_bounds__(kBlock) void dense_assemble(Grid g, int dim, const double *__restrict__ K0,
+                                                       const TE *__restrict__ E, const uint8_t *__restrict__ mask,
+                                                       double *__restrict__ A)
 {
     const int dof = dim == 3 ? 3 : 2, nl = dim == 3 ? 24 : 8;
     const long ndof = g.n_nodes * dof;
@@ -255,7 +258,7 @@ __global__ __launch_bounds__(kBlock) void dense_assemble_f64(Grid g, int dim, co
                     const int ei = i - ox, ej = j - oy, ek = k - oz;
                     if (ei < 0 || ei >= g.nelx || ej < 0 || ej >= g.nely || ek < 0 || ek >= g.nelz)
                         continue;
-                    const double Ee = E[((long)ei * g.nely + ej) * g.nelz + ek];
+                    const double Ee = (double)E[((long)ei * g.nely + ej) * g.nelz + ek];
                     const int a = oz * 4 + (oy ? (ox ? 2 : 3) : (ox ? 1 : 0));
                     for (int cc = 0; cc < 8; ++cc) {
                         const int cx = (cc == 1 || cc == 2 || cc == 5 || cc == 6);
@@ -277,7 +280,7 @@ __global__ __launch_bounds__(kBlock) void dense_assemble_f64(Grid g, int dim, co
                 const int ei = i - ox, ej = j - oy;
                 if (ei < 0 || ei >= g.nelx || ej < 0 || ej >= g.nely)
                     continue;
-                const double Ee = E[(long)ei * g.nely + ej];
+                const double Ee = (double)E[(long)ei * g.nely + ej];
                 const int a = oy ? (ox ? 2 : 3) : (ox ? 1 : 0);
                 for (int cc = 0; cc < 4; ++cc) {
                     const int cx = (cc == 1 || cc == 2), cy = cc >= 2;
@@ -326,8 +329,9 @@ __global__ __launch_bounds__(kBlock) void gj_update(int n, int k, double *__rest
 }
 
 // y = A x for a dense row-major n x n matrix: one wave per row.
-__global__ __launch_bounds__(kBlock) void gemv_f64(int n, const double *__restrict__ A,
-                                                 const double *__restrict__ x, double *__restrict__ y)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void gemv(int n, const double *__restrict__ A, const T *__restrict__ x,
+                                             T *__restrict__ y)
 {
     const int row = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
@@ -336,49 +340,86 @@ __global__ __launch_bounds__(kBlock) void gemv_f64(int n, const double *__restri
     const double *Ar = A + (long)row * n;
     double s = 0.0;
     for (int j = lane; j < n; j += 64)
-        s += Ar[j] * x[j];
+        s += Ar[j] * (double)x[j];
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1)
         s += __shfl_xor(s, off, 64);
     if (lane == 0)
-        y[row] = s;
+        y[row] = (T)s;
 }
 
 static unsigned nblk(long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
-int launch_restrict(const Grid &gf, int dim, int dof, int normalize, const uint8_t *mask_c, const double *rf,
-                    double *rc, hipStream_t s)
+template <typename T>
+static int restrict_t(const Grid &gf, int dim, int dof, int normalize, const uint8_t *mask_c, const T *rf, T *rc,
+                      hipStream_t s)
 {
     if (dim == 3) {
         long ncn = (long)(gf.nelx / 2 + 1) * (gf.nely / 2 + 1) * (gf.nelz / 2 + 1);
-        hipLaunchKernelGGL(restrict3d_f64, dim3(nblk(ncn)), dim3(kBlock), 0, s, gf, dof, normalize, mask_c, rf, rc);
+        hipLaunchKernelGGL(restrict3d<T>, dim3(nblk(ncn)), dim3(kBlock), 0, s, gf, dof, normalize, mask_c, rf, rc);
     } else {
         long ncn = (long)(gf.nelx / 2 + 1) * (gf.nely / 2 + 1);
-        hipLaunchKernelGGL(restrict2d_f64, dim3(nblk(ncn)), dim3(kBlock), 0, s, gf, dof, normalize, mask_c, rf, rc);
+        hipLaunchKernelGGL(restrict2d<T>, dim3(nblk(ncn)), dim3(kBlock), 0, s, gf, dof, normalize, mask_c, rf, rc);
     }
     FANTOM_LAUNCH_CHECK("restrict");
     return kStatusOk;
 }
 
-int launch_prolong(const Grid &gf, int dim, int dof, int normalize, const uint8_t *mask_f, const double *ec,
-                   double *xf, int accumulate, hipStream_t s)
+template <typename T>
+static int prolong_t(const Grid &gf, int dim, int dof, int normalize, const uint8_t *mask_f, const T *ec, T *xf,
+                     int accumulate, hipStream_t s)
 {
     if (dim == 3)
-        hipLaunchKernelGGL(prolong3d_f64, dim3(nblk(gf.n_nodes)), dim3(kBlock), 0, s, gf, dof, normalize, mask_f, ec,
+        hipLaunchKernelGGL(prolong3d<T>, dim3(nblk(gf.n_nodes)), dim3(kBlock), 0, s, gf, dof, normalize, mask_f, ec,
                            xf, accumulate);
     else
-        hipLaunchKernelGGL(prolong2d_f64, dim3(nblk(gf.n_nodes)), dim3(kBlock), 0, s, gf, dof, normalize, mask_f, ec,
+        hipLaunchKernelGGL(prolong2d<T>, dim3(nblk(gf.n_nodes)), dim3(kBlock), 0, s, gf, dof, normalize, mask_f, ec,
                            xf, accumulate);
     FANTOM_LAUNCH_CHECK("prolong");
     return kStatusOk;
 }
 
-int launch_coarsen_E(const Grid &gf, int dim, const double *Ef, double *Ec, hipStream_t s)
+int launch_restrict(const Grid &gf, int dim, int dof, int normalize, const uint8_t *mask_c, const double *rf,
+                    double *rc, hipStream_t s)
+{
+    return restrict_t<double>(gf, dim, dof, normalize, mask_c, rf, rc, s);
+}
+
+int launch_restrict_f32(const Grid &gf, int dim, int dof, int normalize, const uint8_t *mask_c, const float *rf,
+                        float *rc, hipStream_t s)
+{
+    return restrict_t<float>(gf, dim, dof, normalize, mask_c, rf, rc, s);
+}
+
+int launch_prolong(const Grid &gf, int dim, int dof, int normalize, const uint8_t *mask_f, const double *ec,
+                   double *xf, int accumulate, hipStream_t s)
+{
+    return prolong_t<double>(gf, dim, dof, normalize, mask_f, ec, xf, accumulate, s);
+}
+
+int launch_prolong_f32(const Grid &gf, int dim, int dof, int normalize, const uint8_t *mask_f, const float *ec,
+                       float *xf, int accumulate, hipStream_t s)
+{
+    return prolong_t<float>(gf, dim, dof, normalize, mask_f, ec, xf, accumulate, s);
+}
+
+template <typename TI, typename TO>
+static int coarsen_E_t(const Grid &gf, int dim, const TI *Ef, TO *Ec, hipStream_t s)
 {
     long n = (long)(gf.nelx / 2) * (gf.nely / 2) * (dim == 3 ? gf.nelz / 2 : 1);
-    hipLaunchKernelGGL(coarsen_E_f64, dim3(nblk(n)), dim3(kBlock), 0, s, gf, dim, Ef, Ec);
+    hipLaunchKernelGGL((coarsen_E<TI, TO>), dim3(nblk(n)), dim3(kBlock), 0, s, gf, dim, Ef, Ec);
     FANTOM_LAUNCH_CHECK("coarsen_E");
     return kStatusOk;
+}
+
+int launch_coarsen_E(const Grid &gf, int dim, const double *Ef, double *Ec, hipStream_t s)
+{
+    return coarsen_E_t<double, double>(gf, dim, Ef, Ec, s);
+}
+
+int launch_coarsen_E_f32(const Grid &gf, int dim, const float *Ef, float *Ec, hipStream_t s)
+{
+    return coarsen_E_t<float, float>(gf, dim, Ef, Ec, s);
 }
 
 int launch_coarsen_mask(const Grid &gf, int dim, const uint8_t *mf, uint8_t *mc, hipStream_t s)
@@ -389,8 +430,9 @@ int launch_coarsen_mask(const Grid &gf, int dim, const uint8_t *mf, uint8_t *mc,
     return kStatusOk;
 }
 
-int launch_dense_assemble(const Grid &g, int dim, const double *K0, const double *E, const uint8_t *mask,
-                          double *A, hipStream_t s)
+template <typename TE>
+static int dense_assemble_t(const Grid &g, int dim, const double *K0, const TE *E, const uint8_t *mask, double *A,
+                            hipStream_t s)
 {
     const int dof = dim == 3 ? 3 : 2;
     const long ndof = g.n_nodes * dof;
@@ -399,9 +441,21 @@ int launch_dense_assemble(const Grid &g, int dim, const double *K0, const double
         set_error("dense_assemble memset: %s", hipGetErrorString(e));
         return kStatusHipError;
     }
-    hipLaunchKernelGGL(dense_assemble_f64, dim3(nblk(ndof)), dim3(kBlock), 0, s, g, dim, K0, E, mask, A);
+    hipLaunchKernelGGL(dense_assemble<TE>, dim3(nblk(ndof)), dim3(kBlock), 0, s, g, dim, K0, E, mask, A);
     FANTOM_LAUNCH_CHECK("dense_assemble");
     return kStatusOk;
+}
+
+int launch_dense_assemble(const Grid &g, int dim, const double *K0, const double *E, const uint8_t *mask,
+                          double *A, hipStream_t s)
+{
+    return dense_assemble_t<double>(g, dim, K0, E, mask, A, s);
+}
+
+int launch_dense_assemble_f32E(const Grid &g, const double *K0, const float *E, const uint8_t *mask, double *A,
+                               hipStream_t s)
+{
+    return dense_assemble_t<float>(g, 3, K0, E, mask, A, s);
 }
 
 int launch_spd_inverse(int n, double *A, double *work, hipStream_t s)
@@ -418,7 +472,15 @@ int launch_spd_inverse(int n, double *A, double *work, hipStream_t s)
 int launch_gemv(int n, const double *A, const double *x, double *y, hipStream_t s)
 {
     const unsigned blocks = (unsigned)((n + (kBlock / 64) - 1) / (kBlock / 64));
-    hipLaunchKernelGGL(gemv_f64, dim3(blocks), dim3(kBlock), 0, s, n, A, x, y);
+    hipLaunchKernelGGL(gemv<double>, dim3(blocks), dim3(kBlock), 0, s, n, A, x, y);
+    FANTOM_LAUNCH_CHECK("gemv");
+    return kStatusOk;
+}
+
+int launch_gemv_f32(int n, const double *A, const float *x, float *y, hipStream_t s)
+{
+    const unsigned blocks = (unsigned)((n + (kBlock / 64) - 1) / (kBlock / 64));
+    hipLaunchKernelGGL(gemv<float>, dim3(blocks), dim3(kBlock), 0, s, n, A, x, y);
     FANTOM_LAUNCH_CHECK("gemv");
     return kStatusOk;
 }

@@ -1,0 +1,540 @@
+// Native GMG-PCG driver (FP64): the whole solve runs from C++ with all PCG
+// scalars on the device and each PCG iteration captured once into a HIP
+// graph, so an iteration costs one graph launch instead of ~40 kernel
+// launches from Python.
+//
+// Algorithm (reference: solvers/CUDA/_solvers.py:518-660, with the
+// deliberate differences stated in pyfantom_amd/solvers.py):
+//   r = f - K x ; z = V(r) ; p = z ; rz = r.z
+//   repeat: q = K p (fused p.q) ; alpha = rz / p.q ; x += alpha p ;
+//           r -= alpha q (fused r.r) ; stop when sqrt(r.r)/||f|| < tol ;
+//           z = V(r) ; beta = r.z_new / rz ; p = z + beta p
+// V(b) on level l: n_smooth Jacobi sweeps from zero, r_l = b - A_l x,
+// restrict, recurse, prolongate-and-add, n_smooth sweeps; the coarsest level
+// is an exact dense solve with A_c^-1 formed at setup.
+//
+// Convergence is detected on the device: the r.r finalize sets a done flag
+// and records the iteration; the x/r update kernel is a no-op once the flag
+// is set, so iterations launched past convergence leave x unchanged and the
+// result equals stopping at the first converged iteration.  The host reads
+// the flag after batches of iterations sized from the previous solve.
+//
+// Memory: the caller provides one device workspace of
+// fantom_mg_workspace_size() bytes; the handle is host memory.  Nothing is
+// allocated on the device by the library.
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <vector>
+
+#include "../../include/fantom.h"
+#include "common.hpp"
+#include "fantom_internal.hpp"
+#include "vec.hpp"
+
+namespace fantom {
+
+enum { D_DONE = 8, D_ITER = 9, D_RES = 10, D_TOL = 11, D_NB = 12, D_K = 13 };
+
+struct MgLevel {
+    Grid g;
+    int dim, dof;
+    long n_dof;
+    double scale;
+    double *x, *x2, *b, *r, *dinv, *E;
+    uint8_t *mask;
+    double K0dense[576];   // host copy of the scaled dense element matrix
+    double *K0dense_d;     // device copy (diag / dense assembly / node kernel)
+    double Kb[72];         // host block form (tile kernel)
+};
+
+struct MgHandle {
+    int dim, dof, nlev, n_smooth, kernel;
+    double omega;
+    std::vector<MgLevel> lev;
+    double *xs, *r, *z, *p, *q, *f;   // fine-level PCG vectors
+    double *partials;
+    long n_partials;
+    double *scal;                      // FANTOM_SCALARS doubles of PCG state + flags
+    double *Ainv, *gj_work;
+    int nc;
+    hipStream_t stream;
+    hipGraphExec_t iter_exec;
+    bool have_graph;
+    int last_iters;
+};
+
+// ---- small device kernels specific to the driver --------------------------------
+__global__ void mg_conv_k(const double *__restrict__ partials, long np, double *__restrict__ scal)
+{
+    // r.r finalize + convergence test on the device
+    double s = 0.0;
+    for (long i = threadIdx.x; i < np; i += 1024)
+        s += partials[i];
+    s = block_sum<1024>(s);
+    if (threadIdx.x == 0) {
+        scal[D_K] += 1.0;
+        if (scal[D_DONE] == 0.0) {
+            scal[S_RR] = s;
+            const double R = sqrt(fmax(s, 0.0)) / scal[D_NB];
+            scal[D_RES] = R;
+            scal[D_ITER] = scal[D_K];
+            if (R < scal[D_TOL])
+                scal[D_DONE] = 1.0;
+        }
+    }
+}
+
+__global__ void mg_xr_k(long n, double *__restrict__ x, double *__restrict__ r, const double *__restrict__ p,
+                        const double *__restrict__ q, const double *__restrict__ scal, double *__restrict__ partials)
+{
+    const bool frozen = scal[D_DONE] != 0.0;
+    const double alpha = scal[S_ALPHA];
+    double s = 0.0;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        double ri = r[i];
+        if (!frozen) {
+            x[i] += alpha * p[i];
+            ri -= alpha * q[i];
+            r[i] = ri;
+        }
+        s += ri * ri;
+    }
+    s = block_sum<256>(s);
+    if (threadIdx.x == 0)
+        partials[blockIdx.x] = s;
+}
+
+__global__ void mg_init_scal_k(double *__restrict__ scal, double tol)
+{
+    if (threadIdx.x == 0) {
+        scal[D_DONE] = 0.0;
+        scal[D_ITER] = 0.0;
+        scal[D_RES] = 0.0;
+        scal[D_TOL] = tol;
+        scal[D_K] = 0.0;
+    }
+}
+
+__global__ void mg_norm_k(const double *__restrict__ partials, long np, double *__restrict__ scal)
+{
+    double s = 0.0;
+    for (long i = threadIdx.x; i < np; i += 1024)
+        s += partials[i];
+    s = block_sum<1024>(s);
+    if (threadIdx.x == 0) {
+        const double nb = sqrt(fmax(s, 0.0));
+        scal[D_NB] = nb > 0.0 ? nb : 1.0;
+    }
+}
+
+__global__ void mg_copy_k(long n, const double *__restrict__ a, double *__restrict__ b)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        b[i] = a[i];
+}
+
+// ---- helpers ----------------------------------------------------------------------
+static int lev_ku(MgHandle *h, int l, int mode, const double *u, double *y, const double *b, const double *dinv,
+                  double omega, double *partials)
+{
+    MgLevel &L = h->lev[l];
+    if (h->kernel == FANTOM_KU_TILE && L.dim == 3)
+        return launch_ku_tile(mode, L.g, L.Kb, L.E, L.mask, u, y, b, dinv, omega, partials, h->stream);
+    return launch_ku_node(mode, L.g, L.dim, L.K0dense_d, L.E, L.mask, u, y, b, dinv, omega, partials, h->stream);
+}
+
+#define MG_TRY(expr)                                                                                       \
+    do {                                                                                                   \
+        int _s = (expr);                                                                                   \
+        if (_s)                                                                                            \
+            return _s;                                                                                     \
+    } while (0)
+
+// n Jacobi sweeps; *px holds the input (ignored when from_zero) and gets the output
+static int mg_smooth(MgHandle *h, int l, const double *b, double **px, bool from_zero)
+{
+    MgLevel &L = h->lev[l];
+    double *x = *px;
+    double *other = (x == L.x) ? L.x2 : L.x;
+    for (int s = 0; s < h->n_smooth; ++s) {
+        if (s == 0 && from_zero) {
+            MG_TRY(launch_scale(L.n_dof, h->omega, L.dinv, b, x, h->stream));
+            continue;
+        }
+        MG_TRY(lev_ku(h, l, MODE_JAC, x, other, b, L.dinv, h->omega, nullptr));
+        double *t = x;
+        x = other;
+        other = t;
+    }
+    *px = x;
+    return kStatusOk;
+}
+
+static int mg_cycle(MgHandle *h, int l, const double *b, double **out)
+{
+    MgLevel &L = h->lev[l];
+    if (l == h->nlev - 1) {
+        MG_TRY(launch_gemv(h->nc, h->Ainv, b, L.x, h->stream));
+        *out = L.x;
+        return kStatusOk;
+    }
+    double *x = L.x;
+    MG_TRY(mg_smooth(h, l, b, &x, true));
+    MgLevel &C = h->lev[l + 1];
+    MG_TRY(lev_ku(h, l, MODE_RES, x, L.r, b, nullptr, 0.0, nullptr));
+    MG_TRY(launch_restrict(L.g, L.dim, L.dof, 0, C.mask, L.r, C.b, h->stream));
+    double *e = nullptr;
+    MG_TRY(mg_cycle(h, l + 1, C.b, &e));
+    MG_TRY(launch_prolong(L.g, L.dim, L.dof, 0, L.mask, e, x, 1, h->stream));
+    MG_TRY(mg_smooth(h, l, b, &x, false));
+    *out = x;
+    return kStatusOk;
+}
+
+// z = V(r)
+static int mg_precond(MgHandle *h, const double *r, double *z)
+{
+    double *out = nullptr;
+    MG_TRY(mg_cycle(h, 0, r, &out));
+    const long n = h->lev[0].n_dof;
+    hipLaunchKernelGGL(mg_copy_k, dim3(vec_grid(n)), dim3(256), 0, h->stream, n, out, z);
+    FANTOM_LAUNCH_CHECK("mg copy");
+    return kStatusOk;
+}
+
+static int mg_iteration(MgHandle *h)
+{
+    const long n = h->lev[0].n_dof;
+    const long npk = h->kernel == FANTOM_KU_TILE && h->dim == 3 ? ku_tile_partials(h->lev[0].g)
+                                                                : ku_node_partials(h->lev[0].g);
+    MG_TRY(lev_ku(h, 0, MODE_KUDOT, h->p, h->q, nullptr, nullptr, 0.0, h->partials));
+    MG_TRY(launch_finalize(h->partials, npk, h->scal, FIN_ALPHA, 0, h->stream));
+    hipLaunchKernelGGL(mg_xr_k, dim3(vec_grid(n)), dim3(256), 0, h->stream, n, h->xs, h->r, h->p, h->q, h->scal,
+                       h->partials);
+    hipLaunchKernelGGL(mg_conv_k, dim3(1), dim3(1024), 0, h->stream, h->partials, (long)vec_grid(n), h->scal);
+    MG_TRY(mg_precond(h, h->r, h->z));
+    MG_TRY(launch_dot_partials(n, h->r, h->z, h->partials, h->stream));
+    MG_TRY(launch_finalize(h->partials, (long)vec_grid(n), h->scal, FIN_BETA, 0, h->stream));
+    MG_TRY(launch_pcg_p(n, h->p, h->z, h->scal, 0, h->stream));
+    FANTOM_LAUNCH_CHECK("mg iteration");
+    return kStatusOk;
+}
+
+// ---- workspace layout ----------------------------------------------------------------
+struct Carve {
+    char *base;
+    size_t off;
+    template <class T>
+    T *take(size_t count)
+    {
+        off = (off + 255) & ~(size_t)255;
+        T *p = base ? reinterpret_cast<T *>(base + off) : nullptr;
+        off += count * sizeof(T);
+        return p;
+    }
+};
+
+static void layout(MgHandle *h, int nelx, int nely, int nelz, int n_coarse, Carve &cv)
+{
+    h->dim = nelz > 0 ? 3 : 2;
+    h->dof = h->dim;
+    h->nlev = n_coarse + 1;
+    h->lev.resize(h->nlev);
+    for (int l = 0; l < h->nlev; ++l) {
+        MgLevel &L = h->lev[l];
+        L.g = make_grid(nelx >> l, nely >> l, nelz > 0 ? (nelz >> l) : 0);
+        L.dim = h->dim;
+        L.dof = h->dof;
+        L.n_dof = L.g.n_nodes * L.dof;
+        L.scale = h->dim == 3 ? (double)(1L << l) : 1.0;
+        L.x = cv.take<double>(L.n_dof);
+        L.x2 = cv.take<double>(L.n_dof);
+        L.b = cv.take<double>(L.n_dof);
+        L.r = cv.take<double>(L.n_dof);
+        L.dinv = cv.take<double>(L.n_dof);
+        L.E = cv.take<double>(L.g.n_el);
+        L.mask = cv.take<uint8_t>(L.g.n_nodes);
+        L.K0dense_d = cv.take<double>(576);
+    }
+    const long n = h->lev[0].n_dof;
+    h->xs = cv.take<double>(n);
+    h->r = cv.take<double>(n);
+    h->z = cv.take<double>(n);
+    h->p = cv.take<double>(n);
+    h->q = cv.take<double>(n);
+    h->f = cv.take<double>(n);
+    long np = kVecMaxBlocks;
+    np = std::max(np, ku_node_partials(h->lev[0].g));
+    if (h->dim == 3)
+        np = std::max(np, ku_tile_partials(h->lev[0].g));
+    h->n_partials = np;
+    h->partials = cv.take<double>(np);
+    h->scal = cv.take<double>(FANTOM_SCALARS);
+    h->nc = (int)h->lev[h->nlev - 1].n_dof;
+    h->Ainv = cv.take<double>((size_t)h->nc * h->nc);
+    h->gj_work = cv.take<double>(2 * (size_t)h->nc);
+}
+
+static bool levels_ok(int nelx, int nely, int nelz, int n_coarse)
+{
+    const int m = 1 << n_coarse;
+    if (n_coarse < 0 || n_coarse > 10)
+        return false;
+    if (nelx % m || nely % m || (nelz > 0 && nelz % m))
+        return false;
+    return true;
+}
+
+}  // namespace fantom
+
+using namespace fantom;
+
+extern "C" {
+
+long fantom_mg_workspace_size(int nelx, int nely, int nelz, int n_coarse)
+{
+    if (!grid_ok(nelx, nely, nelz) || !levels_ok(nelx, nely, nelz, n_coarse))
+        return -1;
+    MgHandle h;
+    Carve cv{nullptr, 0};
+    layout(&h, nelx, nely, nelz, n_coarse, cv);
+    return (long)cv.off + 256;
+}
+
+int fantom_mg_create(int nelx, int nely, int nelz, int n_coarse, int n_smooth, double omega, int kernel,
+                     const double *K0_host, void *workspace, long workspace_bytes, void **handle)
+{
+    FANTOM_CHECK_ARG(grid_ok(nelx, nely, nelz), "invalid grid %d x %d x %d", nelx, nely, nelz);
+    FANTOM_CHECK_ARG(levels_ok(nelx, nely, nelz, n_coarse), "grid %d x %d x %d does not halve %d times", nelx, nely,
+                     nelz, n_coarse);
+    FANTOM_CHECK_ARG(K0_host && workspace && handle, "null operand");
+    FANTOM_CHECK_ARG(n_smooth >= 1 && n_smooth <= 16, "n_smooth must be in [1, 16]");
+    FANTOM_CHECK_ARG(kernel == FANTOM_KU_NODE || kernel == FANTOM_KU_TILE, "unknown kernel %d", kernel);
+    const long need = fantom_mg_workspace_size(nelx, nely, nelz, n_coarse);
+    FANTOM_CHECK_ARG(workspace_bytes >= need, "workspace of %ld bytes is smaller than the %ld needed",
+                     workspace_bytes, need);
+    MgHandle *h = new MgHandle();
+    Carve cv{reinterpret_cast<char *>(workspace), 0};
+    // align the base
+    cv.off = (256 - (reinterpret_cast<uintptr_t>(workspace) & 255)) & 255;
+    layout(h, nelx, nely, nelz, n_coarse, cv);
+    FANTOM_CHECK_ARG(h->nc <= 20000, "coarsest grid of %d DOF is too large for the dense solve", h->nc);
+    h->n_smooth = n_smooth;
+    h->omega = omega;
+    h->kernel = (h->dim == 3) ? kernel : FANTOM_KU_NODE;
+    h->have_graph = false;
+    h->last_iters = 0;
+    const int nl = h->dim == 3 ? 24 : 8;
+    for (int l = 0; l < h->nlev; ++l) {
+        MgLevel &L = h->lev[l];
+        for (int i = 0; i < nl * nl; ++i)
+            L.K0dense[i] = K0_host[i] * L.scale;
+        if (h->dim == 3) {
+            int st = fantom_hex8_blocks(L.K0dense, L.Kb, nullptr);
+            if (st && h->kernel == FANTOM_KU_TILE) {
+                delete h;
+                return st;
+            }
+        }
+    }
+    hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete h;
+        set_error("stream create: %s", hipGetErrorString(e));
+        return kStatusHipError;
+    }
+    for (int l = 0; l < h->nlev; ++l) {
+        MgLevel &L = h->lev[l];
+        e = hipMemcpyAsync(L.K0dense_d, L.K0dense, sizeof(double) * nl * nl, hipMemcpyHostToDevice, h->stream);
+        if (e != hipSuccess)
+            break;
+    }
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) {
+        hipStreamDestroy(h->stream);
+        delete h;
+        set_error("K0 upload: %s", hipGetErrorString(e));
+        return kStatusHipError;
+    }
+    *handle = h;
+    return kStatusOk;
+}
+
+int fantom_mg_destroy(void *handle)
+{
+    MgHandle *h = reinterpret_cast<MgHandle *>(handle);
+    if (!h)
+        return kStatusOk;
+    hipStreamSynchronize(h->stream);
+    if (h->have_graph)
+        hipGraphExecDestroy(h->iter_exec);
+    hipStreamDestroy(h->stream);
+    delete h;
+    return kStatusOk;
+}
+
+// Make the handle's stream wait for the caller's stream (and vice versa at the end).
+static int join(hipStream_t from, hipStream_t to)
+{
+    hipEvent_t ev;
+    hipError_t e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+    if (e == hipSuccess)
+        e = hipEventRecord(ev, from);
+    if (e == hipSuccess)
+        e = hipStreamWaitEvent(to, ev, 0);
+    hipEventDestroy(ev);
+    if (e != hipSuccess) {
+        set_error("stream join: %s", hipGetErrorString(e));
+        return kStatusHipError;
+    }
+    return kStatusOk;
+}
+
+int fantom_mg_setup(void *handle, const double *E, const uint8_t *mask, void *stream)
+{
+    MgHandle *h = reinterpret_cast<MgHandle *>(handle);
+    FANTOM_CHECK_ARG(h && E, "null operand");
+    MG_TRY(join(as_stream(stream), h->stream));
+    MgLevel &L0 = h->lev[0];
+    hipError_t e = hipMemcpyAsync(L0.E, E, sizeof(double) * L0.g.n_el, hipMemcpyDeviceToDevice, h->stream);
+    if (e == hipSuccess) {
+        if (mask)
+            e = hipMemcpyAsync(L0.mask, mask, L0.g.n_nodes, hipMemcpyDeviceToDevice, h->stream);
+        else
+            e = hipMemsetAsync(L0.mask, 0, L0.g.n_nodes, h->stream);
+    }
+    if (e != hipSuccess) {
+        set_error("setup copy: %s", hipGetErrorString(e));
+        return kStatusHipError;
+    }
+    for (int l = 1; l < h->nlev; ++l) {
+        MgLevel &F = h->lev[l - 1], &C = h->lev[l];
+        MG_TRY(launch_coarsen_E(F.g, h->dim, F.E, C.E, h->stream));
+        MG_TRY(launch_coarsen_mask(F.g, h->dim, F.mask, C.mask, h->stream));
+    }
+    for (int l = 0; l < h->nlev; ++l) {
+        MgLevel &L = h->lev[l];
+        MG_TRY(launch_diag(L.g, h->dim, L.K0dense_d, L.E, L.mask, L.dinv, 1, h->stream));
+    }
+    MgLevel &C = h->lev[h->nlev - 1];
+    MG_TRY(launch_dense_assemble(C.g, h->dim, C.K0dense_d, C.E, C.mask, h->Ainv, h->stream));
+    MG_TRY(launch_spd_inverse(h->nc, h->Ainv, h->gj_work, h->stream));
+    return join(h->stream, as_stream(stream));
+}
+
+int fantom_mg_precondition(void *handle, const double *r, double *z, void *stream)
+{
+    MgHandle *h = reinterpret_cast<MgHandle *>(handle);
+    FANTOM_CHECK_ARG(h && r && z, "null operand");
+    MG_TRY(join(as_stream(stream), h->stream));
+    MG_TRY(mg_precond(h, r, z));
+    return join(h->stream, as_stream(stream));
+}
+
+static int capture_iteration(MgHandle *h)
+{
+    hipGraph_t graph;
+    hipError_t e = hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+        set_error("begin capture: %s", hipGetErrorString(e));
+        return kStatusHipError;
+    }
+    int st = mg_iteration(h);
+    e = hipStreamEndCapture(h->stream, &graph);
+    if (st)
+        return st;
+    if (e != hipSuccess) {
+        set_error("end capture: %s", hipGetErrorString(e));
+        return kStatusHipError;
+    }
+    e = hipGraphInstantiate(&h->iter_exec, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+        set_error("graph instantiate: %s", hipGetErrorString(e));
+        return kStatusHipError;
+    }
+    h->have_graph = true;
+    return kStatusOk;
+}
+
+int fantom_mg_solve(void *handle, const double *f, double *x, double tol, int maxiter, int use_x0,
+                    int use_graph, int *iters, double *rel_res, void *stream)
+{
+    MgHandle *h = reinterpret_cast<MgHandle *>(handle);
+    FANTOM_CHECK_ARG(h && f && x && iters && rel_res, "null operand");
+    FANTOM_CHECK_ARG(maxiter >= 1, "maxiter must be positive");
+    const long n = h->lev[0].n_dof;
+    MG_TRY(join(as_stream(stream), h->stream));
+    hipStream_t s = h->stream;
+    const unsigned G = vec_grid(n);
+    // x0, ||f||, r = f - K x0, z = V(r), p = z, rz = r.z
+    hipLaunchKernelGGL(mg_copy_k, dim3(G), dim3(256), 0, s, n, f, h->f);
+    if (use_x0)
+        hipLaunchKernelGGL(mg_copy_k, dim3(G), dim3(256), 0, s, n, x, h->xs);
+    else
+        hipMemsetAsync(h->xs, 0, sizeof(double) * n, s);
+    hipLaunchKernelGGL(mg_init_scal_k, dim3(1), dim3(64), 0, s, h->scal, tol);
+    MG_TRY(launch_dot_partials(n, h->f, h->f, h->partials, s));
+    hipLaunchKernelGGL(mg_norm_k, dim3(1), dim3(1024), 0, s, h->partials, (long)G, h->scal);
+    MG_TRY(lev_ku(h, 0, MODE_RES, h->xs, h->r, h->f, nullptr, 0.0, nullptr));
+    MG_TRY(mg_precond(h, h->r, h->z));
+    MG_TRY(launch_dot_partials(n, h->r, h->z, h->partials, s));
+    MG_TRY(launch_finalize(h->partials, (long)G, h->scal, FIN_RZ0, 0, s));
+    MG_TRY(launch_pcg_p(n, h->p, h->z, h->scal, 1, s));
+    FANTOM_LAUNCH_CHECK("mg solve init");
+    if (use_graph && !h->have_graph)
+        MG_TRY(capture_iteration(h));
+    double hs[FANTOM_SCALARS];
+    int done_iters = 0;
+    // first batch: close to the previous solve's count, then single steps
+    int batch = h->last_iters > 3 ? h->last_iters - 2 : 1;
+    bool converged = false;
+    while (done_iters < maxiter) {
+        if (batch > maxiter - done_iters)
+            batch = maxiter - done_iters;
+        for (int b = 0; b < batch; ++b) {
+            if (use_graph) {
+                hipError_t e = hipGraphLaunch(h->iter_exec, s);
+                if (e != hipSuccess) {
+                    set_error("graph launch: %s", hipGetErrorString(e));
+                    return kStatusHipError;
+                }
+            } else {
+                MG_TRY(mg_iteration(h));
+            }
+        }
+        done_iters += batch;
+        hipError_t e = hipMemcpyAsync(hs, h->scal, sizeof(hs), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess)
+            e = hipStreamSynchronize(s);
+        if (e != hipSuccess) {
+            set_error("convergence read: %s", hipGetErrorString(e));
+            return kStatusHipError;
+        }
+        if (hs[D_DONE] != 0.0) {
+            converged = true;
+            break;
+        }
+        batch = 1;
+    }
+    *iters = converged ? (int)hs[D_ITER] : done_iters;
+    h->last_iters = *iters;
+    // true residual ||f - K x|| / ||f||
+    MG_TRY(lev_ku(h, 0, MODE_RES, h->xs, h->q, h->f, nullptr, 0.0, nullptr));
+    MG_TRY(launch_dot_partials(n, h->q, h->q, h->partials, s));
+    MG_TRY(launch_finalize(h->partials, (long)G, h->scal, FIN_STORE, S_TMP, s));
+    hipLaunchKernelGGL(mg_copy_k, dim3(G), dim3(256), 0, s, n, h->xs, x);
+    hipError_t e = hipMemcpyAsync(hs, h->scal, sizeof(hs), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+        set_error("final read: %s", hipGetErrorString(e));
+        return kStatusHipError;
+    }
+    *rel_res = sqrt(fmax(hs[S_TMP], 0.0)) / hs[D_NB];
+    return join(h->stream, as_stream(stream));
+}
+
+}  // extern "C"

@@ -1,0 +1,573 @@
+// Mixed-precision solve: FP64 iterative refinement around an FP32 GMG-PCG.
+//
+//   x = x0 (FP64)
+//   loop: r = f - K x            FP64 operator, FP64 true residual
+//         stop if ||r||/||f|| < tol
+//         s = ||r|| ; r32 = r / s
+//         K d = r32 approximately: PCG in FP32 storage (vectors, moduli,
+//           Walsh-Hadamard element contraction with FP32 accumulation, FP32
+//           Jacobi/transfers) with FP64 dot products and step lengths, and the
+//           coarsest solve with the FP64 inverse (FP64 accumulation)
+//         x += s d
+//
+// The inner PCG is the FP64 driver's algorithm on FP32 data (see mgpcg.hip):
+// scalars on the device, each iteration one HIP graph, device-side stop at
+// the inner tolerance with the update frozen afterwards.  The inner solve
+// always starts from zero, so one graph serves every refinement step.
+//
+// FP64 traffic per inner iteration is replaced by FP32 traffic: the fine K.u
+// moves 8 B/DOF + 4 B/element instead of 16 B/DOF + 8 B/element, and every
+// vector kernel half the bytes.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "../../include/fantom.h"
+#include "common.hpp"
+#include "fantom_internal.hpp"
+#include "vec.hpp"
+
+namespace fantom {
+
+enum { X_DONE = 8, X_ITER = 9, X_RES = 10, X_TOL = 11, X_NB = 12, X_K = 13 };
+
+struct MxLevel {
+    Grid g;
+    long n_dof;
+    double scale;
+    float *x, *x2, *b, *r, *dinv, *E;
+    uint8_t *mask;
+    double K0dense[576];
+    double *K0dense_d;
+    double Kb[72];
+};
+
+struct MxHandle {
+    int nlev, n_smooth;
+    double omega;
+    std::vector<MxLevel> lev;
+    // FP64 fine-level data
+    Grid g;
+    long n;
+    double *E64, *x64, *r64, *f64;
+    uint8_t *mask;
+    double Kb64[72];
+    // FP32 inner PCG vectors
+    float *d, *r, *z, *p, *q;
+    double *partials, *scal, *Ainv, *gj_work;
+    int nc;
+    hipStream_t stream;
+    hipGraphExec_t iter_exec;
+    bool have_graph;
+    int last_inner;
+};
+
+__global__ void mx_copy_f32(long n, const float *__restrict__ a, float *__restrict__ b)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        b[i] = a[i];
+}
+
+__global__ void mx_copy_f64(long n, const double *__restrict__ a, double *__restrict__ b)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        b[i] = a[i];
+}
+
+// r32 = r64 / ||r64|| (scal[S_TMP] holds ||r64||^2)
+__global__ void mx_down_k(long n, const double *__restrict__ r64, float *__restrict__ r32,
+                          const double *__restrict__ scal)
+{
+    const double inv = 1.0 / sqrt(scal[S_TMP]);
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        r32[i] = (float)(r64[i] * inv);
+}
+
+// x64 += ||r64|| * d32
+__global__ void mx_up_k(long n, const float *__restrict__ d, double *__restrict__ x, const double *__restrict__ scal)
+{
+    const double s = sqrt(scal[S_TMP]);
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        x[i] += s * (double)d[i];
+}
+
+__global__ void mx_scale_k(long n, float omega, const float *__restrict__ dinv, const float *__restrict__ b,
+                           float *__restrict__ x)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        x[i] = omega * dinv[i] * b[i];
+}
+
+__global__ __launch_bounds__(256) void mx_dot_k(long n, const float *__restrict__ a, const float *__restrict__ b,
+                                               double *__restrict__ partials)
+{
+    double s = 0.0;
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256)
+        s += (double)a[i] * (double)b[i];
+    s = block_sum<256>(s);
+    if (threadIdx.x == 0)
+        partials[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(256) void mx_xr_k(long n, float *__restrict__ x, float *__restrict__ r,
+                                              const float *__restrict__ p, const float *__restrict__ q,
+                                              const double *__restrict__ scal, double *__restrict__ partials)
+{
+    const bool frozen = scal[X_DONE] != 0.0;
+    const double alpha = scal[S_ALPHA];
+    double s = 0.0;
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+        float ri = r[i];
+        if (!frozen) {
+            x[i] = (float)((double)x[i] + alpha * (double)p[i]);
+            ri = (float)((double)ri - alpha * (double)q[i]);
+            r[i] = ri;
+        }
+        s += (double)ri * (double)ri;
+    }
+    s = block_sum<256>(s);
+    if (threadIdx.x == 0)
+        partials[blockIdx.x] = s;
+}
+
+__global__ void mx_p_k(long n, float *__restrict__ p, const float *__restrict__ z, const double *__restrict__ scal,
+                       int first)
+{
+    const double beta = first ? 0.0 : scal[S_BETA];
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        p[i] = first ? z[i] : (float)((double)z[i] + beta * (double)p[i]);
+}
+
+__global__ void mx_conv_k(const double *__restrict__ partials, long np, double *__restrict__ scal)
+{
+    double s = 0.0;
+    for (long i = threadIdx.x; i < np; i += 1024)
+        s += partials[i];
+    s = block_sum<1024>(s);
+    if (threadIdx.x == 0) {
+        scal[X_K] += 1.0;
+        if (scal[X_DONE] == 0.0) {
+            // the inner right-hand side has unit norm
+            const double R = sqrt(fmax(s, 0.0));
+            scal[X_RES] = R;
+            scal[X_ITER] = scal[X_K];
+            if (R < scal[X_TOL])
+                scal[X_DONE] = 1.0;
+        }
+    }
+}
+
+__global__ void mx_init_k(double *__restrict__ scal, double tol)
+{
+    if (threadIdx.x == 0) {
+        scal[X_DONE] = 0.0;
+        scal[X_ITER] = 0.0;
+        scal[X_RES] = 0.0;
+        scal[X_TOL] = tol;
+        scal[X_K] = 0.0;
+    }
+}
+
+__global__ void mx_zero_f32(long n, float *__restrict__ a)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        a[i] = 0.0f;
+}
+
+#define MX_TRY(expr)                                                                                       \
+    do {                                                                                                   \
+        int _s = (expr);                                                                                   \
+        if (_s)                                                                                            \
+            return _s;                                                                                     \
+    } while (0)
+
+static int mx_smooth(MxHandle *h, int l, const float *b, float **px, bool from_zero)
+{
+    MxLevel &L = h->lev[l];
+    float *x = *px;
+    float *other = (x == L.x) ? L.x2 : L.x;
+    for (int s = 0; s < h->n_smooth; ++s) {
+        if (s == 0 && from_zero) {
+            hipLaunchKernelGGL(mx_scale_k, dim3(vec_grid(L.n_dof)), dim3(256), 0, h->stream, L.n_dof,
+                               (float)h->omega, L.dinv, b, x);
+            continue;
+        }
+        MX_TRY(launch_ku_tile_f32(MODE_JAC, L.g, L.Kb, L.E, L.mask, x, other, b, L.dinv, h->omega, nullptr,
+                                  h->stream));
+        float *t = x;
+        x = other;
+        other = t;
+    }
+    *px = x;
+    FANTOM_LAUNCH_CHECK("mx smooth");
+    return kStatusOk;
+}
+
+static int mx_cycle(MxHandle *h, int l, const float *b, float **out)
+{
+    MxLevel &L = h->lev[l];
+    if (l == h->nlev - 1) {
+        MX_TRY(launch_gemv_f32(h->nc, h->Ainv, b, L.x, h->stream));
+        *out = L.x;
+        return kStatusOk;
+    }
+    float *x = L.x;
+    MX_TRY(mx_smooth(h, l, b, &x, true));
+    MxLevel &C = h->lev[l + 1];
+    MX_TRY(launch_ku_tile_f32(MODE_RES, L.g, L.Kb, L.E, L.mask, x, L.r, b, nullptr, 0.0, nullptr, h->stream));
+    MX_TRY(launch_restrict_f32(L.g, 3, 3, 0, C.mask, L.r, C.b, h->stream));
+    float *e = nullptr;
+    MX_TRY(mx_cycle(h, l + 1, C.b, &e));
+    MX_TRY(launch_prolong_f32(L.g, 3, 3, 0, L.mask, e, x, 1, h->stream));
+    MX_TRY(mx_smooth(h, l, b, &x, false));
+    *out = x;
+    return kStatusOk;
+}
+
+static int mx_precond(MxHandle *h, const float *r, float *z)
+{
+    float *out = nullptr;
+    MX_TRY(mx_cycle(h, 0, r, &out));
+    hipLaunchKernelGGL(mx_copy_f32, dim3(vec_grid(h->n)), dim3(256), 0, h->stream, h->n, out, z);
+    FANTOM_LAUNCH_CHECK("mx precond");
+    return kStatusOk;
+}
+
+static int mx_iteration(MxHandle *h)
+{
+    const long n = h->n;
+    const unsigned G = vec_grid(n);
+    MxLevel &L0 = h->lev[0];
+    MX_TRY(launch_ku_tile_f32(MODE_KUDOT, L0.g, L0.Kb, L0.E, L0.mask, h->p, h->q, nullptr, nullptr, 0.0,
+                              h->partials, h->stream));
+    MX_TRY(launch_finalize(h->partials, ku_tile_partials(L0.g), h->scal, FIN_ALPHA, 0, h->stream));
+    hipLaunchKernelGGL(mx_xr_k, dim3(G), dim3(256), 0, h->stream, n, h->d, h->r, h->p, h->q, h->scal, h->partials);
+    hipLaunchKernelGGL(mx_conv_k, dim3(1), dim3(1024), 0, h->stream, h->partials, (long)G, h->scal);
+    MX_TRY(mx_precond(h, h->r, h->z));
+    hipLaunchKernelGGL(mx_dot_k, dim3(G), dim3(256), 0, h->stream, n, h->r, h->z, h->partials);
+    MX_TRY(launch_finalize(h->partials, (long)G, h->scal, FIN_BETA, 0, h->stream));
+    hipLaunchKernelGGL(mx_p_k, dim3(G), dim3(256), 0, h->stream, n, h->p, h->z, h->scal, 0);
+    FANTOM_LAUNCH_CHECK("mx iteration");
+    return kStatusOk;
+}
+
+struct MxCarve {
+    char *base;
+    size_t off;
+    template <class T>
+    T *take(size_t count)
+    {
+        off = (off + 255) & ~(size_t)255;
+        T *p = base ? reinterpret_cast<T *>(base + off) : nullptr;
+        off += count * sizeof(T);
+        return p;
+    }
+};
+
+static void mx_layout(MxHandle *h, int nelx, int nely, int nelz, int n_coarse, MxCarve &cv)
+{
+    h->nlev = n_coarse + 1;
+    h->lev.resize(h->nlev);
+    h->g = make_grid(nelx, nely, nelz);
+    h->n = h->g.n_nodes * 3;
+    for (int l = 0; l < h->nlev; ++l) {
+        MxLevel &L = h->lev[l];
+        L.g = make_grid(nelx >> l, nely >> l, nelz >> l);
+        L.n_dof = L.g.n_nodes * 3;
+        L.scale = (double)(1L << l);
+        L.x = cv.take<float>(L.n_dof);
+        L.x2 = cv.take<float>(L.n_dof);
+        L.b = cv.take<float>(L.n_dof);
+        L.r = cv.take<float>(L.n_dof);
+        L.dinv = cv.take<float>(L.n_dof);
+        L.E = cv.take<float>(L.g.n_el);
+        L.mask = cv.take<uint8_t>(L.g.n_nodes);
+        L.K0dense_d = cv.take<double>(576);
+    }
+    h->E64 = cv.take<double>(h->g.n_el);
+    h->x64 = cv.take<double>(h->n);
+    h->r64 = cv.take<double>(h->n);
+    h->f64 = cv.take<double>(h->n);
+    h->mask = h->lev[0].mask;
+    h->d = cv.take<float>(h->n);
+    h->r = cv.take<float>(h->n);
+    h->z = cv.take<float>(h->n);
+    h->p = cv.take<float>(h->n);
+    h->q = cv.take<float>(h->n);
+    long np = std::max((long)kVecMaxBlocks, ku_tile_partials(h->g));
+    h->partials = cv.take<double>(np);
+    h->scal = cv.take<double>(FANTOM_SCALARS);
+    h->nc = (int)h->lev[h->nlev - 1].n_dof;
+    h->Ainv = cv.take<double>((size_t)h->nc * h->nc);
+    h->gj_work = cv.take<double>(2 * (size_t)h->nc);
+}
+
+static int mx_join(hipStream_t from, hipStream_t to)
+{
+    hipEvent_t ev;
+    hipError_t e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+    if (e == hipSuccess)
+        e = hipEventRecord(ev, from);
+    if (e == hipSuccess)
+        e = hipStreamWaitEvent(to, ev, 0);
+    hipEventDestroy(ev);
+    if (e != hipSuccess) {
+        set_error("stream join: %s", hipGetErrorString(e));
+        return kStatusHipError;
+    }
+    return kStatusOk;
+}
+
+static int mx_read(MxHandle *h, double *hs)
+{
+    hipError_t e = hipMemcpyAsync(hs, h->scal, sizeof(double) * FANTOM_SCALARS, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) {
+        set_error("scalar read: %s", hipGetErrorString(e));
+        return kStatusHipError;
+    }
+    return kStatusOk;
+}
+
+}  // namespace fantom
+
+using namespace fantom;
+
+extern "C" {
+
+long fantom_mx_workspace_size(int nelx, int nely, int nelz, int n_coarse)
+{
+    if (!grid_ok(nelx, nely, nelz) || nelz <= 0 || n_coarse < 0 || n_coarse > 10)
+        return -1;
+    const int m = 1 << n_coarse;
+    if (nelx % m || nely % m || nelz % m)
+        return -1;
+    MxHandle h;
+    MxCarve cv{nullptr, 0};
+    mx_layout(&h, nelx, nely, nelz, n_coarse, cv);
+    return (long)cv.off + 256;
+}
+
+int fantom_mx_create(int nelx, int nely, int nelz, int n_coarse, int n_smooth, double omega, const double *K0_host,
+                     void *workspace, long workspace_bytes, void **handle)
+{
+    const long need = fantom_mx_workspace_size(nelx, nely, nelz, n_coarse);
+    FANTOM_CHECK_ARG(need > 0, "grid %d x %d x %d (3D only) does not halve %d times", nelx, nely, nelz, n_coarse);
+    FANTOM_CHECK_ARG(K0_host && workspace && handle, "null operand");
+    FANTOM_CHECK_ARG(workspace_bytes >= need, "workspace of %ld bytes is smaller than the %ld needed",
+                     workspace_bytes, need);
+    FANTOM_CHECK_ARG(n_smooth >= 1 && n_smooth <= 16, "n_smooth must be in [1, 16]");
+    MxHandle *h = new MxHandle();
+    MxCarve cv{reinterpret_cast<char *>(workspace), 0};
+    cv.off = (256 - (reinterpret_cast<uintptr_t>(workspace) & 255)) & 255;
+    mx_layout(h, nelx, nely, nelz, n_coarse, cv);
+    if (h->nc > 20000) {
+        delete h;
+        set_error("coarsest grid too large for the dense solve");
+        return kStatusBadArg;
+    }
+    h->n_smooth = n_smooth;
+    h->omega = omega;
+    h->have_graph = false;
+    h->last_inner = 0;
+    int st = fantom_hex8_blocks(K0_host, h->Kb64, nullptr);
+    if (st) {
+        delete h;
+        return st;
+    }
+    for (int l = 0; l < h->nlev; ++l) {
+        MxLevel &L = h->lev[l];
+        for (int i = 0; i < 576; ++i)
+            L.K0dense[i] = K0_host[i] * L.scale;
+        for (int i = 0; i < 72; ++i)
+            L.Kb[i] = h->Kb64[i] * L.scale;
+    }
+    hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+    for (int l = 0; l < h->nlev && e == hipSuccess; ++l)
+        e = hipMemcpyAsync(h->lev[l].K0dense_d, h->lev[l].K0dense, sizeof(double) * 576, hipMemcpyHostToDevice,
+                           h->stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) {
+        delete h;
+        set_error("mixed create: %s", hipGetErrorString(e));
+        return kStatusHipError;
+    }
+    *handle = h;
+    return kStatusOk;
+}
+
+int fantom_mx_destroy(void *handle)
+{
+    MxHandle *h = reinterpret_cast<MxHandle *>(handle);
+    if (!h)
+        return kStatusOk;
+    hipStreamSynchronize(h->stream);
+    if (h->have_graph)
+        hipGraphExecDestroy(h->iter_exec);
+    hipStreamDestroy(h->stream);
+    delete h;
+    return kStatusOk;
+}
+
+__global__ void mx_e_down_k(long n, const double *__restrict__ a, float *__restrict__ b)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        b[i] = (float)a[i];
+}
+
+int fantom_mx_setup(void *handle, const double *E, const uint8_t *mask, void *stream)
+{
+    MxHandle *h = reinterpret_cast<MxHandle *>(handle);
+    FANTOM_CHECK_ARG(h && E, "null operand");
+    MX_TRY(mx_join(as_stream(stream), h->stream));
+    hipStream_t s = h->stream;
+    const long ne = h->g.n_el;
+    hipLaunchKernelGGL(mx_copy_f64, dim3(vec_grid(ne)), dim3(256), 0, s, ne, E, h->E64);
+    hipLaunchKernelGGL(mx_e_down_k, dim3(vec_grid(ne)), dim3(256), 0, s, ne, E, h->lev[0].E);
+    hipError_t e = mask ? hipMemcpyAsync(h->mask, mask, h->g.n_nodes, hipMemcpyDeviceToDevice, s)
+                        : hipMemsetAsync(h->mask, 0, h->g.n_nodes, s);
+    if (e != hipSuccess) {
+        set_error("mixed setup copy: %s", hipGetErrorString(e));
+        return kStatusHipError;
+    }
+    for (int l = 1; l < h->nlev; ++l) {
+        MxLevel &F = h->lev[l - 1], &C = h->lev[l];
+        MX_TRY(launch_coarsen_E_f32(F.g, 3, F.E, C.E, s));
+        MX_TRY(launch_coarsen_mask(F.g, 3, F.mask, C.mask, s));
+    }
+    for (int l = 0; l < h->nlev; ++l) {
+        MxLevel &L = h->lev[l];
+        MX_TRY(launch_diag_f32(L.g, L.K0dense_d, L.E, L.mask, L.dinv, s));
+    }
+    // the coarsest operator and its inverse stay FP64 (assembled from the FP32
+    // moduli widened to FP64)
+    MxLevel &C = h->lev[h->nlev - 1];
+    MX_TRY(launch_dense_assemble_f32E(C.g, C.K0dense_d, C.E, C.mask, h->Ainv, s));
+    MX_TRY(launch_spd_inverse(h->nc, h->Ainv, h->gj_work, s));
+    FANTOM_LAUNCH_CHECK("mixed setup");
+    return mx_join(s, as_stream(stream));
+}
+
+static int mx_capture(MxHandle *h)
+{
+    hipGraph_t graph;
+    hipError_t e = hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+        set_error("begin capture: %s", hipGetErrorString(e));
+        return kStatusHipError;
+    }
+    int st = mx_iteration(h);
+    e = hipStreamEndCapture(h->stream, &graph);
+    if (st)
+        return st;
+    if (e != hipSuccess) {
+        set_error("end capture: %s", hipGetErrorString(e));
+        return kStatusHipError;
+    }
+    e = hipGraphInstantiate(&h->iter_exec, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+        set_error("graph instantiate: %s", hipGetErrorString(e));
+        return kStatusHipError;
+    }
+    h->have_graph = true;
+    return kStatusOk;
+}
+
+// inner FP32 PCG on K d = r32 from d = 0; returns the iteration count
+static int mx_inner(MxHandle *h, double inner_tol, int maxiter, int use_graph, int *iters)
+{
+    const long n = h->n;
+    const unsigned G = vec_grid(n);
+    hipStream_t s = h->stream;
+    hipLaunchKernelGGL(mx_zero_f32, dim3(G), dim3(256), 0, s, n, h->d);
+    hipLaunchKernelGGL(mx_init_k, dim3(1), dim3(64), 0, s, h->scal, inner_tol);
+    MX_TRY(mx_precond(h, h->r, h->z));
+    hipLaunchKernelGGL(mx_dot_k, dim3(G), dim3(256), 0, s, n, h->r, h->z, h->partials);
+    MX_TRY(launch_finalize(h->partials, (long)G, h->scal, FIN_RZ0, 0, s));
+    hipLaunchKernelGGL(mx_p_k, dim3(G), dim3(256), 0, s, n, h->p, h->z, h->scal, 1);
+    FANTOM_LAUNCH_CHECK("mx inner init");
+    if (use_graph && !h->have_graph)
+        MX_TRY(mx_capture(h));
+    double hs[FANTOM_SCALARS];
+    int done = 0, batch = h->last_inner > 3 ? h->last_inner - 2 : 1;
+    while (done < maxiter) {
+        batch = std::min(batch, maxiter - done);
+        for (int b = 0; b < batch; ++b) {
+            if (use_graph) {
+                hipError_t e = hipGraphLaunch(h->iter_exec, s);
+                if (e != hipSuccess) {
+                    set_error("graph launch: %s", hipGetErrorString(e));
+                    return kStatusHipError;
+                }
+            } else {
+                MX_TRY(mx_iteration(h));
+            }
+        }
+        done += batch;
+        MX_TRY(mx_read(h, hs));
+        if (hs[X_DONE] != 0.0) {
+            *iters = (int)hs[X_ITER];
+            h->last_inner = *iters;
+            return kStatusOk;
+        }
+        batch = 1;
+    }
+    *iters = done;
+    h->last_inner = done;
+    return kStatusOk;
+}
+
+int fantom_mx_solve(void *handle, const double *f, double *x, double tol, double inner_tol, int max_outer,
+                    int max_inner, int use_x0, int use_graph, int *outer_iters, int *inner_iters, double *rel_res,
+                    void *stream)
+{
+    MxHandle *h = reinterpret_cast<MxHandle *>(handle);
+    FANTOM_CHECK_ARG(h && f && x && outer_iters && inner_iters && rel_res, "null operand");
+    FANTOM_CHECK_ARG(max_outer >= 1 && max_inner >= 1, "iteration limits must be positive");
+    MX_TRY(mx_join(as_stream(stream), h->stream));
+    hipStream_t s = h->stream;
+    const long n = h->n;
+    const unsigned G = vec_grid(n);
+    hipLaunchKernelGGL(mx_copy_f64, dim3(G), dim3(256), 0, s, n, f, h->f64);
+    if (use_x0)
+        hipLaunchKernelGGL(mx_copy_f64, dim3(G), dim3(256), 0, s, n, x, h->x64);
+    else
+        hipMemsetAsync(h->x64, 0, sizeof(double) * n, s);
+    MX_TRY(launch_dot_partials(n, h->f64, h->f64, h->partials, s));
+    MX_TRY(launch_finalize(h->partials, (long)G, h->scal, FIN_STORE, 7, s));
+    double hs[FANTOM_SCALARS];
+    MX_TRY(mx_read(h, hs));
+    double nf = sqrt(fmax(hs[7], 0.0));
+    if (nf == 0.0)
+        nf = 1.0;
+    int total_inner = 0, outer = 0;
+    double R = 0.0;
+    for (outer = 0; outer <= max_outer; ++outer) {
+        // FP64 true residual
+        MX_TRY(launch_ku_tile(MODE_RES, h->g, h->Kb64, h->E64, h->mask, h->x64, h->r64, h->f64, nullptr, 0.0,
+                              nullptr, s));
+        MX_TRY(launch_dot_partials(n, h->r64, h->r64, h->partials, s));
+        MX_TRY(launch_finalize(h->partials, (long)G, h->scal, FIN_STORE, S_TMP, s));
+        MX_TRY(mx_read(h, hs));
+        R = sqrt(fmax(hs[S_TMP], 0.0)) / nf;
+        if (R < tol || outer == max_outer || hs[S_TMP] == 0.0)
+            break;
+        hipLaunchKernelGGL(mx_down_k, dim3(G), dim3(256), 0, s, n, h->r64, h->r, h->scal);
+        int it = 0;
+        MX_TRY(mx_inner(h, inner_tol, max_inner, use_graph, &it));
+        total_inner += it;
+        hipLaunchKernelGGL(mx_up_k, dim3(G), dim3(256), 0, s, n, h->d, h->x64, h->scal);
+        FANTOM_LAUNCH_CHECK("mx refine");
+    }
+    hipLaunchKernelGGL(mx_copy_f64, dim3(G), dim3(256), 0, s, n, h->x64, x);
+    FANTOM_LAUNCH_CHECK("mx solve");
+    *outer_iters = outer;
+    *inner_iters = total_inner;
+    *rel_res = R;
+    return mx_join(s, as_stream(stream));
+}
+
+}  // extern "C"

@@ -112,7 +112,8 @@ class MultiGrid(_DeviceKrylov):
     def __init__(self, mesh, kernel, maxiter=1000, tol=1e-5, n_smooth=3, omega=0.5, n_level=3, cycle="W",
                  w_level=1, coarse_solver="cholmod", matrix_free=False, low_level_tol=1e-8,
                  low_level_maxiter=5000, min_omega=0.4, omega_boost=1.06, omega_schedule="fixed",
-                 check_every=1):
+                 check_every=1, native=True, use_graph=True, precision="fp64", inner_tol=1e-4,
+                 max_outer=20):
         if coarse_solver not in ["cg", "gmres", "splu", "spsolve", "cholmod"]:
             raise ValueError("Coarse solver not recognized.")
         if coarse_solver in ["splu", "spsolve"] and matrix_free:
@@ -143,12 +144,138 @@ class MultiGrid(_DeviceKrylov):
         self._built_version = None
         self.last_iterations = 0
         self.last_recursive_residual = None
+        # the C++ driver runs fixed-omega V/W... cycles; the Python driver
+        # below covers the reference omega schedule and the W-cycle
+        self.native = bool(native) and omega_schedule == "fixed" and cycle != "w"
+        self.use_graph = bool(use_graph)
+        if precision not in ("fp64", "mixed"):
+            raise ValueError("precision must be 'fp64' or 'mixed'")
+        if precision == "mixed" and not self.native:
+            raise ValueError("the mixed-precision solve runs in the native driver (fixed omega, V-cycle)")
+        self.precision = precision
+        self.inner_tol = float(inner_tol)
+        self.max_outer = int(max_outer)
+        self.last_outer_iterations = 0
+        self._nh = None
+        self._nh_key = None
+        self._nh_state = None
+        self._ws = None
 
     def reset(self):
         """Forget the hierarchy and the warm start."""
         self.levels = None
         self._built_version = None
         self.last_x0 = None
+        self._free_native()
+
+    def __del__(self):
+        try:
+            self._free_native()
+        except Exception:
+            pass
+
+    def _free_native(self):
+        if getattr(self, "_nh", None) is not None:
+            if self._nh_key is not None and self._nh_key[-1] == "mixed":
+                _lib.load().fantom_mx_destroy(self._nh)
+            else:
+                _lib.load().fantom_mg_destroy(self._nh)
+            self._nh = None
+            self._nh_state = None
+
+    def _n_coarse(self):
+        nel = list(self.kernel.grid[:self.kernel.dim])
+        n_coarse = self.n_level
+        for l in range(1, self.n_level + 1):
+            if any(n % (2 ** l) for n in nel):
+                n_coarse = l - 1
+                break
+        if n_coarse < self.n_level:
+            warnings.warn(f"grid {nel} halves evenly only {n_coarse} times; using {n_coarse} coarse levels")
+        return n_coarse
+
+    def precondition_native(self, r, z):
+        """z = V(r): one V-cycle of the native FP64 hierarchy for the kernel's
+        current E (sets the hierarchy up if needed)."""
+        if self.precision != "fp64" or not self.native:
+            raise NotImplementedError("precondition_native needs the native FP64 driver")
+        k = self.kernel
+        if self._nh is None or self._nh_state != (k.rho_version, k.con_version, k.rho.data_ptr()):
+            self._native_prepare()
+        _lib.call("fantom_mg_precondition", self._nh, _lib.ptr(r), _lib.ptr(z))
+        return z
+
+    def _native_prepare(self):
+        import ctypes
+        k = self.kernel
+        L = _lib.load()
+        key = (k.grid, self.n_level, self.n_smooth, self.omega, k.kernel_kind, self.precision)
+        if self._nh is None or self._nh_key != key:
+            self._free_native()
+            n_coarse = self._n_coarse()
+            nbytes = L.fantom_mg_workspace_size(*k.grid, n_coarse)
+            self._ws = torch.empty(nbytes, dtype=torch.uint8, device=k.device)
+            h = ctypes.c_void_p()
+            K0 = np.ascontiguousarray(k.K_single, dtype=np.float64)
+            kid = _lib.KU_TILE if k.kernel_kind == "tile" else _lib.KU_NODE
+            _lib.check(L.fantom_mg_create(*k.grid, n_coarse, self.n_smooth, self.omega, kid, K0.ctypes.data,
+                                          self._ws.data_ptr(), nbytes, ctypes.byref(h)))
+            self._nh, self._nh_key, self._nh_state = h, key, None
+        state = (k.rho_version, k.con_version, k.rho.data_ptr())
+        if self._nh_state != state:
+            _lib.call("fantom_mg_setup", self._nh, _lib.ptr(k.rho), _lib.ptr(k.mask) if k.has_con else None)
+            self._nh_state = state
+
+    def _native_solve(self, rhs, use_last):
+        import ctypes
+        k = self.kernel
+        L = _lib.load()
+        mixed = self.precision == "mixed"
+        if mixed and k.dim != 3:
+            raise NotImplementedError("the mixed-precision solve is implemented for hex8 grids")
+        key = (k.grid, self.n_level, self.n_smooth, self.omega, k.kernel_kind, self.precision)
+        pre = "fantom_mx_" if mixed else "fantom_mg_"
+        if self._nh is None or self._nh_key != key:
+            self._free_native()
+            n_coarse = self._n_coarse()
+            nbytes = getattr(L, pre + "workspace_size")(*k.grid, n_coarse)
+            if nbytes <= 0:
+                raise ValueError("grid does not support the requested hierarchy")
+            self._ws = torch.empty(nbytes, dtype=torch.uint8, device=k.device)
+            h = ctypes.c_void_p()
+            K0 = np.ascontiguousarray(k.K_single, dtype=np.float64)
+            if mixed:
+                _lib.check(L.fantom_mx_create(*k.grid, n_coarse, self.n_smooth, self.omega, K0.ctypes.data,
+                                              self._ws.data_ptr(), nbytes, ctypes.byref(h)))
+            else:
+                kid = _lib.KU_TILE if k.kernel_kind == "tile" else _lib.KU_NODE
+                _lib.check(L.fantom_mg_create(*k.grid, n_coarse, self.n_smooth, self.omega, kid, K0.ctypes.data,
+                                              self._ws.data_ptr(), nbytes, ctypes.byref(h)))
+            self._nh = h
+            self._nh_key = key
+            self._nh_state = None
+            self._n_levels = n_coarse + 1
+        state = (k.rho_version, k.con_version, k.rho.data_ptr())
+        if self._nh_state != state:
+            _lib.call(pre + "setup", self._nh, _lib.ptr(k.rho), _lib.ptr(k.mask) if k.has_con else None)
+            self._nh_state = state
+        n = rhs.numel()
+        use_x0 = bool(use_last and self.last_x0 is not None and self.last_x0.numel() == n)
+        x = self.last_x0 if use_x0 else torch.empty_like(rhs)
+        it = ctypes.c_int(0)
+        res = ctypes.c_double(0.0)
+        if mixed:
+            outer = ctypes.c_int(0)
+            _lib.call("fantom_mx_solve", self._nh, _lib.ptr(rhs), _lib.ptr(x), self.tol, self.inner_tol,
+                      self.max_outer, self.maxiter, int(use_x0), int(self.use_graph), ctypes.byref(outer),
+                      ctypes.byref(it), ctypes.byref(res))
+            self.last_outer_iterations = outer.value
+        else:
+            _lib.call("fantom_mg_solve", self._nh, _lib.ptr(rhs), _lib.ptr(x), self.tol, self.maxiter, int(use_x0),
+                      int(self.use_graph), ctypes.byref(it), ctypes.byref(res))
+        self.last_iterations = it.value
+        self.last_x0 = x
+        return x, res.value
 
     # ---- hierarchy ------------------------------------------------------------------
     def _make_levels(self):
@@ -246,6 +373,8 @@ class MultiGrid(_DeviceKrylov):
         if rho is not None:
             k.set_rho(rho)
         rhs = k._as_vec(rhs)
+        if self.native:
+            return self._native_solve(rhs, use_last)
         n = rhs.numel()
         self._alloc(n)
         if self.levels is None or self._built_version != k.rho_version or self.levels[0].n_dof != n:

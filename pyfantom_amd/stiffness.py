@@ -65,6 +65,7 @@ class StructuredStiffnessKernel:
         self.has_rho = False
         self.rho = None
         self.rho_version = 0
+        self.con_version = 0
         self.mat_vec = None
         self.diag = None
         self.dk = None
@@ -103,6 +104,7 @@ class StructuredStiffnessKernel:
         for d in range(self.dof):
             bits |= c[:, d] << d
         self.mask = bits
+        self.con_version += 1
         self.non_con_map = self.idx_map[~self.constraints]
 
     # ---- reference API ---------------------------------------------------------------
