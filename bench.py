@@ -165,7 +165,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": cfg["name"] + "_simp_oc_gmgpcg_1e-8", "elements": list(nel), "dofs": K.shape[0],
-                   "coarsenings": len(solver.levels) - 1, "smoother": "jacobi 2+2 omega 0.6",
+                   "coarsenings": n_level, "smoother": "jacobi 2+2 omega 0.6",
                    "precision": "fp64 plain GMG-PCG"},
         "pcg_iterations_mean": float(np.mean(iters)),
         "compliance": prob.f(),

@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define FANTOM_ABI_VERSION 1
+#define FANTOM_ABI_VERSION 2
 
 /* Number of double slots of the PCG device-scalar array. */
 #define FANTOM_SCALARS 16
@@ -162,10 +162,36 @@ long fantom_oc_partials(long n);
 int fantom_oc_update_f64(long n, const double *x, const double *df, const double *dg, double vf, double move,
                          double *ocp, double *state, double *partials, double *x_new, void *stream);
 
+/* ---- A8: Galerkin coarse operators (hex8) ------------------------------------
+ * Replace get_restricted_l0/l1p (core/CUDA/_mgm.py:19-171, mgm.cu:246-590):
+ * A_{l+1} = P^T A_l P assembled element by element (trilinear P).
+ * galerkin_tables (host): from the fine K0 (24x24, reference corner order)
+ * the 8 child matrices G (8 x 576, natural corner order b = bx+2by+4bz),
+ * their diagonals Gdiag (8 x 24) and the Walsh-Hadamard block table Btab
+ * (576) of the level-1 operator; check[2] gets the symmetry residuals.
+ * gal1_ku: y = A_1 u on the level-1 grid nelx x nely x nelz from the fine
+ * moduli (grid 2x), matrix-free.  gal_build: explicit level-(l+1) element
+ * matrices Kc (n_el x 576) from level-1 moduli (src_kind 0) or level-l
+ * element matrices (src_kind 1).  elem_ku: y = A u from element matrices.
+ * elem_from_fine: explicit level-1 element matrices.  Device pointers except
+ * the table outputs of galerkin_tables (host). */
+int fantom_galerkin_tables(const double *K0, double *G, double *Btab, double *Gdiag, double *check);
+int fantom_gal1_ku_f64(int nelx, int nely, int nelz, const double *Btab, const double *Efine, const uint8_t *mask,
+                       const double *u, double *y, void *stream);
+int fantom_gal_build_f64(int nelx, int nely, int nelz, int src_kind, const double *G, const double *Efine,
+                         const double *Kfine, double *Kc, void *stream);
+int fantom_elem_ku_f64(int nelx, int nely, int nelz, const double *Ke, const uint8_t *mask, const double *u,
+                       double *y, void *stream);
+int fantom_elem_from_fine_f64(int nelx, int nely, int nelz, const double *G, const double *Efine, double *Ke,
+                              void *stream);
+
 /* ---- A9-A11: native GMG-PCG driver ------------------------------------------
  * Replaces MultiGrid._setup / _multi_grid / solve
  * (solvers/CUDA/_solvers.py:442-660): rediscretised matrix-free hierarchy of
- * n_coarse coarsenings (the grid must halve evenly that many times), Jacobi
+ * n_coarse coarsenings (the grid must halve evenly that many times); coarse
+ * operators are Galerkin products (galerkin != 0, hex8: level 1 matrix-free
+ * from the fine moduli, deeper levels explicit element matrices) or
+ * rediscretised from 2^d-averaged moduli (galerkin == 0); Jacobi
  * smoothing (n_smooth sweeps, weight omega), exact dense coarsest solve, PCG
  * with all scalars on the device and each iteration replayed from a HIP
  * graph (use_graph != 0).
@@ -178,9 +204,9 @@ int fantom_oc_update_f64(long n, const double *x, const double *df, const double
  * solve: x = K^-1 f to relative recursive residual tol (x is the initial
  * guess when use_x0 != 0); *iters gets the PCG iteration count and *rel_res
  * the true residual ||f - K x|| / ||f||. */
-long fantom_mg_workspace_size(int nelx, int nely, int nelz, int n_coarse);
+long fantom_mg_workspace_size(int nelx, int nely, int nelz, int n_coarse, int galerkin);
 int fantom_mg_create(int nelx, int nely, int nelz, int n_coarse, int n_smooth, double omega, int kernel,
-                     const double *K0_host, void *workspace, long workspace_bytes, void **handle);
+                     int galerkin, const double *K0_host, void *workspace, long workspace_bytes, void **handle);
 int fantom_mg_destroy(void *handle);
 int fantom_mg_setup(void *handle, const double *E, const uint8_t *mask, void *stream);
 int fantom_mg_precondition(void *handle, const double *r, double *z, void *stream);
@@ -196,9 +222,9 @@ int fantom_mg_solve(void *handle, const double *f, double *x, double tol, int ma
  * solve.  *outer_iters gets the refinement steps, *inner_iters the total
  * inner PCG iterations, *rel_res the final FP64 ||f - K x||/||f||.
  * Workspace / handle / stream rules as for the fantom_mg_* driver. */
-long fantom_mx_workspace_size(int nelx, int nely, int nelz, int n_coarse);
-int fantom_mx_create(int nelx, int nely, int nelz, int n_coarse, int n_smooth, double omega, const double *K0_host,
-                     void *workspace, long workspace_bytes, void **handle);
+long fantom_mx_workspace_size(int nelx, int nely, int nelz, int n_coarse, int galerkin);
+int fantom_mx_create(int nelx, int nely, int nelz, int n_coarse, int n_smooth, double omega, int galerkin,
+                     const double *K0_host, void *workspace, long workspace_bytes, void **handle);
 int fantom_mx_destroy(void *handle);
 int fantom_mx_setup(void *handle, const double *E, const uint8_t *mask, void *stream);
 int fantom_mx_solve(void *handle, const double *f, double *x, double tol, double inner_tol, int max_outer,

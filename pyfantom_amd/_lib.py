@@ -59,16 +59,16 @@ SIGNATURES = {
     "fantom_simp_f64": (_c_int, [_c_long, _vp, _c_int, _c_double, _c_double, _c_double, _c_int, _c_double,
                                  _c_double, _vp, _vp, _vp]),
     "fantom_mul_f64": (_c_int, [_c_long, _vp, _vp, _vp, _vp]),
-    "fantom_mg_workspace_size": (_c_long, [_c_int, _c_int, _c_int, _c_int]),
-    "fantom_mg_create": (_c_int, [_c_int, _c_int, _c_int, _c_int, _c_int, _c_double, _c_int, _vp, _vp, _c_long,
-                                  ctypes.POINTER(_vp)]),
+    "fantom_mg_workspace_size": (_c_long, [_c_int, _c_int, _c_int, _c_int, _c_int]),
+    "fantom_mg_create": (_c_int, [_c_int, _c_int, _c_int, _c_int, _c_int, _c_double, _c_int, _c_int, _vp, _vp,
+                                  _c_long, ctypes.POINTER(_vp)]),
     "fantom_mg_destroy": (_c_int, [_vp]),
     "fantom_mg_setup": (_c_int, [_vp, _vp, _vp, _vp]),
     "fantom_mg_precondition": (_c_int, [_vp, _vp, _vp, _vp]),
     "fantom_mg_solve": (_c_int, [_vp, _vp, _vp, _c_double, _c_int, _c_int, _c_int, ctypes.POINTER(_c_int),
                                  ctypes.POINTER(_c_double), _vp]),
-    "fantom_mx_workspace_size": (_c_long, [_c_int, _c_int, _c_int, _c_int]),
-    "fantom_mx_create": (_c_int, [_c_int, _c_int, _c_int, _c_int, _c_int, _c_double, _vp, _vp, _c_long,
+    "fantom_mx_workspace_size": (_c_long, [_c_int, _c_int, _c_int, _c_int, _c_int]),
+    "fantom_mx_create": (_c_int, [_c_int, _c_int, _c_int, _c_int, _c_int, _c_double, _c_int, _vp, _vp, _c_long,
                                   ctypes.POINTER(_vp)]),
     "fantom_mx_destroy": (_c_int, [_vp]),
     "fantom_mx_setup": (_c_int, [_vp, _vp, _vp, _vp]),
@@ -91,6 +91,11 @@ SIGNATURES = {
     "fantom_net_output": (_c_int, [_c_long, _vp, _vp, _c_int, _vp, _c_int, _vp]),
     "fantom_mse_bf16": (_c_int, [_c_long, _vp, _vp, _vp, _c_int, _vp, _vp, _vp]),
     "fantom_node_density": (_c_int, [_c_int, _c_int, _c_int, _vp, _vp, _vp]),
+    "fantom_galerkin_tables": (_c_int, [_vp, _vp, _vp, _vp, _vp]),
+    "fantom_gal1_ku_f64": (_c_int, [_c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "fantom_gal_build_f64": (_c_int, [_c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp]),
+    "fantom_elem_ku_f64": (_c_int, [_c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp]),
+    "fantom_elem_from_fine_f64": (_c_int, [_c_int, _c_int, _c_int, _vp, _vp, _vp, _vp]),
     "fantom_oc_update_f64": (_c_int, [_c_long, _vp, _vp, _vp, _c_double, _c_double, _vp, _vp, _vp, _vp, _vp]),
 }
 

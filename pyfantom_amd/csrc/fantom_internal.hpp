@@ -45,4 +45,29 @@ int launch_dense_assemble_f32E(const Grid &g, const double *K0, const float *E, 
 int launch_spd_inverse(int n, double *A, double *work, hipStream_t s);
 int launch_gemv(int n, const double *A, const double *x, double *y, hipStream_t s);
 
+// galerkin.hip
+int launch_gal1(int mode, const Grid &g, const double *Btab, const double *Ef, const uint8_t *mask, const double *u,
+                double *y, const double *b, const double *dinv, double omega, double *partials, hipStream_t s);
+int launch_gal1_f32(int mode, const Grid &g, const double *Btab, const float *Ef, const uint8_t *mask, const float *u,
+                    float *y, const float *b, const float *dinv, double omega, double *partials, hipStream_t s);
+long gal1_partials(const Grid &g);
+int launch_elem(int mode, const Grid &g, const double *Ke, const uint8_t *mask, const double *u, double *y,
+                const double *b, const double *dinv, double omega, double *partials, hipStream_t s);
+int launch_elem_f32(int mode, const Grid &g, const double *Ke, const uint8_t *mask, const float *u, float *y,
+                    const float *b, const float *dinv, double omega, double *partials, hipStream_t s);
+long elem_partials(const Grid &g);
+int launch_dinv_gal1(const Grid &g, const double *Gdiag, const double *Ef, const uint8_t *mask, double *d,
+                     hipStream_t s);
+int launch_dinv_gal1_f32(const Grid &g, const double *Gdiag, const float *Ef, const uint8_t *mask, float *d,
+                         hipStream_t s);
+int launch_dinv_elem(const Grid &g, const double *Ke, const uint8_t *mask, double *d, hipStream_t s);
+int launch_dinv_elem_f32(const Grid &g, const double *Ke, const uint8_t *mask, float *d, hipStream_t s);
+int launch_gal_build(const Grid &gc, int src_kind, const double *G, const double *Ef, const double *Kf, double *Kc,
+                     hipStream_t s);
+int launch_gal_build_f32E(const Grid &gc, int src_kind, const double *G, const float *Ef, const double *Kf,
+                          double *Kc, hipStream_t s);
+int launch_elem_from_fine(const Grid &gc, const double *G, const double *Ef, double *Ke, hipStream_t s);
+int launch_elem_from_fine_f32E(const Grid &gc, const double *G, const float *Ef, double *Ke, hipStream_t s);
+int launch_dense_from_elem(const Grid &g, const double *Ke, const uint8_t *mask, double *A, hipStream_t s);
+
 }  // namespace fantom

@@ -1,0 +1,851 @@
+// Galerkin coarse operators for the hex8 multigrid hierarchy.
+//
+// The reference forms A_{l+1} = R A_l R^T as padded CSR on every solve
+// (core/CUDA/_mgm.py:19-171, mgm.cu:246-590).  Here the same product is
+// formed element by element: with trilinear prolongation P (coarse node c on
+// fine node 2c), every fine element lies in one coarse element E, so
+//   A_{l+1} = sum_E P_E^T (sum_{children c of E} K_c) P_E,
+// and coarse element matrices assemble the coarse operator exactly.  (The
+// reference's R = D^-1 P^T differs from P^T by a diagonal scaling that
+// cancels in the V-cycle.)  Constrained coarse DOFs are identity rows and
+// skipped columns, the coincident-node rule.
+//
+// Level 1 is applied without storing matrices.  With child moduli E_c and the
+// constant child matrices G_c = P_c^T K0 P_c, K_E = sum_c E_c G_c.  The
+// reflections of the coarse element permute the children and commute with
+// K0, so G_{c^g} = rho(g) G_c rho(g)^T, and in the Walsh-Hadamard basis
+// (v_{s,d}[corner b, comp d] = (-1)^{popc(b & (s ^ e_d))})
+//   Khat_E[s, s'] = Ehat_{s ^ s'} * Ghat_0[s, s']     (3x3 blocks)
+// where Ehat_t = sum_c (-1)^{popc(c & t)} E_c.  One element costs a WHT of u,
+// a WHT of the 8 child moduli, 64 3x3 block products and a WHT back; only the
+// fine moduli are read.
+// Levels >= 2 store explicit 24x24 element matrices (natural corner order
+// b = bx + 2by + 4bz), built once per setup from the level below.
+#include <cmath>
+
+#include "../../include/fantom.h"
+#include "common.hpp"
+#include "fantom_internal.hpp"
+
+namespace fantom {
+
+constexpr int GTB = 16;             // elements per tile edge
+constexpr int GTN = GTB + 1;
+constexpr int GTOUT = GTB - 1;
+constexpr int GPLANE = GTN * GTN;
+constexpr int GCHUNK = 16;
+constexpr int kGalBlock = GTB * GTB;
+constexpr int kGPre = (3 * GPLANE + kGalBlock - 1) / kGalBlock;
+constexpr int kGPreM = (GPLANE + kGalBlock - 1) / kGalBlock;
+
+template <typename T>
+__device__ __forceinline__ void gwht8(T *w)
+{
+#pragma unroll
+    for (int h = 1; h < 8; h <<= 1) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            if ((i & h) == 0) {
+                const T a = w[i], b = w[i | h];
+                w[i] = a + b;
+                w[i | h] = a - b;
+            }
+        }
+    }
+}
+
+// Level-1 Galerkin K(E)u on the level-1 grid g (its fine grid is 2x in every
+// direction).  Tiling as the fine tile kernel (ku_tile.hip).  Btab: 576
+// doubles, block (s, s') at Btab[(s*8 + s')*9 + 3*d + d'] (scaled by 1/8).
+template <typename T, int MODE>
+__global__ __launch_bounds__(kGalBlock, 2) void ku_gal1(Grid g, const double *__restrict__ Btab,
+                                                      const T *__restrict__ Ef, const uint8_t *__restrict__ mask,
+                                                      const T *__restrict__ u, T *__restrict__ y,
+                                                      const T *__restrict__ bvec, const T *__restrict__ dinv,
+                                                      T omega, double *__restrict__ partials, int tiles_x,
+                                                      int tiles_z, int chunks_y)
+{
+    __shared__ T us[2][3][GPLANE];
+    __shared__ uint8_t ms[2][GPLANE];
+    __shared__ T outs[24][kGalBlock];
+
+    const int per_xcd = gridDim.x / 8;
+    const int bid = blockIdx.x;
+    int logical = (bid % 8) * per_xcd + bid / 8;
+    double part = 0.0;
+    const int tid = threadIdx.x;
+    const int lx = tid / GTB, lz = tid % GTB;
+    const bool valid_tile = logical < tiles_x * tiles_z * chunks_y;
+    if (!valid_tile)
+        logical = 0;
+    const int tz = logical % tiles_z;
+    const int tx = (logical / tiles_z) % tiles_x;
+    const int cy = logical / (tiles_z * tiles_x);
+    const int ex0 = tx * GTOUT - 1, ez0 = tz * GTOUT - 1;
+    const int jc0 = cy * GCHUNK;
+    const int jc1 = min(jc0 + GCHUNK, g.ny);
+    const long nplane = (long)g.nx * g.nz;
+    const int fy = 2 * g.nely, fz = 2 * g.nelz;  // fine element counts (y, z)
+
+    auto load_plane = [&](int j, T *vals, uint8_t *mv) {
+#pragma unroll
+        for (int q = 0; q < kGPre; ++q) {
+            const int idx = tid + q * kGalBlock;
+            T v = T(0);
+            if (idx < 3 * GPLANE && j >= 0 && j < g.ny) {
+                const int a = idx / (3 * GTN), rem = idx % (3 * GTN);
+                const int gx = ex0 + a, gz = ez0 + rem / 3;
+                if (gx >= 0 && gx < g.nx && gz >= 0 && gz < g.nz)
+                    v = u[3 * ((long)j * nplane + (long)gx * g.nz + gz) + rem % 3];
+            }
+            vals[q] = v;
+        }
+#pragma unroll
+        for (int q = 0; q < kGPreM; ++q) {
+            const int idx = tid + q * kGalBlock;
+            uint8_t m = 0;
+            if (mask && idx < GPLANE && j >= 0 && j < g.ny) {
+                const int a = idx / GTN, c = idx % GTN;
+                const int gx = ex0 + a, gz = ez0 + c;
+                if (gx >= 0 && gx < g.nx && gz >= 0 && gz < g.nz)
+                    m = mask[(long)j * nplane + (long)gx * g.nz + gz];
+            }
+            mv[q] = m;
+        }
+    };
+    auto store_plane = [&](int slot, const T *vals, const uint8_t *mv) {
+#pragma unroll
+        for (int q = 0; q < kGPre; ++q) {
+            const int idx = tid + q * kGalBlock;
+            if (idx < 3 * GPLANE) {
+                const int a = idx / (3 * GTN), rem = idx % (3 * GTN);
+                us[slot][rem % 3][a * GTN + rem / 3] = vals[q];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < kGPreM; ++q) {
+            const int idx = tid + q * kGalBlock;
+            if (idx < GPLANE)
+                ms[slot][idx] = mv[q];
+        }
+    };
+
+    T pre[kGPre];
+    uint8_t prem[kGPreM];
+    if (valid_tile) {
+        load_plane(jc0 - 1, pre, prem);
+        store_plane((jc0 + 1) & 1, pre, prem);
+        load_plane(jc0, pre, prem);
+        store_plane(jc0 & 1, pre, prem);
+    }
+    __syncthreads();
+
+    T hi[3] = {T(0), T(0), T(0)};
+    const int gxe = ex0 + lx, gze = ez0 + lz;
+    const bool el_in = gxe >= 0 && gxe < g.nelx && gze >= 0 && gze < g.nelz;
+
+    for (int L = jc0 - 1; L < jc1 && valid_tile; ++L) {
+        const int sl = (L + 2) & 1, su = (L + 3) & 1;
+        if (L + 2 < jc1 + 1)
+            load_plane(L + 2, pre, prem);
+        T uown[3] = {T(0), T(0), T(0)};
+        int mown = 0;
+        if (L >= jc0) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                uown[c] = us[sl][c][lx * GTN + lz];
+            mown = ms[sl][lx * GTN + lz];
+        }
+        const bool active = el_in && L >= 0 && L < g.nely;
+        if (active) {
+            // Walsh transform of the 8 child moduli (child bits cx + 2cy + 4cz)
+            T eh[8];
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                const int cx = c & 1, cyy = (c >> 1) & 1, cz = c >> 2;
+                eh[c] = Ef[((long)(2 * gxe + cx) * fy + (2 * L + cyy)) * fz + (2 * gze + cz)];
+            }
+            gwht8(eh);
+            T w[3][8];
+#pragma unroll
+            for (int b = 0; b < 8; ++b) {
+                const int bx = b & 1, by = (b >> 1) & 1, bz = b >> 2;
+                const int sidx = by ? su : sl;
+                const int p = (lx + bx) * GTN + (lz + bz);
+                const int m = ms[sidx][p];
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+                    w[c][b] = ((m >> c) & 1) ? T(0) : us[sidx][c][p];
+            }
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                gwht8(w[c]);
+            T z[3][8];
+#pragma unroll
+            for (int s = 0; s < 8; ++s) {
+                T a0 = T(0), a1 = T(0), a2 = T(0);
+#pragma unroll
+                for (int s2 = 0; s2 < 8; ++s2) {
+                    const T h0 = w[0][s2 ^ 1], h1 = w[1][s2 ^ 2], h2 = w[2][s2 ^ 4];
+                    const double *Bs = Btab + (s * 8 + s2) * 9;
+                    const T e = eh[s ^ s2];
+                    a0 += e * ((T)Bs[0] * h0 + (T)Bs[1] * h1 + (T)Bs[2] * h2);
+                    a1 += e * ((T)Bs[3] * h0 + (T)Bs[4] * h1 + (T)Bs[5] * h2);
+                    a2 += e * ((T)Bs[6] * h0 + (T)Bs[7] * h1 + (T)Bs[8] * h2);
+                }
+                z[0][s ^ 1] = a0;
+                z[1][s ^ 2] = a1;
+                z[2][s ^ 4] = a2;
+            }
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                gwht8(z[c]);
+#pragma unroll
+                for (int b = 0; b < 8; ++b)
+                    outs[b * 3 + c][tid] = z[c][b];
+            }
+        } else {
+#pragma unroll
+            for (int v = 0; v < 24; ++v)
+                outs[v][tid] = T(0);
+        }
+        __syncthreads();
+        const bool node_thread = lx >= 1 && lz >= 1;
+        if (node_thread) {
+            T lo[3] = {T(0), T(0), T(0)}, up[3] = {T(0), T(0), T(0)};
+#pragma unroll
+            for (int bx = 0; bx < 2; ++bx)
+#pragma unroll
+                for (int bz = 0; bz < 2; ++bz) {
+                    const int t = (lx - bx) * GTB + (lz - bz);
+                    const int bl = bx + 4 * bz, bu = bl + 2;
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        lo[c] += outs[bl * 3 + c][t];
+                        up[c] += outs[bu * 3 + c][t];
+                    }
+                }
+            if (L >= jc0) {
+                const int gx = ex0 + lx, gz = ez0 + lz;
+                if (gx < g.nx && gz < g.nz) {
+                    const long n = (long)L * nplane + (long)gx * g.nz + gz;
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        const long r = 3 * n + c;
+                        const T ku = ((mown >> c) & 1) ? uown[c] : hi[c] + lo[c];
+                        T o;
+                        if (MODE == MODE_JAC)
+                            o = uown[c] + omega * dinv[r] * (bvec[r] - ku);
+                        else if (MODE == MODE_RES)
+                            o = bvec[r] - ku;
+                        else
+                            o = ku;
+                        y[r] = o;
+                        if (MODE == MODE_KUDOT)
+                            part += (double)uown[c] * (double)o;
+                    }
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                hi[c] = up[c];
+        }
+        if (L + 2 < jc1 + 1)
+            store_plane(sl, pre, prem);
+        __syncthreads();
+    }
+    if (MODE == MODE_KUDOT) {
+        const double s = block_sum<kGalBlock>(part);
+        if (threadIdx.x == 0)
+            partials[blockIdx.x] = s;
+    }
+}
+
+// ---- explicit element matrices (levels >= 2) ----------------------------------------
+// corner b of an element sits at node offset (b & 1, (b >> 1) & 1, b >> 2)
+__device__ __forceinline__ long node_of(const Grid &g, int ei, int ej, int ek, int b)
+{
+    return (long)(ej + ((b >> 1) & 1)) * g.nx * g.nz + (long)(ei + (b & 1)) * g.nz + (ek + (b >> 2));
+}
+
+// y = K u with explicit element matrices Ke[e][24][24] (natural order),
+// node-centric gather in a fixed element order; Dirichlet rule as above.
+template <typename T, int MODE>
+__global__ __launch_bounds__(256) void ku_elem(Grid g, const double *__restrict__ Ke,
+                                             const uint8_t *__restrict__ mask, const T *__restrict__ u,
+                                             T *__restrict__ y, const T *__restrict__ bvec,
+                                             const T *__restrict__ dinv, T omega, double *__restrict__ partials)
+{
+    const long n = (long)blockIdx.x * 256 + threadIdx.x;
+    double part = 0.0;
+    if (n < g.n_nodes) {
+        const int k = (int)(n % g.nz);
+        const long t = n / g.nz;
+        const int i = (int)(t % g.nx), j = (int)(t / g.nx);
+        double acc[3] = {0.0, 0.0, 0.0};
+        for (int ox = 1; ox >= 0; --ox)
+            for (int oy = 1; oy >= 0; --oy)
+                for (int oz = 1; oz >= 0; --oz) {
+                    const int ei = i - ox, ej = j - oy, ek = k - oz;
+                    if (ei < 0 || ei >= g.nelx || ej < 0 || ej >= g.nely || ek < 0 || ek >= g.nelz)
+                        continue;
+                    const long e = ((long)ei * g.nely + ej) * g.nelz + ek;
+                    const int a = ox + 2 * oy + 4 * oz;
+                    const double *K = Ke + e * 576 + a * 72;
+                    for (int b = 0; b < 8; ++b) {
+                        const long m = node_of(g, ei, ej, ek, b);
+                        const int mm = mask ? mask[m] : 0;
+                        for (int d = 0; d < 3; ++d) {
+                            if ((mm >> d) & 1)
+                                continue;
+                            const double ud = (double)u[3 * m + d];
+                            acc[0] += K[b * 3 + d] * ud;
+                            acc[1] += K[24 + b * 3 + d] * ud;
+                            acc[2] += K[48 + b * 3 + d] * ud;
+                        }
+                    }
+                }
+        const int mn = mask ? mask[n] : 0;
+        for (int c = 0; c < 3; ++c) {
+            const long r = 3 * n + c;
+            const T ur = u[r];
+            const T ku = ((mn >> c) & 1) ? ur : (T)acc[c];
+            T o;
+            if (MODE == MODE_JAC)
+                o = ur + omega * dinv[r] * (bvec[r] - ku);
+            else if (MODE == MODE_RES)
+                o = bvec[r] - ku;
+            else
+                o = ku;
+            y[r] = o;
+            if (MODE == MODE_KUDOT)
+                part += (double)ur * (double)o;
+        }
+    }
+    if (MODE == MODE_KUDOT) {
+        const double s = block_sum<256>(part);
+        if (threadIdx.x == 0)
+            partials[blockIdx.x] = s;
+    }
+}
+
+// Jacobi weights 1/diag from explicit element matrices (constrained -> 1)
+template <typename T>
+__global__ void dinv_elem(Grid g, const double *__restrict__ Ke, const uint8_t *__restrict__ mask,
+                          T *__restrict__ d)
+{
+    const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= g.n_nodes)
+        return;
+    const int k = (int)(n % g.nz);
+    const long t = n / g.nz;
+    const int i = (int)(t % g.nx), j = (int)(t / g.nx);
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int ox = 1; ox >= 0; --ox)
+        for (int oy = 1; oy >= 0; --oy)
+            for (int oz = 1; oz >= 0; --oz) {
+                const int ei = i - ox, ej = j - oy, ek = k - oz;
+                if (ei < 0 || ei >= g.nelx || ej < 0 || ej >= g.nely || ek < 0 || ek >= g.nelz)
+                    continue;
+                const long e = ((long)ei * g.nely + ej) * g.nelz + ek;
+                const int a = ox + 2 * oy + 4 * oz;
+                for (int c = 0; c < 3; ++c)
+                    acc[c] += Ke[e * 576 + (a * 3 + c) * 24 + a * 3 + c];
+            }
+    const int mn = mask ? mask[n] : 0;
+    for (int c = 0; c < 3; ++c)
+        d[3 * n + c] = ((mn >> c) & 1) ? T(1) : (T)(1.0 / acc[c]);
+}
+
+// Jacobi weights of the level-1 Galerkin operator from the fine moduli:
+// diag K_E[(a,c),(a,c)] = sum_child E_child * Gdiag[child][a*3+c]
+template <typename T>
+__global__ void dinv_gal1(Grid g, const double *__restrict__ Gdiag, const T *__restrict__ Ef,
+                          const uint8_t *__restrict__ mask, T *__restrict__ d)
+{
+    const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= g.n_nodes)
+        return;
+    const int fy = 2 * g.nely, fz = 2 * g.nelz;
+    const int k = (int)(n % g.nz);
+    const long t = n / g.nz;
+    const int i = (int)(t % g.nx), j = (int)(t / g.nx);
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int ox = 1; ox >= 0; --ox)
+        for (int oy = 1; oy >= 0; --oy)
+            for (int oz = 1; oz >= 0; --oz) {
+                const int ei = i - ox, ej = j - oy, ek = k - oz;
+                if (ei < 0 || ei >= g.nelx || ej < 0 || ej >= g.nely || ek < 0 || ek >= g.nelz)
+                    continue;
+                const int a = ox + 2 * oy + 4 * oz;
+                for (int ch = 0; ch < 8; ++ch) {
+                    const double Ec =
+                        (double)Ef[((long)(2 * ei + (ch & 1)) * fy + (2 * ej + ((ch >> 1) & 1))) * fz + (2 * ek + (ch >> 2))];
+                    for (int c = 0; c < 3; ++c)
+                        acc[c] += Ec * Gdiag[ch * 24 + a * 3 + c];
+                }
+            }
+    const int mn = mask ? mask[n] : 0;
+    for (int c = 0; c < 3; ++c)
+        d[3 * n + c] = ((mn >> c) & 1) ? T(1) : (T)(1.0 / acc[c]);
+}
+
+// Build the coarse element matrices of level l+1 (grid gc) from level l:
+//   Kc = sum_{child} P_child^T K_child P_child
+// src_kind 0: children are level-1 elements given by fine moduli Ef and the
+// child tables G (8 x 576, K_child = sum_g E_g G_g); src_kind 1: children are
+// explicit element matrices Kf of the level below (grid 2x gc).
+// One block of 256 threads per coarse element; P weights are computed from
+// the trilinear rule.
+template <typename TE>
+__global__ __launch_bounds__(256) void gal_build(Grid gc, int src_kind, const double *__restrict__ G,
+                                               const TE *__restrict__ Ef, const double *__restrict__ Kf,
+                                               double *__restrict__ Kc)
+{
+    __shared__ double Kch[576];
+    __shared__ double Tm[576];
+    const long E = blockIdx.x;
+    if (E >= gc.n_el)
+        return;
+    const int ei = (int)(E / ((long)gc.nely * gc.nelz));
+    const int r0 = (int)(E % ((long)gc.nely * gc.nelz));
+    const int ej = r0 / gc.nelz, ek = r0 % gc.nelz;
+    // children live on the grid below: 2 nelx x 2 nely x 2 nelz elements
+    const int fnx = 2 * gc.nelx, fny = 2 * gc.nely, fnz = 2 * gc.nelz;
+    (void)fnx;
+    double acc[3] = {0.0, 0.0, 0.0};  // each thread owns up to 3 output entries (576 / 256)
+    for (int ch = 0; ch < 8; ++ch) {
+        const int cx = ch & 1, cy = (ch >> 1) & 1, cz = ch >> 2;
+        const long fe = ((long)(2 * ei + cx) * fny + (2 * ej + cy)) * fnz + (2 * ek + cz);
+        __syncthreads();
+        if (src_kind == 0) {
+            // level-1 child: K = sum over its 8 fine children g of E_g G_g
+            double Eg[8];
+            for (int gg = 0; gg < 8; ++gg) {
+                const int gx = gg & 1, gy = (gg >> 1) & 1, gz = gg >> 2;
+                Eg[gg] = (double)Ef[((long)(2 * (2 * ei + cx) + gx) * (2 * fny) + (2 * (2 * ej + cy) + gy)) * (2 * fnz) +
+                                    (2 * (2 * ek + cz) + gz)];
+            }
+            for (int q = threadIdx.x; q < 576; q += 256) {
+                double s = 0.0;
+                for (int gg = 0; gg < 8; ++gg)
+                    s += Eg[gg] * G[gg * 576 + q];
+                Kch[q] = s;
+            }
+        } else {
+            for (int q = threadIdx.x; q < 576; q += 256)
+                Kch[q] = Kf[fe * 576 + q];
+        }
+        __syncthreads();
+        // T = Kch * P_ch  (P_ch[(gamma, d), (beta, d')] = delta_dd' w(beta, ch + gamma))
+        for (int q = threadIdx.x; q < 576; q += 256) {
+            const int row = q / 24, col = q % 24;          // row: fine corner dof, col: coarse corner dof
+            const int beta = col / 3, dcol = col % 3;
+            double s = 0.0;
+            for (int gam = 0; gam < 8; ++gam) {
+                // weight of coarse corner beta at fine node ch + gam (per axis)
+                double wgt = 1.0;
+                for (int ax = 0; ax < 3; ++ax) {
+                    const int p = ((ch >> ax) & 1) + ((gam >> ax) & 1);  // 0, 1, 2
+                    const int bb = (beta >> ax) & 1;
+                    wgt *= (p == 1) ? 0.5 : ((p == 0) == (bb == 0) ? 1.0 : 0.0);
+                }
+                if (wgt != 0.0)
+                    s += Kch[row * 24 + gam * 3 + dcol] * wgt;
+            }
+            Tm[q] = s;
+        }
+        __syncthreads();
+        // acc += P_ch^T T
+        for (int qi = 0; qi < 3; ++qi) {
+            const int q = threadIdx.x + qi * 256;
+            if (q >= 576)
+                break;
+            const int row = q / 24, col = q % 24;          // row: coarse corner dof
+            const int beta = row / 3, drow = row % 3;
+            double s = 0.0;
+            for (int gam = 0; gam < 8; ++gam) {
+                double wgt = 1.0;
+                for (int ax = 0; ax < 3; ++ax) {
+                    const int p = ((ch >> ax) & 1) + ((gam >> ax) & 1);
+                    const int bb = (beta >> ax) & 1;
+                    wgt *= (p == 1) ? 0.5 : ((p == 0) == (bb == 0) ? 1.0 : 0.0);
+                }
+                if (wgt != 0.0)
+                    s += wgt * Tm[(gam * 3 + drow) * 24 + col];
+            }
+            acc[qi] += s;
+        }
+    }
+    for (int qi = 0; qi < 3; ++qi) {
+        const int q = threadIdx.x + qi * 256;
+        if (q < 576)
+            Kc[E * 576 + q] = acc[qi];
+    }
+}
+
+// Level-1 element matrices materialised from the fine moduli (used when the
+// coarsest level is level 1): K_E = sum_g E_g G_g
+template <typename TE>
+__global__ void elem_from_fine(Grid gc, const double *__restrict__ G, const TE *__restrict__ Ef,
+                               double *__restrict__ Ke)
+{
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= gc.n_el * 576)
+        return;
+    const long E = idx / 576;
+    const int q = (int)(idx % 576);
+    const int ei = (int)(E / ((long)gc.nely * gc.nelz));
+    const int r0 = (int)(E % ((long)gc.nely * gc.nelz));
+    const int ej = r0 / gc.nelz, ek = r0 % gc.nelz;
+    const int fy = 2 * gc.nely, fz = 2 * gc.nelz;
+    double s = 0.0;
+    for (int gg = 0; gg < 8; ++gg) {
+        const double Eg = (double)Ef[((long)(2 * ei + (gg & 1)) * fy + (2 * ej + ((gg >> 1) & 1))) * fz + (2 * ek + (gg >> 2))];
+        s += Eg * G[gg * 576 + q];
+    }
+    Ke[idx] = s;
+}
+
+// dense operator from explicit element matrices (one thread per row)
+__global__ void dense_from_elem(Grid g, const double *__restrict__ Ke, const uint8_t *__restrict__ mask,
+                                double *__restrict__ A)
+{
+    const long ndof = g.n_nodes * 3;
+    const long row = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= ndof)
+        return;
+    const long n = row / 3;
+    const int c = (int)(row % 3);
+    double *Ar = A + row * ndof;
+    const int mn = mask ? mask[n] : 0;
+    if ((mn >> c) & 1) {
+        Ar[row] = 1.0;
+        return;
+    }
+    const int k = (int)(n % g.nz);
+    const long t = n / g.nz;
+    const int i = (int)(t % g.nx), j = (int)(t / g.nx);
+    for (int ox = 1; ox >= 0; --ox)
+        for (int oy = 1; oy >= 0; --oy)
+            for (int oz = 1; oz >= 0; --oz) {
+                const int ei = i - ox, ej = j - oy, ek = k - oz;
+                if (ei < 0 || ei >= g.nelx || ej < 0 || ej >= g.nely || ek < 0 || ek >= g.nelz)
+                    continue;
+                const long e = ((long)ei * g.nely + ej) * g.nelz + ek;
+                const int a = ox + 2 * oy + 4 * oz;
+                for (int b = 0; b < 8; ++b) {
+                    const long m = node_of(g, ei, ej, ek, b);
+                    const int mm = mask ? mask[m] : 0;
+                    for (int d = 0; d < 3; ++d) {
+                        if ((mm >> d) & 1)
+                            continue;
+                        Ar[m * 3 + d] += Ke[e * 576 + (a * 3 + c) * 24 + b * 3 + d];
+                    }
+                }
+            }
+}
+
+static void gal1_grid(const Grid &g, int &tx, int &tz, int &cy)
+{
+    tx = (g.nx + GTOUT - 1) / GTOUT;
+    tz = (g.nz + GTOUT - 1) / GTOUT;
+    cy = (g.ny + GCHUNK - 1) / GCHUNK;
+}
+
+long gal1_partials(const Grid &g)
+{
+    int tx, tz, cy;
+    gal1_grid(g, tx, tz, cy);
+    return ((long)tx * tz * cy + 7) / 8 * 8;
+}
+
+template <typename T>
+static int gal1_launch(int mode, const Grid &g, const double *Btab, const T *Ef, const uint8_t *mask, const T *u,
+                       T *y, const T *b, const T *dinv, double omega, double *partials, hipStream_t s)
+{
+    int tx, tz, cy;
+    gal1_grid(g, tx, tz, cy);
+    const unsigned blocks = (unsigned)gal1_partials(g);
+    const T om = (T)omega;
+#define GAL_CASE(M)                                                                                        \
+    case M:                                                                                                \
+        hipLaunchKernelGGL((ku_gal1<T, M>), dim3(blocks), dim3(kGalBlock), 0, s, g, Btab, Ef, mask, u, y, b, dinv, \
+                           om, partials, tx, tz, cy);                                                      \
+        break;
+    switch (mode) {
+        GAL_CASE(MODE_KU)
+        GAL_CASE(MODE_JAC)
+        GAL_CASE(MODE_RES)
+        GAL_CASE(MODE_KUDOT)
+    default:
+        set_error("unknown mode %d", mode);
+        return kStatusBadArg;
+    }
+#undef GAL_CASE
+    FANTOM_LAUNCH_CHECK("ku_gal1");
+    return kStatusOk;
+}
+
+int launch_gal1(int mode, const Grid &g, const double *Btab, const double *Ef, const uint8_t *mask, const double *u,
+                double *y, const double *b, const double *dinv, double omega, double *partials, hipStream_t s)
+{
+    return gal1_launch<double>(mode, g, Btab, Ef, mask, u, y, b, dinv, omega, partials, s);
+}
+
+int launch_gal1_f32(int mode, const Grid &g, const double *Btab, const float *Ef, const uint8_t *mask, const float *u,
+                    float *y, const float *b, const float *dinv, double omega, double *partials, hipStream_t s)
+{
+    return gal1_launch<float>(mode, g, Btab, Ef, mask, u, y, b, dinv, omega, partials, s);
+}
+
+template <typename T>
+static int elem_launch(int mode, const Grid &g, const double *Ke, const uint8_t *mask, const T *u, T *y, const T *b,
+                       const T *dinv, double omega, double *partials, hipStream_t s)
+{
+    const unsigned blocks = (unsigned)((g.n_nodes + 255) / 256);
+    const T om = (T)omega;
+    switch (mode) {
+    case MODE_KU:
+        hipLaunchKernelGGL((ku_elem<T, MODE_KU>), dim3(blocks), dim3(256), 0, s, g, Ke, mask, u, y, b, dinv, om,
+                           partials);
+        break;
+    case MODE_JAC:
+        hipLaunchKernelGGL((ku_elem<T, MODE_JAC>), dim3(blocks), dim3(256), 0, s, g, Ke, mask, u, y, b, dinv, om,
+                           partials);
+        break;
+    case MODE_RES:
+        hipLaunchKernelGGL((ku_elem<T, MODE_RES>), dim3(blocks), dim3(256), 0, s, g, Ke, mask, u, y, b, dinv, om,
+                           partials);
+        break;
+    case MODE_KUDOT:
+        hipLaunchKernelGGL((ku_elem<T, MODE_KUDOT>), dim3(blocks), dim3(256), 0, s, g, Ke, mask, u, y, b, dinv, om,
+                           partials);
+        break;
+    default:
+        set_error("unknown mode %d", mode);
+        return kStatusBadArg;
+    }
+    FANTOM_LAUNCH_CHECK("ku_elem");
+    return kStatusOk;
+}
+
+int launch_elem(int mode, const Grid &g, const double *Ke, const uint8_t *mask, const double *u, double *y,
+                const double *b, const double *dinv, double omega, double *partials, hipStream_t s)
+{
+    return elem_launch<double>(mode, g, Ke, mask, u, y, b, dinv, omega, partials, s);
+}
+
+int launch_elem_f32(int mode, const Grid &g, const double *Ke, const uint8_t *mask, const float *u, float *y,
+                    const float *b, const float *dinv, double omega, double *partials, hipStream_t s)
+{
+    return elem_launch<float>(mode, g, Ke, mask, u, y, b, dinv, omega, partials, s);
+}
+
+long elem_partials(const Grid &g) { return (g.n_nodes + 255) / 256; }
+
+template <typename T>
+static int dinv_gal1_t(const Grid &g, const double *Gdiag, const T *Ef, const uint8_t *mask, T *d, hipStream_t s)
+{
+    hipLaunchKernelGGL(dinv_gal1<T>, dim3((unsigned)((g.n_nodes + 255) / 256)), dim3(256), 0, s, g, Gdiag, Ef, mask, d);
+    FANTOM_LAUNCH_CHECK("dinv_gal1");
+    return kStatusOk;
+}
+
+int launch_dinv_gal1(const Grid &g, const double *Gdiag, const double *Ef, const uint8_t *mask, double *d,
+                     hipStream_t s)
+{
+    return dinv_gal1_t<double>(g, Gdiag, Ef, mask, d, s);
+}
+
+int launch_dinv_gal1_f32(const Grid &g, const double *Gdiag, const float *Ef, const uint8_t *mask, float *d,
+                         hipStream_t s)
+{
+    return dinv_gal1_t<float>(g, Gdiag, Ef, mask, d, s);
+}
+
+int launch_dinv_elem(const Grid &g, const double *Ke, const uint8_t *mask, double *d, hipStream_t s)
+{
+    hipLaunchKernelGGL(dinv_elem<double>, dim3((unsigned)((g.n_nodes + 255) / 256)), dim3(256), 0, s, g, Ke, mask, d);
+    FANTOM_LAUNCH_CHECK("dinv_elem");
+    return kStatusOk;
+}
+
+int launch_dinv_elem_f32(const Grid &g, const double *Ke, const uint8_t *mask, float *d, hipStream_t s)
+{
+    hipLaunchKernelGGL(dinv_elem<float>, dim3((unsigned)((g.n_nodes + 255) / 256)), dim3(256), 0, s, g, Ke, mask, d);
+    FANTOM_LAUNCH_CHECK("dinv_elem");
+    return kStatusOk;
+}
+
+template <typename TE>
+static int gal_build_t(const Grid &gc, int src_kind, const double *G, const TE *Ef, const double *Kf, double *Kc,
+                       hipStream_t s)
+{
+    hipLaunchKernelGGL(gal_build<TE>, dim3((unsigned)gc.n_el), dim3(256), 0, s, gc, src_kind, G, Ef, Kf, Kc);
+    FANTOM_LAUNCH_CHECK("gal_build");
+    return kStatusOk;
+}
+
+int launch_gal_build(const Grid &gc, int src_kind, const double *G, const double *Ef, const double *Kf, double *Kc,
+                     hipStream_t s)
+{
+    return gal_build_t<double>(gc, src_kind, G, Ef, Kf, Kc, s);
+}
+
+int launch_gal_build_f32E(const Grid &gc, int src_kind, const double *G, const float *Ef, const double *Kf,
+                          double *Kc, hipStream_t s)
+{
+    return gal_build_t<float>(gc, src_kind, G, Ef, Kf, Kc, s);
+}
+
+int launch_elem_from_fine(const Grid &gc, const double *G, const double *Ef, double *Ke, hipStream_t s)
+{
+    const long n = gc.n_el * 576;
+    hipLaunchKernelGGL(elem_from_fine<double>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, gc, G, Ef, Ke);
+    FANTOM_LAUNCH_CHECK("elem_from_fine");
+    return kStatusOk;
+}
+
+int launch_elem_from_fine_f32E(const Grid &gc, const double *G, const float *Ef, double *Ke, hipStream_t s)
+{
+    const long n = gc.n_el * 576;
+    hipLaunchKernelGGL(elem_from_fine<float>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, gc, G, Ef, Ke);
+    FANTOM_LAUNCH_CHECK("elem_from_fine");
+    return kStatusOk;
+}
+
+int launch_dense_from_elem(const Grid &g, const double *Ke, const uint8_t *mask, double *A, hipStream_t s)
+{
+    const long ndof = g.n_nodes * 3;
+    hipError_t e = hipMemsetAsync(A, 0, sizeof(double) * ndof * ndof, s);
+    if (e != hipSuccess) {
+        set_error("dense_from_elem memset: %s", hipGetErrorString(e));
+        return kStatusHipError;
+    }
+    hipLaunchKernelGGL(dense_from_elem, dim3((unsigned)((ndof + 255) / 256)), dim3(256), 0, s, g, Ke, mask, A);
+    FANTOM_LAUNCH_CHECK("dense_from_elem");
+    return kStatusOk;
+}
+
+}  // namespace fantom
+
+using namespace fantom;
+
+// ---- host tables -----------------------------------------------------------------------
+extern "C" int fantom_galerkin_tables(const double *K0, double *G, double *Btab, double *Gdiag, double *check)
+{
+    FANTOM_CHECK_ARG(K0 && G && Btab && Gdiag, "null operand");
+    static const int loc[8] = {0, 1, 3, 2, 4, 5, 7, 6};  // natural corner -> reference local corner
+    double Kn[24][24];
+    for (int b = 0; b < 8; ++b)
+        for (int c = 0; c < 3; ++c)
+            for (int b2 = 0; b2 < 8; ++b2)
+                for (int d = 0; d < 3; ++d)
+                    Kn[b * 3 + c][b2 * 3 + d] = K0[(loc[b] * 3 + c) * 24 + loc[b2] * 3 + d];
+    // child matrices G_ch = P_ch^T Kn P_ch
+    for (int ch = 0; ch < 8; ++ch) {
+        double P[24][24] = {};
+        for (int gam = 0; gam < 8; ++gam)
+            for (int beta = 0; beta < 8; ++beta) {
+                double wgt = 1.0;
+                for (int ax = 0; ax < 3; ++ax) {
+                    const int p = ((ch >> ax) & 1) + ((gam >> ax) & 1);
+                    const int bb = (beta >> ax) & 1;
+                    wgt *= (p == 1) ? 0.5 : ((p == 0) == (bb == 0) ? 1.0 : 0.0);
+                }
+                for (int d = 0; d < 3; ++d)
+                    P[gam * 3 + d][beta * 3 + d] = wgt;
+            }
+        double T[24][24];
+        for (int i = 0; i < 24; ++i)
+            for (int j = 0; j < 24; ++j) {
+                double s = 0.0;
+                for (int k = 0; k < 24; ++k)
+                    s += Kn[i][k] * P[k][j];
+                T[i][j] = s;
+            }
+        for (int i = 0; i < 24; ++i)
+            for (int j = 0; j < 24; ++j) {
+                double s = 0.0;
+                for (int k = 0; k < 24; ++k)
+                    s += P[k][i] * T[k][j];
+                G[ch * 576 + i * 24 + j] = s;
+            }
+        for (int i = 0; i < 24; ++i)
+            Gdiag[ch * 24 + i] = G[ch * 576 + i * 24 + i];
+    }
+    // Ghat_0 = V^T G_0 V with orthonormal V, blocks scaled by 1/8
+    auto H = [](int b, int t) { return (__builtin_popcount(b & t) & 1) ? -1.0 : 1.0; };
+    double maxabs = 0.0, sym_err = 0.0, sum_err = 0.0;
+    for (int s = 0; s < 8; ++s)
+        for (int d = 0; d < 3; ++d)
+            for (int s2 = 0; s2 < 8; ++s2)
+                for (int d2 = 0; d2 < 3; ++d2) {
+                    const int t1 = s ^ (1 << d), t2 = s2 ^ (1 << d2);
+                    double acc = 0.0;
+                    for (int b = 0; b < 8; ++b)
+                        for (int b2 = 0; b2 < 8; ++b2)
+                            acc += H(b, t1) * G[(b * 3 + d) * 24 + b2 * 3 + d2] * H(b2, t2);
+                    acc /= 8.0;  // orthonormal V: (1/sqrt8)^2
+                    Btab[(s * 8 + s2) * 9 + d * 3 + d2] = acc / 8.0;
+                    maxabs = fmax(maxabs, fabs(acc));
+                }
+    // consistency: the child matrices are reflections of G_0
+    for (int ch = 0; ch < 8; ++ch)
+        for (int b = 0; b < 8; ++b)
+            for (int c = 0; c < 3; ++c)
+                for (int b2 = 0; b2 < 8; ++b2)
+                    for (int d = 0; d < 3; ++d) {
+                        const double sgn = (((ch >> c) & 1) ? -1.0 : 1.0) * (((ch >> d) & 1) ? -1.0 : 1.0);
+                        const double ref = sgn * G[((b ^ ch) * 3 + c) * 24 + (b2 ^ ch) * 3 + d];
+                        sym_err = fmax(sym_err, fabs(G[ch * 576 + (b * 3 + c) * 24 + b2 * 3 + d] - ref));
+                    }
+    // consistency: the children sum to the doubled element (3D hex scaling)
+    for (int i = 0; i < 576; ++i) {
+        double s = 0.0;
+        for (int ch = 0; ch < 8; ++ch)
+            s += G[ch * 576 + i];
+        sum_err = fmax(sum_err, fabs(s - 2.0 * Kn[i / 24][i % 24]));
+    }
+    if (check) {
+        check[0] = sym_err / fmax(maxabs, 1e-300);
+        check[1] = sum_err / fmax(maxabs, 1e-300);
+    }
+    if (sym_err > 1e-12 * maxabs || sum_err > 1e-12 * maxabs) {
+        set_error("K0 lacks the box-element symmetry the Galerkin tables need (%.2e, %.2e)", sym_err, sum_err);
+        return kStatusBadArg;
+    }
+    return kStatusOk;
+}
+
+extern "C" int fantom_gal1_ku_f64(int nelx, int nely, int nelz, const double *Btab, const double *Efine,
+                                  const uint8_t *mask, const double *u, double *y, void *stream)
+{
+    FANTOM_CHECK_ARG(grid_ok(nelx, nely, nelz) && nelz > 0 && Btab && Efine && u && y && u != y, "bad operands");
+    return launch_gal1(MODE_KU, make_grid(nelx, nely, nelz), Btab, Efine, mask, u, y, nullptr, nullptr, 0.0, nullptr,
+                       as_stream(stream));
+}
+
+extern "C" int fantom_gal_build_f64(int nelx, int nely, int nelz, int src_kind, const double *G, const double *Efine,
+                                    const double *Kfine, double *Kc, void *stream)
+{
+    FANTOM_CHECK_ARG(grid_ok(nelx, nely, nelz) && nelz > 0 && G && Kc, "bad operands");
+    FANTOM_CHECK_ARG((src_kind == 0 && Efine) || (src_kind == 1 && Kfine), "bad source");
+    return launch_gal_build(make_grid(nelx, nely, nelz), src_kind, G, Efine, Kfine, Kc, as_stream(stream));
+}
+
+extern "C" int fantom_elem_ku_f64(int nelx, int nely, int nelz, const double *Ke, const uint8_t *mask, const double *u,
+                                  double *y, void *stream)
+{
+    FANTOM_CHECK_ARG(grid_ok(nelx, nely, nelz) && nelz > 0 && Ke && u && y && u != y, "bad operands");
+    return launch_elem(MODE_KU, make_grid(nelx, nely, nelz), Ke, mask, u, y, nullptr, nullptr, 0.0, nullptr,
+                       as_stream(stream));
+}
+
+extern "C" int fantom_elem_from_fine_f64(int nelx, int nely, int nelz, const double *G, const double *Efine,
+                                         double *Ke, void *stream)
+{
+    FANTOM_CHECK_ARG(grid_ok(nelx, nely, nelz) && nelz > 0 && G && Efine && Ke, "bad operands");
+    return launch_elem_from_fine(make_grid(nelx, nely, nelz), G, Efine, Ke, as_stream(stream));
+}

@@ -74,9 +74,10 @@ __global__ __launch_bounds__(kTileBlock, 2) void ku3d_tile(Grid g, HexBlocks<T> 
     __shared__ T outs[24][kTileBlock];   // element outputs, SoA
 
     // XCD-aware tile order: the hardware deals workgroups to the 8 XCDs
-    // round-robin, so give each XCD a contiguous range of logical tiles.
-    const int nblocks = gridDim.x;
-    const int per_xcd = (nblocks + 7) / 8;
+    // round-robin, so give each XCD a contiguous range of logical tiles.  The
+    // grid is a multiple of 8 blocks, which makes this map a bijection;
+    // logical ids past the last tile do no work.
+    const int per_xcd = gridDim.x / 8;
     const int bid = blockIdx.x;
     int logical = (bid % 8) * per_xcd + bid / 8;
     double part = 0.0;
@@ -267,12 +268,16 @@ static void tile_grid(const Grid &g, int &tx, int &tz, int &cy)
     cy = (g.ny + CHUNK - 1) / CHUNK;
 }
 
-long ku_tile_partials(const Grid &g)
+// launch size: the tile count rounded up to a multiple of the 8 XCDs
+static unsigned tile_blocks(const Grid &g)
 {
     int tx, tz, cy;
     tile_grid(g, tx, tz, cy);
-    return (long)tx * tz * cy;
+    const long n = (long)tx * tz * cy;
+    return (unsigned)((n + 7) / 8 * 8);
 }
+
+long ku_tile_partials(const Grid &g) { return (long)tile_blocks(g); }
 
 template <typename T>
 int launch_ku_tile_t(int mode, const Grid &g, const double *Kb, const T *E, const uint8_t *mask, const T *u, T *y,
@@ -284,7 +289,7 @@ int launch_ku_tile_t(int mode, const Grid &g, const double *Kb, const T *E, cons
         B.b[i] = (T)Kb[i];
     int tx, tz, cy;
     tile_grid(g, tx, tz, cy);
-    const unsigned blocks = (unsigned)(tx * tz * cy);
+    const unsigned blocks = tile_blocks(g);
     const T om = (T)omega;
 #define FANTOM_TILE_CASE(M)                                                                                \
     case M:                                                                                                \

@@ -46,10 +46,17 @@ struct MgLevel {
     double K0dense[576];   // host copy of the scaled dense element matrix
     double *K0dense_d;     // device copy (diag / dense assembly / node kernel)
     double Kb[72];         // host block form (tile kernel)
+    double *Ke;            // explicit Galerkin element matrices (levels >= 2)
+    int kind;              // LK_* operator of this level
 };
 
+// operator kind of a level
+enum { LK_FINE = 0, LK_REDISC = 1, LK_GAL1 = 2, LK_ELEM = 3 };
+
 struct MgHandle {
-    int dim, dof, nlev, n_smooth, kernel;
+    int dim, dof, nlev, n_smooth, kernel, galerkin;
+    double *Gtab, *Btab, *Gdiag;     // device Galerkin tables (hex8)
+    double *Ke1;                     // level-1 element matrices when level 1 is the coarsest
     double omega;
     std::vector<MgLevel> lev;
     double *xs, *r, *z, *p, *q, *f;   // fine-level PCG vectors
@@ -139,6 +146,10 @@ static int lev_ku(MgHandle *h, int l, int mode, const double *u, double *y, cons
                   double omega, double *partials)
 {
     MgLevel &L = h->lev[l];
+    if (L.kind == LK_GAL1)
+        return launch_gal1(mode, L.g, h->Btab, h->lev[0].E, L.mask, u, y, b, dinv, omega, partials, h->stream);
+    if (L.kind == LK_ELEM)
+        return launch_elem(mode, L.g, L.Ke, L.mask, u, y, b, dinv, omega, partials, h->stream);
     if (h->kernel == FANTOM_KU_TILE && L.dim == 3)
         return launch_ku_tile(mode, L.g, L.Kb, L.E, L.mask, u, y, b, dinv, omega, partials, h->stream);
     return launch_ku_node(mode, L.g, L.dim, L.K0dense_d, L.E, L.mask, u, y, b, dinv, omega, partials, h->stream);
@@ -235,14 +246,17 @@ struct Carve {
     }
 };
 
-static void layout(MgHandle *h, int nelx, int nely, int nelz, int n_coarse, Carve &cv)
+static void layout(MgHandle *h, int nelx, int nely, int nelz, int n_coarse, int galerkin, Carve &cv)
 {
     h->dim = nelz > 0 ? 3 : 2;
     h->dof = h->dim;
     h->nlev = n_coarse + 1;
+    h->galerkin = (galerkin && h->dim == 3) ? 1 : 0;
     h->lev.resize(h->nlev);
     for (int l = 0; l < h->nlev; ++l) {
         MgLevel &L = h->lev[l];
+        L.kind = l == 0 ? LK_FINE : (!h->galerkin ? LK_REDISC : (l == 1 ? LK_GAL1 : LK_ELEM));
+        L.Ke = nullptr;
         L.g = make_grid(nelx >> l, nely >> l, nelz > 0 ? (nelz >> l) : 0);
         L.dim = h->dim;
         L.dof = h->dof;
@@ -256,6 +270,16 @@ static void layout(MgHandle *h, int nelx, int nely, int nelz, int n_coarse, Carv
         L.E = cv.take<double>(L.g.n_el);
         L.mask = cv.take<uint8_t>(L.g.n_nodes);
         L.K0dense_d = cv.take<double>(576);
+        if (L.kind == LK_ELEM)
+            L.Ke = cv.take<double>((size_t)L.g.n_el * 576);
+    }
+    h->Gtab = h->Btab = h->Gdiag = h->Ke1 = nullptr;
+    if (h->galerkin) {
+        h->Gtab = cv.take<double>(8 * 576);
+        h->Btab = cv.take<double>(576);
+        h->Gdiag = cv.take<double>(8 * 24);
+        if (h->nlev == 2)
+            h->Ke1 = cv.take<double>((size_t)h->lev[1].g.n_el * 576);
     }
     const long n = h->lev[0].n_dof;
     h->xs = cv.take<double>(n);
@@ -268,6 +292,11 @@ static void layout(MgHandle *h, int nelx, int nely, int nelz, int n_coarse, Carv
     np = std::max(np, ku_node_partials(h->lev[0].g));
     if (h->dim == 3)
         np = std::max(np, ku_tile_partials(h->lev[0].g));
+    for (int l = 1; l < h->nlev; ++l) {
+        np = std::max(np, elem_partials(h->lev[l].g));
+        if (h->dim == 3)
+            np = std::max(np, gal1_partials(h->lev[l].g));
+    }
     h->n_partials = np;
     h->partials = cv.take<double>(np);
     h->scal = cv.take<double>(FANTOM_SCALARS);
@@ -292,18 +321,18 @@ using namespace fantom;
 
 extern "C" {
 
-long fantom_mg_workspace_size(int nelx, int nely, int nelz, int n_coarse)
+long fantom_mg_workspace_size(int nelx, int nely, int nelz, int n_coarse, int galerkin)
 {
     if (!grid_ok(nelx, nely, nelz) || !levels_ok(nelx, nely, nelz, n_coarse))
         return -1;
     MgHandle h;
     Carve cv{nullptr, 0};
-    layout(&h, nelx, nely, nelz, n_coarse, cv);
+    layout(&h, nelx, nely, nelz, n_coarse, galerkin, cv);
     return (long)cv.off + 256;
 }
 
 int fantom_mg_create(int nelx, int nely, int nelz, int n_coarse, int n_smooth, double omega, int kernel,
-                     const double *K0_host, void *workspace, long workspace_bytes, void **handle)
+                     int galerkin, const double *K0_host, void *workspace, long workspace_bytes, void **handle)
 {
     FANTOM_CHECK_ARG(grid_ok(nelx, nely, nelz), "invalid grid %d x %d x %d", nelx, nely, nelz);
     FANTOM_CHECK_ARG(levels_ok(nelx, nely, nelz, n_coarse), "grid %d x %d x %d does not halve %d times", nelx, nely,
@@ -311,14 +340,14 @@ int fantom_mg_create(int nelx, int nely, int nelz, int n_coarse, int n_smooth, d
     FANTOM_CHECK_ARG(K0_host && workspace && handle, "null operand");
     FANTOM_CHECK_ARG(n_smooth >= 1 && n_smooth <= 16, "n_smooth must be in [1, 16]");
     FANTOM_CHECK_ARG(kernel == FANTOM_KU_NODE || kernel == FANTOM_KU_TILE, "unknown kernel %d", kernel);
-    const long need = fantom_mg_workspace_size(nelx, nely, nelz, n_coarse);
+    const long need = fantom_mg_workspace_size(nelx, nely, nelz, n_coarse, galerkin);
     FANTOM_CHECK_ARG(workspace_bytes >= need, "workspace of %ld bytes is smaller than the %ld needed",
                      workspace_bytes, need);
     MgHandle *h = new MgHandle();
     Carve cv{reinterpret_cast<char *>(workspace), 0};
     // align the base
     cv.off = (256 - (reinterpret_cast<uintptr_t>(workspace) & 255)) & 255;
-    layout(h, nelx, nely, nelz, n_coarse, cv);
+    layout(h, nelx, nely, nelz, n_coarse, galerkin, cv);
     FANTOM_CHECK_ARG(h->nc <= 20000, "coarsest grid of %d DOF is too large for the dense solve", h->nc);
     h->n_smooth = n_smooth;
     h->omega = omega;
@@ -349,6 +378,21 @@ int fantom_mg_create(int nelx, int nely, int nelz, int n_coarse, int n_smooth, d
         e = hipMemcpyAsync(L.K0dense_d, L.K0dense, sizeof(double) * nl * nl, hipMemcpyHostToDevice, h->stream);
         if (e != hipSuccess)
             break;
+    }
+    std::vector<double> tG(8 * 576), tB(576), tD(8 * 24);
+    if (h->galerkin) {
+        int st = fantom_galerkin_tables(K0_host, tG.data(), tB.data(), tD.data(), nullptr);
+        if (st) {
+            hipStreamDestroy(h->stream);
+            delete h;
+            return st;
+        }
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(h->Gtab, tG.data(), sizeof(double) * tG.size(), hipMemcpyHostToDevice, h->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(h->Btab, tB.data(), sizeof(double) * tB.size(), hipMemcpyHostToDevice, h->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(h->Gdiag, tD.data(), sizeof(double) * tD.size(), hipMemcpyHostToDevice, h->stream);
     }
     if (e == hipSuccess)
         e = hipStreamSynchronize(h->stream);
@@ -411,15 +455,38 @@ int fantom_mg_setup(void *handle, const double *E, const uint8_t *mask, void *st
     }
     for (int l = 1; l < h->nlev; ++l) {
         MgLevel &F = h->lev[l - 1], &C = h->lev[l];
-        MG_TRY(launch_coarsen_E(F.g, h->dim, F.E, C.E, h->stream));
+        if (!h->galerkin)
+            MG_TRY(launch_coarsen_E(F.g, h->dim, F.E, C.E, h->stream));
         MG_TRY(launch_coarsen_mask(F.g, h->dim, F.mask, C.mask, h->stream));
+    }
+    if (h->galerkin) {
+        // explicit element matrices from level 2 down
+        for (int l = 2; l < h->nlev; ++l) {
+            MgLevel &C = h->lev[l];
+            if (l == 2)
+                MG_TRY(launch_gal_build(C.g, 0, h->Gtab, L0.E, nullptr, C.Ke, h->stream));
+            else
+                MG_TRY(launch_gal_build(C.g, 1, h->Gtab, nullptr, h->lev[l - 1].Ke, C.Ke, h->stream));
+        }
+        if (h->nlev == 2)
+            MG_TRY(launch_elem_from_fine(h->lev[1].g, h->Gtab, L0.E, h->Ke1, h->stream));
     }
     for (int l = 0; l < h->nlev; ++l) {
         MgLevel &L = h->lev[l];
-        MG_TRY(launch_diag(L.g, h->dim, L.K0dense_d, L.E, L.mask, L.dinv, 1, h->stream));
+        if (L.kind == LK_GAL1)
+            MG_TRY(launch_dinv_gal1(L.g, h->Gdiag, L0.E, L.mask, L.dinv, h->stream));
+        else if (L.kind == LK_ELEM)
+            MG_TRY(launch_dinv_elem(L.g, L.Ke, L.mask, L.dinv, h->stream));
+        else
+            MG_TRY(launch_diag(L.g, h->dim, L.K0dense_d, L.E, L.mask, L.dinv, 1, h->stream));
     }
     MgLevel &C = h->lev[h->nlev - 1];
-    MG_TRY(launch_dense_assemble(C.g, h->dim, C.K0dense_d, C.E, C.mask, h->Ainv, h->stream));
+    if (C.kind == LK_ELEM)
+        MG_TRY(launch_dense_from_elem(C.g, C.Ke, C.mask, h->Ainv, h->stream));
+    else if (C.kind == LK_GAL1)
+        MG_TRY(launch_dense_from_elem(C.g, h->Ke1, C.mask, h->Ainv, h->stream));
+    else
+        MG_TRY(launch_dense_assemble(C.g, h->dim, C.K0dense_d, C.E, C.mask, h->Ainv, h->stream));
     MG_TRY(launch_spd_inverse(h->nc, h->Ainv, h->gj_work, h->stream));
     return join(h->stream, as_stream(stream));
 }

@@ -40,10 +40,13 @@ struct MxLevel {
     double K0dense[576];
     double *K0dense_d;
     double Kb[72];
+    double *Ke;   // explicit Galerkin element matrices (levels >= 2), FP64
+    int kind;     // 0 fine, 1 rediscretised, 2 Galerkin level 1, 3 element matrices
 };
 
 struct MxHandle {
-    int nlev, n_smooth;
+    int nlev, n_smooth, galerkin;
+    double *Gtab, *Btab, *Gdiag, *Ke1;
     double omega;
     std::vector<MxLevel> lev;
     // FP64 fine-level data
@@ -181,6 +184,17 @@ __global__ void mx_zero_f32(long n, float *__restrict__ a)
             return _s;                                                                                     \
     } while (0)
 
+static int mx_ku(MxHandle *h, int l, int mode, const float *u, float *y, const float *b, const float *dinv,
+                 double *partials)
+{
+    MxLevel &L = h->lev[l];
+    if (L.kind == 2)
+        return launch_gal1_f32(mode, L.g, h->Btab, h->lev[0].E, L.mask, u, y, b, dinv, h->omega, partials, h->stream);
+    if (L.kind == 3)
+        return launch_elem_f32(mode, L.g, L.Ke, L.mask, u, y, b, dinv, h->omega, partials, h->stream);
+    return launch_ku_tile_f32(mode, L.g, L.Kb, L.E, L.mask, u, y, b, dinv, h->omega, partials, h->stream);
+}
+
 static int mx_smooth(MxHandle *h, int l, const float *b, float **px, bool from_zero)
 {
     MxLevel &L = h->lev[l];
@@ -192,8 +206,7 @@ static int mx_smooth(MxHandle *h, int l, const float *b, float **px, bool from_z
                                (float)h->omega, L.dinv, b, x);
             continue;
         }
-        MX_TRY(launch_ku_tile_f32(MODE_JAC, L.g, L.Kb, L.E, L.mask, x, other, b, L.dinv, h->omega, nullptr,
-                                  h->stream));
+        MX_TRY(mx_ku(h, l, MODE_JAC, x, other, b, L.dinv, nullptr));
         float *t = x;
         x = other;
         other = t;
@@ -214,7 +227,7 @@ static int mx_cycle(MxHandle *h, int l, const float *b, float **out)
     float *x = L.x;
     MX_TRY(mx_smooth(h, l, b, &x, true));
     MxLevel &C = h->lev[l + 1];
-    MX_TRY(launch_ku_tile_f32(MODE_RES, L.g, L.Kb, L.E, L.mask, x, L.r, b, nullptr, 0.0, nullptr, h->stream));
+    MX_TRY(mx_ku(h, l, MODE_RES, x, L.r, b, nullptr, nullptr));
     MX_TRY(launch_restrict_f32(L.g, 3, 3, 0, C.mask, L.r, C.b, h->stream));
     float *e = nullptr;
     MX_TRY(mx_cycle(h, l + 1, C.b, &e));
@@ -238,8 +251,7 @@ static int mx_iteration(MxHandle *h)
     const long n = h->n;
     const unsigned G = vec_grid(n);
     MxLevel &L0 = h->lev[0];
-    MX_TRY(launch_ku_tile_f32(MODE_KUDOT, L0.g, L0.Kb, L0.E, L0.mask, h->p, h->q, nullptr, nullptr, 0.0,
-                              h->partials, h->stream));
+    MX_TRY(mx_ku(h, 0, MODE_KUDOT, h->p, h->q, nullptr, nullptr, h->partials));
     MX_TRY(launch_finalize(h->partials, ku_tile_partials(L0.g), h->scal, FIN_ALPHA, 0, h->stream));
     hipLaunchKernelGGL(mx_xr_k, dim3(G), dim3(256), 0, h->stream, n, h->d, h->r, h->p, h->q, h->scal, h->partials);
     hipLaunchKernelGGL(mx_conv_k, dim3(1), dim3(1024), 0, h->stream, h->partials, (long)G, h->scal);
@@ -264,9 +276,10 @@ struct MxCarve {
     }
 };
 
-static void mx_layout(MxHandle *h, int nelx, int nely, int nelz, int n_coarse, MxCarve &cv)
+static void mx_layout(MxHandle *h, int nelx, int nely, int nelz, int n_coarse, int galerkin, MxCarve &cv)
 {
     h->nlev = n_coarse + 1;
+    h->galerkin = galerkin ? 1 : 0;
     h->lev.resize(h->nlev);
     h->g = make_grid(nelx, nely, nelz);
     h->n = h->g.n_nodes * 3;
@@ -283,6 +296,16 @@ static void mx_layout(MxHandle *h, int nelx, int nely, int nelz, int n_coarse, M
         L.E = cv.take<float>(L.g.n_el);
         L.mask = cv.take<uint8_t>(L.g.n_nodes);
         L.K0dense_d = cv.take<double>(576);
+        L.kind = l == 0 ? 0 : (!h->galerkin ? 1 : (l == 1 ? 2 : 3));
+        L.Ke = L.kind == 3 ? cv.take<double>((size_t)L.g.n_el * 576) : nullptr;
+    }
+    h->Gtab = h->Btab = h->Gdiag = h->Ke1 = nullptr;
+    if (h->galerkin) {
+        h->Gtab = cv.take<double>(8 * 576);
+        h->Btab = cv.take<double>(576);
+        h->Gdiag = cv.take<double>(8 * 24);
+        if (h->nlev == 2)
+            h->Ke1 = cv.take<double>((size_t)h->lev[1].g.n_el * 576);
     }
     h->E64 = cv.take<double>(h->g.n_el);
     h->x64 = cv.take<double>(h->n);
@@ -295,6 +318,8 @@ static void mx_layout(MxHandle *h, int nelx, int nely, int nelz, int n_coarse, M
     h->p = cv.take<float>(h->n);
     h->q = cv.take<float>(h->n);
     long np = std::max((long)kVecMaxBlocks, ku_tile_partials(h->g));
+    for (int l = 1; l < h->nlev; ++l)
+        np = std::max(np, std::max(elem_partials(h->lev[l].g), gal1_partials(h->lev[l].g)));
     h->partials = cv.take<double>(np);
     h->scal = cv.take<double>(FANTOM_SCALARS);
     h->nc = (int)h->lev[h->nlev - 1].n_dof;
@@ -336,7 +361,7 @@ using namespace fantom;
 
 extern "C" {
 
-long fantom_mx_workspace_size(int nelx, int nely, int nelz, int n_coarse)
+long fantom_mx_workspace_size(int nelx, int nely, int nelz, int n_coarse, int galerkin)
 {
     if (!grid_ok(nelx, nely, nelz) || nelz <= 0 || n_coarse < 0 || n_coarse > 10)
         return -1;
@@ -345,14 +370,14 @@ long fantom_mx_workspace_size(int nelx, int nely, int nelz, int n_coarse)
         return -1;
     MxHandle h;
     MxCarve cv{nullptr, 0};
-    mx_layout(&h, nelx, nely, nelz, n_coarse, cv);
+    mx_layout(&h, nelx, nely, nelz, n_coarse, galerkin, cv);
     return (long)cv.off + 256;
 }
 
-int fantom_mx_create(int nelx, int nely, int nelz, int n_coarse, int n_smooth, double omega, const double *K0_host,
-                     void *workspace, long workspace_bytes, void **handle)
+int fantom_mx_create(int nelx, int nely, int nelz, int n_coarse, int n_smooth, double omega, int galerkin,
+                     const double *K0_host, void *workspace, long workspace_bytes, void **handle)
 {
-    const long need = fantom_mx_workspace_size(nelx, nely, nelz, n_coarse);
+    const long need = fantom_mx_workspace_size(nelx, nely, nelz, n_coarse, galerkin);
     FANTOM_CHECK_ARG(need > 0, "grid %d x %d x %d (3D only) does not halve %d times", nelx, nely, nelz, n_coarse);
     FANTOM_CHECK_ARG(K0_host && workspace && handle, "null operand");
     FANTOM_CHECK_ARG(workspace_bytes >= need, "workspace of %ld bytes is smaller than the %ld needed",
@@ -361,7 +386,7 @@ int fantom_mx_create(int nelx, int nely, int nelz, int n_coarse, int n_smooth, d
     MxHandle *h = new MxHandle();
     MxCarve cv{reinterpret_cast<char *>(workspace), 0};
     cv.off = (256 - (reinterpret_cast<uintptr_t>(workspace) & 255)) & 255;
-    mx_layout(h, nelx, nely, nelz, n_coarse, cv);
+    mx_layout(h, nelx, nely, nelz, n_coarse, galerkin, cv);
     if (h->nc > 20000) {
         delete h;
         set_error("coarsest grid too large for the dense solve");
@@ -383,10 +408,24 @@ int fantom_mx_create(int nelx, int nely, int nelz, int n_coarse, int n_smooth, d
         for (int i = 0; i < 72; ++i)
             L.Kb[i] = h->Kb64[i] * L.scale;
     }
+    std::vector<double> tG(8 * 576), tB(576), tD(8 * 24);
+    if (h->galerkin) {
+        st = fantom_galerkin_tables(K0_host, tG.data(), tB.data(), tD.data(), nullptr);
+        if (st) {
+            delete h;
+            return st;
+        }
+    }
     hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
     for (int l = 0; l < h->nlev && e == hipSuccess; ++l)
         e = hipMemcpyAsync(h->lev[l].K0dense_d, h->lev[l].K0dense, sizeof(double) * 576, hipMemcpyHostToDevice,
                            h->stream);
+    if (h->galerkin && e == hipSuccess)
+        e = hipMemcpyAsync(h->Gtab, tG.data(), sizeof(double) * tG.size(), hipMemcpyHostToDevice, h->stream);
+    if (h->galerkin && e == hipSuccess)
+        e = hipMemcpyAsync(h->Btab, tB.data(), sizeof(double) * tB.size(), hipMemcpyHostToDevice, h->stream);
+    if (h->galerkin && e == hipSuccess)
+        e = hipMemcpyAsync(h->Gdiag, tD.data(), sizeof(double) * tD.size(), hipMemcpyHostToDevice, h->stream);
     if (e == hipSuccess)
         e = hipStreamSynchronize(h->stream);
     if (e != hipSuccess) {
@@ -434,17 +473,38 @@ int fantom_mx_setup(void *handle, const double *E, const uint8_t *mask, void *st
     }
     for (int l = 1; l < h->nlev; ++l) {
         MxLevel &F = h->lev[l - 1], &C = h->lev[l];
-        MX_TRY(launch_coarsen_E_f32(F.g, 3, F.E, C.E, s));
+        if (!h->galerkin)
+            MX_TRY(launch_coarsen_E_f32(F.g, 3, F.E, C.E, s));
         MX_TRY(launch_coarsen_mask(F.g, 3, F.mask, C.mask, s));
+    }
+    if (h->galerkin) {
+        for (int l = 2; l < h->nlev; ++l) {
+            MxLevel &C = h->lev[l];
+            if (l == 2)
+                MX_TRY(launch_gal_build(C.g, 0, h->Gtab, h->E64, nullptr, C.Ke, s));
+            else
+                MX_TRY(launch_gal_build(C.g, 1, h->Gtab, nullptr, h->lev[l - 1].Ke, C.Ke, s));
+        }
+        if (h->nlev == 2)
+            MX_TRY(launch_elem_from_fine(h->lev[1].g, h->Gtab, h->E64, h->Ke1, s));
     }
     for (int l = 0; l < h->nlev; ++l) {
         MxLevel &L = h->lev[l];
-        MX_TRY(launch_diag_f32(L.g, L.K0dense_d, L.E, L.mask, L.dinv, s));
+        if (L.kind == 2)
+            MX_TRY(launch_dinv_gal1_f32(L.g, h->Gdiag, h->lev[0].E, L.mask, L.dinv, s));
+        else if (L.kind == 3)
+            MX_TRY(launch_dinv_elem_f32(L.g, L.Ke, L.mask, L.dinv, s));
+        else
+            MX_TRY(launch_diag_f32(L.g, L.K0dense_d, L.E, L.mask, L.dinv, s));
     }
-    // the coarsest operator and its inverse stay FP64 (assembled from the FP32
-    // moduli widened to FP64)
+    // the coarsest operator and its inverse stay FP64
     MxLevel &C = h->lev[h->nlev - 1];
-    MX_TRY(launch_dense_assemble_f32E(C.g, C.K0dense_d, C.E, C.mask, h->Ainv, s));
+    if (C.kind == 3)
+        MX_TRY(launch_dense_from_elem(C.g, C.Ke, C.mask, h->Ainv, s));
+    else if (C.kind == 2)
+        MX_TRY(launch_dense_from_elem(C.g, h->Ke1, C.mask, h->Ainv, s));
+    else
+        MX_TRY(launch_dense_assemble_f32E(C.g, C.K0dense_d, C.E, C.mask, h->Ainv, s));
     MX_TRY(launch_spd_inverse(h->nc, h->Ainv, h->gj_work, s));
     FANTOM_LAUNCH_CHECK("mixed setup");
     return mx_join(s, as_stream(stream));

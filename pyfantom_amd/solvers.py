@@ -23,9 +23,12 @@ Deliberate differences from the reference (each keeps the solution K^-1 f):
 """
 from __future__ import annotations
 
+import atexit
 import logging
 import math
+import sys
 import warnings
+import weakref
 
 import numpy as np
 import torch
@@ -33,6 +36,19 @@ import torch
 from . import _lib
 
 logger = logging.getLogger(__name__)
+
+# native solver handles still alive; released before interpreter shutdown so
+# that no HIP call runs after the runtime has been torn down
+_LIVE = weakref.WeakSet()
+
+
+@atexit.register
+def _release_native_handles():
+    for s in list(_LIVE):
+        try:
+            s._free_native()
+        except Exception:
+            pass
 
 
 class _Level:
@@ -113,7 +129,7 @@ class MultiGrid(_DeviceKrylov):
                  w_level=1, coarse_solver="cholmod", matrix_free=False, low_level_tol=1e-8,
                  low_level_maxiter=5000, min_omega=0.4, omega_boost=1.06, omega_schedule="fixed",
                  check_every=1, native=True, use_graph=True, precision="fp64", inner_tol=1e-4,
-                 max_outer=20):
+                 max_outer=20, coarse=None):
         if coarse_solver not in ["cg", "gmres", "splu", "spsolve", "cholmod"]:
             raise ValueError("Coarse solver not recognized.")
         if coarse_solver in ["splu", "spsolve"] and matrix_free:
@@ -153,6 +169,13 @@ class MultiGrid(_DeviceKrylov):
         if precision == "mixed" and not self.native:
             raise ValueError("the mixed-precision solve runs in the native driver (fixed omega, V-cycle)")
         self.precision = precision
+        if coarse is None:
+            coarse = "galerkin" if (self.native and len(mesh.nel) == 3) else "rediscretized"
+        if coarse not in ("galerkin", "rediscretized"):
+            raise ValueError("coarse must be 'galerkin' or 'rediscretized'")
+        if coarse == "galerkin" and not self.native:
+            raise ValueError("Galerkin coarse operators run in the native driver")
+        self.coarse = coarse
         self.inner_tol = float(inner_tol)
         self.max_outer = int(max_outer)
         self.last_outer_iterations = 0
@@ -169,6 +192,8 @@ class MultiGrid(_DeviceKrylov):
         self._free_native()
 
     def __del__(self):
+        if sys.is_finalizing():
+            return
         try:
             self._free_native()
         except Exception:
@@ -209,18 +234,20 @@ class MultiGrid(_DeviceKrylov):
         import ctypes
         k = self.kernel
         L = _lib.load()
-        key = (k.grid, self.n_level, self.n_smooth, self.omega, k.kernel_kind, self.precision)
+        gal = int(self.coarse == "galerkin")
+        key = (k.grid, self.n_level, self.n_smooth, self.omega, k.kernel_kind, gal, self.precision)
         if self._nh is None or self._nh_key != key:
             self._free_native()
             n_coarse = self._n_coarse()
-            nbytes = L.fantom_mg_workspace_size(*k.grid, n_coarse)
+            nbytes = L.fantom_mg_workspace_size(*k.grid, n_coarse, gal)
             self._ws = torch.empty(nbytes, dtype=torch.uint8, device=k.device)
             h = ctypes.c_void_p()
             K0 = np.ascontiguousarray(k.K_single, dtype=np.float64)
             kid = _lib.KU_TILE if k.kernel_kind == "tile" else _lib.KU_NODE
-            _lib.check(L.fantom_mg_create(*k.grid, n_coarse, self.n_smooth, self.omega, kid, K0.ctypes.data,
+            _lib.check(L.fantom_mg_create(*k.grid, n_coarse, self.n_smooth, self.omega, kid, gal, K0.ctypes.data,
                                           self._ws.data_ptr(), nbytes, ctypes.byref(h)))
             self._nh, self._nh_key, self._nh_state = h, key, None
+            _LIVE.add(self)
         state = (k.rho_version, k.con_version, k.rho.data_ptr())
         if self._nh_state != state:
             _lib.call("fantom_mg_setup", self._nh, _lib.ptr(k.rho), _lib.ptr(k.mask) if k.has_con else None)
@@ -233,28 +260,30 @@ class MultiGrid(_DeviceKrylov):
         mixed = self.precision == "mixed"
         if mixed and k.dim != 3:
             raise NotImplementedError("the mixed-precision solve is implemented for hex8 grids")
-        key = (k.grid, self.n_level, self.n_smooth, self.omega, k.kernel_kind, self.precision)
+        gal = int(self.coarse == "galerkin")
+        key = (k.grid, self.n_level, self.n_smooth, self.omega, k.kernel_kind, gal, self.precision)
         pre = "fantom_mx_" if mixed else "fantom_mg_"
         if self._nh is None or self._nh_key != key:
             self._free_native()
             n_coarse = self._n_coarse()
-            nbytes = getattr(L, pre + "workspace_size")(*k.grid, n_coarse)
+            nbytes = getattr(L, pre + "workspace_size")(*k.grid, n_coarse, gal)
             if nbytes <= 0:
                 raise ValueError("grid does not support the requested hierarchy")
             self._ws = torch.empty(nbytes, dtype=torch.uint8, device=k.device)
             h = ctypes.c_void_p()
             K0 = np.ascontiguousarray(k.K_single, dtype=np.float64)
             if mixed:
-                _lib.check(L.fantom_mx_create(*k.grid, n_coarse, self.n_smooth, self.omega, K0.ctypes.data,
+                _lib.check(L.fantom_mx_create(*k.grid, n_coarse, self.n_smooth, self.omega, gal, K0.ctypes.data,
                                               self._ws.data_ptr(), nbytes, ctypes.byref(h)))
             else:
                 kid = _lib.KU_TILE if k.kernel_kind == "tile" else _lib.KU_NODE
-                _lib.check(L.fantom_mg_create(*k.grid, n_coarse, self.n_smooth, self.omega, kid, K0.ctypes.data,
-                                              self._ws.data_ptr(), nbytes, ctypes.byref(h)))
+                _lib.check(L.fantom_mg_create(*k.grid, n_coarse, self.n_smooth, self.omega, kid, gal,
+                                              K0.ctypes.data, self._ws.data_ptr(), nbytes, ctypes.byref(h)))
             self._nh = h
             self._nh_key = key
             self._nh_state = None
             self._n_levels = n_coarse + 1
+            _LIVE.add(self)
         state = (k.rho_version, k.con_version, k.rho.data_ptr())
         if self._nh_state != state:
             _lib.call(pre + "setup", self._nh, _lib.ptr(k.rho), _lib.ptr(k.mask) if k.has_con else None)

@@ -32,7 +32,7 @@ def test_binding_table_matches_header():
 
 def test_abi_version():
     from pyfantom_amd import _lib
-    assert _lib.load().fantom_abi_version() == 1
+    assert _lib.load().fantom_abi_version() == 2
 
 
 def test_hex8_blocks_reconstruct(oracle):
@@ -77,3 +77,79 @@ def test_product_filter_taps_match_oracle(oracle):
         o1, w1 = filter_taps(dim, r, sp)
         o2, w2 = oracle.filter_taps(dim, r, sp)
         assert np.array_equal(o1, o2) and np.array_equal(w1, w2)
+
+
+def _galerkin_tables(K0):
+    import ctypes
+    from pyfantom_amd import _lib
+    G = np.zeros(8 * 576)
+    B = np.zeros(576)
+    Gd = np.zeros(8 * 24)
+    chk = np.zeros(2)
+    _lib.check(_lib.load().fantom_galerkin_tables(K0.ctypes.data, G.ctypes.data, B.ctypes.data, Gd.ctypes.data,
+                                                  chk.ctypes.data))
+    return G.reshape(8, 24, 24), B.reshape(8, 8, 3, 3), Gd.reshape(8, 24), chk
+
+
+def _natural(K0):
+    loc = [0, 1, 3, 2, 4, 5, 7, 6]
+    idx = np.array([3 * loc[b] + c for b in range(8) for c in range(3)])
+    return K0[np.ix_(idx, idx)]
+
+
+def test_galerkin_tables_vs_sparse_product(oracle):
+    """Child matrices G_c = P_c^T K0 P_c equal the scipy Galerkin product on a
+    single coarse element (2x2x2 fine elements, explicit trilinear P); the
+    Walsh-Hadamard block form reproduces K_E u for random child moduli."""
+    K0 = np.ascontiguousarray(oracle.k0_exact(3, 0.3))
+    G, B, Gd, chk = _galerkin_tables(K0)
+    assert chk[0] < 1e-13 and chk[1] < 1e-13
+    # Galerkin on one coarse element with random child moduli, assembled
+    # with scipy from the fine mesh and the explicit (unnormalised) P
+    rng = np.random.default_rng(0)
+    Ec = rng.uniform(0.01, 1, 8)
+    nel = (2, 2, 2)
+    el = oracle.elements(nel)
+    # fine element index (x slowest) of child c = cx + 2cy + 4cz
+    Ef = np.zeros(8)
+    for c in range(8):
+        cx, cy, cz = c & 1, (c >> 1) & 1, c >> 2
+        Ef[(cx * 2 + cy) * 2 + cz] = Ec[c]
+    A = oracle.assemble(K0, el, Ef, 27 * 3)
+    P = oracle.transfer_matrix(nel, 3, False).T.toarray()
+    Ac = P.T @ A.toarray() @ P          # coarse element matrix in coarse node order
+    # coarse nodes of a 1x1x1 grid (reference numbering) -> natural corners
+    Ac_nat = _natural_from_nodes(Ac)
+    KE = np.einsum("c,cij->ij", Ec, G)
+    assert np.abs(KE - Ac_nat).max() <= 1e-13 * np.abs(Ac_nat).max()
+    # diagonal table
+    assert np.allclose(Gd, np.einsum("cii->ci", G))
+    # Walsh-Hadamard evaluation used by the level-1 kernel
+    u = rng.standard_normal(24)
+    H = np.array([[(-1) ** bin(b & t).count("1") for t in range(8)] for b in range(8)], float)
+    w = [H.T @ u[d::3] for d in range(3)]            # w_d[t] = sum_b H[b,t] u[b,d]
+    eh = H.T @ Ec
+    z = np.zeros((3, 8))
+    for s in range(8):
+        uh = np.array([w[0][s ^ 1], w[1][s ^ 2], w[2][s ^ 4]])
+        acc = np.zeros(3)
+        for s2 in range(8):
+            uh2 = np.array([w[0][s2 ^ 1], w[1][s2 ^ 2], w[2][s2 ^ 4]])
+            acc += eh[s ^ s2] * (B[s, s2] @ uh2)
+        z[0][s ^ 1], z[1][s ^ 2], z[2][s ^ 4] = acc
+    y = np.zeros(24)
+    for d in range(3):
+        y[d::3] = H @ z[d]
+    assert np.abs(y - KE @ u).max() <= 1e-13 * np.abs(KE @ u).max()
+
+
+def _natural_from_nodes(A):
+    """Reorder a 1x1x1-element coarse matrix from node numbering
+    n = k + 2 i + 4 j (y slowest) to natural corners b = i + 2 j + 4 k."""
+    perm = []
+    for b in range(8):
+        i, j, k = b & 1, (b >> 1) & 1, b >> 2
+        n = k + 2 * i + 4 * j
+        perm += [3 * n, 3 * n + 1, 3 * n + 2]
+    perm = np.array(perm)
+    return A[np.ix_(perm, perm)]

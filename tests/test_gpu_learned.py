@@ -68,7 +68,7 @@ def test_network_vs_oracle():
         assert np.linalg.norm(ab - gBo[name]) <= 3e-2 * max(np.linalg.norm(gBo[name]), 1e-30)
 
 
-def test_training_and_fpcg():
+def test_training_and_fpcg(oracle):
     """Train briefly on (K e, e) pairs of a 32x16x16 cantilever and use the
     network in flexible PCG: it must converge to the FP64 GMG solution, alone
     (through the fallback if it stalls) and as a correction after a V-cycle."""
@@ -82,7 +82,8 @@ def test_training_and_fpcg():
     mesh, K, solver, FE = cantilever(nel, tol=1e-8, n_level=3)
     rng = np.random.default_rng(0)
     xd = rng.uniform(0.01, 1, mesh.n_elements)
-    E = torch.from_numpy(1e-9 + xd ** 3 * (1 - 1e-9)).cuda()
+    from tests.test_gpu_solver import filtered_field
+    E = torch.from_numpy(filtered_field(oracle, nel)).cuda()
     K.set_rho(E)
     net = ConvNet(seed=0)
     run = NetRunner(net, K.grid)

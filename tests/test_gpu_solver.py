@@ -39,6 +39,15 @@ def cantilever(nel, nu=0.3, n_level=4, tol=1e-8, n_smooth=2, omega=0.6, kernel=N
     return mesh, K, solver, FE
 
 
+def filtered_field(oracle, nel, seed=0, r=1.5):
+    """Design variables x ~ U[0.01, 1] (seeded) passed through the density
+    filter and the modified SIMP law, as the SIMP pipeline produces moduli."""
+    x = np.random.default_rng(seed).uniform(0.01, 1.0, int(np.prod(nel)))
+    offs, w = oracle.filter_taps(3, r)
+    rho = oracle.filter_fwd(nel, offs, w, x)
+    return 1e-9 + rho ** 3 * (1 - 1e-9)
+
+
 def oracle_cantilever_inputs(nel):
     nx, ny, nz = (n + 1 for n in nel)
     n_nodes = nx * ny * nz
@@ -95,8 +104,7 @@ def test_cantilever_vs_oracle_solver(oracle):
     nel = (64, 32, 32)
     print("cantilever vs oracle", flush=True)
     mesh, K, solver, FE = cantilever(nel, tol=1e-12, n_level=3)
-    x = np.random.default_rng(0).uniform(0.01, 1.0, mesh.n_elements)
-    E = 1e-9 + x ** 3 * (1 - 1e-9)
+    E = filtered_field(oracle, nel)
     U, res = FE.solve(torch.from_numpy(E).cuda())
     con, f = oracle_cantilever_inputs(nel)
     uo, it, ro = oracle.mgpcg_solve(nel, oracle.k0_exact(3, 0.3), E, con, f, n_level=3, n_smooth=2, omega=0.6,
@@ -114,8 +122,7 @@ def test_cantilever_full_size_properties(oracle):
     nel = (128, 64, 64)
     print("cantilever full size", flush=True)
     mesh, K, solver, FE = cantilever(nel, tol=1e-8, n_level=4)
-    x = np.random.default_rng(0).uniform(0.01, 1.0, mesh.n_elements)
-    E = torch.from_numpy(1e-9 + x ** 3 * (1 - 1e-9)).cuda()
+    E = torch.from_numpy(filtered_field(oracle, nel)).cuda()
     U, res = FE.solve(E)
     assert res < 1e-8 * 1.5
     c1 = float(torch.dot(FE.rhs, U))
@@ -196,10 +203,12 @@ def test_native_graph_matches_eager_and_python(oracle):
     print("native graph vs eager", flush=True)
     mesh, K, _, FE = cantilever(nel, tol=1e-10, n_level=3)
     mk = lambda **kw: pf.MultiGrid(mesh, K, maxiter=500, tol=1e-10, n_smooth=2, omega=0.6, n_level=3, cycle="V", **kw)
-    s_graph, s_eager, s_py = mk(use_graph=True), mk(use_graph=False), mk(native=False)
+    s_graph, s_eager = mk(use_graph=True), mk(use_graph=False)
+    s_py = mk(native=False)
+    s_nat_redisc = mk(coarse="rediscretized")
     rng = np.random.default_rng(7)
     for trial in range(3):
-        E = torch.from_numpy(rng.uniform(0.01, 1, mesh.n_elements) ** 3).cuda()
+        E = torch.from_numpy(filtered_field(oracle, nel, seed=trial)).cuda()
         K.set_rho(E)
         junk = torch.randn(1 << 20, device="cuda", dtype=torch.float64)
         junk.mul_(2.0).sum().item()
@@ -208,10 +217,17 @@ def test_native_graph_matches_eager_and_python(oracle):
         xe, re = s_eager.solve(FE.rhs, use_last=False)
         xe = xe.clone()
         xp, rp = s_py.solve(FE.rhs, use_last=False)
+        xp = xp.clone()
+        xr, rr = s_nat_redisc.solve(FE.rhs, use_last=False)
+        print(f"trial {trial}: galerkin {s_graph.last_iterations} it, rediscretised native "
+              f"{s_nat_redisc.last_iterations} it, python {s_py.last_iterations} it", flush=True)
         assert s_graph.last_iterations == s_eager.last_iterations
         assert torch.equal(xg, xe), trial
-        assert float((xg - xp).abs().max()) <= 1e-12 * float(xp.abs().max())
-        assert rg < 1e-9 and rp < 1e-9
+        # the Python driver runs the rediscretised hierarchy: same kernels, same iterates
+        assert s_py.last_iterations == s_nat_redisc.last_iterations
+        assert float((xr - xp).abs().max()) <= 1e-12 * float(xp.abs().max())
+        assert float((xg - xp).abs().max()) <= 1e-8 * float(xp.abs().max())
+        assert rg < 1e-9 and rp < 1e-9 and rr < 1e-9
 
 
 @pytest.mark.parametrize("field", ["seeded", "void_contrast"])
@@ -224,12 +240,13 @@ def test_mixed_precision_refinement(oracle, field):
     nel = (64, 32, 32)
     print(f"mixed precision {field}", flush=True)
     mesh, K, _, FE = cantilever(nel, tol=1e-12, n_level=4)
-    rng = np.random.default_rng(0)
-    x = rng.uniform(0.01, 1.0, mesh.n_elements)
+    Ef = filtered_field(oracle, nel)
     if field == "void_contrast":
-        # half the design void (x -> Emin) in random blocks: 1e9 modulus contrast
-        x = np.where(rng.uniform(0, 1, mesh.n_elements) < 0.5, 0.0, x)
-    E = torch.from_numpy(1e-9 + x ** 3 * (1 - 1e-9)).cuda()
+        # void blocks (E = Emin) over half of the domain: 1e9 modulus contrast
+        blk = np.random.default_rng(3).uniform(0, 1, (nel[0] // 8, nel[1] // 8, nel[2] // 8)) < 0.5
+        void = np.kron(blk, np.ones((8, 8, 8), bool)).ravel()
+        Ef = np.where(void, 1e-9, Ef)
+    E = torch.from_numpy(Ef).cuda()
     K.set_rho(E)
     ref = pf.MultiGrid(mesh, K, maxiter=2000, tol=1e-12, n_smooth=2, omega=0.6, n_level=4, cycle="V")
     u64, r64 = ref.solve(FE.rhs, use_last=False)
