@@ -1,0 +1,263 @@
+// Tile skeleton shared by the matrix-free hex8 operators (fine K(E)u in
+// ku_tile.hip, level-1 Galerkin operator in galerkin.hip).
+//
+// Both operators act on an element in the Walsh-Hadamard (WHT) basis of its
+// eight corners: w[c][s] = sum_b (-1)^popc(b & s) u[b][c] with the natural
+// corner index b = bx + 2 by + 4 bz.  The transform is a tensor product of
+// 2-point butterflies along z, x and y, and the skeleton exploits that the
+// first two stages are shared between neighbouring elements:
+//
+//   * a workgroup of 16 x 16 threads owns a 16 (x) x 16 (z) block of node
+//     columns and walks a chunk of node planes along y (the slowest node
+//     axis).  Lane layout: lz = lane % 16 (one DPP row), lx = tid / 16.
+//   * per plane, each thread loads its own node (3 doubles + mask byte),
+//     takes the z-butterfly with the node at lz + 1 by a DPP row shift and the
+//     x-butterfly with the column lx + 1 through LDS: the "face transform"
+//     F[c][sx + 2 sz] of the x-z face whose low corner is the thread's node.
+//     A face transform is used by the element layers below and above it.
+//   * per element layer the y-butterfly of the two face transforms gives the
+//     element's 24 coefficients, the operator maps them to 24 output
+//     coefficients, and the inverse y-butterfly splits them into the lower
+//     (this plane) and upper (next plane) face parts; the upper part waits in
+//     registers for the next layer.
+//   * the inverse x/z butterflies and the sum over the four faces around a
+//     node use the mirror-image moves: DPP row shift from lz - 1, LDS from
+//     lx - 1.  Every node sum is taken in a fixed order (deterministic, no
+//     atomics).
+// Thread columns lx, lz in [1, 14] own output nodes; columns 0 and 15 are halo
+// that neighbouring tiles also compute.  LDS traffic is 9 doubles written and
+// read per thread and layer; all other data stays in registers.  Inactive
+// elements (outside the grid or in the halo) get a zero modulus so that their
+// contribution vanishes without divergent code around the cross-lane moves.
+#pragma once
+#include "common.hpp"
+#include "fantom_internal.hpp"
+
+namespace fantom {
+
+constexpr int kTW = 16;                // node columns per tile edge
+constexpr int kTOut = kTW - 2;         // output node columns per tile edge
+constexpr int kTThreads = kTW * kTW;   // threads per workgroup
+
+struct TileShape {
+    int tiles_x, tiles_z, chunks_y, chunk;
+};
+
+// Host: tile counts for grid g.  The y-chunk length trades redundant halo
+// planes (chunk + 2 planes are read for chunk outputs) against the number of
+// workgroups; the default aims at >= 4 workgroups per CU on a 256-CU part.
+TileShape tile_shape(const Grid &g);
+unsigned tile_blocks(const TileShape &ts);
+
+// DPP moves inside rows of 16 lanes; a lane whose source lies outside its row
+// reads 0 (bound_ctrl).
+constexpr int kDppRowShl1 = 0x101;   // lane l <- lane l + 1
+constexpr int kDppRowShr1 = 0x111;   // lane l <- lane l - 1
+
+template <int CTRL>
+__device__ __forceinline__ float dpp_row(float v)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF, true));
+}
+
+template <int CTRL>
+__device__ __forceinline__ double dpp_row(double v)
+{
+    const unsigned long long bits = __builtin_bit_cast(unsigned long long, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)bits, CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(bits >> 32), CTRL, 0xF, 0xF, true);
+    return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned long long)(unsigned)lo);
+}
+
+// The operator policy Op provides
+//   struct Data;                                         per-element inputs
+//   void load(Data &, const Grid &, bool active, int ex, int ey, int ez) const;
+//   void apply(const Data &, const T (&w)[3][8], T (&z)[3][8]) const;
+// where load fills Data for element (ex, ey, ez) (zero modulus when !active)
+// and apply maps WHT coefficients to output coefficients (the 1/8 of the
+// inverse transform is folded into the operator's tables).
+template <typename T, int MODE, class Op>
+__global__ __launch_bounds__(kTThreads) void ku_tile_kernel(Grid g, Op op, const uint8_t *__restrict__ mask,
+                                                          const T *__restrict__ u, T *__restrict__ y,
+                                                          const T *__restrict__ bvec, const T *__restrict__ dinv,
+                                                          T omega, double *__restrict__ partials, TileShape ts)
+{
+    __shared__ T xf[6][kTThreads];   // z-butterflies, exchanged with column lx - 1
+    __shared__ T xr[3][kTThreads];   // upper-x node sums, exchanged with column lx + 1
+
+    // The hardware deals workgroups to the 8 XCDs round-robin; give each XCD
+    // a contiguous range of logical tiles so that tiles sharing halo planes
+    // and columns meet in the same L2.  The grid is a multiple of 8 blocks.
+    const int per_xcd = gridDim.x / 8;
+    const int bid = blockIdx.x;
+    const int logical = (bid % 8) * per_xcd + bid / 8;
+    double part = 0.0;
+    if (logical < ts.tiles_x * ts.tiles_z * ts.chunks_y) {   // uniform per workgroup
+        const int tz = logical % ts.tiles_z;
+        const int tx = (logical / ts.tiles_z) % ts.tiles_x;
+        const int cyk = logical / (ts.tiles_z * ts.tiles_x);
+        const int tid = threadIdx.x, lx = tid >> 4, lz = tid & 15;
+        const int gx = tx * kTOut - 1 + lx, gz = tz * kTOut - 1 + lz;
+        const bool col_in = gx >= 0 && gx < g.nx && gz >= 0 && gz < g.nz;
+        const bool el_col = lx < kTW - 1 && lz < kTW - 1 && gx >= 0 && gx < g.nelx && gz >= 0 && gz < g.nelz;
+        const bool out_col = lx >= 1 && lx < kTW - 1 && lz >= 1 && lz < kTW - 1 && gx < g.nx && gz < g.nz;
+        const long nplane = (long)g.nx * g.nz;
+        const long col = (long)gx * g.nz + gz;
+        const int nb_up = tid + kTW < kTThreads ? tid + kTW : tid;   // column lx + 1 (lx = 15: any valid slot)
+        const int nb_dn = tid >= kTW ? tid - kTW : tid;              // column lx - 1 (lx = 0: any valid slot)
+        const int jc0 = cyk * ts.chunk;
+        const int jc1 = min(jc0 + ts.chunk, g.ny);   // output planes [jc0, jc1)
+
+        auto load_node = [&](int j, T (&v)[3], int &m) {
+            v[0] = v[1] = v[2] = T(0);
+            m = 0;
+            if (col_in && j >= 0 && j < g.ny) {
+                const long n = (long)j * nplane + col;
+                v[0] = u[3 * n];
+                v[1] = u[3 * n + 1];
+                v[2] = u[3 * n + 2];
+                if (mask)
+                    m = mask[n];
+            }
+        };
+        // face transform of a plane from the thread's (unmasked) node values
+        auto face = [&](const T (&v)[3], int m, T (&F)[3][4]) {
+            T zs[3][2];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const T a = ((m >> c) & 1) ? T(0) : v[c];
+                const T b = dpp_row<kDppRowShl1>(a);
+                zs[c][0] = a + b;
+                zs[c][1] = a - b;
+                xf[2 * c][tid] = zs[c][0];
+                xf[2 * c + 1][tid] = zs[c][1];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int sz = 0; sz < 2; ++sz) {
+                    const T nb = xf[2 * c + sz][nb_up];
+                    F[c][2 * sz] = zs[c][sz] + nb;
+                    F[c][2 * sz + 1] = zs[c][sz] - nb;
+                }
+        };
+        auto load_elem = [&](typename Op::Data &d, int L) {
+            op.load(d, g, el_col && L >= 0 && L < g.nely, gx, L, gz);
+        };
+
+        T Fp[3][4], ucur[3], unx[3];
+        int mcur, mnx;
+        typename Op::Data dcur, dnx;
+        load_node(jc0 - 1, ucur, mcur);
+        load_node(jc0, unx, mnx);
+        load_elem(dcur, jc0 - 1);
+        face(ucur, mcur, Fp);
+        __syncthreads();   // the first layer rewrites xf
+        T bcur[3] = {T(0), T(0), T(0)}, dgcur[3] = {T(0), T(0), T(0)};
+        T Zu[3][4];
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                Zu[c][q] = T(0);
+
+        for (int L = jc0 - 1; L < jc1; ++L) {
+            // element layer L lies between node planes L and L + 1; plane L is
+            // written at the end of this step (L >= jc0)
+            T upf[3];
+            int mpf;
+            load_node(L + 2 <= jc1 ? L + 2 : -1, upf, mpf);
+            load_elem(dnx, L + 1 < jc1 ? L + 1 : -1);
+            T bnx[3] = {T(0), T(0), T(0)}, dgnx[3] = {T(0), T(0), T(0)};
+            if ((MODE == MODE_JAC || MODE == MODE_RES) && out_col && L + 1 < jc1) {
+                const long r0 = 3 * ((long)(L + 1) * nplane + col);
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    bnx[c] = bvec[r0 + c];
+                    if (MODE == MODE_JAC)
+                        dgnx[c] = dinv[r0 + c];
+                }
+            }
+
+            T Fc[3][4];
+            face(unx, mnx, Fc);
+
+            T w[3][8], z[3][8];
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int sx = 0; sx < 2; ++sx)
+#pragma unroll
+                    for (int sz = 0; sz < 2; ++sz) {
+                        const int q = sx + 2 * sz;
+                        w[c][sx + 4 * sz] = Fp[c][q] + Fc[c][q];
+                        w[c][sx + 2 + 4 * sz] = Fp[c][q] - Fc[c][q];
+                    }
+            op.apply(dcur, w, z);
+
+            T val[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                T A[4];
+#pragma unroll
+                for (int sx = 0; sx < 2; ++sx)
+#pragma unroll
+                    for (int sz = 0; sz < 2; ++sz) {
+                        const int q = sx + 2 * sz;
+                        const T lo = z[c][sx + 4 * sz] + z[c][sx + 2 + 4 * sz];
+                        const T hi = z[c][sx + 4 * sz] - z[c][sx + 2 + 4 * sz];
+                        A[q] = lo + Zu[c][q];
+                        Zu[c][q] = hi;
+                    }
+                // inverse x butterfly: P[bx][sz]; inverse z butterfly: corner bz
+                const T p00 = A[0] + A[1], p10 = A[0] - A[1];   // sz = 0
+                const T p01 = A[2] + A[3], p11 = A[2] - A[3];   // sz = 1
+                const T q00 = p00 + p01, q01 = p00 - p01;       // bx = 0: corners bz = 0, 1
+                const T q10 = p10 + p11, q11 = p10 - p11;       // bx = 1
+                const T r0 = q00 + dpp_row<kDppRowShr1>(q01);   // node (lx, lz) from this column
+                const T r1 = q10 + dpp_row<kDppRowShr1>(q11);   // node (lx + 1, lz)
+                val[c] = r0;
+                xr[c][tid] = r1;
+            }
+            __syncthreads();
+            if (L >= jc0 && out_col) {
+                const long r0 = 3 * ((long)L * nplane + col);
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const T kuv = ((mcur >> c) & 1) ? ucur[c] : val[c] + xr[c][nb_dn];
+                    T o;
+                    if (MODE == MODE_JAC)
+                        o = ucur[c] + omega * dgcur[c] * (bcur[c] - kuv);
+                    else if (MODE == MODE_RES)
+                        o = bcur[c] - kuv;
+                    else
+                        o = kuv;
+                    y[r0 + c] = o;
+                    if (MODE == MODE_KUDOT)
+                        part += (double)ucur[c] * (double)o;
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    Fp[c][q] = Fc[c][q];
+                ucur[c] = unx[c];
+                unx[c] = upf[c];
+                bcur[c] = bnx[c];
+                dgcur[c] = dgnx[c];
+            }
+            mcur = mnx;
+            mnx = mpf;
+            dcur = dnx;
+        }
+    }
+    if (MODE == MODE_KUDOT) {
+        const double s = block_sum<kTThreads>(part);
+        if (threadIdx.x == 0)
+            partials[blockIdx.x] = s;
+    }
+}
+
+}  // namespace fantom
