@@ -185,6 +185,22 @@ int fantom_elem_ku_f64(int nelx, int nely, int nelz, const double *Ke, const uin
 int fantom_elem_from_fine_f64(int nelx, int nely, int nelz, const double *G, const double *Efine, double *Ke,
                               void *stream);
 
+/* Assembled 27-point node stencils of a Galerkin level (hex8), the form the
+ * native drivers apply on levels >= 2 (the reference applies its coarse CSR
+ * matrices with a generic SpMV, core/CUDA/_mgm.py:19-171).  stencil_build:
+ * S (243 x n_nodes doubles, coefficient k = t*9 + 3c + d of neighbour
+ * t = (dy+1)*9 + (dx+1)*3 + (dz+1), structure of arrays) from element
+ * matrices Ke, with constrained rows set to identity rows and constrained
+ * columns zeroed.  stencil_ku: mode 0 y = A u, 1 one Jacobi sweep
+ * y = u + omega dinv (b - A u), 2 residual y = b - A u, 3 y = A u with the
+ * per-block partial sums of u.(A u) in partials[0:stencil_partials] (as
+ * fantom_ku_f64 / _jacobi / _residual / _dot; unused operands may be NULL). */
+int fantom_stencil_build_f64(int nelx, int nely, int nelz, const double *Ke, const uint8_t *mask, double *S,
+                             void *stream);
+int fantom_stencil_ku_f64(int mode, int nelx, int nely, int nelz, const double *S, const double *u, double *y,
+                          const double *b, const double *dinv, double omega, double *partials, void *stream);
+long fantom_stencil_partials(int nelx, int nely, int nelz);
+
 /* ---- A9-A11: native GMG-PCG driver ------------------------------------------
  * Replaces MultiGrid._setup / _multi_grid / solve
  * (solvers/CUDA/_solvers.py:442-660): rediscretised matrix-free hierarchy of

@@ -96,6 +96,10 @@ SIGNATURES = {
     "fantom_gal_build_f64": (_c_int, [_c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp]),
     "fantom_elem_ku_f64": (_c_int, [_c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp]),
     "fantom_elem_from_fine_f64": (_c_int, [_c_int, _c_int, _c_int, _vp, _vp, _vp, _vp]),
+    "fantom_stencil_build_f64": (_c_int, [_c_int, _c_int, _c_int, _vp, _vp, _vp, _vp]),
+    "fantom_stencil_ku_f64": (_c_int, [_c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _c_double, _vp,
+                                       _vp]),
+    "fantom_stencil_partials": (_c_long, [_c_int, _c_int, _c_int]),
     "fantom_oc_update_f64": (_c_int, [_c_long, _vp, _vp, _vp, _c_double, _c_double, _vp, _vp, _vp, _vp, _vp]),
 }
 

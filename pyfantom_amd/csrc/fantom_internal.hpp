@@ -70,4 +70,15 @@ int launch_elem_from_fine(const Grid &gc, const double *G, const double *Ef, dou
 int launch_elem_from_fine_f32E(const Grid &gc, const double *G, const float *Ef, double *Ke, hipStream_t s);
 int launch_dense_from_elem(const Grid &g, const double *Ke, const uint8_t *mask, double *A, hipStream_t s);
 
+// stencil.hip (assembled 27-point stencils of Galerkin levels >= 2)
+int launch_stencil_build(const Grid &g, const double *Ke, const uint8_t *mask, double *S, hipStream_t s);
+int launch_stencil_build_f32(const Grid &g, const double *Ke, const uint8_t *mask, float *S, hipStream_t s);
+int launch_stencil_ku(int mode, const Grid &g, const double *S, const double *u, double *y, const double *b,
+                      const double *dinv, double omega, double *partials, hipStream_t s);
+int launch_stencil_ku_f32(int mode, const Grid &g, const float *S, const float *u, float *y, const float *b,
+                          const float *dinv, double omega, double *partials, hipStream_t s);
+int launch_dinv_stencil(const Grid &g, const double *S, double *d, hipStream_t s);
+int launch_dinv_stencil_f32(const Grid &g, const float *S, float *d, hipStream_t s);
+long stencil_partials(const Grid &g);
+
 }  // namespace fantom

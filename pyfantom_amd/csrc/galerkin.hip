@@ -24,19 +24,19 @@
 #include <cmath>
 
 #include "../../include/fantom.h"
-#include "common.hpp"
-#include "fantom_internal.hpp"
+#include "tile_skel.hpp"
 
 namespace fantom {
 
-constexpr int GTB = 16;             // elements per tile edge
-constexpr int GTN = GTB + 1;
-constexpr int GTOUT = GTB - 1;
-constexpr int GPLANE = GTN * GTN;
-constexpr int GCHUNK = 16;
-constexpr int kGalBlock = GTB * GTB;
-constexpr int kGPre = (3 * GPLANE + kGalBlock - 1) / kGalBlock;
-constexpr int kGPreM = (GPLANE + kGalBlock - 1) / kGalBlock;
+// Nonzero pattern of the level-1 WHT blocks Ghat_0[s][s'] (bit 3 d + d' of
+// entry s * 8 + s'): the same for every box shape and Poisson ratio (the
+// zeros come from the reflection symmetry and the rigid translations), and
+// checked by fantom_galerkin_tables.  279 of 576 entries are nonzero.
+constexpr unsigned short kGalNZ[64] = {
+    0x1ff, 0x1b6, 0x16d, 0x124, 0x0db, 0x092, 0x049, 0x000, 0x1f8, 0x1b0, 0x168, 0x138, 0x0d8, 0x1d0, 0x048, 0x158,
+    0x1c7, 0x186, 0x145, 0x107, 0x0c3, 0x082, 0x1c1, 0x183, 0x1c0, 0x192, 0x149, 0x11b, 0x0c0, 0x1d2, 0x1c9, 0x1db,
+    0x03f, 0x036, 0x02d, 0x024, 0x01b, 0x017, 0x039, 0x035, 0x038, 0x134, 0x028, 0x13c, 0x059, 0x155, 0x079, 0x17d,
+    0x007, 0x006, 0x125, 0x127, 0x093, 0x097, 0x1b1, 0x1b7, 0x000, 0x116, 0x129, 0x13f, 0x0d1, 0x1d7, 0x1f9, 0x1ff};
 
 template <typename T>
 __device__ __forceinline__ void gwht8(T *w)
@@ -54,212 +54,68 @@ __device__ __forceinline__ void gwht8(T *w)
     }
 }
 
-// Level-1 Galerkin K(E)u on the level-1 grid g (its fine grid is 2x in every
-// direction).  Tiling as the fine tile kernel (ku_tile.hip).  Btab: 576
-// doubles, block (s, s') at Btab[(s*8 + s')*9 + 3*d + d'] (scaled by 1/8).
-template <typename T, int MODE>
-__global__ __launch_bounds__(kGalBlock, 2) void ku_gal1(Grid g, const double *__restrict__ Btab,
-                                                      const T *__restrict__ Ef, const uint8_t *__restrict__ mask,
-                                                      const T *__restrict__ u, T *__restrict__ y,
-                                                      const T *__restrict__ bvec, const T *__restrict__ dinv,
-                                                      T omega, double *__restrict__ partials, int tiles_x,
-                                                      int tiles_z, int chunks_y)
-{
-    __shared__ T us[2][3][GPLANE];
-    __shared__ uint8_t ms[2][GPLANE];
-    __shared__ T outs[24][kGalBlock];
-
-    const int per_xcd = gridDim.x / 8;
-    const int bid = blockIdx.x;
-    int logical = (bid % 8) * per_xcd + bid / 8;
-    double part = 0.0;
-    const int tid = threadIdx.x;
-    const int lx = tid / GTB, lz = tid % GTB;
-    const bool valid_tile = logical < tiles_x * tiles_z * chunks_y;
-    if (!valid_tile)
-        logical = 0;
-    const int tz = logical % tiles_z;
-    const int tx = (logical / tiles_z) % tiles_x;
-    const int cy = logical / (tiles_z * tiles_x);
-    const int ex0 = tx * GTOUT - 1, ez0 = tz * GTOUT - 1;
-    const int jc0 = cy * GCHUNK;
-    const int jc1 = min(jc0 + GCHUNK, g.ny);
-    const long nplane = (long)g.nx * g.nz;
-    const int fy = 2 * g.nely, fz = 2 * g.nelz;  // fine element counts (y, z)
-
-    auto load_plane = [&](int j, T *vals, uint8_t *mv) {
-#pragma unroll
-        for (int q = 0; q < kGPre; ++q) {
-            const int idx = tid + q * kGalBlock;
-            T v = T(0);
-            if (idx < 3 * GPLANE && j >= 0 && j < g.ny) {
-                const int a = idx / (3 * GTN), rem = idx % (3 * GTN);
-                const int gx = ex0 + a, gz = ez0 + rem / 3;
-                if (gx >= 0 && gx < g.nx && gz >= 0 && gz < g.nz)
-                    v = u[3 * ((long)j * nplane + (long)gx * g.nz + gz) + rem % 3];
-            }
-            vals[q] = v;
-        }
-#pragma unroll
-        for (int q = 0; q < kGPreM; ++q) {
-            const int idx = tid + q * kGalBlock;
-            uint8_t m = 0;
-            if (mask && idx < GPLANE && j >= 0 && j < g.ny) {
-                const int a = idx / GTN, c = idx % GTN;
-                const int gx = ex0 + a, gz = ez0 + c;
-                if (gx >= 0 && gx < g.nx && gz >= 0 && gz < g.nz)
-                    m = mask[(long)j * nplane + (long)gx * g.nz + gz];
-            }
-            mv[q] = m;
-        }
+// Level-1 Galerkin operator on the level-1 grid g (its fine grid is 2x in
+// every direction), run on the shared tile skeleton.  Btab: 576 doubles,
+// block (s, s') at Btab[(s*8 + s')*9 + 3*d + d'] (scaled by 1/8); uniform
+// reads, served by the scalar cache.
+template <typename T>
+struct Gal1Op {
+    const double *Btab;
+    const T *Ef;
+    int zero;   // 0, set by the host
+    struct Data {
+        T e[8];   // child moduli, child bits cx + 2 cy + 4 cz
+        int k;    // ey & zero
     };
-    auto store_plane = [&](int slot, const T *vals, const uint8_t *mv) {
+    __device__ __forceinline__ void load(Data &d, const Grid &g, bool active, int ex, int ey, int ez) const
+    {
+        const int fy = 2 * g.nely, fz = 2 * g.nelz;
+        d.k = ey & zero;
 #pragma unroll
-        for (int q = 0; q < kGPre; ++q) {
-            const int idx = tid + q * kGalBlock;
-            if (idx < 3 * GPLANE) {
-                const int a = idx / (3 * GTN), rem = idx % (3 * GTN);
-                us[slot][rem % 3][a * GTN + rem / 3] = vals[q];
-            }
-        }
-#pragma unroll
-        for (int q = 0; q < kGPreM; ++q) {
-            const int idx = tid + q * kGalBlock;
-            if (idx < GPLANE)
-                ms[slot][idx] = mv[q];
-        }
-    };
-
-    T pre[kGPre];
-    uint8_t prem[kGPreM];
-    if (valid_tile) {
-        load_plane(jc0 - 1, pre, prem);
-        store_plane((jc0 + 1) & 1, pre, prem);
-        load_plane(jc0, pre, prem);
-        store_plane(jc0 & 1, pre, prem);
+        for (int c = 0; c < 8; ++c)
+            d.e[c] = active ? Ef[((long)(2 * ex + (c & 1)) * fy + (2 * ey + ((c >> 1) & 1))) * fz + (2 * ez + (c >> 2))]
+                            : T(0);
     }
-    __syncthreads();
-
-    T hi[3] = {T(0), T(0), T(0)};
-    const int gxe = ex0 + lx, gze = ez0 + lz;
-    const bool el_in = gxe >= 0 && gxe < g.nelx && gze >= 0 && gze < g.nelz;
-
-    for (int L = jc0 - 1; L < jc1 && valid_tile; ++L) {
-        const int sl = (L + 2) & 1, su = (L + 3) & 1;
-        if (L + 2 < jc1 + 1)
-            load_plane(L + 2, pre, prem);
-        T uown[3] = {T(0), T(0), T(0)};
-        int mown = 0;
-        if (L >= jc0) {
+    __device__ __forceinline__ void apply(const Data &d, const T (&w)[3][8], T (&z)[3][8]) const
+    {
+        T eh[8];
 #pragma unroll
-            for (int c = 0; c < 3; ++c)
-                uown[c] = us[sl][c][lx * GTN + lz];
-            mown = ms[sl][lx * GTN + lz];
-        }
-        const bool active = el_in && L >= 0 && L < g.nely;
-        if (active) {
-            // Walsh transform of the 8 child moduli (child bits cx + 2cy + 4cz)
-            T eh[8];
+        for (int c = 0; c < 8; ++c)
+            eh[c] = d.e[c];
+        gwht8(eh);
+        // d.k is 0 at run time but depends on the layer, which keeps the 279
+        // uniform table reads inside the layer loop as scalar loads; hoisted
+        // out of the loop they would occupy VGPRs and cut occupancy.
+        typedef const __attribute__((address_space(4))) double ConstD;   // read-only for the whole kernel
+        ConstD *tab = (ConstD *)(Btab + __builtin_amdgcn_readfirstlane(d.k));
 #pragma unroll
-            for (int c = 0; c < 8; ++c) {
-                const int cx = c & 1, cyy = (c >> 1) & 1, cz = c >> 2;
-                eh[c] = Ef[((long)(2 * gxe + cx) * fy + (2 * L + cyy)) * fz + (2 * gze + cz)];
-            }
-            gwht8(eh);
-            T w[3][8];
+        for (int s = 0; s < 8; ++s) {
+            T a[3] = {T(0), T(0), T(0)};
 #pragma unroll
-            for (int b = 0; b < 8; ++b) {
-                const int bx = b & 1, by = (b >> 1) & 1, bz = b >> 2;
-                const int sidx = by ? su : sl;
-                const int p = (lx + bx) * GTN + (lz + bz);
-                const int m = ms[sidx][p];
+            for (int s2 = 0; s2 < 8; ++s2) {
+                const unsigned nz = kGalNZ[s * 8 + s2];
+                if (nz == 0)
+                    continue;
+                const T h[3] = {w[0][s2 ^ 1], w[1][s2 ^ 2], w[2][s2 ^ 4]};
+                ConstD *Bs = tab + (s * 8 + s2) * 9;
 #pragma unroll
-                for (int c = 0; c < 3; ++c)
-                    w[c][b] = ((m >> c) & 1) ? T(0) : us[sidx][c][p];
-            }
+                for (int dr = 0; dr < 3; ++dr) {
+                    if (((nz >> (3 * dr)) & 7) == 0)
+                        continue;
+                    T t = T(0);
 #pragma unroll
-            for (int c = 0; c < 3; ++c)
-                gwht8(w[c]);
-            T z[3][8];
-#pragma unroll
-            for (int s = 0; s < 8; ++s) {
-                T a0 = T(0), a1 = T(0), a2 = T(0);
-#pragma unroll
-                for (int s2 = 0; s2 < 8; ++s2) {
-                    const T h0 = w[0][s2 ^ 1], h1 = w[1][s2 ^ 2], h2 = w[2][s2 ^ 4];
-                    const double *Bs = Btab + (s * 8 + s2) * 9;
-                    const T e = eh[s ^ s2];
-                    a0 += e * ((T)Bs[0] * h0 + (T)Bs[1] * h1 + (T)Bs[2] * h2);
-                    a1 += e * ((T)Bs[3] * h0 + (T)Bs[4] * h1 + (T)Bs[5] * h2);
-                    a2 += e * ((T)Bs[6] * h0 + (T)Bs[7] * h1 + (T)Bs[8] * h2);
-                }
-                z[0][s ^ 1] = a0;
-                z[1][s ^ 2] = a1;
-                z[2][s ^ 4] = a2;
-            }
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                gwht8(z[c]);
-#pragma unroll
-                for (int b = 0; b < 8; ++b)
-                    outs[b * 3 + c][tid] = z[c][b];
-            }
-        } else {
-#pragma unroll
-            for (int v = 0; v < 24; ++v)
-                outs[v][tid] = T(0);
-        }
-        __syncthreads();
-        const bool node_thread = lx >= 1 && lz >= 1;
-        if (node_thread) {
-            T lo[3] = {T(0), T(0), T(0)}, up[3] = {T(0), T(0), T(0)};
-#pragma unroll
-            for (int bx = 0; bx < 2; ++bx)
-#pragma unroll
-                for (int bz = 0; bz < 2; ++bz) {
-                    const int t = (lx - bx) * GTB + (lz - bz);
-                    const int bl = bx + 4 * bz, bu = bl + 2;
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) {
-                        lo[c] += outs[bl * 3 + c][t];
-                        up[c] += outs[bu * 3 + c][t];
-                    }
-                }
-            if (L >= jc0) {
-                const int gx = ex0 + lx, gz = ez0 + lz;
-                if (gx < g.nx && gz < g.nz) {
-                    const long n = (long)L * nplane + (long)gx * g.nz + gz;
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) {
-                        const long r = 3 * n + c;
-                        const T ku = ((mown >> c) & 1) ? uown[c] : hi[c] + lo[c];
-                        T o;
-                        if (MODE == MODE_JAC)
-                            o = uown[c] + omega * dinv[r] * (bvec[r] - ku);
-                        else if (MODE == MODE_RES)
-                            o = bvec[r] - ku;
-                        else
-                            o = ku;
-                        y[r] = o;
-                        if (MODE == MODE_KUDOT)
-                            part += (double)uown[c] * (double)o;
-                    }
+                    for (int dc = 0; dc < 3; ++dc)
+                        if ((nz >> (3 * dr + dc)) & 1)
+                            t += (T)Bs[3 * dr + dc] * h[dc];
+                    a[dr] += eh[s ^ s2] * t;
                 }
             }
-#pragma unroll
-            for (int c = 0; c < 3; ++c)
-                hi[c] = up[c];
+            z[0][s ^ 1] = a[0];
+            z[1][s ^ 2] = a[1];
+            z[2][s ^ 4] = a[2];
         }
-        if (L + 2 < jc1 + 1)
-            store_plane(sl, pre, prem);
-        __syncthreads();
     }
-    if (MODE == MODE_KUDOT) {
-        const double s = block_sum<kGalBlock>(part);
-        if (threadIdx.x == 0)
-            partials[blockIdx.x] = s;
-    }
-}
+};
 
 // ---- explicit element matrices (levels >= 2) ----------------------------------------
 // corner b of an element sits at node offset (b & 1, (b >> 1) & 1, b >> 2)
@@ -546,32 +402,23 @@ __global__ void dense_from_elem(Grid g, const double *__restrict__ Ke, const uin
             }
 }
 
-static void gal1_grid(const Grid &g, int &tx, int &tz, int &cy)
-{
-    tx = (g.nx + GTOUT - 1) / GTOUT;
-    tz = (g.nz + GTOUT - 1) / GTOUT;
-    cy = (g.ny + GCHUNK - 1) / GCHUNK;
-}
-
-long gal1_partials(const Grid &g)
-{
-    int tx, tz, cy;
-    gal1_grid(g, tx, tz, cy);
-    return ((long)tx * tz * cy + 7) / 8 * 8;
-}
+long gal1_partials(const Grid &g) { return (long)tile_blocks(tile_shape(g)); }
 
 template <typename T>
 static int gal1_launch(int mode, const Grid &g, const double *Btab, const T *Ef, const uint8_t *mask, const T *u,
                        T *y, const T *b, const T *dinv, double omega, double *partials, hipStream_t s)
 {
-    int tx, tz, cy;
-    gal1_grid(g, tx, tz, cy);
-    const unsigned blocks = (unsigned)gal1_partials(g);
+    const TileShape ts = tile_shape(g);
+    const unsigned blocks = tile_blocks(ts);
     const T om = (T)omega;
-#define GAL_CASE(M)                                                                                        \
-    case M:                                                                                                \
-        hipLaunchKernelGGL((ku_gal1<T, M>), dim3(blocks), dim3(kGalBlock), 0, s, g, Btab, Ef, mask, u, y, b, dinv, \
-                           om, partials, tx, tz, cy);                                                      \
+    Gal1Op<T> op;
+    op.Btab = Btab;
+    op.Ef = Ef;
+    op.zero = 0;
+#define GAL_CASE(M)                                                                                           \
+    case M:                                                                                                   \
+        hipLaunchKernelGGL((ku_tile_kernel<T, M, Gal1Op<T>>), dim3(blocks), dim3(kTThreads), 0, s, g, op, mask, u, \
+                           y, b, dinv, om, partials, ts);                                                     \
         break;
     switch (mode) {
         GAL_CASE(MODE_KU)
@@ -807,6 +654,15 @@ extern "C" int fantom_galerkin_tables(const double *K0, double *G, double *Btab,
         for (int ch = 0; ch < 8; ++ch)
             s += G[ch * 576 + i];
         sum_err = fmax(sum_err, fabs(s - 2.0 * Kn[i / 24][i % 24]));
+    }
+    double pat_err = 0.0;   // entries outside the fixed nonzero pattern
+    for (int q = 0; q < 64; ++q)
+        for (int i = 0; i < 9; ++i)
+            if (!((kGalNZ[q] >> i) & 1))
+                pat_err = fmax(pat_err, fabs(Btab[q * 9 + i]) * 8.0);
+    if (pat_err > 1e-12 * maxabs) {
+        set_error("level-1 Galerkin blocks leave the box-hex8 nonzero pattern (%.2e)", pat_err);
+        return kStatusBadArg;
     }
     if (check) {
         check[0] = sym_err / fmax(maxabs, 1e-300);

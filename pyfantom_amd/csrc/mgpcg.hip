@@ -47,6 +47,7 @@ struct MgLevel {
     double *K0dense_d;     // device copy (diag / dense assembly / node kernel)
     double Kb[72];         // host block form (tile kernel)
     double *Ke;            // explicit Galerkin element matrices (levels >= 2)
+    double *S;             // assembled stencil of an element-matrix level that is smoothed
     int kind;              // LK_* operator of this level
 };
 
@@ -149,7 +150,8 @@ static int lev_ku(MgHandle *h, int l, int mode, const double *u, double *y, cons
     if (L.kind == LK_GAL1)
         return launch_gal1(mode, L.g, h->Btab, h->lev[0].E, L.mask, u, y, b, dinv, omega, partials, h->stream);
     if (L.kind == LK_ELEM)
-        return launch_elem(mode, L.g, L.Ke, L.mask, u, y, b, dinv, omega, partials, h->stream);
+        return L.S ? launch_stencil_ku(mode, L.g, L.S, u, y, b, dinv, omega, partials, h->stream)
+                   : launch_elem(mode, L.g, L.Ke, L.mask, u, y, b, dinv, omega, partials, h->stream);
     if (h->kernel == FANTOM_KU_TILE && L.dim == 3)
         return launch_ku_tile(mode, L.g, L.Kb, L.E, L.mask, u, y, b, dinv, omega, partials, h->stream);
     return launch_ku_node(mode, L.g, L.dim, L.K0dense_d, L.E, L.mask, u, y, b, dinv, omega, partials, h->stream);
@@ -270,8 +272,12 @@ static void layout(MgHandle *h, int nelx, int nely, int nelz, int n_coarse, int 
         L.E = cv.take<double>(L.g.n_el);
         L.mask = cv.take<uint8_t>(L.g.n_nodes);
         L.K0dense_d = cv.take<double>(576);
-        if (L.kind == LK_ELEM)
+        L.S = nullptr;
+        if (L.kind == LK_ELEM) {
             L.Ke = cv.take<double>((size_t)L.g.n_el * 576);
+            if (l < h->nlev - 1)   // the coarsest level is solved directly
+                L.S = cv.take<double>((size_t)L.g.n_nodes * 243);
+        }
     }
     h->Gtab = h->Btab = h->Gdiag = h->Ke1 = nullptr;
     if (h->galerkin) {
@@ -294,8 +300,10 @@ static void layout(MgHandle *h, int nelx, int nely, int nelz, int n_coarse, int 
         np = std::max(np, ku_tile_partials(h->lev[0].g));
     for (int l = 1; l < h->nlev; ++l) {
         np = std::max(np, elem_partials(h->lev[l].g));
-        if (h->dim == 3)
+        if (h->dim == 3) {
             np = std::max(np, gal1_partials(h->lev[l].g));
+            np = std::max(np, stencil_partials(h->lev[l].g));
+        }
     }
     h->n_partials = np;
     h->partials = cv.take<double>(np);
@@ -475,7 +483,10 @@ int fantom_mg_setup(void *handle, const double *E, const uint8_t *mask, void *st
         MgLevel &L = h->lev[l];
         if (L.kind == LK_GAL1)
             MG_TRY(launch_dinv_gal1(L.g, h->Gdiag, L0.E, L.mask, L.dinv, h->stream));
-        else if (L.kind == LK_ELEM)
+        else if (L.kind == LK_ELEM && L.S) {
+            MG_TRY(launch_stencil_build(L.g, L.Ke, L.mask, L.S, h->stream));
+            MG_TRY(launch_dinv_stencil(L.g, L.S, L.dinv, h->stream));
+        } else if (L.kind == LK_ELEM)
             MG_TRY(launch_dinv_elem(L.g, L.Ke, L.mask, L.dinv, h->stream));
         else
             MG_TRY(launch_diag(L.g, h->dim, L.K0dense_d, L.E, L.mask, L.dinv, 1, h->stream));

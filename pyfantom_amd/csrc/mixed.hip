@@ -41,6 +41,7 @@ struct MxLevel {
     double *K0dense_d;
     double Kb[72];
     double *Ke;   // explicit Galerkin element matrices (levels >= 2), FP64
+    float *S;     // assembled FP32 stencil of an element-matrix level that is smoothed
     int kind;     // 0 fine, 1 rediscretised, 2 Galerkin level 1, 3 element matrices
 };
 
@@ -191,7 +192,8 @@ static int mx_ku(MxHandle *h, int l, int mode, const float *u, float *y, const f
     if (L.kind == 2)
         return launch_gal1_f32(mode, L.g, h->Btab, h->lev[0].E, L.mask, u, y, b, dinv, h->omega, partials, h->stream);
     if (L.kind == 3)
-        return launch_elem_f32(mode, L.g, L.Ke, L.mask, u, y, b, dinv, h->omega, partials, h->stream);
+        return L.S ? launch_stencil_ku_f32(mode, L.g, L.S, u, y, b, dinv, h->omega, partials, h->stream)
+                   : launch_elem_f32(mode, L.g, L.Ke, L.mask, u, y, b, dinv, h->omega, partials, h->stream);
     return launch_ku_tile_f32(mode, L.g, L.Kb, L.E, L.mask, u, y, b, dinv, h->omega, partials, h->stream);
 }
 
@@ -298,6 +300,7 @@ static void mx_layout(MxHandle *h, int nelx, int nely, int nelz, int n_coarse, i
         L.K0dense_d = cv.take<double>(576);
         L.kind = l == 0 ? 0 : (!h->galerkin ? 1 : (l == 1 ? 2 : 3));
         L.Ke = L.kind == 3 ? cv.take<double>((size_t)L.g.n_el * 576) : nullptr;
+        L.S = (L.kind == 3 && l < h->nlev - 1) ? cv.take<float>((size_t)L.g.n_nodes * 243) : nullptr;
     }
     h->Gtab = h->Btab = h->Gdiag = h->Ke1 = nullptr;
     if (h->galerkin) {
@@ -319,7 +322,8 @@ static void mx_layout(MxHandle *h, int nelx, int nely, int nelz, int n_coarse, i
     h->q = cv.take<float>(h->n);
     long np = std::max((long)kVecMaxBlocks, ku_tile_partials(h->g));
     for (int l = 1; l < h->nlev; ++l)
-        np = std::max(np, std::max(elem_partials(h->lev[l].g), gal1_partials(h->lev[l].g)));
+        np = std::max(np, std::max(std::max(elem_partials(h->lev[l].g), gal1_partials(h->lev[l].g)),
+                                   stencil_partials(h->lev[l].g)));
     h->partials = cv.take<double>(np);
     h->scal = cv.take<double>(FANTOM_SCALARS);
     h->nc = (int)h->lev[h->nlev - 1].n_dof;
@@ -492,7 +496,10 @@ int fantom_mx_setup(void *handle, const double *E, const uint8_t *mask, void *st
         MxLevel &L = h->lev[l];
         if (L.kind == 2)
             MX_TRY(launch_dinv_gal1_f32(L.g, h->Gdiag, h->lev[0].E, L.mask, L.dinv, s));
-        else if (L.kind == 3)
+        else if (L.kind == 3 && L.S) {
+            MX_TRY(launch_stencil_build_f32(L.g, L.Ke, L.mask, L.S, s));
+            MX_TRY(launch_dinv_stencil_f32(L.g, L.S, L.dinv, s));
+        } else if (L.kind == 3)
             MX_TRY(launch_dinv_elem_f32(L.g, L.Ke, L.mask, L.dinv, s));
         else
             MX_TRY(launch_diag_f32(L.g, L.K0dense_d, L.E, L.mask, L.dinv, s));

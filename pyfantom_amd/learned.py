@@ -371,8 +371,10 @@ def fpcg_solve(kernel, f, precond, tol=1e-8, maxiter=500, x0=None, fallback=None
     """Flexible PCG (Polak-Ribiere beta) with a nonlinear preconditioner.
 
     fallback: a MultiGrid solver; if the residual does not drop below
-    stall_factor times its value stall_window iterations earlier, the solve
-    continues with plain GMG-PCG from the current iterate.
+    stall_factor times its value stall_window iterations earlier, or the
+    preconditioned direction breaks down (r.z or p.Kp not positive and
+    finite), the solve continues with plain GMG-PCG from the current iterate.
+    Without a fallback a breakdown restarts the recurrence with z = r.
     Returns (x, info) with info = {iterations, fallback, residual, history}."""
     vec = precond.vec if hasattr(precond, "vec") else _Vec(kernel)
     n = f.numel()
@@ -385,33 +387,64 @@ def fpcg_solve(kernel, f, precond, tol=1e-8, maxiter=500, x0=None, fallback=None
     z_old = torch.zeros_like(f)
     p = torch.zeros_like(f)
     q = torch.zeros_like(f)
-    precond(r, z)
-    p.copy_(z)
-    rz = vec.dot(r, z)
+    def ok(v):
+        return math.isfinite(v) and v > 0.0
+
     hist = [math.sqrt(vec.dot(r, r)) / nb]
-    info = {"fallback": False, "fallback_iterations": 0}
+    info = {"fallback": False, "fallback_iterations": 0, "restarts": 0}
+
+    def handover(x):
+        info["fallback"] = True
+        fallback.tol = tol
+        xf, res = _fallback_from(fallback, f, x)
+        info["fallback_iterations"] = fallback.last_iterations
+        hist.append(res)
+        return xf
+
+    def restart():
+        z.copy_(r)
+        p.copy_(z)
+        info["restarts"] += 1
+        return vec.dot(r, z)
+
+    precond(r, z)
+    rz = vec.dot(r, z)
+    p.copy_(z)
     it = 0
+    if not ok(rz) and hist[-1] > 0.0:
+        if fallback is not None:
+            x = handover(x)
+            maxiter = 0
+        else:
+            rz = restart()
     for it in range(1, maxiter + 1):
         vec.ku(p, q)
         pq = vec.dot(p, q)
-        alpha = rz / pq
-        vec.axpby(alpha, p, 1.0, x)
-        vec.axpby(-alpha, q, 1.0, r)
-        R = math.sqrt(max(vec.dot(r, r), 0.0)) / nb
-        hist.append(R)
-        if R < tol:
+        breakdown = not ok(pq)
+        if not breakdown:
+            alpha = rz / pq
+            vec.axpby(alpha, p, 1.0, x)
+            vec.axpby(-alpha, q, 1.0, r)
+            R = math.sqrt(max(vec.dot(r, r), 0.0)) / nb
+            hist.append(R)
+            if R < tol:
+                break
+        stalled = it >= stall_window and hist[-1] > stall_factor * hist[-1 - stall_window]
+        if fallback is not None and (breakdown or stalled):
+            x = handover(x)
             break
-        if fallback is not None and it >= stall_window and R > stall_factor * hist[-1 - stall_window]:
-            info["fallback"] = True
-            fallback.tol = tol
-            xf, res = _fallback_from(fallback, f, x)
-            info["fallback_iterations"] = fallback.last_iterations
-            x = xf
-            hist.append(res)
-            break
+        if breakdown:
+            rz = restart()
+            continue
         z_old.copy_(z)
         precond(r, z)
         rz_new = vec.dot(r, z)
+        if not ok(rz_new):
+            if fallback is not None:
+                x = handover(x)
+                break
+            rz = restart()
+            continue
         beta = (rz_new - vec.dot(r, z_old)) / rz
         rz = rz_new
         vec.axpby(1.0, z, beta, p)

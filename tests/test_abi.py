@@ -30,6 +30,39 @@ def test_binding_table_matches_header():
     assert set(_lib.SIGNATURES) == _declared()
 
 
+def _prototypes():
+    """name -> list of parameter type strings, from the header declarations."""
+    src = open(os.path.join(ROOT, "include", "fantom.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    src = re.sub(r"#.*", "", src)
+    out = {}
+    for m in re.finditer(r"\b(fantom_[A-Za-z0-9_]+)\s*\(([^)]*)\)\s*;", src):
+        params = [p.strip() for p in m.group(2).split(",") if p.strip() and p.strip() != "void"]
+        out[m.group(1)] = params
+    return out
+
+
+def test_binding_arity_and_kinds_match_header():
+    """Every ctypes binding has the header's parameter count, and each slot
+    has the right kind (pointer / floating / integer), so that no argument
+    (the trailing stream above all) is passed with ctypes' default int
+    conversion."""
+    import ctypes
+    from pyfantom_amd import _lib
+    protos = _prototypes()
+    assert set(protos) == set(_lib.SIGNATURES)
+    for name, (_, argtypes) in _lib.SIGNATURES.items():
+        params = protos[name]
+        assert len(argtypes) == len(params), (name, len(argtypes), len(params))
+        for a, p in zip(argtypes, params):
+            if "*" in p:
+                assert a in (ctypes.c_void_p, ctypes.c_char_p) or issubclass(a, ctypes._Pointer), (name, p, a)
+            elif p.split()[0] in ("double", "float"):
+                assert a in (ctypes.c_double, ctypes.c_float), (name, p, a)
+            else:
+                assert a in (ctypes.c_int, ctypes.c_long, ctypes.c_longlong, ctypes.c_size_t), (name, p, a)
+
+
 def test_abi_version():
     from pyfantom_amd import _lib
     assert _lib.load().fantom_abi_version() == 2

@@ -1,0 +1,84 @@
+"""Flexible PCG control logic (pyfantom_amd/learned.py:fpcg_solve) on the CPU
+with a small SPD matrix and a vector backend in torch: convergence with a good
+preconditioner, restarts when the preconditioner breaks down (z = 0), and the
+hand-over to the fallback solver."""
+import numpy as np
+import pytest
+import torch
+
+from pyfantom_amd.learned import fpcg_solve
+
+
+class _Vec:
+    def __init__(self, A):
+        self.A = A
+
+    def residual(self, f, x, r):
+        r.copy_(f - self.A @ x)
+
+    def dot(self, a, b):
+        return float(torch.dot(a, b))
+
+    def ku(self, u, y):
+        y.copy_(self.A @ u)
+
+    def axpby(self, a, x, b, y):
+        y.mul_(b).add_(a * x)
+
+
+class _Precond:
+    def __init__(self, A, kind):
+        self.vec = _Vec(A)
+        self.d = torch.diag(A)
+        self.kind = kind
+        self.calls = 0
+
+    def __call__(self, r, z):
+        self.calls += 1
+        if self.kind == "jacobi":
+            z.copy_(r / self.d)
+        else:
+            z.zero_()
+
+
+class _Fallback:
+    def __init__(self, A):
+        self.A = A
+        self.last_iterations = 0
+        self.tol = None
+        self.last_x0 = None
+
+    def solve(self, f, use_last=True):
+        x = torch.linalg.solve(self.A, f)
+        self.last_iterations = 1
+        return x, 0.0
+
+
+def _spd(n=60, seed=0):
+    rng = np.random.default_rng(seed)
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    A = Q @ np.diag(np.linspace(1.0, 30.0, n)) @ Q.T
+    return torch.from_numpy(A), torch.from_numpy(rng.standard_normal(n))
+
+
+def test_fpcg_converges_with_jacobi():
+    A, f = _spd()
+    x, info = fpcg_solve(None, f, _Precond(A, "jacobi"), tol=1e-10, maxiter=200)
+    assert info["residual"] < 1e-10 and not info["fallback"] and info["restarts"] == 0
+    assert torch.allclose(x, torch.linalg.solve(A, f), atol=1e-8)
+
+
+def test_fpcg_breakdown_restarts_without_fallback():
+    A, f = _spd(seed=1)
+    x, info = fpcg_solve(None, f, _Precond(A, "zero"), tol=1e-8, maxiter=2000)
+    assert info["restarts"] > 0
+    assert info["residual"] < 1e-8
+    assert np.isfinite(x.numpy()).all()
+
+
+def test_fpcg_breakdown_hands_over_to_fallback():
+    A, f = _spd(seed=2)
+    fb = _Fallback(A)
+    x, info = fpcg_solve(None, f, _Precond(A, "zero"), tol=1e-8, maxiter=50, fallback=fb)
+    assert info["fallback"]
+    assert torch.allclose(x, torch.linalg.solve(A, f), atol=1e-10)

@@ -394,3 +394,53 @@ def test_galerkin_levels(oracle, nel):
         # A3 = P^T A2^U P directly from explicit element matrices: compare energies on a random vector
         torch.cuda.synchronize()
         assert torch.isfinite(Ke3).all()
+
+
+@pytest.mark.parametrize("nel", [(16, 8, 12), (32, 16, 16)])
+def test_stencil_levels(oracle, nel):
+    """Assembled 27-point stencil of the level-2 Galerkin operator against
+    scipy P^T A P (K.u), and its Jacobi / residual / K.u-dot epilogues
+    against the same formulas evaluated in numpy."""
+    from pyfantom_amd import _lib
+    torch = _torch()
+    print(f"stencil {nel}", flush=True)
+    K0 = np.ascontiguousarray(oracle.k0_exact(3, 0.3))
+    rng = np.random.default_rng(8)
+    E = rng.uniform(0.01, 1, int(np.prod(nel))) ** 3
+    _, el, _, _, con = _case(oracle, nel, bc="clamp")
+    apply, nels, cons = _galerkin_ref(oracle, nel, E, con)
+    G = np.zeros(8 * 576)
+    B = np.zeros(576)
+    Gd = np.zeros(8 * 24)
+    _lib.check(_lib.load().fantom_galerkin_tables(K0.ctypes.data, G.ctypes.data, B.ctypes.data, Gd.ctypes.data,
+                                                  None))
+    Gd_d, Ed = _dev(G), _dev(E)
+    g2 = nels[2]
+    nn2 = int(np.prod([n + 1 for n in g2]))
+    Ke = torch.empty(int(np.prod(g2)) * 576, dtype=torch.float64, device="cuda")
+    _lib.call("fantom_gal_build_f64", *g2, 0, Gd_d.data_ptr(), Ed.data_ptr(), None, Ke.data_ptr())
+    m2 = _dev(_mask_from_con(cons[2], 3))
+    S = torch.empty(nn2 * 243, dtype=torch.float64, device="cuda")
+    _lib.call("fantom_stencil_build_f64", *g2, Ke.data_ptr(), m2.data_ptr(), S.data_ptr())
+    u = rng.standard_normal(nn2 * 3)
+    b = rng.standard_normal(nn2 * 3)
+    dinv = rng.uniform(0.5, 2.0, nn2 * 3)
+    ud, bd, dd = _dev(u), _dev(b), _dev(dinv)
+    y = torch.empty_like(ud)
+    ref = apply(2, u)
+    _lib.call("fantom_stencil_ku_f64", 0, *g2, S.data_ptr(), ud.data_ptr(), y.data_ptr(), None, None, 0.0, None)
+    assert _rel(_np(y), ref) <= 1e-12
+    _lib.call("fantom_stencil_ku_f64", 1, *g2, S.data_ptr(), ud.data_ptr(), y.data_ptr(), bd.data_ptr(),
+              dd.data_ptr(), 0.6, None)
+    assert _rel(_np(y), u + 0.6 * dinv * (b - ref)) <= 1e-12
+    _lib.call("fantom_stencil_ku_f64", 2, *g2, S.data_ptr(), ud.data_ptr(), y.data_ptr(), bd.data_ptr(), None, 0.0,
+              None)
+    assert _rel(_np(y), b - ref) <= 1e-12
+    npart = _lib.load().fantom_stencil_partials(*g2)
+    part = torch.zeros(npart, dtype=torch.float64, device="cuda")
+    scal = torch.zeros(16, dtype=torch.float64, device="cuda")
+    _lib.call("fantom_stencil_ku_f64", 3, *g2, S.data_ptr(), ud.data_ptr(), y.data_ptr(), None, None, 0.0,
+              part.data_ptr())
+    _lib.call("fantom_finalize_f64", part.data_ptr(), npart, scal.data_ptr(), _lib.FIN_STORE, 5)
+    assert _rel(_np(y), ref) <= 1e-12
+    assert abs(float(scal[5].item()) - u @ ref) <= 1e-12 * abs(u @ ref)
