@@ -119,8 +119,10 @@ int fantom_coarsen_mask(int nelx, int nely, int nelz, const uint8_t *mf, uint8_t
 /* ---- A10: coarsest direct solve ---------------------------------------------
  * Replaces the host CHOLMOD factor/solve of solvers/CUDA/_solvers.py:498-516.
  * dense_assemble writes the dense n_dof x n_dof K(E) with the Dirichlet
- * convention; spd_inverse inverts an SPD matrix in place (Gauss-Jordan,
- * work = 2n doubles); gemv computes y = A x. */
+ * convention; spd_inverse inverts an SPD matrix in place (blocked
+ * Gauss-Jordan without pivoting, 2 ceil(n/32) launches; work = n*n + 1024
+ * doubles);
+ * gemv computes y = A x. */
 int fantom_dense_assemble_f64(int nelx, int nely, int nelz, const double *K0, const double *E,
                               const uint8_t *mask, double *A, void *stream);
 int fantom_spd_inverse_f64(int n, double *A, double *work, void *stream);
@@ -161,6 +163,23 @@ int fantom_filter_adj_f64(int nelx, int nely, int nelz, int ntap, const int32_t 
 long fantom_oc_partials(long n);
 int fantom_oc_update_f64(long n, const double *x, const double *df, const double *dg, double vf, double move,
                          double *ocp, double *state, double *partials, double *x_new, void *stream);
+
+/* ---- A17: MMA update ----------------------------------------------------------
+ * Replaces mmasub/subsolv (mma/CPU/_mma.py:32-165, 288-528) as driven by
+ * Optimizers/CPU/MMA.py:103-180: asymptotes, convex approximation and the
+ * primal-dual interior-point solve of the subproblem, for 1 <= m <= 8
+ * constraints, uniform bounds [xmin, xmax], constants a0, a[m], c[m], d[m]
+ * (host arrays), fval[m] on the host.  Device pointers: x, xold1, xold2
+ * (NULL allowed while iter <= 2), df0dx (n), dfdx (m x n row-major), low/upp
+ * (n, updated in place), x_new (n), work (workspace_size bytes).  iter counts
+ * from 1 as the reference's.  Synchronises the stream once per Newton step
+ * (scalar read-back); newton_steps (optional) gets the number of steps. */
+long fantom_mma_workspace_size(long n, int m);
+int fantom_mma_update_f64(long n, int m, int iter, const double *x, const double *xold1, const double *xold2,
+                          double xmin, double xmax, const double *df0dx, const double *fval, const double *dfdx,
+                          double *low, double *upp, double move, double a0, const double *a, const double *c,
+                          const double *d, double sub_tol, int sub_maxiter, double *x_new, void *work,
+                          long work_bytes, int *newton_steps, void *stream);
 
 /* ---- A8: Galerkin coarse operators (hex8) ------------------------------------
  * Replace get_restricted_l0/l1p (core/CUDA/_mgm.py:19-171, mgm.cu:246-590):
@@ -246,6 +265,14 @@ int fantom_mx_setup(void *handle, const double *E, const uint8_t *mask, void *st
 int fantom_mx_solve(void *handle, const double *f, double *x, double tol, double inner_tol, int max_outer,
                     int max_inner, int use_x0, int use_graph, int *outer_iters, int *inner_iters, double *rel_res,
                     void *stream);
+/* FP64 PCG preconditioned by the handle's FP32 V-cycle (one per iteration):
+ * the fine K.u, the vectors, dot products and step lengths stay FP64, the
+ * V-cycle runs on FP32 copies.  Stops when the recursive relative residual is
+ * below tol; rel_res gets the true FP64 relative residual ||f - K x|| / ||f||.
+ * No counterpart in the reference (its solver is FP64 throughout,
+ * solvers/CUDA/_solvers.py:573-660). */
+int fantom_mx_solve_pcg(void *handle, const double *f, double *x, double tol, int maxiter, int use_x0,
+                        int use_graph, int *iters, double *rel_res, void *stream);
 
 /* ---- A20: learned preconditioner kernels (no reference counterpart) ----------
  * Channels-last BF16 volumes (D, H, W, C), C a multiple of 16; weights

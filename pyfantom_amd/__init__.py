@@ -2,7 +2,7 @@
 finite-element solve path, behind the object API of pyFANTOM's CUDA backend.
 
 >>> from pyfantom_amd import (StructuredMesh3D, StructuredStiffnessKernel, MultiGrid,
-...                           FiniteElement, StructuredFilter3D, MinimumCompliance, OC)
+...                           FiniteElement, StructuredFilter3D, MinimumCompliance, OC, MMA)
 
 The names match ``pyFANTOM.CUDA`` (pyFANTOM/CUDA/__init__.py:10-16) for the
 structured path.  Device arrays are torch tensors on the HIP device; every
@@ -15,7 +15,7 @@ from .stiffness import StructuredStiffnessKernel  # noqa: F401
 from .solvers import CG, MultiGrid  # noqa: F401
 from .fe import FiniteElement  # noqa: F401
 from .problem import MinimumCompliance  # noqa: F401
-from .optimizers import OC  # noqa: F401
+from .optimizers import MMA, OC  # noqa: F401
 
 __all__ = ["StructuredMesh2D", "StructuredMesh3D", "StructuredFilter2D", "StructuredFilter3D", "LinearElasticity",
-           "StructuredStiffnessKernel", "MultiGrid", "CG", "FiniteElement", "MinimumCompliance", "OC"]
+           "StructuredStiffnessKernel", "MultiGrid", "CG", "FiniteElement", "MinimumCompliance", "OC", "MMA"]

@@ -74,6 +74,7 @@ SIGNATURES = {
     "fantom_mx_setup": (_c_int, [_vp, _vp, _vp, _vp]),
     "fantom_mx_solve": (_c_int, [_vp, _vp, _vp, _c_double, _c_double, _c_int, _c_int, _c_int, _c_int,
                                  ctypes.POINTER(_c_int), ctypes.POINTER(_c_int), ctypes.POINTER(_c_double), _vp]),
+    "fantom_mx_solve_pcg": (_c_int, [_vp, _vp, _vp, _c_double, _c_int, _c_int, _c_int, _vp, _vp, _vp]),
     "fantom_conv3d_fwd_bf16": (_c_int, [_c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp,
                                         _c_int, _vp]),
     "fantom_conv3d_wgrad_parts": (_c_long, [_c_int, _c_int, _c_int, _c_int, _c_int, ctypes.POINTER(_c_long)]),
@@ -101,6 +102,10 @@ SIGNATURES = {
                                        _vp]),
     "fantom_stencil_partials": (_c_long, [_c_int, _c_int, _c_int]),
     "fantom_oc_update_f64": (_c_int, [_c_long, _vp, _vp, _vp, _c_double, _c_double, _vp, _vp, _vp, _vp, _vp]),
+    "fantom_mma_workspace_size": (_c_long, [_c_long, _c_int]),
+    "fantom_mma_update_f64": (_c_int, [_c_long, _c_int, _c_int, _vp, _vp, _vp, _c_double, _c_double, _vp, _vp, _vp,
+                                       _vp, _vp, _c_double, _c_double, _vp, _vp, _vp, _c_double, _c_int, _vp, _vp,
+                                       _c_long, _vp, _vp]),
 }
 
 _lib = None

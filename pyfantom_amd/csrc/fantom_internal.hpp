@@ -22,7 +22,8 @@ int launch_ku_tile(int mode, const Grid &g, const double *Kb, const double *E, c
 int launch_ku_tile_f32(int mode, const Grid &g, const double *Kb, const float *E, const uint8_t *mask,
                        const float *u, float *y, const float *b, const float *dinv, double omega, double *partials,
                        hipStream_t s);
-long ku_tile_partials(const Grid &g);
+long ku_tile_partials(const Grid &g);       // FP64 operator
+long ku_tile_partials_f32(const Grid &g);   // FP32 operator
 
 // mg.hip
 int launch_restrict(const Grid &gf, int dim, int dof, int normalize, const uint8_t *mask_c, const double *rf,

@@ -402,13 +402,13 @@ __global__ void dense_from_elem(Grid g, const double *__restrict__ Ke, const uin
             }
 }
 
-long gal1_partials(const Grid &g) { return (long)tile_blocks(tile_shape(g)); }
+long gal1_partials(const Grid &g) { return (long)tile_blocks(tile_shape(g, kGal1PerCU)); }
 
 template <typename T>
 static int gal1_launch(int mode, const Grid &g, const double *Btab, const T *Ef, const uint8_t *mask, const T *u,
                        T *y, const T *b, const T *dinv, double omega, double *partials, hipStream_t s)
 {
-    const TileShape ts = tile_shape(g);
+    const TileShape ts = tile_shape(g, kGal1PerCU);
     const unsigned blocks = tile_blocks(ts);
     const T om = (T)omega;
     Gal1Op<T> op;

@@ -120,7 +120,20 @@ static int pack_blocks(const double *Kb, double *packed)
     return kStatusOk;
 }
 
-TileShape tile_shape(const Grid &g)
+static int device_cus()
+{
+    static const int n = [] {
+        int d = 0, c = 0;
+        if (hipGetDevice(&d) == hipSuccess && hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, d) == hipSuccess &&
+            c > 0)
+            return c;
+        (void)hipGetLastError();
+        return 256;
+    }();
+    return n;
+}
+
+TileShape tile_shape(const Grid &g, int per_cu)
 {
     TileShape ts;
     ts.tiles_x = (g.nx + kTOut - 1) / kTOut;
@@ -129,12 +142,22 @@ TileShape tile_shape(const Grid &g)
         const char *e = getenv("FANTOM_TILE_CHUNK");
         return e ? atoi(e) : 0;
     }();
-    int chunk = forced;
-    if (chunk <= 0) {
-        // >= ~1024 workgroups, chunks of 4..32 planes
+    int chunk = forced > 0 ? (forced < g.ny ? forced : g.ny) : 0;
+    if (chunk == 0) {
         const long cols = (long)ts.tiles_x * ts.tiles_z;
-        chunk = (int)((cols * g.ny) / 1024);
-        chunk = chunk < 4 ? 4 : (chunk > 32 ? 32 : chunk);
+        const long cap = (long)device_cus() * per_cu;
+        long best = -1;
+        for (int n = 1; n <= g.ny; ++n) {
+            const int c = (g.ny + n - 1) / n;
+            if (n > 1 && c == (g.ny + n - 2) / (n - 1))
+                continue;   // same chunk length as n - 1
+            const long rounds = (cols * ((g.ny + c - 1) / c) + cap - 1) / cap;
+            const long cost = rounds * (c + 1);
+            if (best < 0 || cost < best) {
+                best = cost;
+                chunk = c;
+            }
+        }
     }
     ts.chunk = chunk;
     ts.chunks_y = (g.ny + chunk - 1) / chunk;
@@ -147,7 +170,8 @@ unsigned tile_blocks(const TileShape &ts)
     return (unsigned)((n + 7) / 8 * 8);
 }
 
-long ku_tile_partials(const Grid &g) { return (long)tile_blocks(tile_shape(g)); }
+long ku_tile_partials(const Grid &g) { return (long)tile_blocks(tile_shape(g, kFineF64PerCU)); }
+long ku_tile_partials_f32(const Grid &g) { return (long)tile_blocks(tile_shape(g, kFineF32PerCU)); }
 
 template <typename T>
 static int launch_ku_tile_t(int mode, const Grid &g, const double *Kb, const T *E, const uint8_t *mask, const T *u,
@@ -161,7 +185,7 @@ static int launch_ku_tile_t(int mode, const Grid &g, const double *Kb, const T *
     for (int i = 0; i < kPackedBlocks; ++i)
         op.c[i] = (T)packed[i];
     op.E = E;
-    const TileShape ts = tile_shape(g);
+    const TileShape ts = tile_shape(g, sizeof(T) == 8 ? kFineF64PerCU : kFineF32PerCU);
     const unsigned blocks = tile_blocks(ts);
     const T om = (T)omega;
 #define FANTOM_TILE_CASE(M)                                                                                   \

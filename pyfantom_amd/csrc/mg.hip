@@ -17,11 +17,14 @@ constexpr int kBlock = 256;
 __device__ __forceinline__ double tw3(int b) { return b == 3 ? 3.375 : (b == 2 ? 4.5 : (b == 1 ? 6.0 : 8.0)); }
 __device__ __forceinline__ double tw2(int b) { return b == 2 ? 2.25 : (b == 1 ? 3.0 : 4.0); }
 
-// One thread per coarse node.  mask_c (optional): constrained coarse DOFs get 0.
-template <typename T>
-__global__ __launch_bounds__(kBlock) void restrict3d(Grid gf, int dof, int normalize,
-                                                   const uint8_t *__restrict__ mask_c, const T *__restrict__ rf,
-                                                   T *__restrict__ rc)
+// One thread per coarse node.  mask_c (optional): constrained coarse DOFs get
+// 0.  The 27 fine neighbours are read unconditionally (an index outside the
+// grid is clamped to the centre node and weighted 0, which leaves every sum
+// unchanged), so that the loads issue together instead of one latency after
+// another; the summation order is the reference's.
+template <typename T, int DOF>
+__global__ __launch_bounds__(kBlock) void restrict3d(Grid gf, int normalize, const uint8_t *__restrict__ mask_c,
+                                                   const T *__restrict__ rf, T *__restrict__ rc)
 {
     const int nxc = gf.nelx / 2 + 1, nyc = gf.nely / 2 + 1, nzc = gf.nelz / 2 + 1;
     const long ncn = (long)nxc * nyc * nzc;
@@ -32,33 +35,40 @@ __global__ __launch_bounds__(kBlock) void restrict3d(Grid gf, int dof, int norma
     const long t = ic / nzc;
     const int icx = (int)(t % nxc), jc = (int)(t / nxc);
     const int bnd = (icx == 0 || icx == nxc - 1) + (jc == 0 || jc == nyc - 1) + (kc == 0 || kc == nzc - 1);
-    double acc[3] = {0.0, 0.0, 0.0};
+    const long fc = 2 * kc + (long)gf.nz * (2 * icx) + (long)gf.nz * gf.nx * (2 * jc);   // coincident fine node
+    double acc[DOF];
+#pragma unroll
+    for (int d = 0; d < DOF; ++d)
+        acc[d] = 0.0;
     // x outer, y, z inner: the reference's summation order (mgm.cu:28-60)
+#pragma unroll
     for (int di = -1; di <= 1; ++di) {
         const int ii = 2 * icx + di;
-        if (ii < 0 || ii >= gf.nx)
-            continue;
+        const bool vi = ii >= 0 && ii < gf.nx;
+#pragma unroll
         for (int dj = -1; dj <= 1; ++dj) {
             const int jj = 2 * jc + dj;
-            if (jj < 0 || jj >= gf.ny)
-                continue;
+            const bool vj = jj >= 0 && jj < gf.ny;
+#pragma unroll
             for (int dk = -1; dk <= 1; ++dk) {
                 const int kk = 2 * kc + dk;
-                if (kk < 0 || kk >= gf.nz)
-                    continue;
-                const int dist = abs(di) + abs(dj) + abs(dk);
-                const double w = dist == 0 ? 1.0 : (dist == 1 ? 0.5 : (dist == 2 ? 0.25 : 0.125));
-                const long f = kk + (long)gf.nz * ii + (long)gf.nz * gf.nx * jj;
-                for (int d = 0; d < dof; ++d)
-                    acc[d] += w * rf[f * dof + d];
+                const bool v = vi && vj && kk >= 0 && kk < gf.nz;
+                const int dist = (di != 0) + (dj != 0) + (dk != 0);
+                const double w0 = dist == 0 ? 1.0 : (dist == 1 ? 0.5 : (dist == 2 ? 0.25 : 0.125));
+                const double w = v ? w0 : 0.0;
+                const long f = v ? kk + (long)gf.nz * ii + (long)gf.nz * gf.nx * jj : fc;
+#pragma unroll
+                for (int d = 0; d < DOF; ++d)
+                    acc[d] += w * (double)rf[f * DOF + d];
             }
         }
     }
     const int mc = mask_c ? mask_c[ic] : 0;
-    for (int d = 0; d < dof; ++d) {
+#pragma unroll
+    for (int d = 0; d < DOF; ++d) {
         // divide (not multiply by 1/TW) to round like the reference (mgm.cu:66)
         const double v = normalize ? acc[d] / tw3(bnd) : acc[d];
-        rc[ic * dof + d] = ((mc >> d) & 1) ? T(0) : (T)v;
+        rc[ic * DOF + d] = ((mc >> d) & 1) ? T(0) : (T)v;
     }
 }
 
@@ -100,10 +110,9 @@ __global__ __launch_bounds__(kBlock) void restrict2d(Grid gf, int dof, int norma
 // One thread per fine node: average of parent coarse values (each divided by
 // its TW when normalize).  accumulate: xf += P ec, else xf = P ec.
 // mask_f (optional): constrained fine DOFs receive 0.
-template <typename T>
-__global__ __launch_bounds__(kBlock) void prolong3d(Grid gf, int dof, int normalize,
-                                                  const uint8_t *__restrict__ mask_f, const T *__restrict__ ec,
-                                                  T *__restrict__ xf, int accumulate)
+template <typename T, int DOF>
+__global__ __launch_bounds__(kBlock) void prolong3d(Grid gf, int normalize, const uint8_t *__restrict__ mask_f,
+                                                  const T *__restrict__ ec, T *__restrict__ xf, int accumulate)
 {
     const int nxc = gf.nelx / 2 + 1, nyc = gf.nely / 2 + 1, nzc = gf.nelz / 2 + 1;
     const long f = (long)blockIdx.x * kBlock + threadIdx.x;
@@ -112,31 +121,45 @@ __global__ __launch_bounds__(kBlock) void prolong3d(Grid gf, int dof, int normal
     const int k = (int)(f % gf.nz);
     const long t = f / gf.nz;
     const int i = (int)(t % gf.nx), j = (int)(t / gf.nx);
-    double acc[3] = {0.0, 0.0, 0.0};
+    double acc[DOF];
+#pragma unroll
+    for (int d = 0; d < DOF; ++d)
+        acc[d] = 0.0;
     int count = 0;
-    for (int ci = i / 2; ci <= i / 2 + (i & 1); ++ci) {
-        if (ci >= nxc)
-            continue;
-        for (int cj = j / 2; cj <= j / 2 + (j & 1); ++cj) {
-            if (cj >= nyc)
-                continue;
-            for (int ck = k / 2; ck <= k / 2 + (k & 1); ++ck) {
-                if (ck >= nzc)
-                    continue;
+    // parents (i/2 + oi, j/2 + oj, k/2 + ok) with oi <= (i & 1) etc., visited in
+    // a fixed order with the non-parents weighted 0
+    const long c0 = (k / 2) + (long)nzc * (i / 2) + (long)nzc * nxc * (j / 2);
+#pragma unroll
+    for (int oi = 0; oi < 2; ++oi) {
+        const int ci = i / 2 + oi;
+        const bool vi = oi <= (i & 1) && ci < nxc;
+#pragma unroll
+        for (int oj = 0; oj < 2; ++oj) {
+            const int cj = j / 2 + oj;
+            const bool vj = oj <= (j & 1) && cj < nyc;
+#pragma unroll
+            for (int ok = 0; ok < 2; ++ok) {
+                const int ck = k / 2 + ok;
+                const bool v = vi && vj && ok <= (k & 1) && ck < nzc;
                 const int bnd = (ci == 0 || ci == nxc - 1) + (cj == 0 || cj == nyc - 1) + (ck == 0 || ck == nzc - 1);
-                const long c = ck + (long)nzc * ci + (long)nzc * nxc * cj;
-                for (int d = 0; d < dof; ++d)
-                    acc[d] += normalize ? (double)ec[c * dof + d] / tw3(bnd) : (double)ec[c * dof + d];
-                count++;
+                const long c = v ? ck + (long)nzc * ci + (long)nzc * nxc * cj : c0;
+                const double den = normalize ? tw3(bnd) : 1.0;
+#pragma unroll
+                for (int d = 0; d < DOF; ++d) {
+                    const double e = (double)ec[c * DOF + d];
+                    acc[d] += v ? (normalize ? e / den : e) : 0.0;
+                }
+                count += v;
             }
         }
     }
     const int mf = mask_f ? mask_f[f] : 0;
-    for (int d = 0; d < dof; ++d) {
+#pragma unroll
+    for (int d = 0; d < DOF; ++d) {
         double v = count > 0 ? acc[d] / count : 0.0;
         if ((mf >> d) & 1)
             v = 0.0;
-        const long r = f * dof + d;
+        const long r = f * DOF + d;
         xf[r] = accumulate ? (T)((double)xf[r] + v) : (T)v;
     }
 }
@@ -296,36 +319,111 @@ __global__ __launch_bounds__(kBlock) void dense_assemble(Grid g, int dim, const 
     }
 }
 
-// In-place Gauss-Jordan inversion of an SPD matrix (no pivoting needed), one
-// pivot per pair of launches.  work holds 2n doubles: the old pivot column and
-// the scaled pivot row.
-__global__ __launch_bounds__(kBlock) void gj_stage(int n, int k, const double *__restrict__ A,
-                                                 double *__restrict__ work)
+// Blocked Gauss-Jordan inversion of an SPD matrix (no pivoting needed).
+// Per block of kGJ pivots K, with R the remaining indices,
+//   A_KK <- P = A_KK^-1,  A_KR <- P A_KR,  A_RK <- -A_RK P,
+//   A_RR <- A_RR - A_RK P A_KR.
+// gj_pivot (one workgroup) inverts A_KK in LDS and stores P; gj_update (one
+// workgroup per kGJ x kGJ output tile, aligned with the pivot blocks) reads
+// one buffer and writes the other, because every workgroup reads the tiles of
+// row block K and column block K.
+constexpr int kGJ = 32;
+constexpr int kGJPer = kGJ * kGJ / 256;
+
+// One wave inverts the pivot block in registers: lane l holds row l % 32,
+// columns [16 (l / 32), 16 (l / 32) + 16); the pivot row, pivot column and
+// pivot come from other lanes by shuffles.  The step loop is unrolled so that
+// every register index is a constant.
+__global__ __launch_bounds__(64) void gj_pivot(int n, int k0, const double *__restrict__ A, double *__restrict__ Pout)
 {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n)
-        return;
-    const double p = A[(long)k * n + k];
-    work[i] = A[(long)i * n + k];
-    work[n + i] = A[(long)k * n + i] / p;
+    const int lane = threadIdx.x, r = lane & 31, h = lane >> 5;
+    const int b = min(kGJ, n - k0);
+    double v[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        const int c = 16 * h + j;
+        v[j] = (r < b && c < b) ? A[(long)(k0 + r) * n + k0 + c] : (r == c ? 1.0 : 0.0);
+    }
+#pragma unroll
+    for (int k = 0; k < kGJ; ++k) {
+        if (k < b) {   // uniform: b is the same for every lane
+            const int hk = k >> 4, jk = k & 15;
+            double rowk[16];
+#pragma unroll
+            for (int j = 0; j < 16; ++j)
+                rowk[j] = __shfl(v[j], k + 32 * h, 64);           // P[k][16h + j]
+            const double rk = __shfl(v[jk], r + 32 * hk, 64);     // P[r][k]
+            const double pinv = 1.0 / __shfl(v[jk], k + 32 * hk, 64);
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                const int c = 16 * h + j;
+                double o;
+                if (r == k)
+                    o = c == k ? pinv : rowk[j] * pinv;
+                else
+                    o = c == k ? -rk * pinv : v[j] - rk * rowk[j] * pinv;
+                v[j] = o;
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 16; ++j)
+        Pout[r * kGJ + 16 * h + j] = v[j];
 }
 
-__global__ __launch_bounds__(kBlock) void gj_update(int n, int k, double *__restrict__ A,
-                                                  const double *__restrict__ work)
+__global__ __launch_bounds__(256) void gj_update(int n, int k0, const double *__restrict__ A,
+                                               const double *__restrict__ Pin, double *__restrict__ Out)
 {
-    const long idx = (long)blockIdx.x * kBlock + threadIdx.x;
-    if (idx >= (long)n * n)
-        return;
-    const int i = (int)(idx / n), j = (int)(idx % n);
-    const double p = work[k];  // old A[k][k]
-    if (i == k && j == k)
-        A[idx] = 1.0 / p;
-    else if (i == k)
-        A[idx] = work[n + j];
-    else if (j == k)
-        A[idx] = -work[i] / p;
-    else
-        A[idx] -= work[i] * work[n + j];
+    __shared__ double P[kGJ][kGJ + 1];
+    __shared__ double T[kGJ][kGJ + 1];    // A[K, J]
+    __shared__ double X[kGJ][kGJ + 1];    // P A[K, J]
+    __shared__ double Rk[kGJ][kGJ + 1];   // A[I, K]
+    const int b = min(kGJ, n - k0);
+    const int i0 = blockIdx.y * kGJ, j0 = blockIdx.x * kGJ;
+    const bool rowK = i0 == k0, colK = j0 == k0;
+    const int tid = threadIdx.x;
+    for (int q = tid; q < kGJ * kGJ; q += 256) {
+        const int r = q / kGJ, c = q % kGJ;
+        P[r][c] = Pin[q];
+        T[r][c] = (!colK && r < b && j0 + c < n) ? A[(long)(k0 + r) * n + j0 + c] : 0.0;
+        Rk[r][c] = (!rowK && i0 + r < n && c < b) ? A[(long)(i0 + r) * n + k0 + c] : 0.0;
+    }
+    __syncthreads();
+    if (!colK) {
+#pragma unroll
+        for (int m = 0; m < kGJPer; ++m) {
+            const int q = tid + 256 * m, r = q / kGJ, c = q % kGJ;
+            double x = 0.0;
+            for (int t = 0; t < b; ++t)
+                x += P[r][t] * T[t][c];
+            X[r][c] = x;
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int m = 0; m < kGJPer; ++m) {
+        const int q = tid + 256 * m, r = q / kGJ, c = q % kGJ;
+        const int i = i0 + r, j = j0 + c;
+        if (i >= n || j >= n)
+            continue;
+        double o;
+        if (rowK && colK) {
+            o = P[r][c];
+        } else if (rowK) {
+            o = X[r][c];
+        } else if (colK) {
+            double acc = 0.0;
+            for (int t = 0; t < b; ++t)
+                acc += Rk[r][t] * P[t][c];
+            o = -acc;
+        } else {
+            double acc = 0.0;
+            for (int t = 0; t < b; ++t)
+                acc += Rk[r][t] * X[t][c];
+            o = A[(long)i * n + j] - acc;
+        }
+        Out[(long)i * n + j] = o;
+    }
 }
 
 // y = A x for a dense row-major n x n matrix: one wave per row.
@@ -356,7 +454,14 @@ static int restrict_t(const Grid &gf, int dim, int dof, int normalize, const uin
 {
     if (dim == 3) {
         long ncn = (long)(gf.nelx / 2 + 1) * (gf.nely / 2 + 1) * (gf.nelz / 2 + 1);
-        hipLaunchKernelGGL(restrict3d<T>, dim3(nblk(ncn)), dim3(kBlock), 0, s, gf, dof, normalize, mask_c, rf, rc);
+        if (dof == 3)
+            hipLaunchKernelGGL((restrict3d<T, 3>), dim3(nblk(ncn)), dim3(kBlock), 0, s, gf, normalize, mask_c, rf, rc);
+        else if (dof == 1)
+            hipLaunchKernelGGL((restrict3d<T, 1>), dim3(nblk(ncn)), dim3(kBlock), 0, s, gf, normalize, mask_c, rf, rc);
+        else {
+            set_error("3D transfers take dof 1 or 3, not %d", dof);
+            return kStatusBadArg;
+        }
     } else {
         long ncn = (long)(gf.nelx / 2 + 1) * (gf.nely / 2 + 1);
         hipLaunchKernelGGL(restrict2d<T>, dim3(nblk(ncn)), dim3(kBlock), 0, s, gf, dof, normalize, mask_c, rf, rc);
@@ -369,12 +474,21 @@ template <typename T>
 static int prolong_t(const Grid &gf, int dim, int dof, int normalize, const uint8_t *mask_f, const T *ec, T *xf,
                      int accumulate, hipStream_t s)
 {
-    if (dim == 3)
-        hipLaunchKernelGGL(prolong3d<T>, dim3(nblk(gf.n_nodes)), dim3(kBlock), 0, s, gf, dof, normalize, mask_f, ec,
-                           xf, accumulate);
-    else
+    if (dim == 3) {
+        if (dof == 3)
+            hipLaunchKernelGGL((prolong3d<T, 3>), dim3(nblk(gf.n_nodes)), dim3(kBlock), 0, s, gf, normalize, mask_f, ec,
+                               xf, accumulate);
+        else if (dof == 1)
+            hipLaunchKernelGGL((prolong3d<T, 1>), dim3(nblk(gf.n_nodes)), dim3(kBlock), 0, s, gf, normalize, mask_f, ec,
+                               xf, accumulate);
+        else {
+            set_error("3D transfers take dof 1 or 3, not %d", dof);
+            return kStatusBadArg;
+        }
+    } else {
         hipLaunchKernelGGL(prolong2d<T>, dim3(nblk(gf.n_nodes)), dim3(kBlock), 0, s, gf, dof, normalize, mask_f, ec,
                            xf, accumulate);
+    }
     FANTOM_LAUNCH_CHECK("prolong");
     return kStatusOk;
 }
@@ -460,12 +574,24 @@ int launch_dense_assemble_f32E(const Grid &g, const double *K0, const float *E, 
 
 int launch_spd_inverse(int n, double *A, double *work, hipStream_t s)
 {
-    const long nn = (long)n * n;
-    for (int k = 0; k < n; ++k) {
-        hipLaunchKernelGGL(gj_stage, dim3(nblk(n)), dim3(kBlock), 0, s, n, k, A, work);
-        hipLaunchKernelGGL(gj_update, dim3(nblk(nn)), dim3(kBlock), 0, s, n, k, A, work);
+    // work: n*n doubles (the second buffer) + kGJ*kGJ (the pivot-block inverse)
+    const unsigned nt = (unsigned)((n + kGJ - 1) / kGJ);
+    double *src = A, *dst = work, *pinv = work + (size_t)n * n;
+    for (unsigned kb = 0; kb < nt; ++kb) {
+        hipLaunchKernelGGL(gj_pivot, dim3(1), dim3(64), 0, s, n, (int)(kb * kGJ), src, pinv);
+        hipLaunchKernelGGL(gj_update, dim3(nt, nt), dim3(256), 0, s, n, (int)(kb * kGJ), src, pinv, dst);
+        double *t = src;
+        src = dst;
+        dst = t;
     }
     FANTOM_LAUNCH_CHECK("spd_inverse");
+    if (src != A) {
+        const hipError_t e = hipMemcpyAsync(A, src, sizeof(double) * (size_t)n * n, hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess) {
+            set_error("spd_inverse copy: %s", hipGetErrorString(e));
+            return kStatusHipError;
+        }
+    }
     return kStatusOk;
 }
 

@@ -310,7 +310,7 @@ static void layout(MgHandle *h, int nelx, int nely, int nelz, int n_coarse, int 
     h->scal = cv.take<double>(FANTOM_SCALARS);
     h->nc = (int)h->lev[h->nlev - 1].n_dof;
     h->Ainv = cv.take<double>((size_t)h->nc * h->nc);
-    h->gj_work = cv.take<double>(2 * (size_t)h->nc);
+    h->gj_work = cv.take<double>((size_t)h->nc * h->nc + 1024);
 }
 
 static bool levels_ok(int nelx, int nely, int nelz, int n_coarse)

@@ -58,12 +58,14 @@ struct MxHandle {
     double Kb64[72];
     // FP32 inner PCG vectors
     float *d, *r, *z, *p, *q;
+    // FP64 PCG vectors of the FP32-preconditioned solve
+    double *p64, *q64;
     double *partials, *scal, *Ainv, *gj_work;
     int nc;
     hipStream_t stream;
-    hipGraphExec_t iter_exec;
-    bool have_graph;
-    int last_inner;
+    hipGraphExec_t iter_exec, pc_exec;
+    bool have_graph, have_pc_graph;
+    int last_inner, last_pc;
 };
 
 __global__ void mx_copy_f32(long n, const float *__restrict__ a, float *__restrict__ b)
@@ -178,6 +180,91 @@ __global__ void mx_zero_f32(long n, float *__restrict__ a)
         a[i] = 0.0f;
 }
 
+// ---- FP64 PCG with the FP32 V-cycle as preconditioner ---------------------------
+// x += alpha p, r -= alpha q (frozen after convergence), r32 = (float) r, and
+// the partial sums of r.r
+__global__ __launch_bounds__(256) void mxd_xr_k(long n, double *__restrict__ x, double *__restrict__ r,
+                                               float *__restrict__ r32, const double *__restrict__ p,
+                                               const double *__restrict__ q, const double *__restrict__ scal,
+                                               double *__restrict__ partials)
+{
+    const bool frozen = scal[X_DONE] != 0.0;
+    const double alpha = scal[S_ALPHA];
+    double s = 0.0;
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+        double ri = r[i];
+        if (!frozen) {
+            x[i] += alpha * p[i];
+            ri -= alpha * q[i];
+            r[i] = ri;
+        }
+        r32[i] = (float)ri;
+        s += ri * ri;
+    }
+    s = block_sum<256>(s);
+    if (threadIdx.x == 0)
+        partials[blockIdx.x] = s;
+}
+
+// convergence test on ||r|| / ||f|| (scal[X_NB] = ||f||)
+__global__ void mxd_conv_k(const double *__restrict__ partials, long np, double *__restrict__ scal)
+{
+    double s = 0.0;
+    for (long i = threadIdx.x; i < np; i += 1024)
+        s += partials[i];
+    s = block_sum<1024>(s);
+    if (threadIdx.x == 0) {
+        scal[X_K] += 1.0;
+        if (scal[X_DONE] == 0.0) {
+            const double R = sqrt(fmax(s, 0.0)) / scal[X_NB];
+            scal[X_RES] = R;
+            scal[X_ITER] = scal[X_K];
+            if (R < scal[X_TOL])
+                scal[X_DONE] = 1.0;
+        }
+    }
+}
+
+// partial sums of r . z with z the FP32 preconditioned residual
+__global__ __launch_bounds__(256) void mxd_dot_k(long n, const double *__restrict__ r, const float *__restrict__ z,
+                                                double *__restrict__ partials)
+{
+    double s = 0.0;
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256)
+        s += r[i] * (double)z[i];
+    s = block_sum<256>(s);
+    if (threadIdx.x == 0)
+        partials[blockIdx.x] = s;
+}
+
+// p = z + beta p (first: p = z)
+__global__ void mxd_p_k(long n, double *__restrict__ p, const float *__restrict__ z, const double *__restrict__ scal,
+                        int first)
+{
+    const double beta = first ? 0.0 : scal[S_BETA];
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        p[i] = first ? (double)z[i] : (double)z[i] + beta * p[i];
+}
+
+__global__ void mxd_down_k(long n, const double *__restrict__ r, float *__restrict__ r32)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        r32[i] = (float)r[i];
+}
+
+// ||f|| into scal[X_NB] (1 when f = 0)
+__global__ void mxd_norm_k(const double *__restrict__ partials, long np, double *__restrict__ scal)
+{
+    double s = 0.0;
+    for (long i = threadIdx.x; i < np; i += 1024)
+        s += partials[i];
+    s = block_sum<1024>(s);
+    if (threadIdx.x == 0) {
+        const double nb = sqrt(fmax(s, 0.0));
+        scal[X_NB] = nb > 0.0 ? nb : 1.0;
+    }
+}
+
 #define MX_TRY(expr)                                                                                       \
     do {                                                                                                   \
         int _s = (expr);                                                                                   \
@@ -254,7 +341,7 @@ static int mx_iteration(MxHandle *h)
     const unsigned G = vec_grid(n);
     MxLevel &L0 = h->lev[0];
     MX_TRY(mx_ku(h, 0, MODE_KUDOT, h->p, h->q, nullptr, nullptr, h->partials));
-    MX_TRY(launch_finalize(h->partials, ku_tile_partials(L0.g), h->scal, FIN_ALPHA, 0, h->stream));
+    MX_TRY(launch_finalize(h->partials, ku_tile_partials_f32(L0.g), h->scal, FIN_ALPHA, 0, h->stream));
     hipLaunchKernelGGL(mx_xr_k, dim3(G), dim3(256), 0, h->stream, n, h->d, h->r, h->p, h->q, h->scal, h->partials);
     hipLaunchKernelGGL(mx_conv_k, dim3(1), dim3(1024), 0, h->stream, h->partials, (long)G, h->scal);
     MX_TRY(mx_precond(h, h->r, h->z));
@@ -262,6 +349,25 @@ static int mx_iteration(MxHandle *h)
     MX_TRY(launch_finalize(h->partials, (long)G, h->scal, FIN_BETA, 0, h->stream));
     hipLaunchKernelGGL(mx_p_k, dim3(G), dim3(256), 0, h->stream, n, h->p, h->z, h->scal, 0);
     FANTOM_LAUNCH_CHECK("mx iteration");
+    return kStatusOk;
+}
+
+static int mxd_iteration(MxHandle *h)
+{
+    const long n = h->n;
+    const unsigned G = vec_grid(n);
+    hipStream_t s = h->stream;
+    MX_TRY(launch_ku_tile(MODE_KUDOT, h->g, h->Kb64, h->E64, h->mask, h->p64, h->q64, nullptr, nullptr, 0.0,
+                          h->partials, s));
+    MX_TRY(launch_finalize(h->partials, ku_tile_partials(h->g), h->scal, FIN_ALPHA, 0, s));
+    hipLaunchKernelGGL(mxd_xr_k, dim3(G), dim3(256), 0, s, n, h->x64, h->r64, h->r, h->p64, h->q64, h->scal,
+                       h->partials);
+    hipLaunchKernelGGL(mxd_conv_k, dim3(1), dim3(1024), 0, s, h->partials, (long)G, h->scal);
+    MX_TRY(mx_precond(h, h->r, h->z));
+    hipLaunchKernelGGL(mxd_dot_k, dim3(G), dim3(256), 0, s, n, h->r64, h->z, h->partials);
+    MX_TRY(launch_finalize(h->partials, (long)G, h->scal, FIN_BETA, 0, s));
+    hipLaunchKernelGGL(mxd_p_k, dim3(G), dim3(256), 0, s, n, h->p64, h->z, h->scal, 0);
+    FANTOM_LAUNCH_CHECK("mxd iteration");
     return kStatusOk;
 }
 
@@ -320,7 +426,9 @@ static void mx_layout(MxHandle *h, int nelx, int nely, int nelz, int n_coarse, i
     h->z = cv.take<float>(h->n);
     h->p = cv.take<float>(h->n);
     h->q = cv.take<float>(h->n);
-    long np = std::max((long)kVecMaxBlocks, ku_tile_partials(h->g));
+    h->p64 = cv.take<double>(h->n);
+    h->q64 = cv.take<double>(h->n);
+    long np = std::max((long)kVecMaxBlocks, std::max(ku_tile_partials(h->g), ku_tile_partials_f32(h->g)));
     for (int l = 1; l < h->nlev; ++l)
         np = std::max(np, std::max(std::max(elem_partials(h->lev[l].g), gal1_partials(h->lev[l].g)),
                                    stencil_partials(h->lev[l].g)));
@@ -328,7 +436,7 @@ static void mx_layout(MxHandle *h, int nelx, int nely, int nelz, int n_coarse, i
     h->scal = cv.take<double>(FANTOM_SCALARS);
     h->nc = (int)h->lev[h->nlev - 1].n_dof;
     h->Ainv = cv.take<double>((size_t)h->nc * h->nc);
-    h->gj_work = cv.take<double>(2 * (size_t)h->nc);
+    h->gj_work = cv.take<double>((size_t)h->nc * h->nc + 1024);
 }
 
 static int mx_join(hipStream_t from, hipStream_t to)
@@ -399,7 +507,9 @@ int fantom_mx_create(int nelx, int nely, int nelz, int n_coarse, int n_smooth, d
     h->n_smooth = n_smooth;
     h->omega = omega;
     h->have_graph = false;
+    h->have_pc_graph = false;
     h->last_inner = 0;
+    h->last_pc = 0;
     int st = fantom_hex8_blocks(K0_host, h->Kb64, nullptr);
     if (st) {
         delete h;
@@ -449,6 +559,8 @@ int fantom_mx_destroy(void *handle)
     hipStreamSynchronize(h->stream);
     if (h->have_graph)
         hipGraphExecDestroy(h->iter_exec);
+    if (h->have_pc_graph)
+        hipGraphExecDestroy(h->pc_exec);
     hipStreamDestroy(h->stream);
     delete h;
     return kStatusOk;
@@ -517,7 +629,7 @@ int fantom_mx_setup(void *handle, const double *E, const uint8_t *mask, void *st
     return mx_join(s, as_stream(stream));
 }
 
-static int mx_capture(MxHandle *h)
+static int mx_capture(MxHandle *h, bool fp64_pcg = false)
 {
     hipGraph_t graph;
     hipError_t e = hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal);
@@ -525,7 +637,7 @@ static int mx_capture(MxHandle *h)
         set_error("begin capture: %s", hipGetErrorString(e));
         return kStatusHipError;
     }
-    int st = mx_iteration(h);
+    int st = fp64_pcg ? mxd_iteration(h) : mx_iteration(h);
     e = hipStreamEndCapture(h->stream, &graph);
     if (st)
         return st;
@@ -533,13 +645,13 @@ static int mx_capture(MxHandle *h)
         set_error("end capture: %s", hipGetErrorString(e));
         return kStatusHipError;
     }
-    e = hipGraphInstantiate(&h->iter_exec, graph, nullptr, nullptr, 0);
+    e = hipGraphInstantiate(fp64_pcg ? &h->pc_exec : &h->iter_exec, graph, nullptr, nullptr, 0);
     hipGraphDestroy(graph);
     if (e != hipSuccess) {
         set_error("graph instantiate: %s", hipGetErrorString(e));
         return kStatusHipError;
     }
-    h->have_graph = true;
+    (fp64_pcg ? h->have_pc_graph : h->have_graph) = true;
     return kStatusOk;
 }
 
@@ -634,6 +746,82 @@ int fantom_mx_solve(void *handle, const double *f, double *x, double tol, double
     *outer_iters = outer;
     *inner_iters = total_inner;
     *rel_res = R;
+    return mx_join(s, as_stream(stream));
+}
+
+// FP64 PCG preconditioned by one FP32 V-cycle per iteration: FP64 fine K.u,
+// FP64 vectors, dots and step lengths; the V-cycle (smoothing, residuals,
+// transfers and coarse operators) runs on FP32 copies.  Stops on the
+// recursive FP64 residual; returns the true relative residual.
+int fantom_mx_solve_pcg(void *handle, const double *f, double *x, double tol, int maxiter, int use_x0, int use_graph,
+                        int *iters, double *rel_res, void *stream)
+{
+    MxHandle *h = reinterpret_cast<MxHandle *>(handle);
+    FANTOM_CHECK_ARG(h && f && x && iters && rel_res, "null operand");
+    FANTOM_CHECK_ARG(maxiter >= 1, "maxiter must be positive");
+    MX_TRY(mx_join(as_stream(stream), h->stream));
+    hipStream_t s = h->stream;
+    const long n = h->n;
+    const unsigned G = vec_grid(n);
+    hipLaunchKernelGGL(mx_copy_f64, dim3(G), dim3(256), 0, s, n, f, h->f64);
+    if (use_x0)
+        hipLaunchKernelGGL(mx_copy_f64, dim3(G), dim3(256), 0, s, n, x, h->x64);
+    else
+        hipMemsetAsync(h->x64, 0, sizeof(double) * n, s);
+    MX_TRY(launch_dot_partials(n, h->f64, h->f64, h->partials, s));
+    hipLaunchKernelGGL(mxd_norm_k, dim3(1), dim3(1024), 0, s, h->partials, (long)G, h->scal);
+    hipLaunchKernelGGL(mx_init_k, dim3(1), dim3(64), 0, s, h->scal, tol);
+    MX_TRY(launch_ku_tile(MODE_RES, h->g, h->Kb64, h->E64, h->mask, h->x64, h->r64, h->f64, nullptr, 0.0, nullptr,
+                          s));
+    MX_TRY(launch_dot_partials(n, h->r64, h->r64, h->partials, s));
+    MX_TRY(launch_finalize(h->partials, (long)G, h->scal, FIN_STORE, S_TMP, s));
+    double hs[FANTOM_SCALARS];
+    MX_TRY(mx_read(h, hs));
+    int done = 0;
+    if (sqrt(fmax(hs[S_TMP], 0.0)) / hs[X_NB] >= tol) {
+        hipLaunchKernelGGL(mxd_down_k, dim3(G), dim3(256), 0, s, n, h->r64, h->r);
+        MX_TRY(mx_precond(h, h->r, h->z));
+        hipLaunchKernelGGL(mxd_dot_k, dim3(G), dim3(256), 0, s, n, h->r64, h->z, h->partials);
+        MX_TRY(launch_finalize(h->partials, (long)G, h->scal, FIN_RZ0, 0, s));
+        hipLaunchKernelGGL(mxd_p_k, dim3(G), dim3(256), 0, s, n, h->p64, h->z, h->scal, 1);
+        FANTOM_LAUNCH_CHECK("mxd init");
+        if (use_graph && !h->have_pc_graph)
+            MX_TRY(mx_capture(h, true));
+        int batch = h->last_pc > 3 ? h->last_pc - 2 : 1;
+        bool conv = false;
+        while (done < maxiter && !conv) {
+            batch = std::min(batch, maxiter - done);
+            for (int b = 0; b < batch; ++b) {
+                if (use_graph) {
+                    const hipError_t e = hipGraphLaunch(h->pc_exec, s);
+                    if (e != hipSuccess) {
+                        set_error("graph launch: %s", hipGetErrorString(e));
+                        return kStatusHipError;
+                    }
+                } else {
+                    MX_TRY(mxd_iteration(h));
+                }
+            }
+            done += batch;
+            MX_TRY(mx_read(h, hs));
+            if (hs[X_DONE] != 0.0) {
+                done = (int)hs[X_ITER];
+                conv = true;
+            }
+            batch = 1;
+        }
+        h->last_pc = done;
+    }
+    // true FP64 residual of the result
+    MX_TRY(launch_ku_tile(MODE_RES, h->g, h->Kb64, h->E64, h->mask, h->x64, h->r64, h->f64, nullptr, 0.0, nullptr,
+                          s));
+    MX_TRY(launch_dot_partials(n, h->r64, h->r64, h->partials, s));
+    MX_TRY(launch_finalize(h->partials, (long)G, h->scal, FIN_STORE, S_TMP, s));
+    MX_TRY(mx_read(h, hs));
+    hipLaunchKernelGGL(mx_copy_f64, dim3(G), dim3(256), 0, s, n, h->x64, x);
+    FANTOM_LAUNCH_CHECK("mxd solve");
+    *iters = done;
+    *rel_res = sqrt(fmax(hs[S_TMP], 0.0)) / hs[X_NB];
     return mx_join(s, as_stream(stream));
 }
 

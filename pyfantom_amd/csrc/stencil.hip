@@ -68,50 +68,72 @@ __global__ void stencil_from_elem_k(Grid g, const double *__restrict__ Ke, const
         }
 }
 
-// y = K u from the stencil, one thread per node, with the solver epilogues
+// y = K u from the stencil with the solver epilogues.  A block of 192
+// threads covers 64 nodes; wave p (p = 0, 1, 2) takes the 9 neighbours in
+// node plane y + p - 1 of each node, so every thread has one round of
+// independent loads (coarse levels are small and latency-bound), and wave 0
+// adds the three plane sums in a fixed order.
+constexpr int kStNodes = 64;
+constexpr int kStBlock = 3 * kStNodes;
+
 template <typename T, int MODE>
-__global__ __launch_bounds__(256) void ku_stencil_k(Grid g, const T *__restrict__ S, const T *__restrict__ u,
-                                                  T *__restrict__ y, const T *__restrict__ bvec,
-                                                  const T *__restrict__ dinv, T omega, double *__restrict__ partials)
+__global__ __launch_bounds__(kStBlock) void ku_stencil_k(Grid g, const T *__restrict__ S, const T *__restrict__ u,
+                                                       T *__restrict__ y, const T *__restrict__ bvec,
+                                                       const T *__restrict__ dinv, T omega,
+                                                       double *__restrict__ partials)
 {
-    const long n = (long)blockIdx.x * 256 + threadIdx.x;
-    double part = 0.0;
-    if (n < g.n_nodes) {
-        const int k = (int)(n % g.nz);
-        const long q = n / g.nz;
-        const int i = (int)(q % g.nx), j = (int)(q / g.nx);
-        T acc[3] = {T(0), T(0), T(0)};
+    __shared__ T psum[3][3][kStNodes];
+    const int ln = threadIdx.x % kStNodes, ty = threadIdx.x / kStNodes;
+    const long n = (long)blockIdx.x * kStNodes + ln;
+    const bool valid = n < g.n_nodes;
+    const long nn = valid ? n : 0;
+    const int k = (int)(nn % g.nz);
+    const long q = nn / g.nz;
+    const int i = (int)(q % g.nx), j = (int)(q / g.nx);
+    T acc[3] = {T(0), T(0), T(0)};
+    if (valid) {
+        const int dy = ty - 1;
 #pragma unroll
-        for (int t = 0; t < 27; ++t) {
-            const int dy = t / 9 - 1, dx = (t / 3) % 3 - 1, dz = t % 3 - 1;
+        for (int tq = 0; tq < 9; ++tq) {
+            const int t = ty * 9 + tq;
+            const int dx = tq / 3 - 1, dz = tq % 3 - 1;
             const int mi = i + dx, mj = j + dy, mk = k + dz;
-            if (mi < 0 || mi >= g.nx || mj < 0 || mj >= g.ny || mk < 0 || mk >= g.nz)
-                continue;
-            const long m = ((long)mj * g.nx + mi) * g.nz + mk;
+            // outside the grid the coefficients are 0 by construction; read
+            // the node itself so that the loads issue unconditionally
+            const bool in = mi >= 0 && mi < g.nx && mj >= 0 && mj < g.ny && mk >= 0 && mk < g.nz;
+            const long m = in ? ((long)mj * g.nx + mi) * g.nz + mk : nn;
             const T u0 = u[3 * m], u1 = u[3 * m + 1], u2 = u[3 * m + 2];
-            const T *St = S + (long)t * 9 * g.n_nodes + n;
+            const T *St = S + (long)t * 9 * g.n_nodes + nn;
 #pragma unroll
             for (int c = 0; c < 3; ++c)
                 acc[c] += St[(3 * c) * g.n_nodes] * u0 + St[(3 * c + 1) * g.n_nodes] * u1 +
                           St[(3 * c + 2) * g.n_nodes] * u2;
         }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+        psum[ty][c][ln] = acc[c];
+    __syncthreads();
+    double part = 0.0;
+    if (ty == 0 && valid) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
+            const T kuv = psum[0][c][ln] + psum[1][c][ln] + psum[2][c][ln];
             const long r = 3 * n + c;
             T o;
             if (MODE == MODE_JAC)
-                o = u[r] + omega * dinv[r] * (bvec[r] - acc[c]);
+                o = u[r] + omega * dinv[r] * (bvec[r] - kuv);
             else if (MODE == MODE_RES)
-                o = bvec[r] - acc[c];
+                o = bvec[r] - kuv;
             else
-                o = acc[c];
+                o = kuv;
             y[r] = o;
             if (MODE == MODE_KUDOT)
                 part += (double)u[r] * (double)o;
         }
     }
     if (MODE == MODE_KUDOT) {
-        const double s = block_sum<256>(part);
+        const double s = block_sum<kStBlock>(part);
         if (threadIdx.x == 0)
             partials[blockIdx.x] = s;
     }
@@ -128,7 +150,7 @@ __global__ void dinv_stencil_k(Grid g, const T *__restrict__ S, T *__restrict__ 
         d[3 * n + c] = T(1) / S[(long)(13 * 9 + 4 * c) * g.n_nodes + n];
 }
 
-long stencil_partials(const Grid &g) { return (g.n_nodes + 255) / 256; }
+long stencil_partials(const Grid &g) { return (g.n_nodes + kStNodes - 1) / kStNodes; }
 
 template <typename TS>
 static int stencil_build_t(const Grid &g, const double *Ke, const uint8_t *mask, TS *S, hipStream_t s)
@@ -157,7 +179,8 @@ static int stencil_ku_t(int mode, const Grid &g, const T *S, const T *u, T *y, c
     const T om = (T)omega;
 #define ST_CASE(M)                                                                                        \
     case M:                                                                                               \
-        hipLaunchKernelGGL((ku_stencil_k<T, M>), dim3(blocks), dim3(256), 0, s, g, S, u, y, b, dinv, om, partials); \
+        hipLaunchKernelGGL((ku_stencil_k<T, M>), dim3(blocks), dim3(kStBlock), 0, s, g, S, u, y, b, dinv, om,    \
+                           partials);                                                                   \
         break;
     switch (mode) {
         ST_CASE(MODE_KU)

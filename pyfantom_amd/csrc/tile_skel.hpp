@@ -43,11 +43,21 @@ struct TileShape {
     int tiles_x, tiles_z, chunks_y, chunk;
 };
 
-// Host: tile counts for grid g.  The y-chunk length trades redundant halo
-// planes (chunk + 2 planes are read for chunk outputs) against the number of
-// workgroups; the default aims at >= 4 workgroups per CU on a 256-CU part.
-TileShape tile_shape(const Grid &g);
+// Host: tile counts for grid g and an operator with per_cu resident
+// workgroups per CU.  A workgroup walks chunk planes in sequence (chunk + 1
+// element layers, one of them halo), so with n chunks along y the run takes
+// about ceil(cols n / capacity) rounds of (ceil(ny / n) + 1) layer steps;
+// the chunk count minimises that product (fewest workgroups on ties).
+// FANTOM_TILE_CHUNK=<planes> overrides the choice (tuning sweeps).
+TileShape tile_shape(const Grid &g, int per_cu);
 unsigned tile_blocks(const TileShape &ts);
+
+// resident workgroups per CU of the tile operators, from their register use
+// (FP64 fine 154 VGPRs -> 3 waves/SIMD; FP32 fine 76 -> 6; level-1 Galerkin
+// 190-224 -> 2); only the chunk choice depends on them
+constexpr int kFineF64PerCU = 3;
+constexpr int kFineF32PerCU = 6;
+constexpr int kGal1PerCU = 2;
 
 // DPP moves inside rows of 16 lanes; a lane whose source lies outside its row
 // reads 0 (bound_ctrl).

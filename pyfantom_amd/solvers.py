@@ -129,7 +129,7 @@ class MultiGrid(_DeviceKrylov):
                  w_level=1, coarse_solver="cholmod", matrix_free=False, low_level_tol=1e-8,
                  low_level_maxiter=5000, min_omega=0.4, omega_boost=1.06, omega_schedule="fixed",
                  check_every=1, native=True, use_graph=True, precision="fp64", inner_tol=1e-4,
-                 max_outer=20, coarse=None):
+                 max_outer=20, coarse=None, mixed_scheme="pcg"):
         if coarse_solver not in ["cg", "gmres", "splu", "spsolve", "cholmod"]:
             raise ValueError("Coarse solver not recognized.")
         if coarse_solver in ["splu", "spsolve"] and matrix_free:
@@ -169,6 +169,11 @@ class MultiGrid(_DeviceKrylov):
         if precision == "mixed" and not self.native:
             raise ValueError("the mixed-precision solve runs in the native driver (fixed omega, V-cycle)")
         self.precision = precision
+        # mixed precision: "pcg" = FP64 PCG preconditioned by an FP32 V-cycle;
+        # "ir" = FP64 iterative refinement around an FP32 GMG-PCG
+        if mixed_scheme not in ("pcg", "ir"):
+            raise ValueError("mixed_scheme must be 'pcg' or 'ir'")
+        self.mixed_scheme = mixed_scheme
         if coarse is None:
             coarse = "galerkin" if (self.native and len(mesh.nel) == 3) else "rediscretized"
         if coarse not in ("galerkin", "rediscretized"):
@@ -293,7 +298,11 @@ class MultiGrid(_DeviceKrylov):
         x = self.last_x0 if use_x0 else torch.empty_like(rhs)
         it = ctypes.c_int(0)
         res = ctypes.c_double(0.0)
-        if mixed:
+        if mixed and self.mixed_scheme == "pcg":
+            _lib.call("fantom_mx_solve_pcg", self._nh, _lib.ptr(rhs), _lib.ptr(x), self.tol, self.maxiter,
+                      int(use_x0), int(self.use_graph), ctypes.byref(it), ctypes.byref(res))
+            self.last_outer_iterations = 1
+        elif mixed:
             outer = ctypes.c_int(0)
             _lib.call("fantom_mx_solve", self._nh, _lib.ptr(rhs), _lib.ptr(x), self.tol, self.inner_tol,
                       self.max_outer, self.maxiter, int(use_x0), int(self.use_graph), ctypes.byref(outer),
@@ -330,7 +339,7 @@ class MultiGrid(_DeviceKrylov):
             levels.append(L)
         c = levels[-1]
         self._Ainv = torch.zeros((c.n_dof, c.n_dof), dtype=torch.float64, device=dev)
-        self._gj_work = torch.zeros(2 * c.n_dof, dtype=torch.float64, device=dev)
+        self._gj_work = torch.zeros(c.n_dof * c.n_dof + 1024, dtype=torch.float64, device=dev)
         self.levels = levels
 
     def _setup(self):

@@ -230,15 +230,17 @@ def test_native_graph_matches_eager_and_python(oracle):
         assert rg < 1e-9 and rp < 1e-9 and rr < 1e-9
 
 
+@pytest.mark.parametrize("scheme", ["pcg", "ir"])
 @pytest.mark.parametrize("field", ["seeded", "void_contrast"])
-def test_mixed_precision_refinement(oracle, field):
-    """FP32 inner GMG-PCG inside FP64 refinement reaches the FP64 1e-8 true
-    residual; its compliance is within 1e-6 of the FP64 solve (to 1e-12) and
-    the displacement error is reported."""
+def test_mixed_precision_refinement(oracle, field, scheme):
+    """Both mixed-precision schemes -- FP64 PCG with an FP32 V-cycle ("pcg")
+    and FP32 inner GMG-PCG inside FP64 refinement ("ir") -- reach the FP64
+    1e-8 true residual; the compliance is within 1e-6 of the FP64 solve (to
+    1e-12) and the displacement error is reported."""
     import pyfantom_amd as pf
     torch = _torch()
     nel = (64, 32, 32)
-    print(f"mixed precision {field}", flush=True)
+    print(f"mixed precision {field} {scheme}", flush=True)
     mesh, K, _, FE = cantilever(nel, tol=1e-12, n_level=4)
     Ef = filtered_field(oracle, nel)
     if field == "void_contrast":
@@ -252,7 +254,7 @@ def test_mixed_precision_refinement(oracle, field):
     u64, r64 = ref.solve(FE.rhs, use_last=False)
     u64 = u64.clone()
     mx = pf.MultiGrid(mesh, K, maxiter=2000, tol=1e-8, n_smooth=2, omega=0.6, n_level=4, cycle="V",
-                      precision="mixed", inner_tol=1e-4)
+                      precision="mixed", inner_tol=1e-4, mixed_scheme=scheme)
     um, rm = mx.solve(FE.rhs, use_last=False)
     c64 = float(torch.dot(FE.rhs, u64))
     cm = float(torch.dot(FE.rhs, um))
