@@ -37,10 +37,11 @@ CONFIGS = {
 }
 
 
-def build_problem(pf, nel, n_level, tol=1e-8):
+def build_problem(pf, nel, n_level, tol=1e-8, precision="fp64"):
     mesh = pf.StructuredMesh3D(*nel, *nel, physics=pf.LinearElasticity(E=1.0, nu=0.3))
     K = pf.StructuredStiffnessKernel(mesh)
-    solver = pf.MultiGrid(mesh, K, maxiter=1000, tol=tol, n_smooth=2, omega=0.6, n_level=n_level, cycle="V")
+    solver = pf.MultiGrid(mesh, K, maxiter=1000, tol=tol, n_smooth=2, omega=0.6, n_level=n_level, cycle="V",
+                          precision=precision)
     FE = pf.FiniteElement(mesh, K, solver)
     nx, ny, nz = (n + 1 for n in nel)
     j, k = np.meshgrid(np.arange(ny), np.arange(nz), indexing="ij")
@@ -78,6 +79,16 @@ def time_ku(torch, pf, K, mesh, reps=50):
     bytes_alg = 16.0 * n_dof + 8.0 * n_el
     return dict(seconds=t, gdof_s=n_dof / t / 1e9, achieved_gbs=bytes_alg / t / 1e9, bytes=bytes_alg,
                 flops_dense=1152.0 * n_el, kernel="ku3d_tile_f64" if kid == _lib.KU_TILE else "ku3d_node_f64")
+
+
+def measured_traffic(workload):
+    """HBM bytes per launch of the fine K.u for this workload, from the
+    committed counter runs (profiles/ku_tile_f64_traffic.md), or None."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "ku_tile_f64_traffic.json")) as fh:
+            return json.load(fh)[workload]["traffic_bytes"]
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def cpu_baseline(nel, n_level, pcg_iters_per_step, max_iter=4):
@@ -123,6 +134,9 @@ def main():
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--ku-reps", type=int, default=50)
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision SIMP run")
+    ap.add_argument("--ku-only", action="store_true",
+                    help="only the isolated K.u of the workload and a 256 MiB calibration copy (for counter runs)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md)", file=sys.stderr)
@@ -133,22 +147,61 @@ def main():
 
     cfg = CONFIGS[args.config]
     nel, n_level = cfg["nel"], cfg["n_level"]
-    mesh, K, solver, FE, filt, prob = build_problem(pf, nel, n_level)
-    print(f"# setup done: {cfg['name']} dofs={K.shape[0]}", file=sys.stderr, flush=True)
-    opt = pf.OC(prob, move=0.2)
-    for w in range(args.warmup):
-        opt.iter()
-        print(f"# warmup {w}: c={prob.f():.6e} iters={solver.last_iterations}", file=sys.stderr, flush=True)
-    torch.cuda.synchronize()
-    iters = []
-    t0 = time.perf_counter()
-    for s in range(args.steps):
-        opt.iter()
-        iters.append(solver.last_iterations)
-    torch.cuda.synchronize()
-    t = time.perf_counter() - t0
+
+    if args.ku_only:
+        # hardware-counter runs: the K.u launches, then one device copy of a
+        # known size whose FETCH_SIZE / WRITE_SIZE calibrate the counters
+        mesh = pf.StructuredMesh3D(*nel, *nel, physics=pf.LinearElasticity(E=1.0, nu=0.3))
+        K = pf.StructuredStiffnessKernel(mesh)
+        j, k = np.meshgrid(np.arange(nel[1] + 1), np.arange(nel[2] + 1), indexing="ij")
+        clamp = (k + (nel[2] + 1) * (nel[0] + 1) * j).ravel()
+        K.set_constraints(np.concatenate([3 * clamp + d for d in range(3)]))
+        ku = time_ku(torch, pf, K, mesh, args.ku_reps)
+        src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"ku_seconds": ku["seconds"], "ku_algorithmic_bytes": ku["bytes"], "kernel": ku["kernel"],
+                          "calibration_copy_bytes_each_way": src.numel() * 8}), flush=True)
+        return
+
+    def simp_run(precision):
+        """W untimed + K timed SIMP iterations; returns (seconds, iterations, problem objects)."""
+        mesh, K, solver, FE, filt, prob = build_problem(pf, nel, n_level, precision=precision)
+        print(f"# {precision}: setup done: {cfg['name']} dofs={K.shape[0]}", file=sys.stderr, flush=True)
+        opt = pf.OC(prob, move=0.2)
+        for w in range(args.warmup):
+            opt.iter()
+            print(f"# {precision}: warmup {w}: c={prob.f():.6e} iters={solver.last_iterations}", file=sys.stderr,
+                  flush=True)
+        torch.cuda.synchronize()
+        iters, outer = [], []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            opt.iter()
+            iters.append(solver.last_iterations)
+            outer.append(getattr(solver, "last_outer_iterations", None))
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0
+        print(f"# {precision}: timed {args.steps} steps: {t / args.steps * 1e3:.1f} ms/step, pcg iters {iters}",
+              file=sys.stderr, flush=True)
+        return t, iters, outer, (mesh, K, solver, FE, filt, prob)
+
+    t, iters, _, (mesh, K, solver, FE, filt, prob) = simp_run("fp64")
     ms = t / args.steps * 1e3
-    print(f"# timed {args.steps} steps: {ms:.1f} ms/step, pcg iters {iters}", file=sys.stderr, flush=True)
+    mixed = None
+    if not args.no_mixed:
+        tm, it_m, out_m, objs_m = simp_run("mixed")
+        prob_m = objs_m[5]
+        mixed = {"s_per_iter": tm / args.steps, "inner_iterations_mean": float(np.mean(it_m)),
+                 "outer_iterations_mean": float(np.mean([o for o in out_m if o is not None] or [0])),
+                 "speedup_vs_fp64": t / tm, "compliance": prob_m.f(),
+                 "compliance_rel_diff_vs_fp64": abs(prob_m.f() - prob.f()) / abs(prob.f()),
+                 "solve_rel_residual": float(prob_m._residual),
+                 "note": "FP32 inner GMG-PCG (FP32 WHT K.u, FP64 dots) inside FP64 iterative refinement; "
+                         "same SIMP trajectory length"}
+        del objs_m, prob_m
     ku = time_ku(torch, pf, K, mesh, args.ku_reps)
     print(f"# K.u: {ku['seconds']*1e6:.1f} us, {ku['achieved_gbs']:.0f} GB/s", file=sys.stderr, flush=True)
     out = {
@@ -171,9 +224,11 @@ def main():
         "compliance": prob.f(),
         "solve_rel_residual": float(prob._residual),
         "ku": {"gdof_s": ku["gdof_s"], "seconds": ku["seconds"], "kernel": ku["kernel"]},
+        "mixed_precision": mixed,
         "roofline": {"bound": "hbm", "achieved": ku["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": ku["achieved_gbs"] / HBM_PEAK_GBS, "traffic": None, "kernel": ku["kernel"],
-                     "algorithmic_bytes": ku["bytes"]},
+                     "frac": ku["achieved_gbs"] / HBM_PEAK_GBS, "traffic": measured_traffic(cfg["name"]),
+                     "kernel": ku["kernel"], "algorithmic_bytes": ku["bytes"],
+                     "traffic_source": "profiles/ku_tile_f64_traffic.md (FETCH_SIZE/WRITE_SIZE, copy-calibrated)"},
     }
     if not args.no_cpu:
         print("# cpu baseline ...", file=sys.stderr, flush=True)

@@ -78,6 +78,19 @@ class ConvNet:
             self.b[name] = sd[f"b_{name}"].to(self.device).contiguous()
             self.wbf[name] = _bf16(self.w[name]).contiguous()
 
+    def save(self, path, **extra):
+        """Weights and biases as an .npz archive (numpy arrays only)."""
+        np.savez(path, **{k: v.numpy() for k, v in self.state_dict().items()}, **extra)
+
+    @classmethod
+    def from_file(cls, path, device=None):
+        """Network with the weights of an archive written by save()."""
+        net = cls(device=device)
+        with np.load(path, allow_pickle=False) as z:
+            net.load_state_dict({k: torch.from_numpy(z[k].astype(np.float32)) for k in z.files
+                                 if k.startswith(("w_", "b_"))})
+        return net
+
     def n_params(self):
         return sum(v.numel() for v in self.w.values()) + sum(v.numel() for v in self.b.values())
 
@@ -265,11 +278,15 @@ class NetRunner:
         return self.gw, self.gb
 
     def loss_and_grad(self, r, e):
-        """MSE between N(r/rms) and e/rms (first 3 channels); fills the gradients."""
+        """MSE between N(r / rms(r)) and e / rms(e) (first 3 channels); fills
+        the gradients.  Both sides are normalised: flexible PCG is invariant to
+        the scale of z, and the unnormalised error is dominated by the
+        near-void regions where it is orders of magnitude larger."""
         B = self._bufs()
         self.forward(r)
+        self._norm2(e, 2)
         dout = B[(0, 16, 0)]
-        _lib.call("fantom_mse_bf16", self.nv[0], _lib.ptr(self.y32), _lib.ptr(e), _lib.ptr(self.scal), 0,
+        _lib.call("fantom_mse_bf16", self.nv[0], _lib.ptr(self.y32), _lib.ptr(e), _lib.ptr(self.scal), 2,
                   _lib.ptr(dout), _lib.ptr(self.partials))
         dcopy = B[(0, 16, 2)]
         dcopy.copy_(dout)
@@ -330,9 +347,11 @@ class LearnedPreconditioner:
     """z = M(r) for flexible PCG.
 
     mode 'learned': z = N(r), the network alone (replaces smoother and coarse
-    correction).  mode 'hybrid': z = V(r) + N(r - K V(r)), one GMG V-cycle
-    followed by a learned correction of what it leaves.  Constrained DOFs of
-    z are zeroed (the network sees only the free part of the residual)."""
+    correction).  mode 'hybrid': z = V(r) + g c with c = N(t), t = r - K V(r),
+    one GMG V-cycle followed by a learned correction of what it leaves, scaled
+    by the Galerkin step g = t.c / c.Kc (the network predicts directions, not
+    magnitudes).  Constrained DOFs of z are zeroed (the network sees only the
+    free part of the residual)."""
 
     def __init__(self, net: ConvNet, kernel, mg=None, mode="learned"):
         if mode not in ("learned", "hybrid"):
@@ -347,6 +366,7 @@ class LearnedPreconditioner:
         n = kernel.shape[0]
         self._t = torch.zeros(n, dtype=torch.float64, device=kernel.device)
         self._t2 = torch.zeros(n, dtype=torch.float64, device=kernel.device)
+        self._c = torch.zeros(n, dtype=torch.float64, device=kernel.device)
         self._free = (~kernel.constraints).to(torch.float64)
 
     def set_density(self, x_el):
@@ -358,10 +378,15 @@ class LearnedPreconditioner:
         else:
             self.mg.precondition_native(r, z)
             self.vec.ku(z, self._t)
-            # t2 = r - K z
+            # t2 = r - K z, c = N(t2), z += (t2.c / c.Kc) c
             self._t2.copy_(r)
             self.vec.axpby(-1.0, self._t, 1.0, self._t2)
-            self.runner.apply(self._t2, z, accumulate=True)
+            self.runner.apply(self._t2, self._c)
+            self._c.mul_(self._free)
+            self.vec.ku(self._c, self._t)
+            ckc = self.vec.dot(self._c, self._t)
+            if math.isfinite(ckc) and ckc > 0.0:
+                self.vec.axpby(self.vec.dot(self._t2, self._c) / ckc, self._c, 1.0, z)
         z.mul_(self._free)
         return z
 

@@ -169,7 +169,7 @@ def main():
     log = lambda m: print(m, flush=True)
     net, losses = train(tuple(args.train), args.runs, args.steps, log=log)
     if args.out:
-        np.savez(args.out, **{k: v.numpy() for k, v in net.state_dict().items()}, losses=np.array(losses))
+        net.save(args.out, losses=np.array(losses))
     evaluate(net, tuple(args.eval), log=log)
 
 

@@ -127,3 +127,57 @@ def test_training_and_fpcg(oracle):
               flush=True)
         assert info["residual"] < 1e-8 * 1.5
         assert abs(c - c0) <= 1e-6 * abs(c0)
+
+
+def test_trained_network_ab(oracle):
+    """A/B of the committed trained network (pyfantom_amd/data, trained on the
+    GPU by pyfantom_amd/train_learned.py on 96x48x48 SIMP runs) on a held-out
+    problem: a 64x32x32 cantilever after 8 OC iterations.  Compares flexible
+    PCG with the V-cycle + learned correction against the same with an
+    untrained network and against plain GMG-PCG, all to 1e-8; prints the
+    iteration counts and times (the gain, or loss, of the learned part) and
+    checks that every variant reaches the FP64 solution."""
+    import os
+    import time
+    import torch
+    import pyfantom_amd as pf
+    from pyfantom_amd.learned import ConvNet, LearnedPreconditioner, fpcg_solve
+    from pyfantom_amd.train_learned import cantilever_problem
+    _torch()
+    path = os.path.join(os.path.dirname(pf.__file__), "data", "learned_precond_96x48x48.npz")
+    print("trained network A/B", flush=True)
+    nel = (64, 32, 32)
+    mesh, K, solver, FE, filt, prob = cantilever_problem(nel, vf=0.3, load="edge", n_level=4)
+    opt = pf.OC(prob, move=0.2)
+    for _ in range(8):
+        opt.iter()
+    x = prob.get_desvars()
+    plain = pf.MultiGrid(mesh, K, maxiter=1000, tol=1e-8, n_smooth=2, omega=0.6, n_level=4, cycle="V")
+    plain.solve(FE.rhs, use_last=False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    u0, r0 = plain.solve(FE.rhs, use_last=False)
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    u0 = u0.clone()
+    c0 = float(torch.dot(FE.rhs, u0))
+    results = {"plain": (plain.last_iterations, t_plain)}
+    for label, net in (("trained", ConvNet.from_file(path)), ("untrained", ConvNet(seed=0))):
+        mg = pf.MultiGrid(mesh, K, maxiter=1000, tol=1e-8, n_smooth=2, omega=0.6, n_level=4, cycle="V")
+        fb = pf.MultiGrid(mesh, K, maxiter=1000, tol=1e-8, n_smooth=2, omega=0.6, n_level=4, cycle="V")
+        M = LearnedPreconditioner(net, K, mg=mg, mode="hybrid")
+        M.set_density(x)
+        fpcg_solve(K, FE.rhs, M, tol=1e-8, maxiter=3, fallback=None)   # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        u, info = fpcg_solve(K, FE.rhs, M, tol=1e-8, maxiter=300, fallback=fb)
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0
+        c = float(torch.dot(FE.rhs, u))
+        results[label] = (info["iterations"] + info["fallback_iterations"], t)
+        assert info["residual"] < 1.5e-8
+        assert abs(c - c0) <= 1e-6 * abs(c0)
+    print("A/B (iterations, seconds): " + ", ".join(f"{k} {v[0]} / {v[1] * 1e3:.1f} ms" for k, v in results.items()),
+          flush=True)
+    print(f"trained vs untrained iterations: {results['trained'][0]} vs {results['untrained'][0]}; "
+          f"trained vs plain GMG-PCG time ratio {results['trained'][1] / results['plain'][1]:.2f}", flush=True)
