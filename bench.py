@@ -199,8 +199,8 @@ def main():
                  "speedup_vs_fp64": t / tm, "compliance": prob_m.f(),
                  "compliance_rel_diff_vs_fp64": abs(prob_m.f() - prob.f()) / abs(prob.f()),
                  "solve_rel_residual": float(prob_m._residual),
-                 "note": "FP32 inner GMG-PCG (FP32 WHT K.u, FP64 dots) inside FP64 iterative refinement; "
-                         "same SIMP trajectory length"}
+                 "note": "FP64 PCG (FP64 K.u, dots, step lengths, true residual) preconditioned by an FP32 "
+                         "V-cycle (FP32 WHT K.u, Galerkin level 1, stencils, transfers); same SIMP steps"}
         del objs_m, prob_m
     ku = time_ku(torch, pf, K, mesh, args.ku_reps)
     print(f"# K.u: {ku['seconds']*1e6:.1f} us, {ku['achieved_gbs']:.0f} GB/s", file=sys.stderr, flush=True)

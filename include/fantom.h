@@ -75,6 +75,13 @@ int fantom_ku_f64(int kernel, int nelx, int nely, int nelz, const double *K0, co
 int fantom_ku_jacobi_f64(int kernel, int nelx, int nely, int nelz, const double *K0, const double *E,
                          const uint8_t *mask, const double *b, const double *dinv, double omega,
                          const double *x_in, double *x_out, void *stream);
+/* The first Jacobi sweep from the zero guess, x_out = x0 + omega D^-1 (b - K x0)
+ * with x0 = omega D^-1 b formed while loading (the scaling and the sweep of
+ * the reference's smoother, solvers/CUDA/_solvers.py:437-440, in one pass);
+ * tiled hex8 kernel only. */
+int fantom_ku_jacobi0_f64(int kernel, int nelx, int nely, int nelz, const double *K0, const double *E,
+                          const uint8_t *mask, const double *b, const double *dinv, double omega, double *x_out,
+                          void *stream);
 /* Residual fused into the operator: r = b - K x (solvers/CUDA/_solvers.py:538). */
 int fantom_ku_residual_f64(int kernel, int nelx, int nely, int nelz, const double *K0, const double *E,
                            const uint8_t *mask, const double *b, const double *x, double *r, void *stream);
@@ -212,8 +219,9 @@ int fantom_elem_from_fine_f64(int nelx, int nely, int nelz, const double *G, con
  * matrices Ke, with constrained rows set to identity rows and constrained
  * columns zeroed.  stencil_ku: mode 0 y = A u, 1 one Jacobi sweep
  * y = u + omega dinv (b - A u), 2 residual y = b - A u, 3 y = A u with the
- * per-block partial sums of u.(A u) in partials[0:stencil_partials] (as
- * fantom_ku_f64 / _jacobi / _residual / _dot; unused operands may be NULL). */
+ * per-block partial sums of u.(A u) in partials[0:stencil_partials], 4 the
+ * first sweep from u = omega dinv b (u unused) (as fantom_ku_f64 / _jacobi /
+ * _residual / _dot / _jacobi0; unused operands may be NULL). */
 int fantom_stencil_build_f64(int nelx, int nely, int nelz, const double *Ke, const uint8_t *mask, double *S,
                              void *stream);
 int fantom_stencil_ku_f64(int mode, int nelx, int nely, int nelz, const double *S, const double *u, double *y,

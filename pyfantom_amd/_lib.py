@@ -32,6 +32,7 @@ SIGNATURES = {
     "fantom_ku_f64": (_c_int, [_c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
     "fantom_ku_jacobi_f64": (_c_int, [_c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _c_double, _vp,
                                       _vp, _vp]),
+    "fantom_ku_jacobi0_f64": (_c_int, [_c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _c_double, _vp, _vp]),
     "fantom_ku_residual_f64": (_c_int, [_c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "fantom_ku_dot_f64": (_c_int, [_c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "fantom_ku_partials": (_c_long, [_c_int, _c_int, _c_int, _c_int]),

@@ -61,7 +61,7 @@ static int ku_dispatch(int kernel, int mode, int nelx, int nely, int nelz, const
                        double omega, double *partials, void *stream)
 {
     GRID_OR_RETURN(nelx, nely, nelz);
-    FANTOM_CHECK_ARG(K0 && E && u && y, "null operand");
+    FANTOM_CHECK_ARG(K0 && E && (u || mode == MODE_JAC0) && y, "null operand");
     const Grid g = make_grid(nelx, nely, nelz);
     const int dim = nelz > 0 ? 3 : 2;
     if (kernel == FANTOM_KU_TILE) {
@@ -86,6 +86,16 @@ int fantom_ku_jacobi_f64(int kernel, int nelx, int nely, int nelz, const double 
     FANTOM_CHECK_ARG(b && dinv, "null operand");
     FANTOM_CHECK_ARG(x_in != x_out, "x_in and x_out must not alias");
     return ku_dispatch(kernel, MODE_JAC, nelx, nely, nelz, K0, E, mask, x_in, x_out, b, dinv, omega, nullptr, stream);
+}
+
+int fantom_ku_jacobi0_f64(int kernel, int nelx, int nely, int nelz, const double *K0, const double *E,
+                          const uint8_t *mask, const double *b, const double *dinv, double omega, double *x_out,
+                          void *stream)
+{
+    FANTOM_CHECK_ARG(b && dinv && x_out, "null operand");
+    FANTOM_CHECK_ARG(kernel == FANTOM_KU_TILE && nelz > 0, "the fused first sweep is implemented by the tiled kernel");
+    return ku_dispatch(kernel, MODE_JAC0, nelx, nely, nelz, K0, E, mask, nullptr, x_out, b, dinv, omega, nullptr,
+                       stream);
 }
 
 int fantom_ku_residual_f64(int kernel, int nelx, int nely, int nelz, const double *K0, const double *E,

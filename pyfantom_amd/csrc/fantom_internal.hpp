@@ -4,7 +4,11 @@
 
 namespace fantom {
 
-enum { MODE_KU = 0, MODE_JAC = 1, MODE_RES = 2, MODE_KUDOT = 3 };
+// matvec epilogues: y = K u | y = u + omega D^-1 (b - K u) | y = b - K u |
+// y = K u with partial sums of u.(K u) | MODE_JAC0: the Jacobi sweep from
+// x0 = omega D^-1 b, with x0 formed on load (u unused); equals a scaling
+// followed by MODE_JAC, in one pass
+enum { MODE_KU = 0, MODE_JAC = 1, MODE_RES = 2, MODE_KUDOT = 3, MODE_JAC0 = 4 };
 
 // ku_node.hip
 int launch_ku_node(int mode, const Grid &g, int dim, const double *K0, const double *E, const uint8_t *mask,
@@ -49,7 +53,7 @@ int launch_gemv(int n, const double *A, const double *x, double *y, hipStream_t 
 // galerkin.hip
 int launch_gal1(int mode, const Grid &g, const double *Btab, const double *Ef, const uint8_t *mask, const double *u,
                 double *y, const double *b, const double *dinv, double omega, double *partials, hipStream_t s);
-int launch_gal1_f32(int mode, const Grid &g, const double *Btab, const float *Ef, const uint8_t *mask, const float *u,
+int launch_gal1_f32(int mode, const Grid &g, const float *Btab, const float *Ef, const uint8_t *mask, const float *u,
                     float *y, const float *b, const float *dinv, double omega, double *partials, hipStream_t s);
 long gal1_partials(const Grid &g);
 int launch_elem(int mode, const Grid &g, const double *Ke, const uint8_t *mask, const double *u, double *y,

@@ -55,12 +55,13 @@ __device__ __forceinline__ void gwht8(T *w)
 }
 
 // Level-1 Galerkin operator on the level-1 grid g (its fine grid is 2x in
-// every direction), run on the shared tile skeleton.  Btab: 576 doubles,
-// block (s, s') at Btab[(s*8 + s')*9 + 3*d + d'] (scaled by 1/8); uniform
+// every direction), run on the shared tile skeleton.  Btab: 576 entries in
+// the operator's precision, block (s, s') at Btab[(s*8 + s')*9 + 3*d + d']
+// (scaled by 1/8); uniform
 // reads, served by the scalar cache.
 template <typename T>
 struct Gal1Op {
-    const double *Btab;
+    const T *Btab;
     const T *Ef;
     int zero;   // 0, set by the host
     struct Data {
@@ -86,8 +87,8 @@ struct Gal1Op {
         // d.k is 0 at run time but depends on the layer, which keeps the 279
         // uniform table reads inside the layer loop as scalar loads; hoisted
         // out of the loop they would occupy VGPRs and cut occupancy.
-        typedef const __attribute__((address_space(4))) double ConstD;   // read-only for the whole kernel
-        ConstD *tab = (ConstD *)(Btab + __builtin_amdgcn_readfirstlane(d.k));
+        typedef const __attribute__((address_space(4))) T ConstT;   // read-only for the whole kernel
+        ConstT *tab = (ConstT *)(Btab + __builtin_amdgcn_readfirstlane(d.k));
 #pragma unroll
         for (int s = 0; s < 8; ++s) {
             T a[3] = {T(0), T(0), T(0)};
@@ -97,7 +98,7 @@ struct Gal1Op {
                 if (nz == 0)
                     continue;
                 const T h[3] = {w[0][s2 ^ 1], w[1][s2 ^ 2], w[2][s2 ^ 4]};
-                ConstD *Bs = tab + (s * 8 + s2) * 9;
+                ConstT *Bs = tab + (s * 8 + s2) * 9;
 #pragma unroll
                 for (int dr = 0; dr < 3; ++dr) {
                     if (((nz >> (3 * dr)) & 7) == 0)
@@ -106,7 +107,7 @@ struct Gal1Op {
 #pragma unroll
                     for (int dc = 0; dc < 3; ++dc)
                         if ((nz >> (3 * dr + dc)) & 1)
-                            t += (T)Bs[3 * dr + dc] * h[dc];
+                            t += Bs[3 * dr + dc] * h[dc];
                     a[dr] += eh[s ^ s2] * t;
                 }
             }
@@ -405,7 +406,7 @@ __global__ void dense_from_elem(Grid g, const double *__restrict__ Ke, const uin
 long gal1_partials(const Grid &g) { return (long)tile_blocks(tile_shape(g, kGal1PerCU)); }
 
 template <typename T>
-static int gal1_launch(int mode, const Grid &g, const double *Btab, const T *Ef, const uint8_t *mask, const T *u,
+static int gal1_launch(int mode, const Grid &g, const T *Btab, const T *Ef, const uint8_t *mask, const T *u,
                        T *y, const T *b, const T *dinv, double omega, double *partials, hipStream_t s)
 {
     const TileShape ts = tile_shape(g, kGal1PerCU);
@@ -425,6 +426,7 @@ static int gal1_launch(int mode, const Grid &g, const double *Btab, const T *Ef,
         GAL_CASE(MODE_JAC)
         GAL_CASE(MODE_RES)
         GAL_CASE(MODE_KUDOT)
+        GAL_CASE(MODE_JAC0)
     default:
         set_error("unknown mode %d", mode);
         return kStatusBadArg;
@@ -440,7 +442,7 @@ int launch_gal1(int mode, const Grid &g, const double *Btab, const double *Ef, c
     return gal1_launch<double>(mode, g, Btab, Ef, mask, u, y, b, dinv, omega, partials, s);
 }
 
-int launch_gal1_f32(int mode, const Grid &g, const double *Btab, const float *Ef, const uint8_t *mask, const float *u,
+int launch_gal1_f32(int mode, const Grid &g, const float *Btab, const float *Ef, const uint8_t *mask, const float *u,
                     float *y, const float *b, const float *dinv, double omega, double *partials, hipStream_t s)
 {
     return gal1_launch<float>(mode, g, Btab, Ef, mask, u, y, b, dinv, omega, partials, s);

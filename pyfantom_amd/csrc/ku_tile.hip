@@ -198,6 +198,7 @@ static int launch_ku_tile_t(int mode, const Grid &g, const double *Kb, const T *
         FANTOM_TILE_CASE(MODE_JAC)
         FANTOM_TILE_CASE(MODE_RES)
         FANTOM_TILE_CASE(MODE_KUDOT)
+        FANTOM_TILE_CASE(MODE_JAC0)
     default:
         set_error("unknown matvec mode %d", mode);
         return kStatusBadArg;

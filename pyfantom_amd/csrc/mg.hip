@@ -154,9 +154,12 @@ __global__ __launch_bounds__(kBlock) void prolong3d(Grid gf, int normalize, cons
         }
     }
     const int mf = mask_f ? mask_f[f] : 0;
+    // count is 1, 2, 4 or 8: its reciprocal is exact, so the product equals
+    // the reference's division
+    const double inv_count = count == 8 ? 0.125 : count == 4 ? 0.25 : count == 2 ? 0.5 : count == 1 ? 1.0 : 0.0;
 #pragma unroll
     for (int d = 0; d < DOF; ++d) {
-        double v = count > 0 ? acc[d] / count : 0.0;
+        double v = acc[d] * inv_count;
         if ((mf >> d) & 1)
             v = 0.0;
         const long r = f * DOF + d;

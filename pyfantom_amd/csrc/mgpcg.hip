@@ -165,31 +165,53 @@ static int lev_ku(MgHandle *h, int l, int mode, const double *u, double *y, cons
     } while (0)
 
 // n Jacobi sweeps; *px holds the input (ignored when from_zero) and gets the output
-static int mg_smooth(MgHandle *h, int l, const double *b, double **px, bool from_zero)
+// operators whose kernels fuse the zero-guess scaling into the first sweep
+static bool lev_has_jac0(const MgHandle *h, int l)
+{
+    const MgLevel &L = h->lev[l];
+    if (L.kind == LK_GAL1)
+        return true;
+    if (L.kind == LK_ELEM)
+        return L.S != nullptr;
+    return h->kernel == FANTOM_KU_TILE && L.dim == 3;
+}
+
+// n_smooth sweeps on x (*px); the result is returned in *px.  final
+// (optional) receives the last sweep instead of the ping-pong buffer.
+static int mg_smooth(MgHandle *h, int l, const double *b, double **px, bool from_zero, double *final = nullptr)
 {
     MgLevel &L = h->lev[l];
     double *x = *px;
     double *other = (x == L.x) ? L.x2 : L.x;
-    for (int s = 0; s < h->n_smooth; ++s) {
+    int s0 = 0;
+    if (from_zero && h->n_smooth >= 2 && lev_has_jac0(h, l)) {
+        // x = omega D^-1 b followed by one sweep, in one pass
+        MG_TRY(lev_ku(h, l, MODE_JAC0, nullptr, x, b, L.dinv, h->omega, nullptr));
+        s0 = 2;
+    }
+    for (int s = s0; s < h->n_smooth; ++s) {
         if (s == 0 && from_zero) {
             MG_TRY(launch_scale(L.n_dof, h->omega, L.dinv, b, x, h->stream));
             continue;
         }
-        MG_TRY(lev_ku(h, l, MODE_JAC, x, other, b, L.dinv, h->omega, nullptr));
-        double *t = x;
-        x = other;
-        other = t;
+        double *dst = (final && s == h->n_smooth - 1) ? final : other;
+        MG_TRY(lev_ku(h, l, MODE_JAC, x, dst, b, L.dinv, h->omega, nullptr));
+        other = x;
+        x = dst;
     }
     *px = x;
     return kStatusOk;
 }
 
-static int mg_cycle(MgHandle *h, int l, const double *b, double **out)
+// one V-cycle on level l for right-hand side b; final (optional) is where
+// the result should land (the last post-smoothing sweep writes it there)
+static int mg_cycle(MgHandle *h, int l, const double *b, double **out, double *final = nullptr)
 {
     MgLevel &L = h->lev[l];
     if (l == h->nlev - 1) {
-        MG_TRY(launch_gemv(h->nc, h->Ainv, b, L.x, h->stream));
-        *out = L.x;
+        double *dst = final ? final : L.x;
+        MG_TRY(launch_gemv(h->nc, h->Ainv, b, dst, h->stream));
+        *out = dst;
         return kStatusOk;
     }
     double *x = L.x;
@@ -200,7 +222,7 @@ static int mg_cycle(MgHandle *h, int l, const double *b, double **out)
     double *e = nullptr;
     MG_TRY(mg_cycle(h, l + 1, C.b, &e));
     MG_TRY(launch_prolong(L.g, L.dim, L.dof, 0, L.mask, e, x, 1, h->stream));
-    MG_TRY(mg_smooth(h, l, b, &x, false));
+    MG_TRY(mg_smooth(h, l, b, &x, false, final));
     *out = x;
     return kStatusOk;
 }
@@ -209,10 +231,12 @@ static int mg_cycle(MgHandle *h, int l, const double *b, double **out)
 static int mg_precond(MgHandle *h, const double *r, double *z)
 {
     double *out = nullptr;
-    MG_TRY(mg_cycle(h, 0, r, &out));
-    const long n = h->lev[0].n_dof;
-    hipLaunchKernelGGL(mg_copy_k, dim3(vec_grid(n)), dim3(256), 0, h->stream, n, out, z);
-    FANTOM_LAUNCH_CHECK("mg copy");
+    MG_TRY(mg_cycle(h, 0, r, &out, z));
+    if (out != z) {   // no post-smoothing sweep
+        const long n = h->lev[0].n_dof;
+        hipLaunchKernelGGL(mg_copy_k, dim3(vec_grid(n)), dim3(256), 0, h->stream, n, out, z);
+        FANTOM_LAUNCH_CHECK("mg copy");
+    }
     return kStatusOk;
 }
 

@@ -48,6 +48,7 @@ struct MxLevel {
 struct MxHandle {
     int nlev, n_smooth, galerkin;
     double *Gtab, *Btab, *Gdiag, *Ke1;
+    float *Btab32;   // the level-1 table in the FP32 operator's precision
     double omega;
     std::vector<MxLevel> lev;
     // FP64 fine-level data
@@ -277,40 +278,49 @@ static int mx_ku(MxHandle *h, int l, int mode, const float *u, float *y, const f
 {
     MxLevel &L = h->lev[l];
     if (L.kind == 2)
-        return launch_gal1_f32(mode, L.g, h->Btab, h->lev[0].E, L.mask, u, y, b, dinv, h->omega, partials, h->stream);
+        return launch_gal1_f32(mode, L.g, h->Btab32, h->lev[0].E, L.mask, u, y, b, dinv, h->omega, partials, h->stream);
     if (L.kind == 3)
         return L.S ? launch_stencil_ku_f32(mode, L.g, L.S, u, y, b, dinv, h->omega, partials, h->stream)
                    : launch_elem_f32(mode, L.g, L.Ke, L.mask, u, y, b, dinv, h->omega, partials, h->stream);
     return launch_ku_tile_f32(mode, L.g, L.Kb, L.E, L.mask, u, y, b, dinv, h->omega, partials, h->stream);
 }
 
-static int mx_smooth(MxHandle *h, int l, const float *b, float **px, bool from_zero)
+// n_smooth sweeps on x (*px); final (optional) receives the last sweep
+static int mx_smooth(MxHandle *h, int l, const float *b, float **px, bool from_zero, float *final = nullptr)
 {
     MxLevel &L = h->lev[l];
     float *x = *px;
     float *other = (x == L.x) ? L.x2 : L.x;
-    for (int s = 0; s < h->n_smooth; ++s) {
+    int s0 = 0;
+    // every kind of the FP32 hierarchy fuses the zero-guess scaling except
+    // element-matrix levels without a stencil
+    if (from_zero && h->n_smooth >= 2 && (L.kind != 3 || L.S != nullptr)) {
+        MX_TRY(mx_ku(h, l, MODE_JAC0, nullptr, x, b, L.dinv, nullptr));
+        s0 = 2;
+    }
+    for (int s = s0; s < h->n_smooth; ++s) {
         if (s == 0 && from_zero) {
             hipLaunchKernelGGL(mx_scale_k, dim3(vec_grid(L.n_dof)), dim3(256), 0, h->stream, L.n_dof,
                                (float)h->omega, L.dinv, b, x);
             continue;
         }
-        MX_TRY(mx_ku(h, l, MODE_JAC, x, other, b, L.dinv, nullptr));
-        float *t = x;
-        x = other;
-        other = t;
+        float *dst = (final && s == h->n_smooth - 1) ? final : other;
+        MX_TRY(mx_ku(h, l, MODE_JAC, x, dst, b, L.dinv, nullptr));
+        other = x;
+        x = dst;
     }
     *px = x;
     FANTOM_LAUNCH_CHECK("mx smooth");
     return kStatusOk;
 }
 
-static int mx_cycle(MxHandle *h, int l, const float *b, float **out)
+static int mx_cycle(MxHandle *h, int l, const float *b, float **out, float *final = nullptr)
 {
     MxLevel &L = h->lev[l];
     if (l == h->nlev - 1) {
-        MX_TRY(launch_gemv_f32(h->nc, h->Ainv, b, L.x, h->stream));
-        *out = L.x;
+        float *dst = final ? final : L.x;
+        MX_TRY(launch_gemv_f32(h->nc, h->Ainv, b, dst, h->stream));
+        *out = dst;
         return kStatusOk;
     }
     float *x = L.x;
@@ -321,7 +331,7 @@ static int mx_cycle(MxHandle *h, int l, const float *b, float **out)
     float *e = nullptr;
     MX_TRY(mx_cycle(h, l + 1, C.b, &e));
     MX_TRY(launch_prolong_f32(L.g, 3, 3, 0, L.mask, e, x, 1, h->stream));
-    MX_TRY(mx_smooth(h, l, b, &x, false));
+    MX_TRY(mx_smooth(h, l, b, &x, false, final));
     *out = x;
     return kStatusOk;
 }
@@ -329,8 +339,9 @@ static int mx_cycle(MxHandle *h, int l, const float *b, float **out)
 static int mx_precond(MxHandle *h, const float *r, float *z)
 {
     float *out = nullptr;
-    MX_TRY(mx_cycle(h, 0, r, &out));
-    hipLaunchKernelGGL(mx_copy_f32, dim3(vec_grid(h->n)), dim3(256), 0, h->stream, h->n, out, z);
+    MX_TRY(mx_cycle(h, 0, r, &out, z));
+    if (out != z)   // no post-smoothing sweep
+        hipLaunchKernelGGL(mx_copy_f32, dim3(vec_grid(h->n)), dim3(256), 0, h->stream, h->n, out, z);
     FANTOM_LAUNCH_CHECK("mx precond");
     return kStatusOk;
 }
@@ -409,9 +420,11 @@ static void mx_layout(MxHandle *h, int nelx, int nely, int nelz, int n_coarse, i
         L.S = (L.kind == 3 && l < h->nlev - 1) ? cv.take<float>((size_t)L.g.n_nodes * 243) : nullptr;
     }
     h->Gtab = h->Btab = h->Gdiag = h->Ke1 = nullptr;
+    h->Btab32 = nullptr;
     if (h->galerkin) {
         h->Gtab = cv.take<double>(8 * 576);
         h->Btab = cv.take<double>(576);
+        h->Btab32 = cv.take<float>(576);
         h->Gdiag = cv.take<double>(8 * 24);
         if (h->nlev == 2)
             h->Ke1 = cv.take<double>((size_t)h->lev[1].g.n_el * 576);
@@ -538,6 +551,9 @@ int fantom_mx_create(int nelx, int nely, int nelz, int n_coarse, int n_smooth, d
         e = hipMemcpyAsync(h->Gtab, tG.data(), sizeof(double) * tG.size(), hipMemcpyHostToDevice, h->stream);
     if (h->galerkin && e == hipSuccess)
         e = hipMemcpyAsync(h->Btab, tB.data(), sizeof(double) * tB.size(), hipMemcpyHostToDevice, h->stream);
+    const std::vector<float> tB32(tB.begin(), tB.end());
+    if (h->galerkin && e == hipSuccess)
+        e = hipMemcpyAsync(h->Btab32, tB32.data(), sizeof(float) * tB32.size(), hipMemcpyHostToDevice, h->stream);
     if (h->galerkin && e == hipSuccess)
         e = hipMemcpyAsync(h->Gdiag, tD.data(), sizeof(double) * tD.size(), hipMemcpyHostToDevice, h->stream);
     if (e == hipSuccess)

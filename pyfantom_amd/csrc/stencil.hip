@@ -102,7 +102,16 @@ __global__ __launch_bounds__(kStBlock) void ku_stencil_k(Grid g, const T *__rest
             // the node itself so that the loads issue unconditionally
             const bool in = mi >= 0 && mi < g.nx && mj >= 0 && mj < g.ny && mk >= 0 && mk < g.nz;
             const long m = in ? ((long)mj * g.nx + mi) * g.nz + mk : nn;
-            const T u0 = u[3 * m], u1 = u[3 * m + 1], u2 = u[3 * m + 2];
+            T u0, u1, u2;
+            if (MODE == MODE_JAC0) {
+                u0 = omega * dinv[3 * m] * bvec[3 * m];
+                u1 = omega * dinv[3 * m + 1] * bvec[3 * m + 1];
+                u2 = omega * dinv[3 * m + 2] * bvec[3 * m + 2];
+            } else {
+                u0 = u[3 * m];
+                u1 = u[3 * m + 1];
+                u2 = u[3 * m + 2];
+            }
             const T *St = S + (long)t * 9 * g.n_nodes + nn;
 #pragma unroll
             for (int c = 0; c < 3; ++c)
@@ -123,6 +132,10 @@ __global__ __launch_bounds__(kStBlock) void ku_stencil_k(Grid g, const T *__rest
             T o;
             if (MODE == MODE_JAC)
                 o = u[r] + omega * dinv[r] * (bvec[r] - kuv);
+            else if (MODE == MODE_JAC0) {
+                const T x0 = omega * dinv[r] * bvec[r];
+                o = x0 + omega * dinv[r] * (bvec[r] - kuv);
+            }
             else if (MODE == MODE_RES)
                 o = bvec[r] - kuv;
             else
@@ -187,6 +200,7 @@ static int stencil_ku_t(int mode, const Grid &g, const T *S, const T *u, T *y, c
         ST_CASE(MODE_JAC)
         ST_CASE(MODE_RES)
         ST_CASE(MODE_KUDOT)
+        ST_CASE(MODE_JAC0)
     default:
         set_error("unknown mode %d", mode);
         return kStatusBadArg;
@@ -236,10 +250,10 @@ extern "C" int fantom_stencil_ku_f64(int mode, int nelx, int nely, int nelz, con
                                      void *stream)
 {
     FANTOM_CHECK_ARG(grid_ok(nelx, nely, nelz) && nelz > 0, "bad 3D grid");
-    FANTOM_CHECK_ARG(S && u && y, "null operand");
-    FANTOM_CHECK_ARG(mode >= MODE_KU && mode <= MODE_KUDOT, "unknown mode %d", mode);
-    FANTOM_CHECK_ARG((mode != MODE_JAC && mode != MODE_RES) || b, "mode needs b");
-    FANTOM_CHECK_ARG(mode != MODE_JAC || dinv, "mode needs dinv");
+    FANTOM_CHECK_ARG(S && (u || mode == MODE_JAC0) && y, "null operand");
+    FANTOM_CHECK_ARG(mode >= MODE_KU && mode <= MODE_JAC0, "unknown mode %d", mode);
+    FANTOM_CHECK_ARG((mode != MODE_JAC && mode != MODE_JAC0 && mode != MODE_RES) || b, "mode needs b");
+    FANTOM_CHECK_ARG((mode != MODE_JAC && mode != MODE_JAC0) || dinv, "mode needs dinv");
     FANTOM_CHECK_ARG(mode != MODE_KUDOT || partials, "mode needs partials");
     return launch_stencil_ku(mode, make_grid(nelx, nely, nelz), S, u, y, b, dinv, omega, partials,
                              as_stream(stream));

@@ -123,9 +123,15 @@ __global__ __launch_bounds__(kTThreads) void ku_tile_kernel(Grid g, Op op, const
             m = 0;
             if (col_in && j >= 0 && j < g.ny) {
                 const long n = (long)j * nplane + col;
-                v[0] = u[3 * n];
-                v[1] = u[3 * n + 1];
-                v[2] = u[3 * n + 2];
+                if (MODE == MODE_JAC0) {
+#pragma unroll
+                    for (int c = 0; c < 3; ++c)
+                        v[c] = omega * dinv[3 * n + c] * bvec[3 * n + c];
+                } else {
+                    v[0] = u[3 * n];
+                    v[1] = u[3 * n + 1];
+                    v[2] = u[3 * n + 2];
+                }
                 if (mask)
                     m = mask[n];
             }
@@ -180,12 +186,12 @@ __global__ __launch_bounds__(kTThreads) void ku_tile_kernel(Grid g, Op op, const
             load_node(L + 2 <= jc1 ? L + 2 : -1, upf, mpf);
             load_elem(dnx, L + 1 < jc1 ? L + 1 : -1);
             T bnx[3] = {T(0), T(0), T(0)}, dgnx[3] = {T(0), T(0), T(0)};
-            if ((MODE == MODE_JAC || MODE == MODE_RES) && out_col && L + 1 < jc1) {
+            if ((MODE == MODE_JAC || MODE == MODE_JAC0 || MODE == MODE_RES) && out_col && L + 1 < jc1) {
                 const long r0 = 3 * ((long)(L + 1) * nplane + col);
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
                     bnx[c] = bvec[r0 + c];
-                    if (MODE == MODE_JAC)
+                    if (MODE == MODE_JAC || MODE == MODE_JAC0)
                         dgnx[c] = dinv[r0 + c];
                 }
             }
@@ -237,7 +243,7 @@ __global__ __launch_bounds__(kTThreads) void ku_tile_kernel(Grid g, Op op, const
                 for (int c = 0; c < 3; ++c) {
                     const T kuv = ((mcur >> c) & 1) ? ucur[c] : val[c] + xr[c][nb_dn];
                     T o;
-                    if (MODE == MODE_JAC)
+                    if (MODE == MODE_JAC || MODE == MODE_JAC0)
                         o = ucur[c] + omega * dgcur[c] * (bcur[c] - kuv);
                     else if (MODE == MODE_RES)
                         o = bcur[c] - kuv;

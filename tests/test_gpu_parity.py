@@ -436,6 +436,10 @@ def test_stencil_levels(oracle, nel):
     _lib.call("fantom_stencil_ku_f64", 2, *g2, S.data_ptr(), ud.data_ptr(), y.data_ptr(), bd.data_ptr(), None, 0.0,
               None)
     assert _rel(_np(y), b - ref) <= 1e-12
+    _lib.call("fantom_stencil_ku_f64", 4, *g2, S.data_ptr(), None, y.data_ptr(), bd.data_ptr(), dd.data_ptr(), 0.6,
+              None)
+    x0 = 0.6 * dinv * b
+    assert _rel(_np(y), x0 + 0.6 * dinv * (b - apply(2, x0))) <= 1e-12
     npart = _lib.load().fantom_stencil_partials(*g2)
     part = torch.zeros(npart, dtype=torch.float64, device="cuda")
     scal = torch.zeros(16, dtype=torch.float64, device="cuda")
@@ -444,3 +448,27 @@ def test_stencil_levels(oracle, nel):
     _lib.call("fantom_finalize_f64", part.data_ptr(), npart, scal.data_ptr(), _lib.FIN_STORE, 5)
     assert _rel(_np(y), ref) <= 1e-12
     assert abs(float(scal[5].item()) - u @ ref) <= 1e-12 * abs(u @ ref)
+
+
+@pytest.mark.parametrize("nel", [(8, 4, 4), (30, 14, 45)])
+def test_jacobi0_matches_scale_then_sweep(oracle, nel):
+    """The fused first sweep (x0 = omega D^-1 b formed on load) equals the
+    scaling followed by one Jacobi sweep, against the oracle's assembled K."""
+    from pyfantom_amd import _lib
+    from pyfantom_amd.stiffness import hex8_blocks
+    torch = _torch()
+    print(f"jacobi0 {nel}", flush=True)
+    K0, el, E, u, con = _case(oracle, nel, bc="clamp", seed=6)
+    n = len(u)
+    rng = np.random.default_rng(9)
+    b = rng.standard_normal(n)
+    dinv = rng.uniform(0.5, 2.0, n)
+    A = oracle.assemble(K0, el, E, n, con)
+    x0 = 0.6 * dinv * b
+    ref = x0 + 0.6 * dinv * (b - A @ x0)
+    kb = hex8_blocks(K0)
+    Ed, bd, dd, md = _dev(E), _dev(b), _dev(dinv), _dev(_mask_from_con(con, 3))
+    y = torch.empty(n, dtype=torch.float64, device="cuda")
+    _lib.call("fantom_ku_jacobi0_f64", _lib.KU_TILE, *nel, kb.ctypes.data, Ed.data_ptr(), md.data_ptr(),
+              bd.data_ptr(), dd.data_ptr(), 0.6, y.data_ptr())
+    assert _rel(_np(y), ref) <= 1e-13
