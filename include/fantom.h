@@ -210,6 +210,14 @@ int fantom_elem_ku_f64(int nelx, int nely, int nelz, const double *Ke, const uin
                        double *y, void *stream);
 int fantom_elem_from_fine_f64(int nelx, int nely, int nelz, const double *G, const double *Efine, double *Ke,
                               void *stream);
+/* galerkin_tables2 (host): the 64 level-2 tables G2 (64 x 576, entry
+ * 8 c + g = P_c^T G_g P_c for level-1 child c and fine child g) from the 8
+ * child tables G; gal2_build: level-2 element matrices Kc (n_el x 576) as
+ * sum_f E_f G2[f] over the 64 fine moduli of each element (grid = level 2,
+ * the fine grid is 4x in every direction). */
+int fantom_galerkin_tables2(const double *G, double *G2);
+int fantom_gal2_build_f64(int nelx, int nely, int nelz, const double *G2, const double *Efine, double *Kc,
+                          void *stream);
 
 /* Assembled 27-point node stencils of a Galerkin level (hex8), the form the
  * native drivers apply on levels >= 2 (the reference applies its coarse CSR

@@ -98,6 +98,8 @@ SIGNATURES = {
     "fantom_gal_build_f64": (_c_int, [_c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp]),
     "fantom_elem_ku_f64": (_c_int, [_c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp]),
     "fantom_elem_from_fine_f64": (_c_int, [_c_int, _c_int, _c_int, _vp, _vp, _vp, _vp]),
+    "fantom_galerkin_tables2": (_c_int, [_vp, _vp]),
+    "fantom_gal2_build_f64": (_c_int, [_c_int, _c_int, _c_int, _vp, _vp, _vp, _vp]),
     "fantom_stencil_build_f64": (_c_int, [_c_int, _c_int, _c_int, _vp, _vp, _vp, _vp]),
     "fantom_stencil_ku_f64": (_c_int, [_c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _c_double, _vp,
                                        _vp]),

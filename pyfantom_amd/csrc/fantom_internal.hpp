@@ -73,6 +73,9 @@ int launch_gal_build_f32E(const Grid &gc, int src_kind, const double *G, const f
                           double *Kc, hipStream_t s);
 int launch_elem_from_fine(const Grid &gc, const double *G, const double *Ef, double *Ke, hipStream_t s);
 int launch_elem_from_fine_f32E(const Grid &gc, const double *G, const float *Ef, double *Ke, hipStream_t s);
+int galerkin_tables2(const double *G, double *G2);   // host: 64 x 576 level-2 tables from the 8 child tables
+int launch_gal2_build(const Grid &gc, const double *G2, const double *Ef, double *Kc, hipStream_t s);
+int launch_gal2_build_f32E(const Grid &gc, const double *G2, const float *Ef, double *Kc, hipStream_t s);
 int launch_dense_from_elem(const Grid &g, const double *Ke, const uint8_t *mask, double *A, hipStream_t s);
 
 // stencil.hip (assembled 27-point stencils of Galerkin levels >= 2)

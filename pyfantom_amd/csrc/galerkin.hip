@@ -22,6 +22,7 @@
 // Levels >= 2 store explicit 24x24 element matrices (natural corner order
 // b = bx + 2by + 4bz), built once per setup from the level below.
 #include <cmath>
+#include <vector>
 
 #include "../../include/fantom.h"
 #include "tile_skel.hpp"
@@ -341,6 +342,67 @@ __global__ __launch_bounds__(256) void gal_build(Grid gc, int src_kind, const do
     }
 }
 
+// Level-2 element matrices straight from the fine moduli: with the 64 fine
+// elements f = 8 c + g of a level-2 element (c its level-1 child, g that
+// child's fine child), K = sum_f E_f G2[f], G2[8c + g] = P_c^T G_g P_c
+// (galerkin_tables2).  A block of 256 threads builds 8 coarse elements: the
+// 8 x 64 moduli wait in LDS, each thread owns up to 3 of the 576 entries for
+// all 8 elements and streams the tables from L2.
+constexpr int kG2Elems = 8;
+
+template <typename TE>
+__global__ __launch_bounds__(256) void gal2_from_fine(Grid gc, const double *__restrict__ G2,
+                                                    const TE *__restrict__ Ef, double *__restrict__ Kc)
+{
+    __shared__ double Es[kG2Elems][64];
+    const long e0 = (long)blockIdx.x * kG2Elems;
+    const int fny = 4 * gc.nely, fnz = 4 * gc.nelz;   // fine element counts along y and z
+    for (int q = threadIdx.x; q < kG2Elems * 64; q += 256) {
+        const int el = q / 64, f = q % 64, c = f >> 3, gch = f & 7;
+        const long E = e0 + el;
+        double v = 0.0;
+        if (E < gc.n_el) {
+            const int ei = (int)(E / ((long)gc.nely * gc.nelz));
+            const int r0 = (int)(E % ((long)gc.nely * gc.nelz));
+            const int ej = r0 / gc.nelz, ek = r0 % gc.nelz;
+            const int fx = 2 * (2 * ei + (c & 1)) + (gch & 1);
+            const int fy = 2 * (2 * ej + ((c >> 1) & 1)) + ((gch >> 1) & 1);
+            const int fz = 2 * (2 * ek + (c >> 2)) + (gch >> 2);
+            v = (double)Ef[((long)fx * fny + fy) * fnz + fz];
+        }
+        Es[el][f] = v;
+    }
+    __syncthreads();
+    double acc[3][kG2Elems];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int el = 0; el < kG2Elems; ++el)
+            acc[k][el] = 0.0;
+    const int q0 = threadIdx.x, q1 = threadIdx.x + 256, q2 = threadIdx.x + 512;
+    for (int f = 0; f < 64; ++f) {
+        const double g0 = G2[f * 576 + q0], g1 = G2[f * 576 + q1];
+        const double g2 = q2 < 576 ? G2[f * 576 + q2] : 0.0;
+#pragma unroll
+        for (int el = 0; el < kG2Elems; ++el) {
+            const double e = Es[el][f];
+            acc[0][el] += e * g0;
+            acc[1][el] += e * g1;
+            acc[2][el] += e * g2;
+        }
+    }
+#pragma unroll
+    for (int el = 0; el < kG2Elems; ++el) {
+        const long E = e0 + el;
+        if (E >= gc.n_el)
+            break;
+        Kc[E * 576 + q0] = acc[0][el];
+        Kc[E * 576 + q1] = acc[1][el];
+        if (q2 < 576)
+            Kc[E * 576 + q2] = acc[2][el];
+    }
+}
+
 // Level-1 element matrices materialised from the fine moduli (used when the
 // coarsest level is level 1): K_E = sum_g E_g G_g
 template <typename TE>
@@ -536,6 +598,25 @@ static int gal_build_t(const Grid &gc, int src_kind, const double *G, const TE *
     return kStatusOk;
 }
 
+template <typename TE>
+static int gal2_launch(const Grid &gc, const double *G2, const TE *Ef, double *Kc, hipStream_t s)
+{
+    const unsigned blocks = (unsigned)((gc.n_el + kG2Elems - 1) / kG2Elems);
+    hipLaunchKernelGGL(gal2_from_fine<TE>, dim3(blocks), dim3(256), 0, s, gc, G2, Ef, Kc);
+    FANTOM_LAUNCH_CHECK("gal2_from_fine");
+    return kStatusOk;
+}
+
+int launch_gal2_build(const Grid &gc, const double *G2, const double *Ef, double *Kc, hipStream_t s)
+{
+    return gal2_launch<double>(gc, G2, Ef, Kc, s);
+}
+
+int launch_gal2_build_f32E(const Grid &gc, const double *G2, const float *Ef, double *Kc, hipStream_t s)
+{
+    return gal2_launch<float>(gc, G2, Ef, Kc, s);
+}
+
 int launch_gal_build(const Grid &gc, int src_kind, const double *G, const double *Ef, const double *Kf, double *Kc,
                      hipStream_t s)
 {
@@ -582,17 +663,11 @@ int launch_dense_from_elem(const Grid &g, const double *Ke, const uint8_t *mask,
 using namespace fantom;
 
 // ---- host tables -----------------------------------------------------------------------
-extern "C" int fantom_galerkin_tables(const double *K0, double *G, double *Btab, double *Gdiag, double *check)
+// out[ch] = P_ch^T Kn P_ch (24 x 24 each, natural corner order) for the 8
+// children ch of a coarse element, P_ch the trilinear prolongation from the
+// coarse element's corners to child ch's corners.
+static void child_products(const double *Kn, double *out)
 {
-    FANTOM_CHECK_ARG(K0 && G && Btab && Gdiag, "null operand");
-    static const int loc[8] = {0, 1, 3, 2, 4, 5, 7, 6};  // natural corner -> reference local corner
-    double Kn[24][24];
-    for (int b = 0; b < 8; ++b)
-        for (int c = 0; c < 3; ++c)
-            for (int b2 = 0; b2 < 8; ++b2)
-                for (int d = 0; d < 3; ++d)
-                    Kn[b * 3 + c][b2 * 3 + d] = K0[(loc[b] * 3 + c) * 24 + loc[b2] * 3 + d];
-    // child matrices G_ch = P_ch^T Kn P_ch
     for (int ch = 0; ch < 8; ++ch) {
         double P[24][24] = {};
         for (int gam = 0; gam < 8; ++gam)
@@ -611,7 +686,7 @@ extern "C" int fantom_galerkin_tables(const double *K0, double *G, double *Btab,
             for (int j = 0; j < 24; ++j) {
                 double s = 0.0;
                 for (int k = 0; k < 24; ++k)
-                    s += Kn[i][k] * P[k][j];
+                    s += Kn[i * 24 + k] * P[k][j];
                 T[i][j] = s;
             }
         for (int i = 0; i < 24; ++i)
@@ -619,11 +694,38 @@ extern "C" int fantom_galerkin_tables(const double *K0, double *G, double *Btab,
                 double s = 0.0;
                 for (int k = 0; k < 24; ++k)
                     s += P[k][i] * T[k][j];
-                G[ch * 576 + i * 24 + j] = s;
+                out[ch * 576 + i * 24 + j] = s;
             }
+    }
+}
+
+int fantom::galerkin_tables2(const double *G, double *G2)
+{
+    std::vector<double> tmp(8 * 576);
+    for (int g = 0; g < 8; ++g) {
+        child_products(G + g * 576, tmp.data());
+        for (int c = 0; c < 8; ++c)
+            for (int q = 0; q < 576; ++q)
+                G2[(c * 8 + g) * 576 + q] = tmp[c * 576 + q];
+    }
+    return kStatusOk;
+}
+
+extern "C" int fantom_galerkin_tables(const double *K0, double *G, double *Btab, double *Gdiag, double *check)
+{
+    FANTOM_CHECK_ARG(K0 && G && Btab && Gdiag, "null operand");
+    static const int loc[8] = {0, 1, 3, 2, 4, 5, 7, 6};  // natural corner -> reference local corner
+    double Kn[24][24];
+    for (int b = 0; b < 8; ++b)
+        for (int c = 0; c < 3; ++c)
+            for (int b2 = 0; b2 < 8; ++b2)
+                for (int d = 0; d < 3; ++d)
+                    Kn[b * 3 + c][b2 * 3 + d] = K0[(loc[b] * 3 + c) * 24 + loc[b2] * 3 + d];
+    // child matrices G_ch = P_ch^T Kn P_ch
+    child_products(&Kn[0][0], G);
+    for (int ch = 0; ch < 8; ++ch)
         for (int i = 0; i < 24; ++i)
             Gdiag[ch * 24 + i] = G[ch * 576 + i * 24 + i];
-    }
     // Ghat_0 = V^T G_0 V with orthonormal V, blocks scaled by 1/8
     auto H = [](int b, int t) { return (__builtin_popcount(b & t) & 1) ? -1.0 : 1.0; };
     double maxabs = 0.0, sym_err = 0.0, sum_err = 0.0;
@@ -675,6 +777,19 @@ extern "C" int fantom_galerkin_tables(const double *K0, double *G, double *Btab,
         return kStatusBadArg;
     }
     return kStatusOk;
+}
+
+extern "C" int fantom_galerkin_tables2(const double *G, double *G2)
+{
+    FANTOM_CHECK_ARG(G && G2, "null operand");
+    return galerkin_tables2(G, G2);
+}
+
+extern "C" int fantom_gal2_build_f64(int nelx, int nely, int nelz, const double *G2, const double *Efine, double *Kc,
+                                     void *stream)
+{
+    FANTOM_CHECK_ARG(grid_ok(nelx, nely, nelz) && nelz > 0 && G2 && Efine && Kc, "bad operands");
+    return launch_gal2_build(make_grid(nelx, nely, nelz), G2, Efine, Kc, as_stream(stream));
 }
 
 extern "C" int fantom_gal1_ku_f64(int nelx, int nely, int nelz, const double *Btab, const double *Efine,

@@ -57,6 +57,7 @@ enum { LK_FINE = 0, LK_REDISC = 1, LK_GAL1 = 2, LK_ELEM = 3 };
 struct MgHandle {
     int dim, dof, nlev, n_smooth, kernel, galerkin;
     double *Gtab, *Btab, *Gdiag;     // device Galerkin tables (hex8)
+    double *G2tab;                   // 64 x 576 level-2 tables (levels >= 2 built from the fine moduli)
     double *Ke1;                     // level-1 element matrices when level 1 is the coarsest
     double omega;
     std::vector<MgLevel> lev;
@@ -303,9 +304,10 @@ static void layout(MgHandle *h, int nelx, int nely, int nelz, int n_coarse, int 
                 L.S = cv.take<double>((size_t)L.g.n_nodes * 243);
         }
     }
-    h->Gtab = h->Btab = h->Gdiag = h->Ke1 = nullptr;
+    h->Gtab = h->Btab = h->Gdiag = h->Ke1 = h->G2tab = nullptr;
     if (h->galerkin) {
         h->Gtab = cv.take<double>(8 * 576);
+        h->G2tab = cv.take<double>(64 * 576);
         h->Btab = cv.take<double>(576);
         h->Gdiag = cv.take<double>(8 * 24);
         if (h->nlev == 2)
@@ -411,7 +413,7 @@ int fantom_mg_create(int nelx, int nely, int nelz, int n_coarse, int n_smooth, d
         if (e != hipSuccess)
             break;
     }
-    std::vector<double> tG(8 * 576), tB(576), tD(8 * 24);
+    std::vector<double> tG(8 * 576), tB(576), tD(8 * 24), tG2(64 * 576);
     if (h->galerkin) {
         int st = fantom_galerkin_tables(K0_host, tG.data(), tB.data(), tD.data(), nullptr);
         if (st) {
@@ -421,6 +423,9 @@ int fantom_mg_create(int nelx, int nely, int nelz, int n_coarse, int n_smooth, d
         }
         if (e == hipSuccess)
             e = hipMemcpyAsync(h->Gtab, tG.data(), sizeof(double) * tG.size(), hipMemcpyHostToDevice, h->stream);
+        galerkin_tables2(tG.data(), tG2.data());
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(h->G2tab, tG2.data(), sizeof(double) * tG2.size(), hipMemcpyHostToDevice, h->stream);
         if (e == hipSuccess)
             e = hipMemcpyAsync(h->Btab, tB.data(), sizeof(double) * tB.size(), hipMemcpyHostToDevice, h->stream);
         if (e == hipSuccess)
@@ -496,7 +501,7 @@ int fantom_mg_setup(void *handle, const double *E, const uint8_t *mask, void *st
         for (int l = 2; l < h->nlev; ++l) {
             MgLevel &C = h->lev[l];
             if (l == 2)
-                MG_TRY(launch_gal_build(C.g, 0, h->Gtab, L0.E, nullptr, C.Ke, h->stream));
+                MG_TRY(launch_gal2_build(C.g, h->G2tab, L0.E, C.Ke, h->stream));
             else
                 MG_TRY(launch_gal_build(C.g, 1, h->Gtab, nullptr, h->lev[l - 1].Ke, C.Ke, h->stream));
         }

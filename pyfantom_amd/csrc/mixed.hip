@@ -48,6 +48,7 @@ struct MxLevel {
 struct MxHandle {
     int nlev, n_smooth, galerkin;
     double *Gtab, *Btab, *Gdiag, *Ke1;
+    double *G2tab;   // 64 x 576 level-2 tables
     float *Btab32;   // the level-1 table in the FP32 operator's precision
     double omega;
     std::vector<MxLevel> lev;
@@ -419,10 +420,11 @@ static void mx_layout(MxHandle *h, int nelx, int nely, int nelz, int n_coarse, i
         L.Ke = L.kind == 3 ? cv.take<double>((size_t)L.g.n_el * 576) : nullptr;
         L.S = (L.kind == 3 && l < h->nlev - 1) ? cv.take<float>((size_t)L.g.n_nodes * 243) : nullptr;
     }
-    h->Gtab = h->Btab = h->Gdiag = h->Ke1 = nullptr;
+    h->Gtab = h->Btab = h->Gdiag = h->Ke1 = h->G2tab = nullptr;
     h->Btab32 = nullptr;
     if (h->galerkin) {
         h->Gtab = cv.take<double>(8 * 576);
+        h->G2tab = cv.take<double>(64 * 576);
         h->Btab = cv.take<double>(576);
         h->Btab32 = cv.take<float>(576);
         h->Gdiag = cv.take<double>(8 * 24);
@@ -549,6 +551,11 @@ int fantom_mx_create(int nelx, int nely, int nelz, int n_coarse, int n_smooth, d
                            h->stream);
     if (h->galerkin && e == hipSuccess)
         e = hipMemcpyAsync(h->Gtab, tG.data(), sizeof(double) * tG.size(), hipMemcpyHostToDevice, h->stream);
+    std::vector<double> tG2(64 * 576);
+    if (h->galerkin)
+        galerkin_tables2(tG.data(), tG2.data());
+    if (h->galerkin && e == hipSuccess)
+        e = hipMemcpyAsync(h->G2tab, tG2.data(), sizeof(double) * tG2.size(), hipMemcpyHostToDevice, h->stream);
     if (h->galerkin && e == hipSuccess)
         e = hipMemcpyAsync(h->Btab, tB.data(), sizeof(double) * tB.size(), hipMemcpyHostToDevice, h->stream);
     const std::vector<float> tB32(tB.begin(), tB.end());
@@ -613,7 +620,7 @@ int fantom_mx_setup(void *handle, const double *E, const uint8_t *mask, void *st
         for (int l = 2; l < h->nlev; ++l) {
             MxLevel &C = h->lev[l];
             if (l == 2)
-                MX_TRY(launch_gal_build(C.g, 0, h->Gtab, h->E64, nullptr, C.Ke, s));
+                MX_TRY(launch_gal2_build(C.g, h->G2tab, h->E64, C.Ke, s));
             else
                 MX_TRY(launch_gal_build(C.g, 1, h->Gtab, nullptr, h->lev[l - 1].Ke, C.Ke, s));
         }

@@ -186,3 +186,33 @@ def _natural_from_nodes(A):
         perm += [3 * n, 3 * n + 1, 3 * n + 2]
     perm = np.array(perm)
     return A[np.ix_(perm, perm)]
+
+
+def test_level2_tables_vs_sparse_product(oracle):
+    """The 64 level-2 tables (fantom_galerkin_tables2) give the element matrix
+    of one level-2 element over 4x4x4 fine elements equal to the scipy
+    two-level Galerkin product P1^T P0^T A P0 P1 with random fine moduli."""
+    from pyfantom_amd import _lib
+    K0 = np.ascontiguousarray(oracle.k0_exact(3, 0.3))
+    G, B, Gd, chk = _galerkin_tables(K0)
+    G2 = np.zeros(64 * 576)
+    _lib.check(_lib.load().fantom_galerkin_tables2(np.ascontiguousarray(G).ctypes.data, G2.ctypes.data))
+    G2 = G2.reshape(64, 24, 24)
+    rng = np.random.default_rng(1)
+    nel = (4, 4, 4)
+    Ef = rng.uniform(0.01, 1, 64)          # fine moduli, reference element order (x slowest)
+    A = oracle.assemble(K0, oracle.elements(nel), Ef, 125 * 3).toarray()
+    P0 = oracle.transfer_matrix(nel, 3, False).T.toarray()
+    P1 = oracle.transfer_matrix((2, 2, 2), 3, False).T.toarray()
+    Ac = P1.T @ P0.T @ A @ P0 @ P1
+    Ac_nat = _natural_from_nodes(Ac)
+    # table index f = 8 c + g: level-1 child c, fine child g (bits x + 2y + 4z)
+    w = np.zeros(64)
+    for c in range(8):
+        for g in range(8):
+            fx = 2 * (c & 1) + (g & 1)
+            fy = 2 * ((c >> 1) & 1) + ((g >> 1) & 1)
+            fz = 2 * (c >> 2) + (g >> 2)
+            w[8 * c + g] = Ef[(fx * 4 + fy) * 4 + fz]
+    K2 = np.einsum("f,fij->ij", w, G2)
+    assert np.abs(K2 - Ac_nat).max() <= 1e-13 * np.abs(Ac_nat).max()

@@ -379,6 +379,16 @@ def test_galerkin_levels(oracle, nel):
     y2 = torch.empty_like(u2d)
     _lib.call("fantom_elem_ku_f64", *g2, Ke.data_ptr(), m2.data_ptr(), u2d.data_ptr(), y2.data_ptr())
     assert _rel(_np(y2), apply(2, u2)) <= 1e-12
+    # the same matrices from the 64 level-2 tables (the drivers' build)
+    G2 = np.zeros(64 * 576)
+    _lib.check(_lib.load().fantom_galerkin_tables2(G.ctypes.data, G2.ctypes.data))
+    G2d = _dev(G2)
+    Ke2 = torch.empty_like(Ke)
+    _lib.call("fantom_gal2_build_f64", *g2, G2d.data_ptr(), Ed.data_ptr(), Ke2.data_ptr())
+    torch.cuda.synchronize()
+    assert _rel(_np(Ke2), _np(Ke)) <= 1e-13
+    _lib.call("fantom_elem_ku_f64", *g2, Ke2.data_ptr(), m2.data_ptr(), u2d.data_ptr(), y2.data_ptr())
+    assert _rel(_np(y2), apply(2, u2)) <= 1e-12
     # level-1 explicit matrices agree with the matrix-free form
     Ke1 = torch.empty(int(np.prod(g1)) * 576, dtype=torch.float64, device="cuda")
     _lib.call("fantom_elem_from_fine_f64", *g1, Gd_d.data_ptr(), Ed.data_ptr(), Ke1.data_ptr())
