@@ -7,8 +7,9 @@ namespace fantom {
 // matvec epilogues: y = K u | y = u + omega D^-1 (b - K u) | y = b - K u |
 // y = K u with partial sums of u.(K u) | MODE_JAC0: the Jacobi sweep from
 // x0 = omega D^-1 b, with x0 formed on load (u unused); equals a scaling
-// followed by MODE_JAC, in one pass
-enum { MODE_KU = 0, MODE_JAC = 1, MODE_RES = 2, MODE_KUDOT = 3, MODE_JAC0 = 4 };
+// followed by MODE_JAC, in one pass | MODE_JACDOT: MODE_JAC with the partial
+// sums of b.y (the PCG r.z when the sweep ends a V-cycle on r), tiled kernel
+enum { MODE_KU = 0, MODE_JAC = 1, MODE_RES = 2, MODE_KUDOT = 3, MODE_JAC0 = 4, MODE_JACDOT = 5 };
 
 // ku_node.hip
 int launch_ku_node(int mode, const Grid &g, int dim, const double *K0, const double *E, const uint8_t *mask,

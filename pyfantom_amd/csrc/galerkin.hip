@@ -62,6 +62,9 @@ __device__ __forceinline__ void gwht8(T *w)
 // reads, served by the scalar cache.
 template <typename T>
 struct Gal1Op {
+    // resident waves per SIMD the kernels are built for (FP64 192-224 VGPRs:
+    // 2; FP32 104-120: 4)
+    static constexpr int kWavesPerEU = sizeof(T) == 8 ? 2 : 4;
     const T *Btab;
     const T *Ef;
     int zero;   // 0, set by the host

@@ -31,6 +31,9 @@ static const int kPackRC[kPackedBlocks] = {0, 1, 2, 4, 5, 8, 4, 5, 8, 0, 2, 8, 0
 
 template <typename T>
 struct FineOp {
+    // resident waves per SIMD the kernels are built for (FP64 152-169 VGPRs:
+    // 3; FP32 76-95: 5)
+    static constexpr int kWavesPerEU = sizeof(T) == 8 ? 3 : 5;
     T c[kPackedBlocks];
     const T *E;
     struct Data {
@@ -199,6 +202,7 @@ static int launch_ku_tile_t(int mode, const Grid &g, const double *Kb, const T *
         FANTOM_TILE_CASE(MODE_RES)
         FANTOM_TILE_CASE(MODE_KUDOT)
         FANTOM_TILE_CASE(MODE_JAC0)
+        FANTOM_TILE_CASE(MODE_JACDOT)
     default:
         set_error("unknown matvec mode %d", mode);
         return kStatusBadArg;

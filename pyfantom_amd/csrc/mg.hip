@@ -17,23 +17,25 @@ constexpr int kBlock = 256;
 __device__ __forceinline__ double tw3(int b) { return b == 3 ? 3.375 : (b == 2 ? 4.5 : (b == 1 ? 6.0 : 8.0)); }
 __device__ __forceinline__ double tw2(int b) { return b == 2 ? 2.25 : (b == 1 ? 3.0 : 4.0); }
 
-// One thread per coarse node.  mask_c (optional): constrained coarse DOFs get
-// 0.  The 27 fine neighbours are read unconditionally (an index outside the
-// grid is clamped to the centre node and weighted 0, which leaves every sum
-// unchanged), so that the loads issue together instead of one latency after
-// another; the summation order is the reference's.
+// One thread per coarse node; blockIdx.y is the coarse node plane (y), so
+// that the in-plane index arithmetic stays 32-bit.  mask_c (optional):
+// constrained coarse DOFs get 0.  The 27 fine neighbours are read
+// unconditionally (an index outside the grid is clamped to the centre node
+// and weighted 0, which leaves every sum unchanged), so that the loads issue
+// together instead of one latency after another; the summation order is the
+// reference's.
 template <typename T, int DOF>
 __global__ __launch_bounds__(kBlock) void restrict3d(Grid gf, int normalize, const uint8_t *__restrict__ mask_c,
                                                    const T *__restrict__ rf, T *__restrict__ rc)
 {
     const int nxc = gf.nelx / 2 + 1, nyc = gf.nely / 2 + 1, nzc = gf.nelz / 2 + 1;
-    const long ncn = (long)nxc * nyc * nzc;
-    const long ic = (long)blockIdx.x * kBlock + threadIdx.x;
-    if (ic >= ncn)
+    const int pc = blockIdx.x * kBlock + threadIdx.x;
+    if (pc >= nxc * nzc)
         return;
-    const int kc = (int)(ic % nzc);
-    const long t = ic / nzc;
-    const int icx = (int)(t % nxc), jc = (int)(t / nxc);
+    const int jc = blockIdx.y;
+    const int icx = pc / nzc;
+    const int kc = pc - icx * nzc;
+    const long ic = (long)jc * nxc * nzc + pc;
     const int bnd = (icx == 0 || icx == nxc - 1) + (jc == 0 || jc == nyc - 1) + (kc == 0 || kc == nzc - 1);
     const long fc = 2 * kc + (long)gf.nz * (2 * icx) + (long)gf.nz * gf.nx * (2 * jc);   // coincident fine node
     double acc[DOF];
@@ -110,17 +112,21 @@ __global__ __launch_bounds__(kBlock) void restrict2d(Grid gf, int dof, int norma
 // One thread per fine node: average of parent coarse values (each divided by
 // its TW when normalize).  accumulate: xf += P ec, else xf = P ec.
 // mask_f (optional): constrained fine DOFs receive 0.
+// blockIdx.y is the fine node plane (y): the in-plane index arithmetic
+// stays 32-bit (a 64-bit division per thread cost more than the memory
+// traffic of the whole pass).
 template <typename T, int DOF>
 __global__ __launch_bounds__(kBlock) void prolong3d(Grid gf, int normalize, const uint8_t *__restrict__ mask_f,
                                                   const T *__restrict__ ec, T *__restrict__ xf, int accumulate)
 {
     const int nxc = gf.nelx / 2 + 1, nyc = gf.nely / 2 + 1, nzc = gf.nelz / 2 + 1;
-    const long f = (long)blockIdx.x * kBlock + threadIdx.x;
-    if (f >= gf.n_nodes)
+    const int p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= gf.nx * gf.nz)
         return;
-    const int k = (int)(f % gf.nz);
-    const long t = f / gf.nz;
-    const int i = (int)(t % gf.nx), j = (int)(t / gf.nx);
+    const int j = blockIdx.y;
+    const int i = p / gf.nz;
+    const int k = p - i * gf.nz;
+    const long f = (long)j * gf.nx * gf.nz + p;
     double acc[DOF];
 #pragma unroll
     for (int d = 0; d < DOF; ++d)
@@ -451,16 +457,28 @@ __global__ __launch_bounds__(kBlock) void gemv(int n, const double *__restrict__
 
 static unsigned nblk(long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
+// the 3D transfers launch one workgroup row per node plane (grid y)
+static int check_planes(const Grid &gf)
+{
+    if (gf.ny > 65535 || (long)gf.nx * gf.nz > (1L << 30)) {
+        set_error("3D transfer: %d node planes of %d x %d nodes exceed the launch limits", gf.ny, gf.nx, gf.nz);
+        return kStatusBadArg;
+    }
+    return kStatusOk;
+}
+
 template <typename T>
 static int restrict_t(const Grid &gf, int dim, int dof, int normalize, const uint8_t *mask_c, const T *rf, T *rc,
                       hipStream_t s)
 {
     if (dim == 3) {
-        long ncn = (long)(gf.nelx / 2 + 1) * (gf.nely / 2 + 1) * (gf.nelz / 2 + 1);
+        if (int st = check_planes(gf))
+            return st;
+        const dim3 grid(nblk((long)(gf.nelx / 2 + 1) * (gf.nelz / 2 + 1)), gf.nely / 2 + 1);
         if (dof == 3)
-            hipLaunchKernelGGL((restrict3d<T, 3>), dim3(nblk(ncn)), dim3(kBlock), 0, s, gf, normalize, mask_c, rf, rc);
+            hipLaunchKernelGGL((restrict3d<T, 3>), grid, dim3(kBlock), 0, s, gf, normalize, mask_c, rf, rc);
         else if (dof == 1)
-            hipLaunchKernelGGL((restrict3d<T, 1>), dim3(nblk(ncn)), dim3(kBlock), 0, s, gf, normalize, mask_c, rf, rc);
+            hipLaunchKernelGGL((restrict3d<T, 1>), grid, dim3(kBlock), 0, s, gf, normalize, mask_c, rf, rc);
         else {
             set_error("3D transfers take dof 1 or 3, not %d", dof);
             return kStatusBadArg;
@@ -478,12 +496,13 @@ static int prolong_t(const Grid &gf, int dim, int dof, int normalize, const uint
                      int accumulate, hipStream_t s)
 {
     if (dim == 3) {
+        if (int st = check_planes(gf))
+            return st;
+        const dim3 grid(nblk((long)gf.nx * gf.nz), gf.ny);
         if (dof == 3)
-            hipLaunchKernelGGL((prolong3d<T, 3>), dim3(nblk(gf.n_nodes)), dim3(kBlock), 0, s, gf, normalize, mask_f, ec,
-                               xf, accumulate);
+            hipLaunchKernelGGL((prolong3d<T, 3>), grid, dim3(kBlock), 0, s, gf, normalize, mask_f, ec, xf, accumulate);
         else if (dof == 1)
-            hipLaunchKernelGGL((prolong3d<T, 1>), dim3(nblk(gf.n_nodes)), dim3(kBlock), 0, s, gf, normalize, mask_f, ec,
-                               xf, accumulate);
+            hipLaunchKernelGGL((prolong3d<T, 1>), grid, dim3(kBlock), 0, s, gf, normalize, mask_f, ec, xf, accumulate);
         else {
             set_error("3D transfers take dof 1 or 3, not %d", dof);
             return kStatusBadArg;

@@ -178,8 +178,11 @@ static bool lev_has_jac0(const MgHandle *h, int l)
 }
 
 // n_smooth sweeps on x (*px); the result is returned in *px.  final
-// (optional) receives the last sweep instead of the ping-pong buffer.
-static int mg_smooth(MgHandle *h, int l, const double *b, double **px, bool from_zero, double *final = nullptr)
+// (optional) receives the last sweep instead of the ping-pong buffer.  With
+// rz (level 0 on the tiled 3D kernel only) that last sweep also leaves the
+// block partials of b.final in h->partials and *rz is set.
+static int mg_smooth(MgHandle *h, int l, const double *b, double **px, bool from_zero, double *final = nullptr,
+                     bool *rz = nullptr)
 {
     MgLevel &L = h->lev[l];
     double *x = *px;
@@ -195,8 +198,14 @@ static int mg_smooth(MgHandle *h, int l, const double *b, double **px, bool from
             MG_TRY(launch_scale(L.n_dof, h->omega, L.dinv, b, x, h->stream));
             continue;
         }
-        double *dst = (final && s == h->n_smooth - 1) ? final : other;
-        MG_TRY(lev_ku(h, l, MODE_JAC, x, dst, b, L.dinv, h->omega, nullptr));
+        const bool last = final && s == h->n_smooth - 1;
+        double *dst = last ? final : other;
+        if (last && rz && l == 0 && h->kernel == FANTOM_KU_TILE && L.dim == 3) {
+            MG_TRY(lev_ku(h, l, MODE_JACDOT, x, dst, b, L.dinv, h->omega, h->partials));
+            *rz = true;
+        } else {
+            MG_TRY(lev_ku(h, l, MODE_JAC, x, dst, b, L.dinv, h->omega, nullptr));
+        }
         other = x;
         x = dst;
     }
@@ -206,7 +215,7 @@ static int mg_smooth(MgHandle *h, int l, const double *b, double **px, bool from
 
 // one V-cycle on level l for right-hand side b; final (optional) is where
 // the result should land (the last post-smoothing sweep writes it there)
-static int mg_cycle(MgHandle *h, int l, const double *b, double **out, double *final = nullptr)
+static int mg_cycle(MgHandle *h, int l, const double *b, double **out, double *final = nullptr, bool *rz = nullptr)
 {
     MgLevel &L = h->lev[l];
     if (l == h->nlev - 1) {
@@ -223,20 +232,32 @@ static int mg_cycle(MgHandle *h, int l, const double *b, double **out, double *f
     double *e = nullptr;
     MG_TRY(mg_cycle(h, l + 1, C.b, &e));
     MG_TRY(launch_prolong(L.g, L.dim, L.dof, 0, L.mask, e, x, 1, h->stream));
-    MG_TRY(mg_smooth(h, l, b, &x, false, final));
+    MG_TRY(mg_smooth(h, l, b, &x, false, final, rz));
     *out = x;
     return kStatusOk;
 }
 
-// z = V(r)
-static int mg_precond(MgHandle *h, const double *r, double *z)
+// z = V(r).  With np_rz, also the block partials of r.z in h->partials
+// (fused into the last smoothing sweep when the fine level allows it) and
+// their count in *np_rz.
+static int mg_precond(MgHandle *h, const double *r, double *z, long *np_rz = nullptr)
 {
     double *out = nullptr;
-    MG_TRY(mg_cycle(h, 0, r, &out, z));
+    bool rz = false;
+    MG_TRY(mg_cycle(h, 0, r, &out, z, np_rz ? &rz : nullptr));
     if (out != z) {   // no post-smoothing sweep
         const long n = h->lev[0].n_dof;
         hipLaunchKernelGGL(mg_copy_k, dim3(vec_grid(n)), dim3(256), 0, h->stream, n, out, z);
         FANTOM_LAUNCH_CHECK("mg copy");
+    }
+    if (!np_rz)
+        return kStatusOk;
+    if (rz) {
+        *np_rz = ku_tile_partials(h->lev[0].g);
+    } else {
+        const long n = h->lev[0].n_dof;
+        MG_TRY(launch_dot_partials(n, r, z, h->partials, h->stream));
+        *np_rz = (long)vec_grid(n);
     }
     return kStatusOk;
 }
@@ -251,9 +272,9 @@ static int mg_iteration(MgHandle *h)
     hipLaunchKernelGGL(mg_xr_k, dim3(vec_grid(n)), dim3(256), 0, h->stream, n, h->xs, h->r, h->p, h->q, h->scal,
                        h->partials);
     hipLaunchKernelGGL(mg_conv_k, dim3(1), dim3(1024), 0, h->stream, h->partials, (long)vec_grid(n), h->scal);
-    MG_TRY(mg_precond(h, h->r, h->z));
-    MG_TRY(launch_dot_partials(n, h->r, h->z, h->partials, h->stream));
-    MG_TRY(launch_finalize(h->partials, (long)vec_grid(n), h->scal, FIN_BETA, 0, h->stream));
+    long nrz = 0;
+    MG_TRY(mg_precond(h, h->r, h->z, &nrz));
+    MG_TRY(launch_finalize(h->partials, nrz, h->scal, FIN_BETA, 0, h->stream));
     MG_TRY(launch_pcg_p(n, h->p, h->z, h->scal, 0, h->stream));
     FANTOM_LAUNCH_CHECK("mg iteration");
     return kStatusOk;
@@ -586,9 +607,9 @@ int fantom_mg_solve(void *handle, const double *f, double *x, double tol, int ma
     MG_TRY(launch_dot_partials(n, h->f, h->f, h->partials, s));
     hipLaunchKernelGGL(mg_norm_k, dim3(1), dim3(1024), 0, s, h->partials, (long)G, h->scal);
     MG_TRY(lev_ku(h, 0, MODE_RES, h->xs, h->r, h->f, nullptr, 0.0, nullptr));
-    MG_TRY(mg_precond(h, h->r, h->z));
-    MG_TRY(launch_dot_partials(n, h->r, h->z, h->partials, s));
-    MG_TRY(launch_finalize(h->partials, (long)G, h->scal, FIN_RZ0, 0, s));
+    long nrz = 0;
+    MG_TRY(mg_precond(h, h->r, h->z, &nrz));
+    MG_TRY(launch_finalize(h->partials, nrz, h->scal, FIN_RZ0, 0, s));
     MG_TRY(launch_pcg_p(n, h->p, h->z, h->scal, 1, s));
     FANTOM_LAUNCH_CHECK("mg solve init");
     if (use_graph && !h->have_graph)

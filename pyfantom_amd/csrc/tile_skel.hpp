@@ -186,12 +186,13 @@ __global__ __launch_bounds__(kTThreads) void ku_tile_kernel(Grid g, Op op, const
             load_node(L + 2 <= jc1 ? L + 2 : -1, upf, mpf);
             load_elem(dnx, L + 1 < jc1 ? L + 1 : -1);
             T bnx[3] = {T(0), T(0), T(0)}, dgnx[3] = {T(0), T(0), T(0)};
-            if ((MODE == MODE_JAC || MODE == MODE_JAC0 || MODE == MODE_RES) && out_col && L + 1 < jc1) {
+            if ((MODE == MODE_JAC || MODE == MODE_JAC0 || MODE == MODE_JACDOT || MODE == MODE_RES) && out_col &&
+                L + 1 < jc1) {
                 const long r0 = 3 * ((long)(L + 1) * nplane + col);
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
                     bnx[c] = bvec[r0 + c];
-                    if (MODE == MODE_JAC || MODE == MODE_JAC0)
+                    if (MODE == MODE_JAC || MODE == MODE_JAC0 || MODE == MODE_JACDOT)
                         dgnx[c] = dinv[r0 + c];
                 }
             }
@@ -243,7 +244,7 @@ __global__ __launch_bounds__(kTThreads) void ku_tile_kernel(Grid g, Op op, const
                 for (int c = 0; c < 3; ++c) {
                     const T kuv = ((mcur >> c) & 1) ? ucur[c] : val[c] + xr[c][nb_dn];
                     T o;
-                    if (MODE == MODE_JAC || MODE == MODE_JAC0)
+                    if (MODE == MODE_JAC || MODE == MODE_JAC0 || MODE == MODE_JACDOT)
                         o = ucur[c] + omega * dgcur[c] * (bcur[c] - kuv);
                     else if (MODE == MODE_RES)
                         o = bcur[c] - kuv;
@@ -252,6 +253,8 @@ __global__ __launch_bounds__(kTThreads) void ku_tile_kernel(Grid g, Op op, const
                     y[r0 + c] = o;
                     if (MODE == MODE_KUDOT)
                         part += (double)ucur[c] * (double)o;
+                    if (MODE == MODE_JACDOT)
+                        part += (double)bcur[c] * (double)o;
                 }
             }
 #pragma unroll
@@ -269,7 +272,7 @@ __global__ __launch_bounds__(kTThreads) void ku_tile_kernel(Grid g, Op op, const
             dcur = dnx;
         }
     }
-    if (MODE == MODE_KUDOT) {
+    if (MODE == MODE_KUDOT || MODE == MODE_JACDOT) {
         const double s = block_sum<kTThreads>(part);
         if (threadIdx.x == 0)
             partials[blockIdx.x] = s;
