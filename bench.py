@@ -34,21 +34,40 @@ CONFIGS = {
     "c2": dict(nel=(128, 64, 64), n_level=4, name="cantilever_128x64x64"),
     "c3": dict(nel=(256, 128, 128), n_level=5, name="cantilever_256x128x128"),
     "small": dict(nel=(32, 16, 16), n_level=3, name="cantilever_32x16x16"),
+    # combined run: bridge, MMA, filter r = 2, volume fraction 0.15
+    "c5": dict(nel=(384, 192, 192), n_level=6, name="bridge_384x192x192", kind="bridge", optimizer="mma",
+               vf=0.15, r=2.0, omega=0.5),
+    "bridge_small": dict(nel=(48, 24, 24), n_level=3, name="bridge_48x24x24", kind="bridge", optimizer="mma",
+                         vf=0.15, r=2.0, omega=0.5),
 }
 
 
-def build_problem(pf, nel, n_level, tol=1e-8, precision="fp64"):
+def build_problem(pf, nel, n_level, tol=1e-8, precision="fp64", kind="cantilever", vf=0.3, r=1.5, omega=0.6,
+                  omega_schedule="reference"):
+    """cantilever: x = 0 face clamped, total load -1 in y over the nodes of
+    the bottom edge (y = 0) of the x = L face.  bridge: the four bottom
+    corner nodes (y = 0, x in {0, L}, z in {0, W}) pinned in all DOFs, total
+    load -1 in y spread evenly over the nodes of the top surface y = H."""
     mesh = pf.StructuredMesh3D(*nel, *nel, physics=pf.LinearElasticity(E=1.0, nu=0.3))
     K = pf.StructuredStiffnessKernel(mesh)
-    solver = pf.MultiGrid(mesh, K, maxiter=1000, tol=tol, n_smooth=2, omega=0.6, n_level=n_level, cycle="V",
-                          precision=precision)
+    solver = pf.MultiGrid(mesh, K, maxiter=1000, tol=tol, n_smooth=2, omega=omega, n_level=n_level, cycle="V",
+                          precision=precision, omega_schedule=omega_schedule)
     FE = pf.FiniteElement(mesh, K, solver)
     nx, ny, nz = (n + 1 for n in nel)
-    j, k = np.meshgrid(np.arange(ny), np.arange(nz), indexing="ij")
-    FE.add_dirichlet_boundary_condition(node_ids=(k + nz * nx * j).ravel(), rhs=0)
-    FE.add_point_forces(forces=np.array([[0.0, -1.0 / nz, 0.0]]), node_ids=np.arange(nz) + nz * (nx - 1))
-    filt = pf.StructuredFilter3D(mesh, 1.5)
-    prob = pf.MinimumCompliance(FE, filt, E_mul=[1.0], void=1e-9, penalty=3.0, volume_fraction=[0.3],
+    if kind == "cantilever":
+        j, k = np.meshgrid(np.arange(ny), np.arange(nz), indexing="ij")
+        FE.add_dirichlet_boundary_condition(node_ids=(k + nz * nx * j).ravel(), rhs=0)
+        FE.add_point_forces(forces=np.array([[0.0, -1.0 / nz, 0.0]]), node_ids=np.arange(nz) + nz * (nx - 1))
+    elif kind == "bridge":
+        corners = np.array([k + nz * i for i in (0, nx - 1) for k in (0, nz - 1)])   # y = 0 plane
+        FE.add_dirichlet_boundary_condition(node_ids=corners, rhs=0)
+        i, k = np.meshgrid(np.arange(nx), np.arange(nz), indexing="ij")
+        top = (k + nz * i + nz * nx * (ny - 1)).ravel()
+        FE.add_point_forces(forces=np.array([[0.0, -1.0 / top.size, 0.0]]), node_ids=top)
+    else:
+        raise ValueError(kind)
+    filt = pf.StructuredFilter3D(mesh, r)
+    prob = pf.MinimumCompliance(FE, filt, E_mul=[1.0], void=1e-9, penalty=3.0, volume_fraction=[vf],
                                 heavyside=False, law="modified")
     return mesh, K, solver, FE, filt, prob
 
@@ -79,6 +98,25 @@ def time_ku(torch, pf, K, mesh, reps=50):
     bytes_alg = 16.0 * n_dof + 8.0 * n_el
     return dict(seconds=t, gdof_s=n_dof / t / 1e9, achieved_gbs=bytes_alg / t / 1e9, bytes=bytes_alg,
                 flops_dense=1152.0 * n_el, kernel="ku3d_tile_f64" if kid == _lib.KU_TILE else "ku3d_node_f64")
+
+
+def precision_errors(torch, mesh, FE_m, FE):
+    """One solve of the seeded field (x ~ U[0.01, 1], default_rng(0)) with
+    the mixed-precision solver and with the FP64 one (both cold-started):
+    relative errors of u (L2), compliance and the element sensitivity
+    -u_e^T K0 u_e (L2) against FP64."""
+    x = np.random.default_rng(0).uniform(0.01, 1.0, mesh.n_elements)
+    E = torch.from_numpy(1e-9 + x ** 3 * (1 - 1e-9)).cuda()
+    out = {}
+    for tag, fe in (("mixed", FE_m), ("fp64", FE)):
+        fe.solver.reset()
+        U, res = fe.solve(E)
+        out[tag] = (U.clone(), float(torch.dot(fe.rhs, U)), fe.kernel.process_grad(U).clone(), res)
+    (um, cm, dm, rm), (u6, c6, d6, r6) = out["mixed"], out["fp64"]
+    nrm = torch.linalg.vector_norm
+    return {"u_rel_l2": float(nrm(um - u6) / nrm(u6)), "compliance_rel": abs(cm - c6) / abs(c6),
+            "dc_rel_l2": float(nrm(dm - d6) / nrm(d6)), "rel_residual_mixed": float(rm),
+            "rel_residual_fp64": float(r6)}
 
 
 def measured_traffic(workload):
@@ -135,6 +173,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--ku-reps", type=int, default=50)
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision SIMP run")
+    ap.add_argument("--progress", action="store_true", help="print compliance and PCG iterations every step")
+    ap.add_argument("--omega-schedule", default="reference", choices=["reference", "fixed"],
+                    help="Jacobi weight: the reference's adaptive schedule (omega is its maximum) or fixed omega")
     ap.add_argument("--ku-only", action="store_true",
                     help="only the isolated K.u of the workload and a 256 MiB calibration copy (for counter runs)")
     args = ap.parse_args()
@@ -166,22 +207,33 @@ def main():
                           "calibration_copy_bytes_each_way": src.numel() * 8}), flush=True)
         return
 
+    fallbacks = []
+
     def simp_run(precision):
         """W untimed + K timed SIMP iterations; returns (seconds, iterations, problem objects)."""
-        mesh, K, solver, FE, filt, prob = build_problem(pf, nel, n_level, precision=precision)
+        mesh, K, solver, FE, filt, prob = build_problem(pf, nel, n_level, precision=precision,
+                                                        kind=cfg.get("kind", "cantilever"), vf=cfg.get("vf", 0.3),
+                                                        r=cfg.get("r", 1.5), omega=cfg.get("omega", 0.6),
+                                                        omega_schedule=args.omega_schedule)
         print(f"# {precision}: setup done: {cfg['name']} dofs={K.shape[0]}", file=sys.stderr, flush=True)
-        opt = pf.OC(prob, move=0.2)
+        opt = pf.MMA(prob) if cfg.get("optimizer") == "mma" else pf.OC(prob, move=0.2)
         for w in range(args.warmup):
             opt.iter()
             print(f"# {precision}: warmup {w}: c={prob.f():.6e} iters={solver.last_iterations}", file=sys.stderr,
                   flush=True)
         torch.cuda.synchronize()
         iters, outer = [], []
+        fallbacks.clear()
         t0 = time.perf_counter()
-        for _ in range(args.steps):
+        for s in range(args.steps):
             opt.iter()
             iters.append(solver.last_iterations)
             outer.append(getattr(solver, "last_outer_iterations", None))
+            if solver.last_fallback:
+                fallbacks.append((s, solver.last_fallback_iterations))
+            if args.progress:
+                print(f"# {precision}: step {s}: c={prob.f():.6e} iters={solver.last_iterations}", file=sys.stderr,
+                      flush=True)
         torch.cuda.synchronize()
         t = time.perf_counter() - t0
         print(f"# {precision}: timed {args.steps} steps: {t / args.steps * 1e3:.1f} ms/step, pcg iters {iters}",
@@ -193,12 +245,22 @@ def main():
     mixed = None
     if not args.no_mixed:
         tm, it_m, out_m, objs_m = simp_run("mixed")
+        fb_m = list(fallbacks)
         prob_m = objs_m[5]
+
+        def rel(a, b):
+            return float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))
+
         mixed = {"s_per_iter": tm / args.steps, "inner_iterations_mean": float(np.mean(it_m)),
                  "outer_iterations_mean": float(np.mean([o for o in out_m if o is not None] or [0])),
                  "speedup_vs_fp64": t / tm, "compliance": prob_m.f(),
                  "compliance_rel_diff_vs_fp64": abs(prob_m.f() - prob.f()) / abs(prob.f()),
                  "solve_rel_residual": float(prob_m._residual),
+                 # end state of the same number of SIMP steps from the same start
+                 "u_rel_l2_vs_fp64": rel(prob_m._U, prob._U),
+                 "dc_rel_l2_vs_fp64": rel(prob_m.nabla_f(), prob.nabla_f()),
+                 "seeded_field": precision_errors(torch, mesh, objs_m[3], FE),
+                 "fp64_fallbacks": [{"step": s, "fp64_iterations": n} for s, n in fb_m],
                  "note": "FP64 PCG (FP64 K.u, dots, step lengths, true residual) preconditioned by an FP32 "
                          "V-cycle (FP32 WHT K.u, Galerkin level 1, stencils, transfers); same SIMP steps"}
         del objs_m, prob_m
@@ -217,8 +279,10 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": cfg["name"] + "_simp_oc_gmgpcg_1e-8", "elements": list(nel), "dofs": K.shape[0],
-                   "coarsenings": n_level, "smoother": "jacobi 2+2 omega 0.6",
+        "config": {"workload": cfg["name"] + ("_simp_mma" if cfg.get("optimizer") == "mma" else "_simp_oc")
+                   + "_gmgpcg_1e-8", "elements": list(nel), "dofs": K.shape[0],
+                   "coarsenings": n_level,
+                   "smoother": f"jacobi 2+2 omega {cfg.get('omega', 0.6)} ({args.omega_schedule} schedule)",
                    "precision": "fp64 plain GMG-PCG"},
         "pcg_iterations_mean": float(np.mean(iters)),
         "compliance": prob.f(),
@@ -230,7 +294,7 @@ def main():
                      "kernel": ku["kernel"], "algorithmic_bytes": ku["bytes"],
                      "traffic_source": "profiles/ku_tile_f64_traffic.md (FETCH_SIZE/WRITE_SIZE, copy-calibrated)"},
     }
-    if not args.no_cpu:
+    if not args.no_cpu and cfg.get("kind", "cantilever") == "cantilever":
         print("# cpu baseline ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(nel, n_level, float(np.mean(iters)))
     print(json.dumps(out), flush=True)

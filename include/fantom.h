@@ -259,6 +259,16 @@ long fantom_mg_workspace_size(int nelx, int nely, int nelz, int n_coarse, int ga
 int fantom_mg_create(int nelx, int nely, int nelz, int n_coarse, int n_smooth, double omega, int kernel,
                      int galerkin, const double *K0_host, void *workspace, long workspace_bytes, void **handle);
 int fantom_mg_destroy(void *handle);
+/* Jacobi weight schedule of the solve (replaces the omega update of
+ * MultiGrid.solve, solvers/CUDA/_solvers.py:626-648, and the per-level
+ * factor 1.06^level of _multi_grid, :524-526).  adaptive = 0: the fixed
+ * weight omega of create on every level (the default).  adaptive = 1: the
+ * reference's schedule -- min_omega for the first cycle, then after each
+ * iteration's test on R = ||r|| / ||f||
+ *   omega = min_omega + (omega - min_omega) / 2 * exp(500 (tol - clip(R, tol, 0.1))),
+ * times level_boost^level on each level.  The weight lives in a device
+ * scalar, so batched and graph-replayed iterations follow it. */
+int fantom_mg_set_omega_schedule(void *handle, int adaptive, double min_omega, double level_boost);
 int fantom_mg_setup(void *handle, const double *E, const uint8_t *mask, void *stream);
 int fantom_mg_precondition(void *handle, const double *r, double *z, void *stream);
 int fantom_mg_solve(void *handle, const double *f, double *x, double tol, int maxiter, int use_x0, int use_graph,
@@ -277,6 +287,11 @@ long fantom_mx_workspace_size(int nelx, int nely, int nelz, int n_coarse, int ga
 int fantom_mx_create(int nelx, int nely, int nelz, int n_coarse, int n_smooth, double omega, int galerkin,
                      const double *K0_host, void *workspace, long workspace_bytes, void **handle);
 int fantom_mx_destroy(void *handle);
+/* As fantom_mg_set_omega_schedule, for the FP32 V-cycle; the adaptive
+ * schedule follows the residual of the PCG that the V-cycle preconditions
+ * (the FP64 PCG of fantom_mx_solve_pcg, the inner FP32 PCG of
+ * fantom_mx_solve, restarted at min_omega on each refinement step). */
+int fantom_mx_set_omega_schedule(void *handle, int adaptive, double min_omega, double level_boost);
 int fantom_mx_setup(void *handle, const double *E, const uint8_t *mask, void *stream);
 int fantom_mx_solve(void *handle, const double *f, double *x, double tol, double inner_tol, int max_outer,
                     int max_inner, int use_x0, int use_graph, int *outer_iters, int *inner_iters, double *rel_res,
@@ -284,7 +299,9 @@ int fantom_mx_solve(void *handle, const double *f, double *x, double tol, double
 /* FP64 PCG preconditioned by the handle's FP32 V-cycle (one per iteration):
  * the fine K.u, the vectors, dot products and step lengths stay FP64, the
  * V-cycle runs on FP32 copies.  Stops when the recursive relative residual is
- * below tol; rel_res gets the true FP64 relative residual ||f - K x|| / ||f||.
+ * below tol, or early when it has not dropped by 10% over 20 iterations
+ * (stagnation: the caller sees rel_res >= tol and falls back to the FP64
+ * solver); rel_res gets the true FP64 relative residual ||f - K x|| / ||f||.
  * No counterpart in the reference (its solver is FP64 throughout,
  * solvers/CUDA/_solvers.py:573-660). */
 int fantom_mx_solve_pcg(void *handle, const double *f, double *x, double tol, int maxiter, int use_x0,

@@ -37,11 +37,22 @@ inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s);
     } while (0)
 
 // Structured grid of nelx x nely x nelz elements (nelz = 0 marks 2D).
+// omega_scale (optional, device memory): the current value of an adaptive
+// Jacobi weight; the smoothing kernels multiply their weight argument by it.
 struct Grid {
     int nelx, nely, nelz;
     int nx, ny, nz;  // nodes per axis (nz = 1 in 2D)
     long n_nodes, n_el;
+    const double *omega_scale = nullptr;
 };
+
+// Jacobi weight of a smoothing kernel: the argument, times the device scalar
+// of an adaptive schedule when the grid carries one (a uniform scalar load).
+template <typename T>
+__device__ __forceinline__ T jacobi_weight(const Grid &g, T omega)
+{
+    return g.omega_scale ? (T)((double)omega * g.omega_scale[0]) : omega;
+}
 
 inline Grid make_grid(int nelx, int nely, int nelz)
 {

@@ -11,6 +11,23 @@ namespace fantom {
 // sums of b.y (the PCG r.z when the sweep ends a V-cycle on r), tiled kernel
 enum { MODE_KU = 0, MODE_JAC = 1, MODE_RES = 2, MODE_KUDOT = 3, MODE_JAC0 = 4, MODE_JACDOT = 5 };
 
+// Adaptive Jacobi weight of the reference's MultiGrid.solve
+// (solvers/CUDA/_solvers.py:626-648): omega = min_omega before the first
+// cycle and, after each iteration's convergence test on R = ||r|| / ||f||,
+// omega = min_omega + (max_omega - min_omega) / 2 * exp(500 (tol - clip(R, tol, 0.1))),
+// times 1.06^level on each level (:524-526).
+struct OmegaSchedule {
+    int on;
+    double min, delta;   // min_omega, max_omega - min_omega
+};
+
+__device__ __forceinline__ double schedule_omega(const OmegaSchedule &o, double R, double tol)
+{
+    const double Rc = fmin(fmax(R, tol), 0.1);
+    return o.min + o.delta / 2 * exp((-Rc + tol) * 500.0);
+}
+
+
 // ku_node.hip
 int launch_ku_node(int mode, const Grid &g, int dim, const double *K0, const double *E, const uint8_t *mask,
                    const double *u, double *y, const double *b, const double *dinv, double omega,

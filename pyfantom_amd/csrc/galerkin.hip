@@ -65,6 +65,7 @@ struct Gal1Op {
     // resident waves per SIMD the kernels are built for (FP64 192-224 VGPRs:
     // 2; FP32 104-120: 4)
     static constexpr int kWavesPerEU = sizeof(T) == 8 ? 2 : 4;
+    static constexpr bool kScaled = false;
     const T *Btab;
     const T *Ef;
     int zero;   // 0, set by the host
@@ -137,6 +138,8 @@ __global__ __launch_bounds__(256) void ku_elem(Grid g, const double *__restrict_
                                              T *__restrict__ y, const T *__restrict__ bvec,
                                              const T *__restrict__ dinv, T omega, double *__restrict__ partials)
 {
+    if (MODE == MODE_JAC)
+        omega = jacobi_weight(g, omega);
     const long n = (long)blockIdx.x * 256 + threadIdx.x;
     double part = 0.0;
     if (n < g.n_nodes) {

@@ -41,6 +41,8 @@ __global__ __launch_bounds__(kNodeBlock) void ku3d_node_f64(Grid g, const double
                                                           const double *__restrict__ dinv, double omega,
                                                           double *__restrict__ partials)
 {
+    if (MODE == MODE_JAC)
+        omega = jacobi_weight(g, omega);
     __shared__ double K0[576];
     for (int t = threadIdx.x; t < 576; t += kNodeBlock)
         K0[t] = K0g[t];
@@ -124,6 +126,8 @@ __global__ __launch_bounds__(kNodeBlock) void ku2d_node_f64(Grid g, const double
                                                           const double *__restrict__ dinv, double omega,
                                                           double *__restrict__ partials)
 {
+    if (MODE == MODE_JAC)
+        omega = jacobi_weight(g, omega);
     __shared__ double K0[64];
     if (threadIdx.x < 64)
         K0[threadIdx.x] = K0g[threadIdx.x];

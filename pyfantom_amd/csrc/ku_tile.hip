@@ -34,6 +34,8 @@ struct FineOp {
     // resident waves per SIMD the kernels are built for (FP64 152-169 VGPRs:
     // 3; FP32 76-95: 5)
     static constexpr int kWavesPerEU = sizeof(T) == 8 ? 3 : 5;
+    // apply leaves out the element modulus; the skeleton multiplies it in
+    static constexpr bool kScaled = true;
     T c[kPackedBlocks];
     const T *E;
     struct Data {
@@ -43,57 +45,58 @@ struct FineOp {
     {
         d.e = active ? E[((long)ex * g.nely + ey) * g.nelz + ez] : T(0);
     }
+    __device__ __forceinline__ T scale(const Data &d) const { return d.e; }
     // block s reads h_k = w[k][s ^ (1 << k)] and writes z[k][s ^ (1 << k)]
+    // (without the factor d.e)
     __device__ __forceinline__ void apply(const Data &d, const T (&w)[3][8], T (&z)[3][8]) const
     {
-        const T e = d.e;
         {   // s = 0
             const T h0 = w[0][1], h1 = w[1][2], h2 = w[2][4];
-            z[0][1] = e * (c[0] * h0 + c[1] * h1 + c[2] * h2);
-            z[1][2] = e * (c[1] * h0 + c[3] * h1 + c[4] * h2);
-            z[2][4] = e * (c[2] * h0 + c[4] * h1 + c[5] * h2);
+            z[0][1] = c[0] * h0 + c[1] * h1 + c[2] * h2;
+            z[1][2] = c[1] * h0 + c[3] * h1 + c[4] * h2;
+            z[2][4] = c[2] * h0 + c[4] * h1 + c[5] * h2;
         }
         {   // s = 1 (row/column 0 vanish)
             const T h1 = w[1][3], h2 = w[2][5];
             z[0][0] = T(0);
-            z[1][3] = e * (c[6] * h1 + c[7] * h2);
-            z[2][5] = e * (c[7] * h1 + c[8] * h2);
+            z[1][3] = c[6] * h1 + c[7] * h2;
+            z[2][5] = c[7] * h1 + c[8] * h2;
         }
         {   // s = 2
             const T h0 = w[0][3], h2 = w[2][6];
-            z[0][3] = e * (c[9] * h0 + c[10] * h2);
+            z[0][3] = c[9] * h0 + c[10] * h2;
             z[1][0] = T(0);
-            z[2][6] = e * (c[10] * h0 + c[11] * h2);
+            z[2][6] = c[10] * h0 + c[11] * h2;
         }
         {   // s = 3
             const T h0 = w[0][2], h1 = w[1][1], h2 = w[2][7];
-            z[0][2] = e * (c[12] * h0 + c[13] * h1);
-            z[1][1] = e * (c[13] * h0 + c[14] * h1);
-            z[2][7] = e * (c[15] * h2);
+            z[0][2] = c[12] * h0 + c[13] * h1;
+            z[1][1] = c[13] * h0 + c[14] * h1;
+            z[2][7] = c[15] * h2;
         }
         {   // s = 4
             const T h0 = w[0][5], h1 = w[1][6];
-            z[0][5] = e * (c[16] * h0 + c[17] * h1);
-            z[1][6] = e * (c[17] * h0 + c[18] * h1);
+            z[0][5] = c[16] * h0 + c[17] * h1;
+            z[1][6] = c[17] * h0 + c[18] * h1;
             z[2][0] = T(0);
         }
         {   // s = 5
             const T h0 = w[0][4], h1 = w[1][7], h2 = w[2][1];
-            z[0][4] = e * (c[19] * h0 + c[20] * h2);
-            z[1][7] = e * (c[21] * h1);
-            z[2][1] = e * (c[20] * h0 + c[22] * h2);
+            z[0][4] = c[19] * h0 + c[20] * h2;
+            z[1][7] = c[21] * h1;
+            z[2][1] = c[20] * h0 + c[22] * h2;
         }
         {   // s = 6
             const T h0 = w[0][7], h1 = w[1][4], h2 = w[2][2];
-            z[0][7] = e * (c[23] * h0);
-            z[1][4] = e * (c[24] * h1 + c[25] * h2);
-            z[2][2] = e * (c[25] * h1 + c[26] * h2);
+            z[0][7] = c[23] * h0;
+            z[1][4] = c[24] * h1 + c[25] * h2;
+            z[2][2] = c[25] * h1 + c[26] * h2;
         }
         {   // s = 7
             const T h0 = w[0][6], h1 = w[1][5], h2 = w[2][3];
-            z[0][6] = e * (c[27] * h0 + c[28] * h1 + c[29] * h2);
-            z[1][5] = e * (c[28] * h0 + c[30] * h1 + c[31] * h2);
-            z[2][3] = e * (c[29] * h0 + c[31] * h1 + c[32] * h2);
+            z[0][6] = c[27] * h0 + c[28] * h1 + c[29] * h2;
+            z[1][5] = c[28] * h0 + c[30] * h1 + c[31] * h2;
+            z[2][3] = c[29] * h0 + c[31] * h1 + c[32] * h2;
         }
     }
 };

@@ -34,7 +34,8 @@
 
 namespace fantom {
 
-enum { D_DONE = 8, D_ITER = 9, D_RES = 10, D_TOL = 11, D_NB = 12, D_K = 13 };
+enum { D_DONE = 8, D_ITER = 9, D_RES = 10, D_TOL = 11, D_NB = 12, D_K = 13, D_OMEGA = 14 };
+
 
 struct MgLevel {
     Grid g;
@@ -49,6 +50,7 @@ struct MgLevel {
     double *Ke;            // explicit Galerkin element matrices (levels >= 2)
     double *S;             // assembled stencil of an element-matrix level that is smoothed
     int kind;              // LK_* operator of this level
+    double om;             // weight argument of the smoothing kernels (see OmegaSchedule)
 };
 
 // operator kind of a level
@@ -60,6 +62,7 @@ struct MgHandle {
     double *G2tab;                   // 64 x 576 level-2 tables (levels >= 2 built from the fine moduli)
     double *Ke1;                     // level-1 element matrices when level 1 is the coarsest
     double omega;
+    OmegaSchedule sched;
     std::vector<MgLevel> lev;
     double *xs, *r, *z, *p, *q, *f;   // fine-level PCG vectors
     double *partials;
@@ -74,7 +77,7 @@ struct MgHandle {
 };
 
 // ---- small device kernels specific to the driver --------------------------------
-__global__ void mg_conv_k(const double *__restrict__ partials, long np, double *__restrict__ scal)
+__global__ void mg_conv_k(const double *__restrict__ partials, long np, double *__restrict__ scal, OmegaSchedule om)
 {
     // r.r finalize + convergence test on the device
     double s = 0.0;
@@ -90,6 +93,8 @@ __global__ void mg_conv_k(const double *__restrict__ partials, long np, double *
             scal[D_ITER] = scal[D_K];
             if (R < scal[D_TOL])
                 scal[D_DONE] = 1.0;
+            else if (om.on)
+                scal[D_OMEGA] = schedule_omega(om, R, scal[D_TOL]);
         }
     }
 }
@@ -114,9 +119,10 @@ __global__ void mg_xr_k(long n, double *__restrict__ x, double *__restrict__ r, 
         partials[blockIdx.x] = s;
 }
 
-__global__ void mg_init_scal_k(double *__restrict__ scal, double tol)
+__global__ void mg_init_scal_k(double *__restrict__ scal, double tol, double omega0)
 {
     if (threadIdx.x == 0) {
+        scal[D_OMEGA] = omega0;
         scal[D_DONE] = 0.0;
         scal[D_ITER] = 0.0;
         scal[D_RES] = 0.0;
@@ -190,21 +196,21 @@ static int mg_smooth(MgHandle *h, int l, const double *b, double **px, bool from
     int s0 = 0;
     if (from_zero && h->n_smooth >= 2 && lev_has_jac0(h, l)) {
         // x = omega D^-1 b followed by one sweep, in one pass
-        MG_TRY(lev_ku(h, l, MODE_JAC0, nullptr, x, b, L.dinv, h->omega, nullptr));
+        MG_TRY(lev_ku(h, l, MODE_JAC0, nullptr, x, b, L.dinv, L.om, nullptr));
         s0 = 2;
     }
     for (int s = s0; s < h->n_smooth; ++s) {
         if (s == 0 && from_zero) {
-            MG_TRY(launch_scale(L.n_dof, h->omega, L.dinv, b, x, h->stream));
+            MG_TRY(launch_scale(L.n_dof, L.om, L.dinv, b, x, h->stream, L.g.omega_scale));
             continue;
         }
         const bool last = final && s == h->n_smooth - 1;
         double *dst = last ? final : other;
         if (last && rz && l == 0 && h->kernel == FANTOM_KU_TILE && L.dim == 3) {
-            MG_TRY(lev_ku(h, l, MODE_JACDOT, x, dst, b, L.dinv, h->omega, h->partials));
+            MG_TRY(lev_ku(h, l, MODE_JACDOT, x, dst, b, L.dinv, L.om, h->partials));
             *rz = true;
         } else {
-            MG_TRY(lev_ku(h, l, MODE_JAC, x, dst, b, L.dinv, h->omega, nullptr));
+            MG_TRY(lev_ku(h, l, MODE_JAC, x, dst, b, L.dinv, L.om, nullptr));
         }
         other = x;
         x = dst;
@@ -271,7 +277,8 @@ static int mg_iteration(MgHandle *h)
     MG_TRY(launch_finalize(h->partials, npk, h->scal, FIN_ALPHA, 0, h->stream));
     hipLaunchKernelGGL(mg_xr_k, dim3(vec_grid(n)), dim3(256), 0, h->stream, n, h->xs, h->r, h->p, h->q, h->scal,
                        h->partials);
-    hipLaunchKernelGGL(mg_conv_k, dim3(1), dim3(1024), 0, h->stream, h->partials, (long)vec_grid(n), h->scal);
+    hipLaunchKernelGGL(mg_conv_k, dim3(1), dim3(1024), 0, h->stream, h->partials, (long)vec_grid(n), h->scal,
+                       h->sched);
     long nrz = 0;
     MG_TRY(mg_precond(h, h->r, h->z, &nrz));
     MG_TRY(launch_finalize(h->partials, nrz, h->scal, FIN_BETA, 0, h->stream));
@@ -406,6 +413,11 @@ int fantom_mg_create(int nelx, int nely, int nelz, int n_coarse, int n_smooth, d
     FANTOM_CHECK_ARG(h->nc <= 20000, "coarsest grid of %d DOF is too large for the dense solve", h->nc);
     h->n_smooth = n_smooth;
     h->omega = omega;
+    h->sched = OmegaSchedule{0, omega, 0.0};
+    for (auto &L : h->lev) {
+        L.om = omega;
+        L.g.omega_scale = nullptr;
+    }
     h->kernel = (h->dim == 3) ? kernel : FANTOM_KU_NODE;
     h->have_graph = false;
     h->last_iters = 0;
@@ -461,6 +473,33 @@ int fantom_mg_create(int nelx, int nely, int nelz, int n_coarse, int n_smooth, d
         return kStatusHipError;
     }
     *handle = h;
+    return kStatusOk;
+}
+
+int fantom_mg_set_omega_schedule(void *handle, int adaptive, double min_omega, double level_boost)
+{
+    MgHandle *h = reinterpret_cast<MgHandle *>(handle);
+    FANTOM_CHECK_ARG(h, "null handle");
+    FANTOM_CHECK_ARG(!adaptive || (min_omega > 0.0 && level_boost > 0.0),
+                     "adaptive schedule needs min_omega > 0 and level_boost > 0");
+    h->sched = adaptive ? OmegaSchedule{1, min_omega, h->omega - min_omega} : OmegaSchedule{0, h->omega, 0.0};
+    double boost = 1.0;
+    for (auto &L : h->lev) {
+        L.om = adaptive ? boost : h->omega;
+        L.g.omega_scale = adaptive ? h->scal + D_OMEGA : nullptr;
+        boost *= level_boost;
+    }
+    if (adaptive) {   // the weight of a V-cycle applied before any solve
+        const hipError_t e = hipMemcpy(h->scal + D_OMEGA, &min_omega, sizeof(double), hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            set_error("omega schedule: %s", hipGetErrorString(e));
+            return kStatusHipError;
+        }
+    }
+    if (h->have_graph) {   // the captured iteration holds the old weights
+        hipGraphExecDestroy(h->iter_exec);
+        h->have_graph = false;
+    }
     return kStatusOk;
 }
 
@@ -603,7 +642,7 @@ int fantom_mg_solve(void *handle, const double *f, double *x, double tol, int ma
         hipLaunchKernelGGL(mg_copy_k, dim3(G), dim3(256), 0, s, n, x, h->xs);
     else
         hipMemsetAsync(h->xs, 0, sizeof(double) * n, s);
-    hipLaunchKernelGGL(mg_init_scal_k, dim3(1), dim3(64), 0, s, h->scal, tol);
+    hipLaunchKernelGGL(mg_init_scal_k, dim3(1), dim3(64), 0, s, h->scal, tol, h->sched.min);
     MG_TRY(launch_dot_partials(n, h->f, h->f, h->partials, s));
     hipLaunchKernelGGL(mg_norm_k, dim3(1), dim3(1024), 0, s, h->partials, (long)G, h->scal);
     MG_TRY(lev_ku(h, 0, MODE_RES, h->xs, h->r, h->f, nullptr, 0.0, nullptr));

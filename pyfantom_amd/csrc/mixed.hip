@@ -29,7 +29,7 @@
 
 namespace fantom {
 
-enum { X_DONE = 8, X_ITER = 9, X_RES = 10, X_TOL = 11, X_NB = 12, X_K = 13 };
+enum { X_DONE = 8, X_ITER = 9, X_RES = 10, X_TOL = 11, X_NB = 12, X_K = 13, X_OMEGA = 14 };
 
 struct MxLevel {
     Grid g;
@@ -43,6 +43,7 @@ struct MxLevel {
     double *Ke;   // explicit Galerkin element matrices (levels >= 2), FP64
     float *S;     // assembled FP32 stencil of an element-matrix level that is smoothed
     int kind;     // 0 fine, 1 rediscretised, 2 Galerkin level 1, 3 element matrices
+    double om;    // weight argument of the smoothing kernels (see OmegaSchedule)
 };
 
 struct MxHandle {
@@ -51,6 +52,7 @@ struct MxHandle {
     double *G2tab;   // 64 x 576 level-2 tables
     float *Btab32;   // the level-1 table in the FP32 operator's precision
     double omega;
+    OmegaSchedule sched;   // adaptive weight of the FP64-PCG scheme
     std::vector<MxLevel> lev;
     // FP64 fine-level data
     Grid g;
@@ -99,9 +101,11 @@ __global__ void mx_up_k(long n, const float *__restrict__ d, double *__restrict_
         x[i] += s * (double)d[i];
 }
 
-__global__ void mx_scale_k(long n, float omega, const float *__restrict__ dinv, const float *__restrict__ b,
-                           float *__restrict__ x)
+__global__ void mx_scale_k(long n, float omega, const double *__restrict__ omega_scale, const float *__restrict__ dinv,
+                           const float *__restrict__ b, float *__restrict__ x)
 {
+    if (omega_scale)
+        omega = (float)((double)omega * omega_scale[0]);
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
         x[i] = omega * dinv[i] * b[i];
 }
@@ -146,7 +150,7 @@ __global__ void mx_p_k(long n, float *__restrict__ p, const float *__restrict__ 
         p[i] = first ? z[i] : (float)((double)z[i] + beta * (double)p[i]);
 }
 
-__global__ void mx_conv_k(const double *__restrict__ partials, long np, double *__restrict__ scal)
+__global__ void mx_conv_k(const double *__restrict__ partials, long np, double *__restrict__ scal, OmegaSchedule om)
 {
     double s = 0.0;
     for (long i = threadIdx.x; i < np; i += 1024)
@@ -161,13 +165,16 @@ __global__ void mx_conv_k(const double *__restrict__ partials, long np, double *
             scal[X_ITER] = scal[X_K];
             if (R < scal[X_TOL])
                 scal[X_DONE] = 1.0;
+            else if (om.on)
+                scal[X_OMEGA] = schedule_omega(om, R, scal[X_TOL]);
         }
     }
 }
 
-__global__ void mx_init_k(double *__restrict__ scal, double tol)
+__global__ void mx_init_k(double *__restrict__ scal, double tol, double omega0)
 {
     if (threadIdx.x == 0) {
+        scal[X_OMEGA] = omega0;
         scal[X_DONE] = 0.0;
         scal[X_ITER] = 0.0;
         scal[X_RES] = 0.0;
@@ -209,7 +216,7 @@ __global__ __launch_bounds__(256) void mxd_xr_k(long n, double *__restrict__ x, 
 }
 
 // convergence test on ||r|| / ||f|| (scal[X_NB] = ||f||)
-__global__ void mxd_conv_k(const double *__restrict__ partials, long np, double *__restrict__ scal)
+__global__ void mxd_conv_k(const double *__restrict__ partials, long np, double *__restrict__ scal, OmegaSchedule om)
 {
     double s = 0.0;
     for (long i = threadIdx.x; i < np; i += 1024)
@@ -223,6 +230,8 @@ __global__ void mxd_conv_k(const double *__restrict__ partials, long np, double 
             scal[X_ITER] = scal[X_K];
             if (R < scal[X_TOL])
                 scal[X_DONE] = 1.0;
+            else if (om.on)
+                scal[X_OMEGA] = schedule_omega(om, R, scal[X_TOL]);
         }
     }
 }
@@ -279,11 +288,11 @@ static int mx_ku(MxHandle *h, int l, int mode, const float *u, float *y, const f
 {
     MxLevel &L = h->lev[l];
     if (L.kind == 2)
-        return launch_gal1_f32(mode, L.g, h->Btab32, h->lev[0].E, L.mask, u, y, b, dinv, h->omega, partials, h->stream);
+        return launch_gal1_f32(mode, L.g, h->Btab32, h->lev[0].E, L.mask, u, y, b, dinv, L.om, partials, h->stream);
     if (L.kind == 3)
-        return L.S ? launch_stencil_ku_f32(mode, L.g, L.S, u, y, b, dinv, h->omega, partials, h->stream)
-                   : launch_elem_f32(mode, L.g, L.Ke, L.mask, u, y, b, dinv, h->omega, partials, h->stream);
-    return launch_ku_tile_f32(mode, L.g, L.Kb, L.E, L.mask, u, y, b, dinv, h->omega, partials, h->stream);
+        return L.S ? launch_stencil_ku_f32(mode, L.g, L.S, u, y, b, dinv, L.om, partials, h->stream)
+                   : launch_elem_f32(mode, L.g, L.Ke, L.mask, u, y, b, dinv, L.om, partials, h->stream);
+    return launch_ku_tile_f32(mode, L.g, L.Kb, L.E, L.mask, u, y, b, dinv, L.om, partials, h->stream);
 }
 
 // n_smooth sweeps on x (*px); final (optional) receives the last sweep
@@ -302,7 +311,7 @@ static int mx_smooth(MxHandle *h, int l, const float *b, float **px, bool from_z
     for (int s = s0; s < h->n_smooth; ++s) {
         if (s == 0 && from_zero) {
             hipLaunchKernelGGL(mx_scale_k, dim3(vec_grid(L.n_dof)), dim3(256), 0, h->stream, L.n_dof,
-                               (float)h->omega, L.dinv, b, x);
+                               (float)L.om, L.g.omega_scale, L.dinv, b, x);
             continue;
         }
         float *dst = (final && s == h->n_smooth - 1) ? final : other;
@@ -355,7 +364,7 @@ static int mx_iteration(MxHandle *h)
     MX_TRY(mx_ku(h, 0, MODE_KUDOT, h->p, h->q, nullptr, nullptr, h->partials));
     MX_TRY(launch_finalize(h->partials, ku_tile_partials_f32(L0.g), h->scal, FIN_ALPHA, 0, h->stream));
     hipLaunchKernelGGL(mx_xr_k, dim3(G), dim3(256), 0, h->stream, n, h->d, h->r, h->p, h->q, h->scal, h->partials);
-    hipLaunchKernelGGL(mx_conv_k, dim3(1), dim3(1024), 0, h->stream, h->partials, (long)G, h->scal);
+    hipLaunchKernelGGL(mx_conv_k, dim3(1), dim3(1024), 0, h->stream, h->partials, (long)G, h->scal, h->sched);
     MX_TRY(mx_precond(h, h->r, h->z));
     hipLaunchKernelGGL(mx_dot_k, dim3(G), dim3(256), 0, h->stream, n, h->r, h->z, h->partials);
     MX_TRY(launch_finalize(h->partials, (long)G, h->scal, FIN_BETA, 0, h->stream));
@@ -374,7 +383,7 @@ static int mxd_iteration(MxHandle *h)
     MX_TRY(launch_finalize(h->partials, ku_tile_partials(h->g), h->scal, FIN_ALPHA, 0, s));
     hipLaunchKernelGGL(mxd_xr_k, dim3(G), dim3(256), 0, s, n, h->x64, h->r64, h->r, h->p64, h->q64, h->scal,
                        h->partials);
-    hipLaunchKernelGGL(mxd_conv_k, dim3(1), dim3(1024), 0, s, h->partials, (long)G, h->scal);
+    hipLaunchKernelGGL(mxd_conv_k, dim3(1), dim3(1024), 0, s, h->partials, (long)G, h->scal, h->sched);
     MX_TRY(mx_precond(h, h->r, h->z));
     hipLaunchKernelGGL(mxd_dot_k, dim3(G), dim3(256), 0, s, n, h->r64, h->z, h->partials);
     MX_TRY(launch_finalize(h->partials, (long)G, h->scal, FIN_BETA, 0, s));
@@ -521,6 +530,11 @@ int fantom_mx_create(int nelx, int nely, int nelz, int n_coarse, int n_smooth, d
     }
     h->n_smooth = n_smooth;
     h->omega = omega;
+    h->sched = OmegaSchedule{0, omega, 0.0};
+    for (auto &L : h->lev) {
+        L.om = omega;
+        L.g.omega_scale = nullptr;
+    }
     h->have_graph = false;
     h->have_pc_graph = false;
     h->last_inner = 0;
@@ -571,6 +585,37 @@ int fantom_mx_create(int nelx, int nely, int nelz, int n_coarse, int n_smooth, d
         return kStatusHipError;
     }
     *handle = h;
+    return kStatusOk;
+}
+
+int fantom_mx_set_omega_schedule(void *handle, int adaptive, double min_omega, double level_boost)
+{
+    MxHandle *h = reinterpret_cast<MxHandle *>(handle);
+    FANTOM_CHECK_ARG(h, "null handle");
+    FANTOM_CHECK_ARG(!adaptive || (min_omega > 0.0 && level_boost > 0.0),
+                     "adaptive schedule needs min_omega > 0 and level_boost > 0");
+    h->sched = adaptive ? OmegaSchedule{1, min_omega, h->omega - min_omega} : OmegaSchedule{0, h->omega, 0.0};
+    double boost = 1.0;
+    for (auto &L : h->lev) {
+        L.om = adaptive ? boost : h->omega;
+        L.g.omega_scale = adaptive ? h->scal + X_OMEGA : nullptr;
+        boost *= level_boost;
+    }
+    if (adaptive) {   // the weight of a V-cycle applied before any solve
+        const hipError_t e = hipMemcpy(h->scal + X_OMEGA, &min_omega, sizeof(double), hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            set_error("omega schedule: %s", hipGetErrorString(e));
+            return kStatusHipError;
+        }
+    }
+    if (h->have_pc_graph) {   // the captured iterations hold the old weights
+        hipGraphExecDestroy(h->pc_exec);
+        h->have_pc_graph = false;
+    }
+    if (h->have_graph) {
+        hipGraphExecDestroy(h->iter_exec);
+        h->have_graph = false;
+    }
     return kStatusOk;
 }
 
@@ -685,7 +730,7 @@ static int mx_inner(MxHandle *h, double inner_tol, int maxiter, int use_graph, i
     const unsigned G = vec_grid(n);
     hipStream_t s = h->stream;
     hipLaunchKernelGGL(mx_zero_f32, dim3(G), dim3(256), 0, s, n, h->d);
-    hipLaunchKernelGGL(mx_init_k, dim3(1), dim3(64), 0, s, h->scal, inner_tol);
+    hipLaunchKernelGGL(mx_init_k, dim3(1), dim3(64), 0, s, h->scal, inner_tol, h->sched.min);
     MX_TRY(mx_precond(h, h->r, h->z));
     hipLaunchKernelGGL(mx_dot_k, dim3(G), dim3(256), 0, s, n, h->r, h->z, h->partials);
     MX_TRY(launch_finalize(h->partials, (long)G, h->scal, FIN_RZ0, 0, s));
@@ -793,7 +838,7 @@ int fantom_mx_solve_pcg(void *handle, const double *f, double *x, double tol, in
         hipMemsetAsync(h->x64, 0, sizeof(double) * n, s);
     MX_TRY(launch_dot_partials(n, h->f64, h->f64, h->partials, s));
     hipLaunchKernelGGL(mxd_norm_k, dim3(1), dim3(1024), 0, s, h->partials, (long)G, h->scal);
-    hipLaunchKernelGGL(mx_init_k, dim3(1), dim3(64), 0, s, h->scal, tol);
+    hipLaunchKernelGGL(mx_init_k, dim3(1), dim3(64), 0, s, h->scal, tol, h->sched.min);
     MX_TRY(launch_ku_tile(MODE_RES, h->g, h->Kb64, h->E64, h->mask, h->x64, h->r64, h->f64, nullptr, 0.0, nullptr,
                           s));
     MX_TRY(launch_dot_partials(n, h->r64, h->r64, h->partials, s));
@@ -812,6 +857,11 @@ int fantom_mx_solve_pcg(void *handle, const double *f, double *x, double tol, in
             MX_TRY(mx_capture(h, true));
         int batch = h->last_pc > 3 ? h->last_pc - 2 : 1;
         bool conv = false;
+        // stagnation: the recursive residual has not dropped by 10% over
+        // kStallWindow iterations -> stop (the caller sees rel_res >= tol)
+        constexpr int kStallWindow = 20;
+        int w0 = 0;
+        double r0 = INFINITY;
         while (done < maxiter && !conv) {
             batch = std::min(batch, maxiter - done);
             for (int b = 0; b < batch; ++b) {
@@ -830,6 +880,14 @@ int fantom_mx_solve_pcg(void *handle, const double *f, double *x, double tol, in
             if (hs[X_DONE] != 0.0) {
                 done = (int)hs[X_ITER];
                 conv = true;
+            } else if (done - w0 >= kStallWindow) {
+                if (!(hs[X_RES] < 0.9 * r0) && r0 != INFINITY)
+                    break;
+                w0 = done;
+                r0 = hs[X_RES];
+            } else if (r0 == INFINITY) {
+                w0 = done;
+                r0 = hs[X_RES];
             }
             batch = 1;
         }

@@ -82,6 +82,8 @@ __global__ __launch_bounds__(kStBlock) void ku_stencil_k(Grid g, const T *__rest
                                                        const T *__restrict__ dinv, T omega,
                                                        double *__restrict__ partials)
 {
+    if (MODE == MODE_JAC || MODE == MODE_JAC0)
+        omega = jacobi_weight(g, omega);
     __shared__ T psum[3][3][kStNodes];
     const int ln = threadIdx.x % kStNodes, ty = threadIdx.x / kStNodes;
     const long n = (long)blockIdx.x * kStNodes + ln;

@@ -83,15 +83,23 @@ __device__ __forceinline__ double dpp_row(double v)
 //   struct Data;                                         per-element inputs
 //   void load(Data &, const Grid &, bool active, int ex, int ey, int ez) const;
 //   void apply(const Data &, const T (&w)[3][8], T (&z)[3][8]) const;
+//   static constexpr bool kScaled;  T scale(const Data &) const;
+//   static constexpr int kWavesPerEU;
 // where load fills Data for element (ex, ey, ez) (zero modulus when !active)
 // and apply maps WHT coefficients to output coefficients (the 1/8 of the
-// inverse transform is folded into the operator's tables).
+// inverse transform is folded into the operator's tables).  With kScaled the
+// outputs of apply still lack one per-element factor, scale(d), which the
+// inverse y-butterfly folds in (one FMA and one product per coefficient pair
+// instead of a product per coefficient and an add).  kWavesPerEU is the
+// occupancy the register budget is held to.
 template <typename T, int MODE, class Op>
-__global__ __launch_bounds__(kTThreads) void ku_tile_kernel(Grid g, Op op, const uint8_t *__restrict__ mask,
+__global__ __launch_bounds__(kTThreads) __attribute__((amdgpu_waves_per_eu(Op::kWavesPerEU, 8))) void ku_tile_kernel(Grid g, Op op, const uint8_t *__restrict__ mask,
                                                           const T *__restrict__ u, T *__restrict__ y,
                                                           const T *__restrict__ bvec, const T *__restrict__ dinv,
                                                           T omega, double *__restrict__ partials, TileShape ts)
 {
+    if (MODE == MODE_JAC || MODE == MODE_JAC0 || MODE == MODE_JACDOT)
+        omega = jacobi_weight(g, omega);
     __shared__ T xf[6][kTThreads];   // z-butterflies, exchanged with column lx - 1
     __shared__ T xr[3][kTThreads];   // upper-x node sums, exchanged with column lx + 1
 
@@ -224,8 +232,14 @@ __global__ __launch_bounds__(kTThreads) void ku_tile_kernel(Grid g, Op op, const
                         const int q = sx + 2 * sz;
                         const T lo = z[c][sx + 4 * sz] + z[c][sx + 2 + 4 * sz];
                         const T hi = z[c][sx + 4 * sz] - z[c][sx + 2 + 4 * sz];
-                        A[q] = lo + Zu[c][q];
-                        Zu[c][q] = hi;
+                        if constexpr (Op::kScaled) {
+                            const T e = op.scale(dcur);
+                            A[q] = __builtin_fma(e, lo, Zu[c][q]);
+                            Zu[c][q] = e * hi;
+                        } else {
+                            A[q] = lo + Zu[c][q];
+                            Zu[c][q] = hi;
+                        }
                     }
                 // inverse x butterfly: P[bx][sz]; inverse z butterfly: corner bz
                 const T p00 = A[0] + A[1], p10 = A[0] - A[1];   // sz = 0

@@ -85,9 +85,12 @@ __global__ __launch_bounds__(kVecBlock) void pcg_p_f64(long n, double *__restric
 }
 
 // x = omega * dinv * b  (first Jacobi sweep from a zero guess)
-__global__ __launch_bounds__(kVecBlock) void scale_f64(long n, double omega, const double *__restrict__ dinv,
-                                                     const double *__restrict__ b, double *__restrict__ x)
+__global__ __launch_bounds__(kVecBlock) void scale_f64(long n, double omega, const double *__restrict__ omega_scale,
+                                                     const double *__restrict__ dinv, const double *__restrict__ b,
+                                                     double *__restrict__ x)
 {
+    if (omega_scale)
+        omega *= omega_scale[0];
     for (long i = (long)blockIdx.x * kVecBlock + threadIdx.x; i < n; i += (long)gridDim.x * kVecBlock)
         x[i] = omega * dinv[i] * b[i];
 }
@@ -139,9 +142,10 @@ int launch_pcg_p(long n, double *p, const double *z, const double *scal, int fir
     return kStatusOk;
 }
 
-int launch_scale(long n, double omega, const double *dinv, const double *b, double *x, hipStream_t s)
+int launch_scale(long n, double omega, const double *dinv, const double *b, double *x, hipStream_t s,
+                 const double *omega_scale)
 {
-    hipLaunchKernelGGL(scale_f64, dim3(vec_grid(n)), dim3(kVecBlock), 0, s, n, omega, dinv, b, x);
+    hipLaunchKernelGGL(scale_f64, dim3(vec_grid(n)), dim3(kVecBlock), 0, s, n, omega, omega_scale, dinv, b, x);
     FANTOM_LAUNCH_CHECK("scale");
     return kStatusOk;
 }

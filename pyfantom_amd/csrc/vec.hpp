@@ -16,7 +16,9 @@ int launch_finalize(const double *partials, long np, double *scal, int op, int s
 int launch_pcg_xr(long n, double *x, double *r, const double *p, const double *q, const double *scal,
                   double *partials, hipStream_t s);
 int launch_pcg_p(long n, double *p, const double *z, const double *scal, int first, hipStream_t s);
-int launch_scale(long n, double omega, const double *dinv, const double *b, double *x, hipStream_t s);
+// x = omega * dinv * b; omega_scale (optional, device): factor on omega
+int launch_scale(long n, double omega, const double *dinv, const double *b, double *x, hipStream_t s,
+                 const double *omega_scale = nullptr);
 int launch_axpby(long n, double a, const double *x, double b, double *y, hipStream_t s);
 
 // filter.hip

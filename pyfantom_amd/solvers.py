@@ -15,8 +15,9 @@ Deliberate differences from the reference (each keeps the solution K^-1 f):
     TW normalisation cancels in the V-cycle);
   * the coarsest level is solved exactly on the device (dense SPD inverse)
     instead of a host CHOLMOD factor of K + 1e-6 I;
-  * omega is fixed (the reference's adaptive schedule is available with
-    ``omega_schedule='reference'``);
+  * omega follows the reference's adaptive schedule by default (min_omega
+    first, then min_omega + (omega - min_omega)/2 exp(500 (tol - clip(R))),
+    times 1.06^level); ``omega_schedule='fixed'`` keeps omega on every level;
   * cycle='W' runs a V-cycle and cycle='w' a W-cycle at ``w_level`` -- this
     is the reference's observed behaviour (it compares with 'w');
   * the returned residual is the true ||f - K u|| / ||f||.
@@ -127,9 +128,9 @@ class _DeviceKrylov(Solver):
 class MultiGrid(_DeviceKrylov):
     def __init__(self, mesh, kernel, maxiter=1000, tol=1e-5, n_smooth=3, omega=0.5, n_level=3, cycle="W",
                  w_level=1, coarse_solver="cholmod", matrix_free=False, low_level_tol=1e-8,
-                 low_level_maxiter=5000, min_omega=0.4, omega_boost=1.06, omega_schedule="fixed",
+                 low_level_maxiter=5000, min_omega=0.4, omega_boost=1.06, omega_schedule="reference",
                  check_every=1, native=True, use_graph=True, precision="fp64", inner_tol=1e-4,
-                 max_outer=20, coarse=None, mixed_scheme="pcg"):
+                 max_outer=20, coarse=None, mixed_scheme="pcg", fallback=True, fallback_maxiter=1000):
         if coarse_solver not in ["cg", "gmres", "splu", "spsolve", "cholmod"]:
             raise ValueError("Coarse solver not recognized.")
         if coarse_solver in ["splu", "spsolve"] and matrix_free:
@@ -160,14 +161,14 @@ class MultiGrid(_DeviceKrylov):
         self._built_version = None
         self.last_iterations = 0
         self.last_recursive_residual = None
-        # the C++ driver runs fixed-omega V/W... cycles; the Python driver
-        # below covers the reference omega schedule and the W-cycle
-        self.native = bool(native) and omega_schedule == "fixed" and cycle != "w"
+        # the C++ driver runs V cycles with either weight schedule; the
+        # Python driver below also covers the reference's live 'w' cycle
+        self.native = bool(native) and cycle != "w"
         self.use_graph = bool(use_graph)
         if precision not in ("fp64", "mixed"):
             raise ValueError("precision must be 'fp64' or 'mixed'")
         if precision == "mixed" and not self.native:
-            raise ValueError("the mixed-precision solve runs in the native driver (fixed omega, V-cycle)")
+            raise ValueError("the mixed-precision solve runs in the native driver (V-cycle)")
         self.precision = precision
         # mixed precision: "pcg" = FP64 PCG preconditioned by an FP32 V-cycle;
         # "ir" = FP64 iterative refinement around an FP32 GMG-PCG
@@ -183,6 +184,13 @@ class MultiGrid(_DeviceKrylov):
         self.coarse = coarse
         self.inner_tol = float(inner_tol)
         self.max_outer = int(max_outer)
+        # mixed precision: a solve that stagnates or ends above tol is
+        # finished by an FP64 GMG-PCG of the same hierarchy from its iterate
+        self.fallback = bool(fallback)
+        self.fallback_maxiter = int(fallback_maxiter)
+        self._fb = None
+        self.last_fallback = False
+        self.last_fallback_iterations = 0
         self.last_outer_iterations = 0
         self._nh = None
         self._nh_key = None
@@ -195,6 +203,8 @@ class MultiGrid(_DeviceKrylov):
         self._built_version = None
         self.last_x0 = None
         self._free_native()
+        if getattr(self, "_fb", None) is not None:
+            self._fb.reset()
 
     def __del__(self):
         if sys.is_finalizing():
@@ -235,12 +245,18 @@ class MultiGrid(_DeviceKrylov):
         _lib.call("fantom_mg_precondition", self._nh, _lib.ptr(r), _lib.ptr(z))
         return z
 
+    def _native_schedule(self, L, pre, h):
+        adaptive = self.omega_schedule == "reference"
+        _lib.check(getattr(L, pre + "set_omega_schedule")(h, int(adaptive), self.min_omega if adaptive else 0.0,
+                                                              self.omega_boost if adaptive else 1.0))
+
     def _native_prepare(self):
         import ctypes
         k = self.kernel
         L = _lib.load()
         gal = int(self.coarse == "galerkin")
-        key = (k.grid, self.n_level, self.n_smooth, self.omega, k.kernel_kind, gal, self.precision)
+        key = (k.grid, self.n_level, self.n_smooth, self.omega, k.kernel_kind, gal, self.omega_schedule,
+               self.min_omega, self.omega_boost, self.precision)
         if self._nh is None or self._nh_key != key:
             self._free_native()
             n_coarse = self._n_coarse()
@@ -253,6 +269,7 @@ class MultiGrid(_DeviceKrylov):
                                           self._ws.data_ptr(), nbytes, ctypes.byref(h)))
             self._nh, self._nh_key, self._nh_state = h, key, None
             _LIVE.add(self)
+            self._native_schedule(L, "fantom_mg_", h)
         state = (k.rho_version, k.con_version, k.rho.data_ptr())
         if self._nh_state != state:
             _lib.call("fantom_mg_setup", self._nh, _lib.ptr(k.rho), _lib.ptr(k.mask) if k.has_con else None)
@@ -266,7 +283,8 @@ class MultiGrid(_DeviceKrylov):
         if mixed and k.dim != 3:
             raise NotImplementedError("the mixed-precision solve is implemented for hex8 grids")
         gal = int(self.coarse == "galerkin")
-        key = (k.grid, self.n_level, self.n_smooth, self.omega, k.kernel_kind, gal, self.precision)
+        key = (k.grid, self.n_level, self.n_smooth, self.omega, k.kernel_kind, gal, self.omega_schedule,
+               self.min_omega, self.omega_boost, self.precision)
         pre = "fantom_mx_" if mixed else "fantom_mg_"
         if self._nh is None or self._nh_key != key:
             self._free_native()
@@ -289,6 +307,7 @@ class MultiGrid(_DeviceKrylov):
             self._nh_state = None
             self._n_levels = n_coarse + 1
             _LIVE.add(self)
+            self._native_schedule(L, pre, h)
         state = (k.rho_version, k.con_version, k.rho.data_ptr())
         if self._nh_state != state:
             _lib.call(pre + "setup", self._nh, _lib.ptr(k.rho), _lib.ptr(k.mask) if k.has_con else None)
@@ -313,6 +332,20 @@ class MultiGrid(_DeviceKrylov):
                       int(self.use_graph), ctypes.byref(it), ctypes.byref(res))
         self.last_iterations = it.value
         self.last_x0 = x
+        self.last_fallback = False
+        self.last_fallback_iterations = 0
+        if mixed and self.fallback and not res.value < self.tol:
+            if self._fb is None:
+                self._fb = MultiGrid(self.mesh, k, maxiter=self.fallback_maxiter, tol=self.tol, n_smooth=self.n_smooth,
+                                     omega=self.omega, n_level=self.n_level, cycle=self.cycle,
+                                     use_graph=self.use_graph, coarse=self.coarse)
+            self._fb.tol = self.tol
+            self._fb.last_x0 = x
+            x, r = self._fb.solve(rhs, use_last=True)
+            self.last_fallback = True
+            self.last_fallback_iterations = self._fb.last_iterations
+            self.last_x0 = x
+            return x, r
         return x, res.value
 
     # ---- hierarchy ------------------------------------------------------------------

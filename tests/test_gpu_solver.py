@@ -192,17 +192,22 @@ def test_mbb_pipeline_vs_oracle(oracle):
     print(f"compliance after 5 iterations {prob.f()!r}", flush=True)
 
 
-def test_native_graph_matches_eager_and_python(oracle):
+@pytest.mark.parametrize("schedule", ["reference", "fixed"])
+def test_native_graph_matches_eager_and_python(oracle, schedule):
     """The C++ driver replaying one HIP graph per PCG iteration gives the same
     iterates as launching the same kernels eagerly (bitwise), and the same
     solution as the Python driver; checked on every one of several solves
-    with unrelated eager device work between them."""
+    with unrelated eager device work between them.  With the reference's
+    adaptive Jacobi weight (a device scalar updated after each convergence
+    test, times 1.06^level) the native driver takes the Python driver's
+    iteration count, i.e. the same weight sequence."""
     import pyfantom_amd as pf
     torch = _torch()
     nel = (32, 16, 16)
-    print("native graph vs eager", flush=True)
+    print(f"native graph vs eager, omega schedule {schedule}", flush=True)
     mesh, K, _, FE = cantilever(nel, tol=1e-10, n_level=3)
-    mk = lambda **kw: pf.MultiGrid(mesh, K, maxiter=500, tol=1e-10, n_smooth=2, omega=0.6, n_level=3, cycle="V", **kw)
+    mk = lambda **kw: pf.MultiGrid(mesh, K, maxiter=500, tol=1e-10, n_smooth=2, omega=0.6, n_level=3, cycle="V",
+                                   omega_schedule=schedule, **kw)
     s_graph, s_eager = mk(use_graph=True), mk(use_graph=False)
     s_py = mk(native=False)
     s_nat_redisc = mk(coarse="rediscretized")
@@ -261,5 +266,35 @@ def test_mixed_precision_refinement(oracle, field, scheme):
     du = float(torch.linalg.norm(um - u64) / torch.linalg.norm(u64))
     print(f"fp64 iters {ref.last_iterations} mixed outer {mx.last_outer_iterations} inner {mx.last_iterations} "
           f"res {rm:.2e} |dc|/c {abs(cm - c64) / abs(c64):.2e} |du|/|u| {du:.2e}", flush=True)
+    assert not mx.last_fallback          # the mixed path converged by itself
     assert rm < 1e-8
     assert abs(cm - c64) <= 1e-6 * abs(c64)
+
+
+def test_mixed_precision_fallback(oracle):
+    """A mixed-precision solve that ends above tol (here: cut off after 3
+    iterations) is finished by the FP64 GMG-PCG from its iterate; the result
+    meets tol and matches the FP64 solve."""
+    import pyfantom_amd as pf
+    torch = _torch()
+    nel = (32, 16, 16)
+    print("mixed precision fallback", flush=True)
+    mesh, K, _, FE = cantilever(nel, tol=1e-10, n_level=3)
+    E = torch.from_numpy(filtered_field(oracle, nel)).cuda()
+    K.set_rho(E)
+    ref = pf.MultiGrid(mesh, K, maxiter=500, tol=1e-10, n_smooth=2, omega=0.6, n_level=3, cycle="V")
+    u64, _ = ref.solve(FE.rhs, use_last=False)
+    u64 = u64.clone()
+    mx = pf.MultiGrid(mesh, K, maxiter=3, tol=1e-10, n_smooth=2, omega=0.6, n_level=3, cycle="V",
+                      precision="mixed")
+    um, rm = mx.solve(FE.rhs, use_last=False)
+    print(f"mixed its {mx.last_iterations} fallback {mx.last_fallback} +{mx.last_fallback_iterations} "
+          f"res {rm:.2e}", flush=True)
+    assert mx.last_fallback and mx.last_fallback_iterations > 0
+    assert rm < 1e-10
+    assert float(torch.linalg.norm(um - u64) / torch.linalg.norm(u64)) < 1e-8
+    # the no-fallback switch returns the cut-off iterate as it is
+    mx2 = pf.MultiGrid(mesh, K, maxiter=3, tol=1e-10, n_smooth=2, omega=0.6, n_level=3, cycle="V",
+                       precision="mixed", fallback=False)
+    _, rm2 = mx2.solve(FE.rhs, use_last=False)
+    assert not mx2.last_fallback and rm2 > 1e-10
