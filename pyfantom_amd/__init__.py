@@ -16,6 +16,7 @@ from .solvers import CG, MultiGrid  # noqa: F401
 from .fe import FiniteElement  # noqa: F401
 from .problem import MinimumCompliance  # noqa: F401
 from .optimizers import MMA, OC  # noqa: F401
+from . import checkpoint  # noqa: F401
 
 __all__ = ["StructuredMesh2D", "StructuredMesh3D", "StructuredFilter2D", "StructuredFilter3D", "LinearElasticity",
            "StructuredStiffnessKernel", "MultiGrid", "CG", "FiniteElement", "MinimumCompliance", "OC", "MMA"]

@@ -109,68 +109,106 @@ __global__ __launch_bounds__(kBlock) void restrict2d(Grid gf, int dof, int norma
     }
 }
 
-// One thread per fine node: average of parent coarse values (each divided by
-// its TW when normalize).  accumulate: xf += P ec, else xf = P ec.
-// mask_f (optional): constrained fine DOFs receive 0.
-// blockIdx.y is the fine node plane (y): the in-plane index arithmetic
-// stays 32-bit (a 64-bit division per thread cost more than the memory
-// traffic of the whole pass).
+// Fine value = average of its parent coarse values (each divided by its TW
+// when normalize); accumulate: xf += P ec, else xf = P ec; mask_f (optional):
+// constrained fine DOFs receive 0.  One thread per coarse node c (blockIdx.y
+// is the coarse plane): it loads the 2x2x2 coarse nodes c + {0,1}^3 once and
+// writes the up to 8 fine nodes 2c + o, o in {0,1}^3, whose parents are
+// c + o' with o' <= o -- 3 coarse loads per fine node instead of 24, and
+// each wave writes contiguous runs of z-node pairs.  Every fine value sums
+// its parents in the order x outer, y, z inner, as one thread per fine node
+// would, so results are unchanged bit for bit.
 template <typename T, int DOF>
 __global__ __launch_bounds__(kBlock) void prolong3d(Grid gf, int normalize, const uint8_t *__restrict__ mask_f,
                                                   const T *__restrict__ ec, T *__restrict__ xf, int accumulate)
 {
     const int nxc = gf.nelx / 2 + 1, nyc = gf.nely / 2 + 1, nzc = gf.nelz / 2 + 1;
-    const int p = blockIdx.x * kBlock + threadIdx.x;
-    if (p >= gf.nx * gf.nz)
+    const int pc = blockIdx.x * kBlock + threadIdx.x;
+    if (pc >= nxc * nzc)
         return;
-    const int j = blockIdx.y;
-    const int i = p / gf.nz;
-    const int k = p - i * gf.nz;
-    const long f = (long)j * gf.nx * gf.nz + p;
-    double acc[DOF];
+    const int jc = blockIdx.y;
+    const int ic = pc / nzc;
+    const int kc = pc - ic * nzc;
+    // parent values, clamped to the grid (a clamped copy is never used)
+    double ev[2][2][2][DOF];
 #pragma unroll
-    for (int d = 0; d < DOF; ++d)
-        acc[d] = 0.0;
-    int count = 0;
-    // parents (i/2 + oi, j/2 + oj, k/2 + ok) with oi <= (i & 1) etc., visited in
-    // a fixed order with the non-parents weighted 0
-    const long c0 = (k / 2) + (long)nzc * (i / 2) + (long)nzc * nxc * (j / 2);
+    for (int a = 0; a < 2; ++a)
 #pragma unroll
-    for (int oi = 0; oi < 2; ++oi) {
-        const int ci = i / 2 + oi;
-        const bool vi = oi <= (i & 1) && ci < nxc;
+        for (int b = 0; b < 2; ++b)
 #pragma unroll
-        for (int oj = 0; oj < 2; ++oj) {
-            const int cj = j / 2 + oj;
-            const bool vj = oj <= (j & 1) && cj < nyc;
-#pragma unroll
-            for (int ok = 0; ok < 2; ++ok) {
-                const int ck = k / 2 + ok;
-                const bool v = vi && vj && ok <= (k & 1) && ck < nzc;
+            for (int d = 0; d < 2; ++d) {
+                const int ci = min(ic + a, nxc - 1), cj = min(jc + b, nyc - 1), ck = min(kc + d, nzc - 1);
+                const long c = ck + (long)nzc * ci + (long)nzc * nxc * cj;
                 const int bnd = (ci == 0 || ci == nxc - 1) + (cj == 0 || cj == nyc - 1) + (ck == 0 || ck == nzc - 1);
-                const long c = v ? ck + (long)nzc * ci + (long)nzc * nxc * cj : c0;
                 const double den = normalize ? tw3(bnd) : 1.0;
 #pragma unroll
-                for (int d = 0; d < DOF; ++d) {
-                    const double e = (double)ec[c * DOF + d];
-                    acc[d] += v ? (normalize ? e / den : e) : 0.0;
+                for (int q = 0; q < DOF; ++q) {
+                    const double e = (double)ec[c * DOF + q];
+                    ev[a][b][d][q] = normalize ? e / den : e;
                 }
-                count += v;
             }
-        }
-    }
-    const int mf = mask_f ? mask_f[f] : 0;
-    // count is 1, 2, 4 or 8: its reciprocal is exact, so the product equals
-    // the reference's division
-    const double inv_count = count == 8 ? 0.125 : count == 4 ? 0.25 : count == 2 ? 0.5 : count == 1 ? 1.0 : 0.0;
+    // the fine values being accumulated into and the masks, all loaded
+    // before the first store (stores to xf would otherwise order every later
+    // load of xf behind them)
+    const long plane = (long)gf.nx * gf.nz;
+    T xo[2][2][2][DOF];
+    int mo[2][2][2];
 #pragma unroll
-    for (int d = 0; d < DOF; ++d) {
-        double v = acc[d] * inv_count;
-        if ((mf >> d) & 1)
-            v = 0.0;
-        const long r = f * DOF + d;
-        xf[r] = accumulate ? (T)((double)xf[r] + v) : (T)v;
-    }
+    for (int ox = 0; ox < 2; ++ox)
+#pragma unroll
+        for (int oy = 0; oy < 2; ++oy)
+#pragma unroll
+            for (int oz = 0; oz < 2; ++oz) {
+                const int i = min(2 * ic + ox, gf.nx - 1), j = min(2 * jc + oy, gf.ny - 1),
+                          k = min(2 * kc + oz, gf.nz - 1);
+                const long f = k + (long)gf.nz * i + plane * j;
+#pragma unroll
+                for (int q = 0; q < DOF; ++q)
+                    xo[ox][oy][oz][q] = accumulate ? xf[f * DOF + q] : T(0);
+                mo[ox][oy][oz] = mask_f ? mask_f[f] : 0;
+            }
+#pragma unroll
+    for (int ox = 0; ox < 2; ++ox)
+#pragma unroll
+        for (int oy = 0; oy < 2; ++oy)
+#pragma unroll
+            for (int oz = 0; oz < 2; ++oz) {
+                const int i = 2 * ic + ox, j = 2 * jc + oy, k = 2 * kc + oz;
+                if (i >= gf.nx || j >= gf.ny || k >= gf.nz)
+                    continue;
+                const long f = k + (long)gf.nz * i + plane * j;
+                double acc[DOF];
+#pragma unroll
+                for (int q = 0; q < DOF; ++q)
+                    acc[q] = 0.0;
+                int count = 0;
+#pragma unroll
+                for (int a = 0; a <= ox; ++a)
+#pragma unroll
+                    for (int b = 0; b <= oy; ++b)
+#pragma unroll
+                        for (int d = 0; d <= oz; ++d) {
+                            // a parent past the last coarse node exists only
+                            // on grids with an odd element count
+                            const bool v = ic + a < nxc && jc + b < nyc && kc + d < nzc;
+#pragma unroll
+                            for (int q = 0; q < DOF; ++q)
+                                acc[q] += v ? ev[a][b][d][q] : 0.0;
+                            count += v;
+                        }
+                // count is 1, 2, 4 or 8: its reciprocal is exact, so the
+                // product equals the reference's division
+                const double inv_count =
+                    count == 8 ? 0.125 : count == 4 ? 0.25 : count == 2 ? 0.5 : count == 1 ? 1.0 : 0.0;
+                const int mf = mo[ox][oy][oz];
+#pragma unroll
+                for (int q = 0; q < DOF; ++q) {
+                    double v = acc[q] * inv_count;
+                    if ((mf >> q) & 1)
+                        v = 0.0;
+                    xf[f * DOF + q] = accumulate ? (T)((double)xo[ox][oy][oz][q] + v) : (T)v;
+                }
+            }
 }
 
 template <typename T>
@@ -498,7 +536,7 @@ static int prolong_t(const Grid &gf, int dim, int dof, int normalize, const uint
     if (dim == 3) {
         if (int st = check_planes(gf))
             return st;
-        const dim3 grid(nblk((long)gf.nx * gf.nz), gf.ny);
+        const dim3 grid(nblk((long)(gf.nelx / 2 + 1) * (gf.nelz / 2 + 1)), gf.nely / 2 + 1);
         if (dof == 3)
             hipLaunchKernelGGL((prolong3d<T, 3>), grid, dim3(kBlock), 0, s, gf, normalize, mask_f, ec, xf, accumulate);
         else if (dof == 1)

@@ -167,4 +167,12 @@ class MinimumCompliance:
         return True
 
     def logs(self):
-        return {"iteration": int(self.iteration), "residual": float(self._residual)}
+        """The reference's keys (Problem/CPU/MinimumCompliance.py:639-657) plus
+        the solve's PCG iteration count, precision and FP64 fallback."""
+        out = {"iteration": int(self.iteration), "residual": float(self._residual)}
+        solver = self.FE.solver
+        out["cg_iterations"] = int(getattr(solver, "last_iterations", 0))
+        out["precision"] = getattr(solver, "precision", "fp64")
+        if getattr(solver, "last_fallback", False):
+            out["fp64_fallback_iterations"] = int(solver.last_fallback_iterations)
+        return out

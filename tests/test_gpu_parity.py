@@ -171,7 +171,7 @@ def test_diag_dc(oracle, nel):
     assert _rel(_np(dc), oracle.process_dk(K0, el, u)) <= 1e-14
 
 
-@pytest.mark.parametrize("nel", [(2, 2, 2), (8, 4, 6), (32, 16, 16), (6, 4), (120, 40)])
+@pytest.mark.parametrize("nel", [(2, 2, 2), (8, 4, 6), (32, 16, 16), (5, 3, 7), (9, 6, 5), (6, 4), (120, 40)])
 @pytest.mark.parametrize("normalize", [0, 1])
 def test_transfers(oracle, nel, normalize):
     from pyfantom_amd import _lib
