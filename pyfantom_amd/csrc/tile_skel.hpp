@@ -25,8 +25,11 @@
 //     lx - 1.  Every node sum is taken in a fixed order (deterministic, no
 //     atomics).
 // Thread columns lx, lz in [1, 14] own output nodes; columns 0 and 15 are halo
-// that neighbouring tiles also compute.  LDS traffic is 9 doubles written and
-// read per thread and layer; all other data stays in registers.  Inactive
+// that neighbouring tiles also compute.  Per thread and layer 9 doubles are
+// written to LDS and 15 read back; all other data stays in registers.  The
+// LDS buffers alternate between layers, so one barrier per layer orders both
+// exchanges: the z-butterflies of the plane two above are written at the end
+// of a layer, together with its x-sums.  Inactive
 // elements (outside the grid or in the halo) get a zero modulus so that their
 // contribution vanishes without divergent code around the cross-lane moves.
 #pragma once
@@ -100,8 +103,10 @@ __global__ __launch_bounds__(kTThreads) __attribute__((amdgpu_waves_per_eu(Op::k
 {
     if (MODE == MODE_JAC || MODE == MODE_JAC0 || MODE == MODE_JACDOT)
         omega = jacobi_weight(g, omega);
-    __shared__ T xf[6][kTThreads];   // z-butterflies, exchanged with column lx - 1
-    __shared__ T xr[3][kTThreads];   // upper-x node sums, exchanged with column lx + 1
+    // double-buffered by layer parity, so that one barrier per layer orders
+    // both exchanges
+    __shared__ T xf[2][6][kTThreads];   // z-butterflies, exchanged with column lx - 1
+    __shared__ T xr[2][3][kTThreads];   // upper-x node sums, exchanged with column lx + 1
 
     // The hardware deals workgroups to the 8 XCDs round-robin; give each XCD
     // a contiguous range of logical tiles so that tiles sharing halo planes
@@ -144,26 +149,27 @@ __global__ __launch_bounds__(kTThreads) __attribute__((amdgpu_waves_per_eu(Op::k
                     m = mask[n];
             }
         };
-        // face transform of a plane from the thread's (unmasked) node values
-        auto face = [&](const T (&v)[3], int m, T (&F)[3][4]) {
-            T zs[3][2];
+        // face transform of a plane in two halves: face_write takes the
+        // z-butterflies (DPP) of the thread's (unmasked) node values into LDS,
+        // face_read (after a barrier) adds the x-butterflies with column lx + 1
+        auto face_write = [&](const T (&v)[3], int m, int buf) {
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 const T a = ((m >> c) & 1) ? T(0) : v[c];
                 const T b = dpp_row<kDppRowShl1>(a);
-                zs[c][0] = a + b;
-                zs[c][1] = a - b;
-                xf[2 * c][tid] = zs[c][0];
-                xf[2 * c + 1][tid] = zs[c][1];
+                xf[buf][2 * c][tid] = a + b;
+                xf[buf][2 * c + 1][tid] = a - b;
             }
-            __syncthreads();
+        };
+        auto face_read = [&](int buf, T (&F)[3][4]) {
 #pragma unroll
             for (int c = 0; c < 3; ++c)
 #pragma unroll
                 for (int sz = 0; sz < 2; ++sz) {
-                    const T nb = xf[2 * c + sz][nb_up];
-                    F[c][2 * sz] = zs[c][sz] + nb;
-                    F[c][2 * sz + 1] = zs[c][sz] - nb;
+                    const T own = xf[buf][2 * c + sz][tid];
+                    const T nb = xf[buf][2 * c + sz][nb_up];
+                    F[c][2 * sz] = own + nb;
+                    F[c][2 * sz + 1] = own - nb;
                 }
         };
         auto load_elem = [&](typename Op::Data &d, int L) {
@@ -176,8 +182,12 @@ __global__ __launch_bounds__(kTThreads) __attribute__((amdgpu_waves_per_eu(Op::k
         load_node(jc0 - 1, ucur, mcur);
         load_node(jc0, unx, mnx);
         load_elem(dcur, jc0 - 1);
-        face(ucur, mcur, Fp);
-        __syncthreads();   // the first layer rewrites xf
+        face_write(ucur, mcur, 0);
+        __syncthreads();
+        face_read(0, Fp);
+        face_write(unx, mnx, 1);
+        __syncthreads();
+        int pb = 1, rb = 0;   // LDS buffers of plane L + 1's z-butterflies and of this layer's x-sums
         T bcur[3] = {T(0), T(0), T(0)}, dgcur[3] = {T(0), T(0), T(0)};
         T Zu[3][4];
 #pragma unroll
@@ -206,7 +216,7 @@ __global__ __launch_bounds__(kTThreads) __attribute__((amdgpu_waves_per_eu(Op::k
             }
 
             T Fc[3][4];
-            face(unx, mnx, Fc);
+            face_read(pb, Fc);
 
             T w[3][8], z[3][8];
 #pragma unroll
@@ -249,14 +259,15 @@ __global__ __launch_bounds__(kTThreads) __attribute__((amdgpu_waves_per_eu(Op::k
                 const T r0 = q00 + dpp_row<kDppRowShr1>(q01);   // node (lx, lz) from this column
                 const T r1 = q10 + dpp_row<kDppRowShr1>(q11);   // node (lx + 1, lz)
                 val[c] = r0;
-                xr[c][tid] = r1;
+                xr[rb][c][tid] = r1;
             }
+            face_write(upf, mpf, pb ^ 1);   // plane L + 2, read by the next layer
             __syncthreads();
             if (L >= jc0 && out_col) {
                 const long r0 = 3 * ((long)L * nplane + col);
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
-                    const T kuv = ((mcur >> c) & 1) ? ucur[c] : val[c] + xr[c][nb_dn];
+                    const T kuv = ((mcur >> c) & 1) ? ucur[c] : val[c] + xr[rb][c][nb_dn];
                     T o;
                     if (MODE == MODE_JAC || MODE == MODE_JAC0 || MODE == MODE_JACDOT)
                         o = ucur[c] + omega * dgcur[c] * (bcur[c] - kuv);
@@ -284,6 +295,8 @@ __global__ __launch_bounds__(kTThreads) __attribute__((amdgpu_waves_per_eu(Op::k
             mcur = mnx;
             mnx = mpf;
             dcur = dnx;
+            pb ^= 1;
+            rb ^= 1;
         }
     }
     if (MODE == MODE_KUDOT || MODE == MODE_JACDOT) {
