@@ -192,7 +192,8 @@ def main():
                     help="only the isolated K.u of the workload and a 256 MiB calibration copy (for counter runs)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md)", file=sys.stderr)
+        print("bench.py runs on one GPU only: there is no multi-GPU path (the sharding is a design, not code)",
+              file=sys.stderr)
         sys.exit(2)
 
     import torch

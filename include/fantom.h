@@ -71,7 +71,10 @@ int fantom_stream_sync(void *stream);
 int fantom_ku_f64(int kernel, int nelx, int nely, int nelz, const double *K0, const double *E,
                   const uint8_t *mask, const double *u, double *y, void *stream);
 /* Jacobi sweep fused into the operator: x_out = x_in + omega*dinv*(b - K x_in)
- * (solvers/CUDA/_solvers.py:437-440). x_out must not alias x_in. */
+ * (solvers/CUDA/_solvers.py:437-440). x_out must not alias x_in.  With the
+ * tiled kernel dinv may be NULL: D^-1 is then formed in the sweep from E
+ * (diagonal d_c * sum of E over the node's elements, 1 on constrained DOFs),
+ * which saves reading a vector. */
 int fantom_ku_jacobi_f64(int kernel, int nelx, int nely, int nelz, const double *K0, const double *E,
                          const uint8_t *mask, const double *b, const double *dinv, double omega,
                          const double *x_in, double *x_out, void *stream);

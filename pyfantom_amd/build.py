@@ -22,7 +22,8 @@ FLAGS = [
     "-std=c++17",
     "-fPIC",
     "-shared",
-    # packed FP32 VALU ops stay off (see the build notes in DESIGN.md)
+    # packed FP32 VALU ops (v_pk_fma_f32 and kin) stay off: some of their
+    # forms have given wrong results on this hardware without any error
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result",
 ]

@@ -83,7 +83,7 @@ int fantom_ku_jacobi_f64(int kernel, int nelx, int nely, int nelz, const double 
                          const uint8_t *mask, const double *b, const double *dinv, double omega, const double *x_in,
                          double *x_out, void *stream)
 {
-    FANTOM_CHECK_ARG(b && dinv, "null operand");
+    FANTOM_CHECK_ARG(b && (dinv || kernel == FANTOM_KU_TILE), "null operand");
     FANTOM_CHECK_ARG(x_in != x_out, "x_in and x_out must not alias");
     return ku_dispatch(kernel, MODE_JAC, nelx, nely, nelz, K0, E, mask, x_in, x_out, b, dinv, omega, nullptr, stream);
 }

@@ -9,7 +9,22 @@ namespace fantom {
 // x0 = omega D^-1 b, with x0 formed on load (u unused); equals a scaling
 // followed by MODE_JAC, in one pass | MODE_JACDOT: MODE_JAC with the partial
 // sums of b.y (the PCG r.z when the sweep ends a V-cycle on r), tiled kernel
-enum { MODE_KU = 0, MODE_JAC = 1, MODE_RES = 2, MODE_KUDOT = 3, MODE_JAC0 = 4, MODE_JACDOT = 5 };
+// | MODE_JACOWN: MODE_JAC with D^-1 formed from the moduli (fine hex8 tile
+// operator; launch_ku_tile picks it for MODE_JAC with dinv = NULL).  On the
+// fine hex8 tile operator MODE_JACDOT forms D^-1 the same way | MODE_JAC0OWN:
+// MODE_JAC0 for a dinv that is the operator's own D^-1 (the drivers' fine
+// level): the sweep reuses x0 = omega D^-1 b, y = 2 x0 - omega D^-1 K x0 with
+// D^-1 formed from the moduli, so b and dinv are read once, not twice.
+enum {
+    MODE_KU = 0,
+    MODE_JAC = 1,
+    MODE_RES = 2,
+    MODE_KUDOT = 3,
+    MODE_JAC0 = 4,
+    MODE_JACDOT = 5,
+    MODE_JACOWN = 6,
+    MODE_JAC0OWN = 7
+};
 
 // Adaptive Jacobi weight of the reference's MultiGrid.solve
 // (solvers/CUDA/_solvers.py:626-648): omega = min_omega before the first

@@ -66,6 +66,8 @@ struct Gal1Op {
     // 2; FP32 104-120: 4)
     static constexpr int kWavesPerEU = sizeof(T) == 8 ? 2 : 4;
     static constexpr bool kScaled = false;
+    static constexpr bool kDiagFromModuli = false;   // the Galerkin diagonal is read (dinv_gal1)
+    __device__ __forceinline__ T inv_diag(int, T) const { return T(0); }
     const T *Btab;
     const T *Ef;
     int zero;   // 0, set by the host

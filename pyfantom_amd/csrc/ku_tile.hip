@@ -36,7 +36,11 @@ struct FineOp {
     static constexpr int kWavesPerEU = sizeof(T) == 8 ? 3 : 5;
     // apply leaves out the element modulus; the skeleton multiplies it in
     static constexpr bool kScaled = true;
+    // every corner of a box hex8 has the same diagonal entry per component,
+    // d_c = K0[3b + c][3b + c]; rd holds 1 / d_c
+    static constexpr bool kDiagFromModuli = true;
     T c[kPackedBlocks];
+    T rd[3];
     const T *E;
     struct Data {
         T e;
@@ -46,6 +50,7 @@ struct FineOp {
         d.e = active ? E[((long)ex * g.nely + ey) * g.nelz + ez] : T(0);
     }
     __device__ __forceinline__ T scale(const Data &d) const { return d.e; }
+    __device__ __forceinline__ T inv_diag(int comp, T inv_sum) const { return rd[comp] * inv_sum; }
     // block s reads h_k = w[k][s ^ (1 << k)] and writes z[k][s ^ (1 << k)]
     // (without the factor d.e)
     __device__ __forceinline__ void apply(const Data &d, const T (&w)[3][8], T (&z)[3][8]) const
@@ -190,7 +195,16 @@ static int launch_ku_tile_t(int mode, const Grid &g, const double *Kb, const T *
     FineOp<T> op;
     for (int i = 0; i < kPackedBlocks; ++i)
         op.c[i] = (T)packed[i];
+    // K0[3b + c][3b + c] = sum over the blocks s of their (c, c) entries
+    for (int c = 0; c < 3; ++c) {
+        double d = 0.0;
+        for (int sb = 0; sb < 8; ++sb)
+            d += Kb[9 * sb + 4 * c];
+        op.rd[c] = (T)(1.0 / d);
+    }
     op.E = E;
+    if (mode == MODE_JAC && dinv == nullptr)
+        mode = MODE_JACOWN;
     const TileShape ts = tile_shape(g, sizeof(T) == 8 ? kFineF64PerCU : kFineF32PerCU);
     const unsigned blocks = tile_blocks(ts);
     const T om = (T)omega;
@@ -206,6 +220,8 @@ static int launch_ku_tile_t(int mode, const Grid &g, const double *Kb, const T *
         FANTOM_TILE_CASE(MODE_KUDOT)
         FANTOM_TILE_CASE(MODE_JAC0)
         FANTOM_TILE_CASE(MODE_JACDOT)
+        FANTOM_TILE_CASE(MODE_JACOWN)
+        FANTOM_TILE_CASE(MODE_JAC0OWN)
     default:
         set_error("unknown matvec mode %d", mode);
         return kStatusBadArg;

