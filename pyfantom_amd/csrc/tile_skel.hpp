@@ -143,12 +143,25 @@ __global__ __launch_bounds__(kTThreads) __attribute__((amdgpu_waves_per_eu(Op::k
         const int jc0 = cyk * ts.chunk;
         const int jc1 = min(jc0 + ts.chunk, g.ny);   // output planes [jc0, jc1)
 
-        auto load_node = [&](int j, T (&v)[3], int &m) {
+        // zero-guess sweeps: the node value is x0 = omega D^-1 b.  load_node
+        // leaves b in v and D^-1 in vd and x0 is formed when the value is
+        // first used (form_x0), so that the loads stay in flight meanwhile;
+        // the fine operator's sweep with a read D^-1 (register-bound) forms
+        // it on load instead.
+        constexpr bool kDeferX0 = kZeroGuess && (MODE == MODE_JAC0OWN || !Op::kDiagFromModuli);
+        T vd_cur[3] = {T(0), T(0), T(0)}, vd_nx[3] = {T(0), T(0), T(0)}, vd_pf[3] = {T(0), T(0), T(0)};
+        auto load_node = [&](int j, T (&v)[3], int &m, T (&vd)[3]) {
             v[0] = v[1] = v[2] = T(0);
             m = 0;
             if (col_in && j >= 0 && j < g.ny) {
                 const long n = (long)j * nplane + col;
-                if (kZeroGuess) {
+                if (kDeferX0) {
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        v[c] = bvec[3 * n + c];
+                        vd[c] = dinv[3 * n + c];
+                    }
+                } else if (kZeroGuess) {
 #pragma unroll
                     for (int c = 0; c < 3; ++c)
                         v[c] = omega * dinv[3 * n + c] * bvec[3 * n + c];
@@ -159,6 +172,13 @@ __global__ __launch_bounds__(kTThreads) __attribute__((amdgpu_waves_per_eu(Op::k
                 }
                 if (mask)
                     m = mask[n];
+            }
+        };
+        auto form_x0 = [&](T (&v)[3], const T (&vd)[3]) {
+            if (kDeferX0) {
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+                    v[c] = omega * vd[c] * v[c];
             }
         };
         // face transform of a plane in two halves: face_write takes the
@@ -191,8 +211,10 @@ __global__ __launch_bounds__(kTThreads) __attribute__((amdgpu_waves_per_eu(Op::k
         T Fp[3][4], ucur[3], unx[3];
         int mcur, mnx;
         typename Op::Data dcur, dnx;
-        load_node(jc0 - 1, ucur, mcur);
-        load_node(jc0, unx, mnx);
+        load_node(jc0 - 1, ucur, mcur, vd_cur);
+        load_node(jc0, unx, mnx, vd_nx);
+        form_x0(ucur, vd_cur);
+        form_x0(unx, vd_nx);
         load_elem(dcur, jc0 - 1);
         face_write(ucur, mcur, 0);
         __syncthreads();
@@ -214,7 +236,7 @@ __global__ __launch_bounds__(kTThreads) __attribute__((amdgpu_waves_per_eu(Op::k
             // written at the end of this step (L >= jc0)
             T upf[3];
             int mpf;
-            load_node(L + 2 <= jc1 ? L + 2 : -1, upf, mpf);
+            load_node(L + 2 <= jc1 ? L + 2 : -1, upf, mpf, vd_pf);
             load_elem(dnx, L + 1 < jc1 ? L + 1 : -1);
             T bnx[3] = {T(0), T(0), T(0)}, dgnx[3] = {T(0), T(0), T(0)};
             if (kEpiB && out_col && L + 1 < jc1) {
@@ -273,6 +295,7 @@ __global__ __launch_bounds__(kTThreads) __attribute__((amdgpu_waves_per_eu(Op::k
                 val[c] = r0;
                 xr[rb][c][tid] = r1;
             }
+            form_x0(upf, vd_pf);
             face_write(upf, mpf, pb ^ 1);   // plane L + 2, read by the next layer
             if constexpr (kOwnDiag) {
                 const T e = op.scale(dcur);
