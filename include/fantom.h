@@ -110,6 +110,18 @@ int fantom_diag_f64(int nelx, int nely, int nelz, const double *K0, const double
  * core/CUDA/Kernels/grad.cu:1-16). */
 int fantom_dc_f64(int nelx, int nely, int nelz, const double *K0, const double *u, double *dc, void *stream);
 
+/* ---- element post-processing of a solution ------------------------------------
+ * Replaces FEA_locals_node_basis_parallel (core/CPU/_ops.py:1030-1058,
+ * FEA.cu:1-39) and the von Mises formulas of MinimumCompliance.FEA
+ * (Problem/CPU/MinimumCompliance.py:659-736) on structured grids: per element
+ * strain = B u_e (6 or 3 components), stress = D strain, energy =
+ * 1/2 u_e^T (w_e K0) u_e, von Mises of the stress.  K0 (24x24 / 8x8), D (6x6 /
+ * 3x3) and B (6x24 / 3x8, the Gauss-point-averaged operator) are device
+ * arrays in the reference's local order; w holds the element weights. */
+int fantom_fea_locals_f64(int nelx, int nely, int nelz, const double *K0, const double *D, const double *B,
+                          const double *w, const double *u, double *strain, double *stress, double *energy,
+                          double *von_mises, void *stream);
+
 /* ---- A7: grid transfers (fine grid nelx x nely x nelz) ---------------------
  * Replace apply_restriction_cuda / apply_prolongation_cuda
  * (core/CUDA/_mgm.py:173-264, core/CUDA/Kernels/mgm.cu:1-244).

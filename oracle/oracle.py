@@ -466,6 +466,75 @@ def num_threads():
 
 
 # Which check pins each oracle function (see tests/test_oracle.py).
+# --------------------------------------------------------------------------
+# Element post-processing (core/CPU/_ops.py:1030-1058,
+# Problem/CPU/MinimumCompliance.py:659-736)
+# --------------------------------------------------------------------------
+def b_center(dim: int, hx: float = 1.0, hy: float = 1.0, hz: float = 1.0) -> np.ndarray:
+    """The reference's Gauss-point-averaged B operator of a box element
+    (physics/LinearElasticity.py:780-829 for hex8, :405-424 for quad4).  The
+    derivatives of multilinear shape functions are multilinear in the other
+    coordinates, so their average over the symmetric Gauss points is their
+    value at the centre: dN_a/dx_p = (2 b_p - 1) / (2^(dim-1) h_p) for corner
+    bits b.  Rows (Voigt, engineering shear): 3D xx, yy, zz, xy, yz, xz;
+    2D xx, yy, xy."""
+    if dim == 3:
+        corners = [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)]
+        h = (hx, hy, hz)
+        B = np.zeros((6, 24))
+        for a, bits in enumerate(corners):
+            dn = [(2 * bits[p] - 1) / (4.0 * h[p]) for p in range(3)]
+            for i in range(3):
+                B[i, 3 * a + i] = dn[i]
+            B[3, 3 * a + 0], B[3, 3 * a + 1] = dn[1], dn[0]     # du/dy + dv/dx
+            B[4, 3 * a + 1], B[4, 3 * a + 2] = dn[2], dn[1]     # dv/dz + dw/dy
+            B[5, 3 * a + 0], B[5, 3 * a + 2] = dn[2], dn[0]     # du/dz + dw/dx
+        return B
+    corners = [(0, 0), (1, 0), (1, 1), (0, 1)]
+    h = (hx, hy)
+    B = np.zeros((3, 8))
+    for a, bits in enumerate(corners):
+        dn = [(2 * bits[p] - 1) / (2.0 * h[p]) for p in range(2)]
+        B[0, 2 * a] = dn[0]
+        B[1, 2 * a + 1] = dn[1]
+        B[2, 2 * a], B[2, 2 * a + 1] = dn[1], dn[0]
+    return B
+
+
+def d_iso(dim: int, nu: float, E: float = 1.0) -> np.ndarray:
+    """Constitutive matrix: 3D isotropic (physics/LinearElasticity.py:744-765),
+    2D plane stress (:358-364), Voigt order with engineering shear."""
+    if dim == 3:
+        f = E / ((1 + nu) * (1 - 2 * nu))
+        D = np.zeros((6, 6))
+        D[:3, :3] = f * nu
+        D[[0, 1, 2], [0, 1, 2]] = f * (1 - nu)
+        D[[3, 4, 5], [3, 4, 5]] = f * (1 - 2 * nu) / 2
+        return D
+    c = E / (1 - nu * nu)
+    return np.array([[c, c * nu, 0.0], [c * nu, c, 0.0], [0.0, 0.0, c * (1 - nu) / 2]])
+
+
+def fea_locals(K0, D, B, elems, w, u):
+    """strain_e = B u_e, stress_e = D strain_e, energy_e = 1/2 u_e^T w_e K0 u_e
+    (core/CPU/_ops.py:1030-1058) and the von Mises stress of
+    MinimumCompliance.FEA (Problem/CPU/MinimumCompliance.py:719-722)."""
+    npe = elems.shape[1]
+    dof = K0.shape[0] // npe
+    edofs = (elems[:, :, None] * dof + np.arange(dof)[None, None, :]).reshape(len(elems), -1)
+    ue = u[edofs]
+    strain = ue @ B.T
+    stress = strain @ D.T
+    energy = 0.5 * w * np.einsum("ei,ij,ej->e", ue, K0, ue)
+    s = stress
+    if dof == 3:
+        vm = np.sqrt(0.5 * ((s[:, 0] - s[:, 1]) ** 2 + (s[:, 1] - s[:, 2]) ** 2 + (s[:, 2] - s[:, 0]) ** 2
+                            + 6 * (s[:, 3] ** 2 + s[:, 4] ** 2 + s[:, 5] ** 2)))
+    else:
+        vm = np.sqrt(s[:, 0] ** 2 + s[:, 1] ** 2 - s[:, 0] * s[:, 1] + 3 * s[:, 2] ** 2)
+    return strain, stress, energy, vm
+
+
 PINNED = {
     "elements": "brute-force coordinate lookup (elements_bruteforce) + SHA-256 fixture",
     "k0 (product K0)": "exact sympy integration; top88 closed form up to node permutation; rigid-body nullspace",
@@ -477,4 +546,7 @@ PINNED = {
     "filter": "dense brute-force H; adjoint identity; H 1 = 1",
     "oc_update": "hand-checked 4-element case; volume feasibility and bisection invariant",
     "mgpcg_solve": "scipy spsolve of the same system",
+    "b_center / d_iso": "patch test (linear field -> exact constant strain); constant-strain energy u^T K0 u = "
+                        "V eps^T D eps against the exactly integrated K0",
+    "fea_locals": "energy sum equals 1/2 u^T K u of the scipy assembly; von Mises of a uniaxial stress is |sigma|",
 }

@@ -39,6 +39,7 @@ SIGNATURES = {
     "fantom_hex8_blocks": (_c_int, [_vp, _vp, _vp]),
     "fantom_diag_f64": (_c_int, [_c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _c_int, _vp]),
     "fantom_dc_f64": (_c_int, [_c_int, _c_int, _c_int, _vp, _vp, _vp, _vp]),
+    "fantom_fea_locals_f64": (_c_int, [_c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "fantom_restrict_f64": (_c_int, [_c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp]),
     "fantom_prolong_f64": (_c_int, [_c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _c_int, _vp]),
     "fantom_coarsen_E_f64": (_c_int, [_c_int, _c_int, _c_int, _vp, _vp, _vp]),

@@ -166,6 +166,50 @@ class MinimumCompliance:
             return self.penalty_schedule(self.penalty, self.iteration) == self.penalty
         return True
 
+    # ---- post-processing ----------------------------------------------------------------
+    def FEA(self, thresshold: bool = True):
+        """Solve the thresholded design and evaluate per-element strain,
+        stress, strain energy and von Mises stress
+        (Problem/CPU/MinimumCompliance.py:659-736).  As the reference: the
+        element moduli are (x > 0.5) + void (times E_mul), the solve gets four
+        times the solver's iteration limit, stress = D strain with the unit
+        modulus D and B averaged over the Gauss points, strain energy scaled by
+        the modulus.  ``thresshold=False`` uses the raw design variables (the
+        reference leaves that case undefined).  Returns a dict with 'strain',
+        'stress', 'strain_energy', 'von_mises' (device tensors), 'compliance'
+        and 'Displacements'."""
+        if self.desvars is None:
+            raise ValueError("Design variables are not initialized. Call init_desvars() or set_desvars() first.")
+        x = self.get_desvars()
+        rho = (x > 0.5).to(torch.float64) + self.void if thresshold else x.to(torch.float64).clone()
+        rho = rho * self.E_mul
+        solver = self.FE.solver
+        maxiter = getattr(solver, "maxiter", None)
+        if maxiter is not None:
+            solver.maxiter = maxiter * 4
+        try:
+            U, residual = self.FE.solve(rho)
+        finally:
+            if maxiter is not None:
+                solver.maxiter = maxiter
+        compliance = self._dot(self.FE.rhs, U)
+        if residual > 1e-5:
+            print(f"Solver residual is above 1e-5 ({residual:.4e}). Consider higher iterations (rerun this function "
+                  "and more iteration from prior solve will be applied).")
+        mesh, k = self.FE.mesh, self.FE.kernel
+        D, B = (torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(self.device) for a in mesh.locals)
+        K0 = torch.from_numpy(np.ascontiguousarray(mesh.K_single, dtype=np.float64)).to(self.device)
+        ns = D.shape[0]
+        strain = torch.empty((self.nel, ns), dtype=torch.float64, device=self.device)
+        stress = torch.empty_like(strain)
+        energy = torch.empty(self.nel, dtype=torch.float64, device=self.device)
+        vm = torch.empty_like(energy)
+        U = k._as_vec(U)
+        _lib.call("fantom_fea_locals_f64", *k.grid, _lib.ptr(K0), _lib.ptr(D), _lib.ptr(B), _lib.ptr(rho),
+                  _lib.ptr(U), _lib.ptr(strain), _lib.ptr(stress), _lib.ptr(energy), _lib.ptr(vm))
+        return {"strain": strain, "stress": stress, "strain_energy": energy, "von_mises": vm,
+                "compliance": float(compliance.item()), "Displacements": U}
+
     def logs(self):
         """The reference's keys (Problem/CPU/MinimumCompliance.py:639-657) plus
         the solve's PCG iteration count, precision and FP64 fallback."""
