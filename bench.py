@@ -87,7 +87,9 @@ def time_ku(torch, pf, K, mesh, reps=50):
     kid, k0 = K.op_args()
     args = (kid, *K.grid, K.k0_ptr(k0), E.data_ptr(), K.mask.data_ptr(), u.data_ptr(), y.data_ptr())
     stream = torch.cuda.current_stream()
-    for _ in range(5):
+    # steady state: the first launches on freshly allocated buffers run up
+    # to 20% slower (measured: 82 -> 67 us at 256x128x128 over ~100 launches)
+    for _ in range(100):
         _lib.call("fantom_ku_f64", *args)
     torch.cuda.synchronize()
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -182,7 +184,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="c3", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
-    ap.add_argument("--ku-reps", type=int, default=50)
+    ap.add_argument("--ku-reps", type=int, default=100)
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision SIMP run")
     ap.add_argument("--progress", action="store_true", help="print compliance and PCG iterations every step")
     ap.add_argument("--omega-schedule", default=None, choices=["reference", "fixed"],
