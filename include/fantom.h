@@ -122,6 +122,23 @@ int fantom_fea_locals_f64(int nelx, int nely, int nelz, const double *K0, const 
                           const double *w, const double *u, double *strain, double *stress, double *energy,
                           double *von_mises, void *stream);
 
+/* ---- explicit CSR of K(E) for direct-solver users ------------------------------
+ * Replaces StructuredStiffnessKernel.construct (stiffness/CUDA/_FEA.py:484-516)
+ * with its two kernels, matmat_node_basis_nnz_per_row_wcon_kernel
+ * (core/CUDA/Kernels/_matmat.py:258-300) and
+ * matmat_node_basis_parallel_wcon_kernel (core/CUDA/Kernels/matmat.cu:54-126),
+ * for the product K(E) @ I.  Dirichlet convention of the matrix-free operator:
+ * a constrained DOF's row is the unit row and its column is dropped from the
+ * free rows; explicit zeros of K0 stay in the pattern, as in the reference.
+ * csr_row_nnz writes the length of every row (n_dof ints); the caller forms
+ * indptr[0] = 0, indptr[r + 1] = indptr[r] + counts[r] (n_dof + 1 int64) and
+ * sizes indices (int32) and data to indptr[n_dof].  Columns come out in
+ * ascending order within each row (the reference keeps the order of first
+ * visit: the same matrix); values are summed in a fixed order. */
+int fantom_csr_row_nnz(int nelx, int nely, int nelz, const uint8_t *mask, int *counts, void *stream);
+int fantom_csr_assemble_f64(int nelx, int nely, int nelz, const double *K0, const double *E, const uint8_t *mask,
+                            const long long *indptr, int *indices, double *data, void *stream);
+
 /* ---- A7: grid transfers (fine grid nelx x nely x nelz) ---------------------
  * Replace apply_restriction_cuda / apply_prolongation_cuda
  * (core/CUDA/_mgm.py:173-264, core/CUDA/Kernels/mgm.cu:1-244).

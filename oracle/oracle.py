@@ -245,6 +245,39 @@ def assemble(K0, elems, E, n_dof, con=None):
     return K
 
 
+def csr_assemble(K0, elems, E, n_dof, con=None):
+    """K(E) @ I in the CSR form of the reference's construct
+    (stiffness/CUDA/_FEA.py:484-516; pattern from
+    core/CUDA/Kernels/_matmat.py:258-300, values from
+    core/CUDA/Kernels/matmat.cu:54-126): the pattern holds every pair of DOFs
+    of one element with both DOFs free, explicit zeros of K0 included; a
+    constrained DOF's row is the unit row.  Element scatter with duplicates
+    summed; columns ascending within a row.  Returns (indptr int64, indices
+    int32, data float64)."""
+    npe = elems.shape[1]
+    dof = K0.shape[0] // npe
+    nl = npe * dof
+    edofs = (elems[:, :, None].astype(np.int64) * dof + np.arange(dof)[None, None, :]).reshape(len(elems), -1)
+    rows = np.repeat(edofs, nl, axis=1).ravel()
+    cols = np.tile(edofs, (1, nl)).ravel()
+    vals = (np.asarray(E, np.float64)[:, None, None] * K0[None]).reshape(len(elems), -1).ravel()
+    if con is not None:
+        con = np.asarray(con, bool)
+        keep = ~con[rows] & ~con[cols]
+        c = np.flatnonzero(con)
+        rows = np.concatenate([rows[keep], c])
+        cols = np.concatenate([cols[keep], c])
+        vals = np.concatenate([vals[keep], np.ones(len(c))])
+    key = rows * n_dof + cols
+    order = np.argsort(key, kind="stable")
+    key, vals = key[order], vals[order]
+    uniq, start = np.unique(key, return_index=True)
+    data = np.add.reduceat(vals, start) if len(vals) else vals
+    indptr = np.zeros(n_dof + 1, np.int64)
+    np.cumsum(np.bincount(uniq // n_dof, minlength=n_dof), out=indptr[1:])
+    return indptr, (uniq % n_dof).astype(np.int32), data
+
+
 # --------------------------------------------------------------------------
 # C-library wrappers
 # --------------------------------------------------------------------------
@@ -549,4 +582,6 @@ PINNED = {
     "b_center / d_iso": "patch test (linear field -> exact constant strain); constant-strain energy u^T K0 u = "
                         "V eps^T D eps against the exactly integrated K0",
     "fea_locals": "energy sum equals 1/2 u^T K u of the scipy assembly; von Mises of a uniaxial stress is |sigma|",
+    "csr_assemble": "closed-form nonzero counts (free grid 9 prod(3n-2), clamped face); dense matrix equals the "
+                    "scipy assembly; K u equals the C matvec with constraints; symmetric pattern",
 }

@@ -45,6 +45,8 @@ SIGNATURES = {
     "fantom_coarsen_E_f64": (_c_int, [_c_int, _c_int, _c_int, _vp, _vp, _vp]),
     "fantom_coarsen_mask": (_c_int, [_c_int, _c_int, _c_int, _vp, _vp, _vp]),
     "fantom_dense_assemble_f64": (_c_int, [_c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp]),
+    "fantom_csr_row_nnz": (_c_int, [_c_int, _c_int, _c_int, _vp, _vp, _vp]),
+    "fantom_csr_assemble_f64": (_c_int, [_c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "fantom_spd_inverse_f64": (_c_int, [_c_int, _vp, _vp, _vp]),
     "fantom_gemv_f64": (_c_int, [_c_int, _vp, _vp, _vp, _vp]),
     "fantom_vec_partials": (_c_long, [_c_long]),

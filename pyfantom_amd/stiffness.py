@@ -69,6 +69,10 @@ class StructuredStiffnessKernel:
         self.mat_vec = None
         self.diag = None
         self.dk = None
+        self._csr_pattern = None
+        self.CSR = None
+        self.ptr = None
+        self.has_been_constructed = False
         self.matvec = self.dot
         self.rmatvec = self.dot
         self.matmat = self.dot
@@ -139,11 +143,22 @@ class StructuredStiffnessKernel:
         """K(rho) @ rhs (stiffness/CUDA/_FEA.py:623-673); returns a reused buffer."""
         if not self.has_rho:
             raise ValueError("Rho has not been set. dot works only after setting rho.")
+        if isinstance(rhs, torch.Tensor) and rhs.layout in (torch.sparse_csr, torch.sparse_coo):
+            # K @ B = construct(rho) @ B, constrained rows taking B's rows
+            # (core/CUDA/Kernels/matmat.cu:54-126 for a general B)
+            if rhs.shape[0] != self.n_nodes * self.dof:
+                raise ValueError("Shape of the input matrix does not match the number of nodes and dof.")
+            A = self.construct(self.rho)
+            B = rhs.to(device=self.device, dtype=torch.float64).to_sparse_csr()
+            it = A.crow_indices().dtype
+            if B.crow_indices().dtype != it:
+                B = torch.sparse_csr_tensor(B.crow_indices().to(it), B.col_indices().to(it), B.values(), size=B.shape)
+            return torch.sparse.mm(A, B)
         if isinstance(rhs, torch.Tensor):
             if rhs.ndim == 1 and rhs.shape[0] == self.n_nodes * self.dof:
                 return self._matvec(self.rho, self._as_vec(rhs))
             raise ValueError("Shape of the input vector does not match the number of nodes and dof.")
-        raise NotImplementedError("Only device tensors as vectors are supported.")
+        raise NotImplementedError("Only device tensors as vectors and sparse matrices are supported.")
 
     def __matmul__(self, rhs):
         return self.dot(rhs)
@@ -187,7 +202,39 @@ class StructuredStiffnessKernel:
         return self.dk
 
     def construct(self, rho):
-        raise NotImplementedError("explicit assembly is not part of the matrix-free structured path")
+        """Explicit K(rho) as a torch sparse CSR tensor on the device
+        (stiffness/CUDA/_FEA.py:484-516): the product K(rho) @ I of the
+        reference, with the Dirichlet convention of ``dot`` (unit rows for
+        constrained DOFs, their columns dropped elsewhere).  The pattern is
+        kept while the constraints stay the same; columns are ascending within
+        each row.  Index dtype int32 while the nonzeros fit, else int64."""
+        if not isinstance(rho, torch.Tensor):
+            rho = torch.as_tensor(np.asarray(rho))
+        rho = rho.to(device=self.device, dtype=torch.float64).contiguous()
+        if rho.shape[0] != self.n_elements:
+            raise ValueError("rho must have one entry per element.")
+        n_dof = self.shape[0]
+        mask = _lib.ptr(self.mask) if self.has_con else None
+        if self._csr_pattern is None or self._csr_pattern[0] != (self.con_version, self.has_con):
+            counts = torch.empty(n_dof, dtype=torch.int32, device=self.device)
+            _lib.call("fantom_csr_row_nnz", *self.grid, mask, _lib.ptr(counts))
+            indptr = torch.zeros(n_dof + 1, dtype=torch.int64, device=self.device)
+            torch.cumsum(counts, 0, out=indptr[1:])
+            nnz = int(indptr[-1])
+            indices = torch.empty(nnz, dtype=torch.int32, device=self.device)
+            self._csr_pattern = ((self.con_version, self.has_con), indptr, indices, None)
+        _, indptr, indices, crow = self._csr_pattern
+        data = torch.empty(indices.shape[0], dtype=torch.float64, device=self.device)
+        _lib.call("fantom_csr_assemble_f64", *self.grid, _lib.ptr(self.K0), _lib.ptr(rho), mask,
+                  _lib.ptr(indptr), _lib.ptr(indices), _lib.ptr(data))
+        if crow is None:
+            # torch wants one index dtype for both arrays
+            crow = (indptr.to(torch.int32), indices) if indices.shape[0] < 2 ** 31 else (indptr, indices.long())
+            self._csr_pattern = self._csr_pattern[:3] + (crow,)
+        self.CSR = torch.sparse_csr_tensor(crow[0], crow[1], data, size=self.shape)
+        self.ptr = crow[0]
+        self.has_been_constructed = True
+        return self.CSR
 
     def reset(self):
         self.has_rho = False

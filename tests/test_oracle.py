@@ -367,3 +367,65 @@ def test_product_locals_match_oracle(oracle, dim, h):
     D, B = LinearElasticity(E=1.0, nu=0.3).locals(X)
     assert np.allclose(D, oracle.d_iso(dim, 0.3), rtol=1e-14, atol=0)
     assert np.allclose(B, oracle.b_center(dim, *h), rtol=0, atol=1e-14)
+
+
+# ------------------------------------------------------- explicit CSR assembly
+def _clamp_x0(nel):
+    """Per-DOF constraint flags clamping the x = 0 node face (3D or 2D)."""
+    X = np.asarray(nel)
+    nodes_per_axis = X + 1
+    dim = len(nel)
+    n_nodes = int(np.prod(nodes_per_axis))
+    con = np.zeros(n_nodes * dim, bool)
+    P = None
+    if dim == 3:
+        # node n(i, j, k) = k + nz i + nz nx j; x index i
+        nx, nz = nodes_per_axis[0], nodes_per_axis[2]
+        ids = np.arange(n_nodes)
+        P = (ids % (nx * nz)) // nz
+    else:
+        nx = nodes_per_axis[0]
+        P = np.arange(n_nodes) % nx
+    for d in range(dim):
+        con[np.flatnonzero(P == 0) * dim + d] = True
+    return con
+
+
+@pytest.mark.parametrize("nel", [(1, 1, 1), (3, 2, 4), (2, 3), (4, 1)])
+def test_csr_assemble_pattern_and_values(oracle, nel):
+    """Nonzero counts against closed forms, the dense matrix against the scipy
+    assembly, K u against the C matvec: a wrong pattern rule (dropped explicit
+    zeros, constrained columns kept, a missing unit row) changes the counts; a
+    wrong local index or weight changes the values."""
+    dim = len(nel)
+    K0 = oracle.k0_exact(dim, 0.3)
+    el = oracle.elements(nel)
+    rng = np.random.default_rng(3)
+    E = rng.uniform(0.1, 1.0, len(el))
+    n = np.asarray(nel) + 1
+    n_dof = int(np.prod(n)) * dim
+    # free grid: every pair of nodes within one step per axis, dim x dim DOFs
+    indptr, idx, data = oracle.csr_assemble(K0, el, E, n_dof)
+    assert indptr[-1] == dim * dim * np.prod(3 * n - 2)
+    # clamped x = 0 face: free rows see x-neighbours in [1, nx - 1] only
+    con = _clamp_x0(nel)
+    indptr_c, idx_c, data_c = oracle.csr_assemble(K0, el, E, n_dof, con)
+    m = np.concatenate([[3 * (n[0] - 1) - 2], 3 * n[1:] - 2])
+    assert indptr_c[-1] == dim * dim * np.prod(m) + con.sum()
+    import scipy.sparse as sps
+    for (p, j, x), c in (((indptr, idx, data), None), ((indptr_c, idx_c, data_c), con)):
+        K = sps.csr_matrix((x, j, p), shape=(n_dof, n_dof))
+        for r in range(n_dof):
+            assert np.all(np.diff(j[p[r]:p[r + 1]]) > 0)  # ascending, no duplicates
+        Kd = K.toarray()
+        A = oracle.assemble(K0, el, E, n_dof, c).toarray()
+        assert np.abs(Kd - A).max() <= 1e-14 * np.abs(A).max()
+        pat = sps.csr_matrix((np.ones_like(x), j, p), shape=(n_dof, n_dof)).toarray()
+        assert np.all(pat[A != 0] == 1)  # every nonzero is in the pattern
+        assert np.array_equal(pat, pat.T)
+        u = rng.standard_normal(n_dof)
+        assert np.allclose(K @ u, oracle.matvec(K0, el, E, u, c), rtol=0, atol=1e-13 * np.abs(u).max())
+    # constrained rows are unit rows
+    rows_c = np.flatnonzero(con)
+    assert np.array_equal(np.diff(indptr_c)[rows_c], np.ones(len(rows_c), np.int64))
+    assert np.array_equal(idx_c[indptr_c[rows_c]], rows_c) and np.all(data_c[indptr_c[rows_c]] == 1.0)
