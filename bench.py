@@ -105,6 +105,40 @@ def time_ku(torch, pf, K, mesh, reps=50):
                 flops_dense=1152.0 * n_el, kernel="ku3d_tile_f64" if kid == _lib.KU_TILE else "ku3d_node_f64")
 
 
+def time_construct(torch, K, mesh, reps):
+    """Explicit CSR assembly of the seeded field: construct() once (row
+    lengths, scan, fill), then the fill kernel alone timed with HIP events on
+    its stream.  Algorithmic bytes: 12 B per nonzero (value, int32 column)
+    written, the 8 B row pointers and the 8 B moduli read."""
+    from pyfantom_amd import _lib
+    x = np.random.default_rng(0).uniform(0.01, 1.0, mesh.n_elements)
+    E = torch.from_numpy(1e-9 + x ** 3 * (1 - 1e-9)).cuda()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    A = K.construct(E)
+    torch.cuda.synchronize()
+    t_construct = time.perf_counter() - t0
+    _, indptr, indices, _ = K._csr_pattern
+    data = A.values()
+    args = (*K.grid, K.K0.data_ptr(), E.data_ptr(), K.mask.data_ptr(), indptr.data_ptr(), indices.data_ptr(),
+            data.data_ptr())
+    for _ in range(5):
+        _lib.call("fantom_csr_assemble_f64", *args)
+    stream = torch.cuda.current_stream()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = max(1, min(reps, 20))
+    ev0.record(stream)
+    for _ in range(reps):
+        _lib.call("fantom_csr_assemble_f64", *args)
+    ev1.record(stream)
+    ev1.synchronize()
+    t = ev0.elapsed_time(ev1) / 1e3 / reps
+    nnz = int(indices.shape[0])
+    alg = 12.0 * nnz + 8.0 * (K.shape[0] + 1) + 8.0 * mesh.n_elements
+    return {"construct_seconds": t_construct, "fill_seconds": t, "nnz": nnz, "algorithmic_bytes": alg,
+            "achieved_gbs": alg / t / 1e9, "hbm_fraction": alg / t / 8e12, "kernel": "csr_fill_k<3>"}
+
+
 def precision_errors(torch, pf, mesh, FE_m, FE, n_level):
     """One cold solve of the seeded field (x ~ U[0.01, 1], default_rng(0),
     E = Emin + x^3) with the mixed-precision solver and with the FP64 one:
@@ -192,6 +226,8 @@ def main():
                          "(default: fixed for the cantilevers, as specified for them; reference for the bridge)")
     ap.add_argument("--ku-only", action="store_true",
                     help="only the isolated K.u of the workload and a 256 MiB calibration copy (for counter runs)")
+    ap.add_argument("--construct-only", action="store_true",
+                    help="only the explicit CSR assembly K(E) @ I of the workload (construct())")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: there is no multi-GPU path (the sharding is a design, not code)",
@@ -205,6 +241,15 @@ def main():
     nel, n_level = cfg["nel"], cfg["n_level"]
     if args.omega_schedule is None:
         args.omega_schedule = "reference" if cfg.get("kind") == "bridge" else "fixed"
+
+    if args.construct_only:
+        mesh = pf.StructuredMesh3D(*nel, *nel, physics=pf.LinearElasticity(E=1.0, nu=0.3))
+        K = pf.StructuredStiffnessKernel(mesh)
+        j, k = np.meshgrid(np.arange(nel[1] + 1), np.arange(nel[2] + 1), indexing="ij")
+        clamp = (k + (nel[2] + 1) * (nel[0] + 1) * j).ravel()
+        K.set_constraints(np.concatenate([3 * clamp + d for d in range(3)]))
+        print(json.dumps(time_construct(torch, K, mesh, args.ku_reps)), flush=True)
+        return
 
     if args.ku_only:
         # hardware-counter runs: the K.u launches, then one device copy of a

@@ -9,11 +9,10 @@
 // Two passes, as the reference: the row lengths (free DOFs of the up to 27 / 9
 // neighbour nodes that share an element with the row's node; 1 for a
 // constrained row), an exclusive scan by the caller, then the fill.  The fill
-// gathers instead of scattering: one thread per row walks the neighbour nodes
-// in ascending node order, sums E_e K0 over the elements the two nodes share
-// and writes the row's free columns in ascending order (the reference keeps
-// the order of first visit; the matrix is the same).  No atomics, so the
-// values are deterministic.
+// gathers instead of scattering: each (row, neighbour node) pair sums E_e K0
+// over the elements the two nodes share, and the row's free columns come out
+// in ascending order (the reference keeps the order of first visit; the
+// matrix is the same).  No atomics, so the values are deterministic.
 #include "../../include/fantom.h"
 #include "common.hpp"
 
@@ -79,46 +78,75 @@ __global__ __launch_bounds__(kCsrBlock) void csr_row_nnz_k(Grid g, const unsigne
         counts[DIM * p + d] = (mp >> d) & 1u ? 1 : cnt;
 }
 
+// Fill: a block takes kCsrRows consecutive rows; one thread per (row,
+// neighbour node) pair computes that neighbour's DIM column values from the
+// elements the two nodes share and stages them in LDS at their place in the
+// row.  The group's rows occupy one contiguous range of the CSR arrays, which
+// the block then writes with unit-stride streaming stores.
+constexpr int kCsrRows = 16;   // 3D: 432 pairs on 448 threads; 2D: 144 on 192
+constexpr int kCsrGroups = 8;  // row groups per block (one K0 staging each)
+
 template <int DIM>
-__global__ __launch_bounds__(kCsrBlock) void csr_fill_k(Grid g, const double *__restrict__ K0g,
-                                                      const double *__restrict__ E,
-                                                      const unsigned char *__restrict__ mask,
-                                                      const long long *__restrict__ indptr, int *__restrict__ indices,
-                                                      double *__restrict__ data)
+__global__ __launch_bounds__(448) void csr_fill_k(
+    Grid g, const double *__restrict__ K0g, const double *__restrict__ E, const unsigned char *__restrict__ mask,
+    const long long *__restrict__ indptr, int *__restrict__ indices, double *__restrict__ data)
 {
-    constexpr int NL = DIM == 3 ? 24 : 8;  // element DOFs
+    constexpr int NL = DIM == 3 ? 24 : 8;     // element DOFs
+    constexpr int NB = DIM == 3 ? 27 : 9;     // neighbour nodes
+    constexpr int ITEMS = kCsrRows * NB;      // (row, neighbour) pairs per block
+    constexpr unsigned kAll = (1u << DIM) - 1;
     __shared__ double K0[NL * NL];
-    for (int t = threadIdx.x; t < NL * NL; t += kCsrBlock)
+    __shared__ double sdata[kCsrRows * NB * DIM];
+    __shared__ int sidx[kCsrRows * NB * DIM];
+    __shared__ unsigned char scnt[ITEMS];
+    const int tid = threadIdx.x;
+    for (int t = tid; t < NL * NL; t += blockDim.x)
         K0[t] = K0g[t];
-    __syncthreads();
-    const long row = (long)blockIdx.x * kCsrBlock + threadIdx.x;
-    if (row >= (long)DIM * g.n_nodes)
-        return;
-    const long p = row / DIM;
-    const int d = (int)(row - p * DIM);
-    long pos = indptr[row];
-    const unsigned mp = mask ? mask[p] : 0u;
-    if ((mp >> d) & 1u) {
-        indices[pos] = (int)row;
-        data[pos] = 1.0;
-        return;
-    }
-    int i, j, k;
-    node_coords<DIM>(g, p, i, j, k);
-    for (int dj = -1; dj <= 1; ++dj) {
-        const int jj = j + dj;
-        if (jj < 0 || jj >= g.ny)
-            continue;
-        for (int di = -1; di <= 1; ++di) {
-            const int ii = i + di;
-            if (ii < 0 || ii >= g.nx)
-                continue;
-            for (int dk = (DIM == 3 ? -1 : 0); dk <= (DIM == 3 ? 1 : 0); ++dk) {
-                const int kk = k + dk;
-                if (kk < 0 || kk >= g.nz)
-                    continue;
-                const long q = DIM == 3 ? (long)jj * g.nx * g.nz + (long)ii * g.nz + kk : (long)jj * g.nx + ii;
-                const unsigned mq = mask ? mask[q] : 0u;
+    const long n_rows = (long)DIM * g.n_nodes;
+    for (int grp = 0; grp < kCsrGroups; ++grp) {
+        const long row0 = ((long)blockIdx.x * kCsrGroups + grp) * kCsrRows;
+        if (row0 >= n_rows)
+            break;
+        const long row_end = min(row0 + kCsrRows, n_rows);
+        const long long base = indptr[row0];
+        // this thread's pair
+        const int r_loc = tid / NB, nb = tid - r_loc * NB;
+        const long row = row0 + r_loc;
+        const bool active = tid < ITEMS && row < row_end;
+        long p = 0, q = 0;
+        int d = 0, i = 0, j = 0, k = 0, ii = 0, jj = 0, kk = 0;
+        unsigned mp = 0, mq = kAll;  // out-of-grid neighbour: nothing free
+        if (active) {
+            p = row / DIM;
+            d = (int)(row - p * DIM);
+            node_coords<DIM>(g, p, i, j, k);
+            const int dj = nb / (DIM == 3 ? 9 : 3) - 1;
+            const int di = (DIM == 3 ? (nb / 3) % 3 : nb % 3) - 1;
+            const int dk = DIM == 3 ? nb % 3 - 1 : 0;
+            jj = j + dj;
+            ii = i + di;
+            kk = k + dk;
+            mp = mask ? mask[p] : 0u;
+            if (jj >= 0 && jj < g.ny && ii >= 0 && ii < g.nx && kk >= 0 && kk < g.nz) {
+                q = DIM == 3 ? (long)jj * g.nx * g.nz + (long)ii * g.nz + kk : (long)jj * g.nx + ii;
+                mq = mask ? mask[q] : 0u;
+            }
+        }
+        const bool row_free = !((mp >> d) & 1u);
+        if (tid < ITEMS)
+            scnt[tid] = active && row_free ? (unsigned char)__popc(kAll & ~mq) : 0;
+        __syncthreads();
+        if (active) {
+            // place of this pair in the block's range
+            long long off = indptr[row] - base;
+            if (!row_free) {
+                if (nb == NB / 2) {
+                    sidx[off] = (int)row;
+                    sdata[off] = 1.0;
+                }
+            } else if (scnt[tid]) {
+                for (int t = r_loc * NB; t < tid; ++t)
+                    off += scnt[t];
                 double v[DIM];
 #pragma unroll
                 for (int c = 0; c < DIM; ++c)
@@ -148,13 +176,20 @@ __global__ __launch_bounds__(kCsrBlock) void csr_fill_k(Grid g, const double *__
 #pragma unroll
                 for (int c = 0; c < DIM; ++c) {
                     if (!((mq >> c) & 1u)) {
-                        indices[pos] = (int)(DIM * q + c);
-                        data[pos] = v[c];
-                        ++pos;
+                        sidx[off] = (int)(DIM * q + c);
+                        sdata[off] = v[c];
+                        ++off;
                     }
                 }
             }
         }
+        __syncthreads();
+        const int total = (int)(indptr[row_end] - base);
+        for (int t = tid; t < total; t += blockDim.x) {
+            __builtin_nontemporal_store(sidx[t], indices + base + t);
+            __builtin_nontemporal_store(sdata[t], data + base + t);
+        }
+        __syncthreads();  // the next group reuses the staging buffers
     }
 }
 
@@ -185,13 +220,14 @@ int fantom_csr_assemble_f64(int nelx, int nely, int nelz, const double *K0, cons
     FANTOM_CHECK_ARG(K0 && E && indptr && indices && data, "null operand");
     const Grid g = make_grid(nelx, nely, nelz);
     const int dim = nelz > 0 ? 3 : 2;
-    const unsigned blocks = (unsigned)((dim * g.n_nodes + kCsrBlock - 1) / kCsrBlock);
+    const long rows_per_block = (long)kCsrRows * kCsrGroups;
+    const unsigned blocks = (unsigned)((dim * g.n_nodes + rows_per_block - 1) / rows_per_block);
     if (dim == 3)
-        hipLaunchKernelGGL(csr_fill_k<3>, dim3(blocks), dim3(kCsrBlock), 0, as_stream(stream), g, K0, E, mask, indptr,
-                           indices, data);
+        hipLaunchKernelGGL(csr_fill_k<3>, dim3(blocks), dim3(448), 0, as_stream(stream), g, K0, E, mask,
+                           indptr, indices, data);
     else
-        hipLaunchKernelGGL(csr_fill_k<2>, dim3(blocks), dim3(kCsrBlock), 0, as_stream(stream), g, K0, E, mask, indptr,
-                           indices, data);
+        hipLaunchKernelGGL(csr_fill_k<2>, dim3(blocks), dim3(192), 0, as_stream(stream), g, K0, E, mask,
+                           indptr, indices, data);
     FANTOM_LAUNCH_CHECK("csr_assemble");
     return kStatusOk;
 }
