@@ -122,6 +122,18 @@ int fantom_fea_locals_f64(int nelx, int nely, int nelz, const double *K0, const 
                           const double *w, const double *u, double *strain, double *stress, double *energy,
                           double *von_mises, void *stream);
 
+/* ---- structured mesh arrays (A1 numbering) -------------------------------------
+ * Replace generate_elements_2d / generate_elements_3d and the linspace /
+ * meshgrid node positions of generate_structured_mesh_cuda
+ * (core/CUDA/Kernels/_geom.py:3-118; numbering core/CPU/_geom.py:4-98).
+ * elements: int32 [n_el][8] (3D) or [n_el][4] (2D), 16-byte aligned;
+ * nodes: [n_nodes][3] or [n_nodes][2] with coordinate i (L / n) per axis and
+ * the last one exactly L, in the output type (numpy's linspace of a length of
+ * that type).  nelz = 0 selects 2D (lz ignored). */
+int fantom_mesh_elements(int nelx, int nely, int nelz, int32_t *elements, void *stream);
+int fantom_mesh_nodes_f64(int nelx, int nely, int nelz, double lx, double ly, double lz, double *nodes, void *stream);
+int fantom_mesh_nodes_f32(int nelx, int nely, int nelz, double lx, double ly, double lz, float *nodes, void *stream);
+
 /* ---- explicit CSR of K(E) for direct-solver users ------------------------------
  * Replaces StructuredStiffnessKernel.construct (stiffness/CUDA/_FEA.py:484-516)
  * with its two kernels, matmat_node_basis_nnz_per_row_wcon_kernel

@@ -105,9 +105,10 @@ def elements(nel) -> np.ndarray:
     return out
 
 
-def nodes(nel, dim_len) -> np.ndarray:
-    """Node coordinates, meshgrid 'xy' flatten order (core/CPU/_geom.py:74-91)."""
-    axes = [np.linspace(0, L, n + 1) for L, n in zip(dim_len, nel)]
+def nodes(nel, dim_len, dtype=np.float64) -> np.ndarray:
+    """Node coordinates, meshgrid 'xy' flatten order (core/CPU/_geom.py:74-91):
+    numpy's linspace of each length taken in the mesh's dtype."""
+    axes = [np.linspace(0, dtype(L), n + 1, dtype=dtype) for L, n in zip(dim_len, nel)]
     grids = np.meshgrid(*axes)
     return np.stack([g.flatten() for g in grids], axis=-1)
 

@@ -46,6 +46,9 @@ SIGNATURES = {
     "fantom_coarsen_mask": (_c_int, [_c_int, _c_int, _c_int, _vp, _vp, _vp]),
     "fantom_dense_assemble_f64": (_c_int, [_c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp]),
     "fantom_csr_row_nnz": (_c_int, [_c_int, _c_int, _c_int, _vp, _vp, _vp]),
+    "fantom_mesh_elements": (_c_int, [_c_int, _c_int, _c_int, _vp, _vp]),
+    "fantom_mesh_nodes_f64": (_c_int, [_c_int, _c_int, _c_int, _c_double, _c_double, _c_double, _vp, _vp]),
+    "fantom_mesh_nodes_f32": (_c_int, [_c_int, _c_int, _c_int, _c_double, _c_double, _c_double, _vp, _vp]),
     "fantom_csr_assemble_f64": (_c_int, [_c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "fantom_spd_inverse_f64": (_c_int, [_c_int, _vp, _vp, _vp]),
     "fantom_gemv_f64": (_c_int, [_c_int, _vp, _vp, _vp, _vp]),

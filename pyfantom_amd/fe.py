@@ -6,7 +6,9 @@ positions, optional per-DOF flags, prescribed values), ``add_point_forces``,
 ``reset_forces``, ``reset_dirichlet_boundary_conditions`` and
 ``solve(rho) -> (U, residual)``.  Positions are mapped to nodes by nearest
 grid node (the reference queries a KDTree over the node coordinates; on a
-structured grid the nearest node is the rounded grid index).
+structured grid the nearest node is the rounded grid index).  The constraint
+flags, prescribed values and load vector are assembled with device tensor
+operations: no host round trip.
 """
 from __future__ import annotations
 
@@ -29,24 +31,32 @@ class FiniteElement:
         self.is_3D = len(mesh.nel) == 3
         self.dof = mesh.dof
 
-    def nearest_nodes(self, positions) -> np.ndarray:
-        """Node ids nearest to the given coordinates (KDTree query semantics)."""
-        P = np.asarray(positions.detach().cpu().numpy() if isinstance(positions, torch.Tensor) else positions,
-                       dtype=np.float64).reshape(-1, len(self.mesh.nel))
-        h = np.asarray(self.mesh.spacing, dtype=np.float64)
-        idx = np.rint(P / h[None, :]).astype(np.int64)
-        idx = np.clip(idx, 0, np.asarray(self.mesh.nel, dtype=np.int64)[None, :])
+    def _dev(self, a, dtype):
+        """A host array, list or tensor as a device tensor of the given dtype."""
+        if isinstance(a, torch.Tensor):
+            return a.to(device=self.kernel.device, dtype=dtype)
+        return torch.as_tensor(np.asarray(a), dtype=dtype, device=self.kernel.device)
+
+    def nearest_nodes(self, positions) -> torch.Tensor:
+        """Node ids nearest to the given coordinates (KDTree query semantics,
+        FiniteElement/CPU/FiniteElement.py:179-184): on a structured grid the
+        nearest node is the rounded grid index, computed on the device."""
+        d = len(self.mesh.nel)
+        P = self._dev(positions, torch.float64).reshape(-1, d)
+        h = torch.tensor([float(v) for v in self.mesh.spacing], dtype=torch.float64, device=P.device)
+        nel = torch.tensor([int(v) for v in self.mesh.nel], dtype=torch.int64, device=P.device)
+        idx = torch.minimum(torch.round(P / h).to(torch.int64).clamp_min(0), nel)
         nx = int(self.mesh.nel[0]) + 1
         if self.is_3D:
             nz = int(self.mesh.nel[2]) + 1
             return idx[:, 2] + nz * idx[:, 0] + nz * nx * idx[:, 1]
         return idx[:, 1] * nx + idx[:, 0]
 
-    def _ids(self, node_ids):
-        return np.asarray(node_ids.detach().cpu().numpy() if isinstance(node_ids, torch.Tensor) else node_ids,
-                          dtype=np.int64).reshape(-1)
+    def _ids(self, node_ids) -> torch.Tensor:
+        return self._dev(node_ids, torch.int64).reshape(-1)
 
     def add_dirichlet_boundary_condition(self, node_ids=None, positions=None, dofs=None, rhs=0):
+        """FiniteElement/CPU/FiniteElement.py:108-188, on device tensors."""
         if node_ids is None and positions is None:
             raise ValueError("Either node_ids or positions must be provided.")
         if node_ids is not None and positions is not None:
@@ -57,33 +67,32 @@ class FiniteElement:
         n_con = node_ids.shape[0]
         rhs_arr = None
         if isinstance(rhs, (np.ndarray, torch.Tensor)):
-            rhs_arr = np.asarray(rhs.detach().cpu().numpy() if isinstance(rhs, torch.Tensor) else rhs, dtype=np.float64)
+            rhs_arr = self._dev(rhs, torch.float64)
             if rhs_arr.shape[0] != n_con:
                 raise ValueError("rhs must have the same length as node_ids or positions.")
+            rhs_arr = rhs_arr.reshape(n_con, -1)
         if dofs is not None:
-            dofs = np.asarray(dofs.detach().cpu().numpy() if isinstance(dofs, torch.Tensor) else dofs)
+            dofs = self._dev(dofs, torch.int64)
             if dofs.ndim == 1:
                 dofs = dofs[None, :]
             if dofs.shape[0] != n_con and dofs.shape[0] != 1:
                 raise ValueError("dofs must have the same length as node_ids.")
             if dofs.shape[0] == 1:
-                dofs = np.tile(dofs, (n_con, 1))
-        dev = self.kernel.device
-        d_rhs = self.d_rhs.cpu().numpy()
+                dofs = dofs.expand(n_con, -1)
         for i in range(self.dof):
-            sel = np.ones(n_con, dtype=bool) if dofs is None else (dofs[:, i] == 1)
+            sel = slice(None) if dofs is None else (dofs[:, i] == 1)
             cons = node_ids[sel] * self.dof + i
-            self.kernel.add_constraints(torch.as_tensor(cons, device=dev))
+            self.kernel.add_constraints(cons)
             if rhs_arr is not None:
-                d_rhs[cons] = rhs_arr[sel, i]
+                self.d_rhs[cons] = rhs_arr[sel, i]
             else:
-                d_rhs[cons] = rhs + np.nan_to_num(d_rhs[cons], nan=0)
-        self.d_rhs = torch.from_numpy(d_rhs).to(dev)
+                self.d_rhs[cons] = float(rhs) + torch.nan_to_num(self.d_rhs[cons], nan=0.0)
 
     def add_neumann_boundary_condition(self, **kwargs):
         raise NotImplementedError("Neumann boundary conditions are not implemented in this version of FiniteElement.")
 
     def add_point_forces(self, forces, node_ids=None, positions=None):
+        """FiniteElement/CPU/FiniteElement.py:190-259, on device tensors."""
         if node_ids is None and positions is None:
             raise ValueError("Either node_ids or positions must be provided.")
         if node_ids is not None and positions is not None:
@@ -91,23 +100,25 @@ class FiniteElement:
         if positions is not None:
             return self.add_point_forces(forces=forces, node_ids=self.nearest_nodes(positions))
         node_ids = self._ids(node_ids)
-        forces = np.asarray(forces.detach().cpu().numpy() if isinstance(forces, torch.Tensor) else forces,
-                            dtype=np.float64)
+        forces = self._dev(forces, torch.float64)
         if forces.ndim == 1:
             forces = forces[None, :]
         if (forces.shape[0] != node_ids.shape[0] and forces.shape[0] != 1) or forces.shape[1] != self.dof:
             raise ValueError("forces must have shape (N_forces, mesh.dof).")
         if forces.shape[0] == 1:
-            forces = np.tile(forces, (node_ids.shape[0], 1))
-        rhs = self.rhs.cpu().numpy()
+            forces = forces.expand(node_ids.shape[0], -1)
+        # the reference's buffered fancy-index add (rhs[idx] += f): a node
+        # listed twice receives one force, its last occurrence's
+        uniq, inv = torch.unique(node_ids, return_inverse=True)
+        pos = torch.arange(node_ids.shape[0], device=node_ids.device)
+        last = torch.full((uniq.shape[0],), -1, dtype=torch.int64, device=node_ids.device)
+        last.scatter_reduce_(0, inv, pos, reduce="amax")
+        f = forces[last]
         for i in range(self.dof):
-            # buffered fancy-index add, as the reference does: a node listed
-            # twice receives its force once
-            rhs[node_ids * self.dof + i] += forces[:, i]
-        d_rhs = self.d_rhs.cpu().numpy()
-        dir_dofs = ~np.isnan(d_rhs)
-        rhs[dir_dofs] = d_rhs[dir_dofs]
-        self.rhs = torch.from_numpy(rhs).to(self.kernel.device)
+            idx = uniq * self.dof + i
+            self.rhs[idx] = self.rhs[idx] + f[:, i]
+        dir_dofs = ~torch.isnan(self.d_rhs)
+        self.rhs[dir_dofs] = self.d_rhs[dir_dofs]
 
     def reset_forces(self):
         self.rhs.zero_()

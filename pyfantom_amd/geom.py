@@ -4,8 +4,9 @@
 (geom/CPU/_mesh.py:10-210, geom/CUDA/_mesh.py:10-170): same constructor,
 attributes and node/element numbering (core/CPU/_geom.py:4-98).  The kernels
 never read the connectivity -- they index the grid implicitly -- so
-``elements`` / ``nodes`` are built on first access only, on the host, as
-vectorised index arithmetic, then moved to the device.
+``elements`` / ``nodes`` are generated on first access only, on the device
+(fantom_mesh_elements / fantom_mesh_nodes_*, the counterpart of
+generate_structured_mesh_cuda, core/CUDA/Kernels/_geom.py:54-118).
 
 ``StructuredFilter2D/3D`` mirror geom/CPU/_filters.py:14-194 and
 geom/commons/_filters.py:19-157: cone weights over integer offsets,
@@ -18,32 +19,6 @@ import torch
 
 from . import _lib
 from .physics import LinearElasticity
-
-
-def generate_elements(nel) -> np.ndarray:
-    """Connectivity int32[n_el, 2^d] in the reference's order (core/CPU/_geom.py:4-59)."""
-    nel = [int(v) for v in nel]
-    if len(nel) == 3:
-        nelx, nely, nelz = nel
-        nx, nz = nelx + 1, nelz + 1
-        i, j, k = np.meshgrid(np.arange(nelx), np.arange(nely), np.arange(nelz), indexing="ij")
-        i, j, k = (a.reshape(-1).astype(np.int64) for a in (i, j, k))
-        base = nz * i + nz * nx * j + k
-        d = [0, nz, nz + nz * nx, nz * nx, 1, nz + 1, nz + nz * nx + 1, nz * nx + 1]
-        return np.stack([base + o for o in d], axis=1).astype(np.int32)
-    nelx, nely = nel
-    nx = nelx + 1
-    i, j = np.meshgrid(np.arange(nelx), np.arange(nely), indexing="ij")
-    i, j = i.reshape(-1).astype(np.int64), j.reshape(-1).astype(np.int64)
-    base = j * nx + i
-    return np.stack([base, base + 1, base + nx + 1, base + nx], axis=1).astype(np.int32)
-
-
-def generate_nodes(nel, lengths, dtype=np.float64) -> np.ndarray:
-    """Node coordinates in meshgrid('xy') flatten order (core/CPU/_geom.py:74-91)."""
-    axes = [np.linspace(0, L, n + 1, dtype=dtype) for L, n in zip(lengths, nel)]
-    grids = np.meshgrid(*axes)
-    return np.stack([g.flatten() for g in grids], axis=-1)
 
 
 _HEX_BITS = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]],
@@ -82,14 +57,24 @@ class _StructuredMesh:
 
     @property
     def elements(self) -> torch.Tensor:
+        """Connectivity int32 [n_el, 2^d] (core/CPU/_geom.py:4-59)."""
         if self._elements is None:
-            self._elements = torch.from_numpy(generate_elements(self.nel)).to(_lib.device())
+            el = torch.empty((self.n_elements, self.elements_size), dtype=torch.int32, device=_lib.device())
+            _lib.call("fantom_mesh_elements", *self.grid, _lib.ptr(el))
+            self._elements = el
         return self._elements
 
     @property
     def nodes(self) -> torch.Tensor:
+        """Node coordinates [n_nodes, d] in meshgrid('xy') flatten order
+        (core/CPU/_geom.py:74-91)."""
         if self._nodes is None:
-            self._nodes = torch.from_numpy(generate_nodes(self.nel, self.dim, self.dtype)).to(_lib.device())
+            f32 = np.dtype(self.dtype) == np.float32
+            X = torch.empty((self.n_nodes, len(self.nel)), dtype=torch.float32 if f32 else torch.float64,
+                            device=_lib.device())
+            L = [float(v) for v in self.dim] + [0.0] * (3 - len(self.nel))
+            _lib.call("fantom_mesh_nodes_f32" if f32 else "fantom_mesh_nodes_f64", *self.grid, *L, _lib.ptr(X))
+            self._nodes = X
         return self._nodes
 
     @property

@@ -98,12 +98,6 @@ def test_hex8_blocks_rejects_non_box():
         hex8_blocks(K + K.T)
 
 
-def test_product_connectivity_matches_oracle(oracle):
-    from pyfantom_amd.geom import generate_elements
-    for nel in ((3, 2, 4), (5, 3), (128, 64, 64)):
-        assert np.array_equal(generate_elements(nel), oracle.elements(nel))
-
-
 def test_product_filter_taps_match_oracle(oracle):
     from pyfantom_amd.geom import filter_taps
     for dim, r, sp in ((3, 1.5, (1, 1, 1)), (2, 1.5, (1, 1)), (3, 2.0, (1, 1, 1)), (3, 2.0, (1, 0.5, 2))):

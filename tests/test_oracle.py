@@ -36,6 +36,28 @@ def test_connectivity_golden_hash(oracle):
         assert hashlib.sha256(el.tobytes()).hexdigest() == h, key
 
 
+@pytest.mark.parametrize("nel", [(3, 2, 4), (4, 3)])
+def test_nodes_numbering_and_spacing(oracle, nel):
+    """Node n(i, j, k) = k + nz i + nz nx j (2D: j nx + i) sits at (i h_x,
+    j h_y, k h_z), the last one of each axis exactly at the length (catches a
+    transposed meshgrid or a step formed as L * (i / n))."""
+    L = (1.3, 0.7, 0.1)[:len(nel)]
+    for dt in (np.float64, np.float32):
+        X = oracle.nodes(nel, L, dt)
+        n = [v + 1 for v in nel]
+        for idx in np.ndindex(*n):
+            if len(nel) == 3:
+                i, j, k = idx
+                node = k + n[2] * i + n[2] * n[0] * j
+            else:
+                i, j = idx
+                node = j * n[0] + i
+            for a, c in enumerate(idx):
+                step = dt(dt(L[a]) / dt(nel[a]))
+                want = dt(L[a]) if c == nel[a] else dt(dt(c) * step)
+                assert X[node, a] == want
+
+
 def test_element_volumes_positive(oracle):
     """Every hex has positive Jacobian volume with the reference node order."""
     nel = (3, 2, 2)
