@@ -208,7 +208,7 @@ __global__ __launch_bounds__(kTThreads) __attribute__((amdgpu_waves_per_eu(Op::k
             op.load(d, g, el_col && L >= 0 && L < g.nely, gx, L, gz);
         };
 
-        T Fp[3][4], ucur[3], unx[3];
+        T FpA[3][4], FpB[3][4], ucur[3], unx[3];
         int mcur, mnx;
         typename Op::Data dcur, dnx;
         load_node(jc0 - 1, ucur, mcur, vd_cur);
@@ -218,7 +218,7 @@ __global__ __launch_bounds__(kTThreads) __attribute__((amdgpu_waves_per_eu(Op::k
         load_elem(dcur, jc0 - 1);
         face_write(ucur, mcur, 0);
         __syncthreads();
-        face_read(0, Fp);
+        face_read(0, FpA);
         face_write(unx, mnx, 1);
         __syncthreads();
         int pb = 1, rb = 0;   // LDS buffers of plane L + 1's z-butterflies and of this layer's x-sums
@@ -231,7 +231,10 @@ __global__ __launch_bounds__(kTThreads) __attribute__((amdgpu_waves_per_eu(Op::k
             for (int q = 0; q < 4; ++q)
                 Zu[c][q] = T(0);
 
-        for (int L = jc0 - 1; L < jc1; ++L) {
+        // one element layer: Fp holds the face transform of plane L, the
+        // layer leaves that of plane L + 1 in Fc.  The loop alternates the two
+        // face buffers instead of copying one into the other.
+        auto layer = [&](const int L, const T (&Fp)[3][4], T (&Fc)[3][4]) {
             // element layer L lies between node planes L and L + 1; plane L is
             // written at the end of this step (L >= jc0)
             T upf[3];
@@ -249,7 +252,6 @@ __global__ __launch_bounds__(kTThreads) __attribute__((amdgpu_waves_per_eu(Op::k
                 }
             }
 
-            T Fc[3][4];
             face_read(pb, Fc);
 
             T w[3][8], z[3][8];
@@ -335,9 +337,6 @@ __global__ __launch_bounds__(kTThreads) __attribute__((amdgpu_waves_per_eu(Op::k
             }
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-#pragma unroll
-                for (int q = 0; q < 4; ++q)
-                    Fp[c][q] = Fc[c][q];
                 ucur[c] = unx[c];
                 unx[c] = upf[c];
                 bcur[c] = bnx[c];
@@ -348,7 +347,14 @@ __global__ __launch_bounds__(kTThreads) __attribute__((amdgpu_waves_per_eu(Op::k
             dcur = dnx;
             pb ^= 1;
             rb ^= 1;
+        };
+        int L = jc0 - 1;
+        for (; L + 1 < jc1; L += 2) {
+            layer(L, FpA, FpB);
+            layer(L + 1, FpB, FpA);
         }
+        if (L < jc1)
+            layer(L, FpA, FpB);
     }
     if (MODE == MODE_KUDOT || MODE == MODE_JACDOT) {
         const double s = block_sum<kTThreads>(part);
