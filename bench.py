@@ -87,11 +87,14 @@ def time_ku(torch, pf, K, mesh, reps=50):
     kid, k0 = K.op_args()
     args = (kid, *K.grid, K.k0_ptr(k0), E.data_ptr(), K.mask.data_ptr(), u.data_ptr(), y.data_ptr())
     stream = torch.cuda.current_stream()
-    # steady state: the first launches on freshly allocated buffers run up
-    # to 20% slower (measured: 82 -> 67 us at 256x128x128 over ~100 launches)
-    for _ in range(100):
-        _lib.call("fantom_ku_f64", *args)
-    torch.cuda.synchronize()
+    # steady state: the first few hundred launches run up to 20% slower
+    # (per-launch trace at 256x128x128: 72 -> 88 -> 71 us over the first 100),
+    # so the kernel runs back to back for 0.3 s before the timed launches
+    t_warm = time.perf_counter() + 0.3
+    while time.perf_counter() < t_warm:
+        for _ in range(50):
+            _lib.call("fantom_ku_f64", *args)
+        torch.cuda.synchronize()
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     ev0.record(stream)
     for _ in range(reps):
