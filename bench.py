@@ -75,37 +75,50 @@ def build_problem(pf, nel, n_level, tol=1e-8, precision="fp64", kind="cantilever
     return mesh, K, solver, FE, filt, prob
 
 
-def time_ku(torch, pf, K, mesh, reps=50):
-    """Average duration of the K.u kernel from HIP events on its stream."""
+def time_ku(torch, pf, K, mesh, reps=100):
+    """Average duration of the K.u kernel from HIP events on its stream.
+
+    The launches rotate over enough independent (E, u, y) sets that their
+    footprint is three times the 256 MB Infinity Cache, so every launch reads
+    its inputs from HBM: with one set reused, part of the 240 MB working set
+    of the 256x128x128 K.u stays in that cache and the kernel measures 60 us
+    instead of 80.  The first set is the seeded field (x ~ U[0.01, 1],
+    default_rng(0), E = Emin + x^3; u ~ N(0, 1), default_rng(1), zero on the
+    constrained DOFs); the others use seeds 2, 3, ...  Before the timed
+    launches the kernel runs back to back for 0.3 s (steady clocks)."""
     from pyfantom_amd import _lib
-    x = np.random.default_rng(0).uniform(0.01, 1.0, mesh.n_elements)
-    E = torch.from_numpy(1e-9 + x ** 3 * (1 - 1e-9)).cuda()
-    K.set_rho(E)
-    u = torch.from_numpy(np.random.default_rng(1).standard_normal(K.shape[0])).cuda()
-    u[K.constraints] = 0
-    y = torch.empty_like(u)
+    n_dof, n_el = K.shape[0], mesh.n_elements
+    bytes_alg = 16.0 * n_dof + 8.0 * n_el
+    n_sets = max(3, int(np.ceil(3 * 256e6 / bytes_alg)))
     kid, k0 = K.op_args()
-    args = (kid, *K.grid, K.k0_ptr(k0), E.data_ptr(), K.mask.data_ptr(), u.data_ptr(), y.data_ptr())
+    sets, args = [], []
+    for s in range(n_sets):
+        x = np.random.default_rng(0 if s == 0 else 2 * s).uniform(0.01, 1.0, n_el)
+        E = torch.from_numpy(1e-9 + x ** 3 * (1 - 1e-9)).cuda()
+        u = torch.from_numpy(np.random.default_rng(1 if s == 0 else 2 * s + 1).standard_normal(n_dof)).cuda()
+        u[K.constraints] = 0
+        y = torch.empty_like(u)
+        sets.append((E, u, y))
+        args.append((kid, *K.grid, K.k0_ptr(k0), E.data_ptr(), K.mask.data_ptr(), u.data_ptr(), y.data_ptr()))
+    K.set_rho(sets[0][0])
     stream = torch.cuda.current_stream()
-    # steady state: the first few hundred launches run up to 20% slower
-    # (per-launch trace at 256x128x128: 72 -> 88 -> 71 us over the first 100),
-    # so the kernel runs back to back for 0.3 s before the timed launches
     t_warm = time.perf_counter() + 0.3
+    i = 0
     while time.perf_counter() < t_warm:
         for _ in range(50):
-            _lib.call("fantom_ku_f64", *args)
+            _lib.call("fantom_ku_f64", *args[i % n_sets])
+            i += 1
         torch.cuda.synchronize()
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     ev0.record(stream)
-    for _ in range(reps):
-        _lib.call("fantom_ku_f64", *args)
+    for r in range(reps):
+        _lib.call("fantom_ku_f64", *args[r % n_sets])
     ev1.record(stream)
     ev1.synchronize()
     t = ev0.elapsed_time(ev1) / 1e3 / reps
-    n_dof, n_el = K.shape[0], mesh.n_elements
-    bytes_alg = 16.0 * n_dof + 8.0 * n_el
     return dict(seconds=t, gdof_s=n_dof / t / 1e9, achieved_gbs=bytes_alg / t / 1e9, bytes=bytes_alg,
-                flops_dense=1152.0 * n_el, kernel="ku3d_tile_f64" if kid == _lib.KU_TILE else "ku3d_node_f64")
+                flops_dense=1152.0 * n_el, buffer_sets=n_sets,
+                kernel="ku3d_tile_f64" if kid == _lib.KU_TILE else "ku3d_node_f64")
 
 
 def time_construct(torch, K, mesh, reps):
@@ -352,7 +365,8 @@ def main():
         "pcg_iterations_mean": float(np.mean(iters)),
         "compliance": prob.f(),
         "solve_rel_residual": float(prob._residual),
-        "ku": {"gdof_s": ku["gdof_s"], "seconds": ku["seconds"], "kernel": ku["kernel"]},
+        "ku": {"gdof_s": ku["gdof_s"], "seconds": ku["seconds"], "kernel": ku["kernel"],
+               "buffer_sets": ku["buffer_sets"]},
         "mixed_precision": mixed,
         "roofline": {"bound": "hbm", "achieved": ku["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ku["achieved_gbs"] / HBM_PEAK_GBS, "traffic": measured_traffic(cfg["name"]),

@@ -349,12 +349,25 @@ __global__ __launch_bounds__(kTThreads) __attribute__((amdgpu_waves_per_eu(Op::k
             rb ^= 1;
         };
         int L = jc0 - 1;
-        for (; L + 1 < jc1; L += 2) {
-            layer(L, FpA, FpB);
-            layer(L + 1, FpB, FpA);
+        if constexpr (!(MODE == MODE_JAC || MODE == MODE_JAC0)) {
+            for (; L + 1 < jc1; L += 2) {
+                layer(L, FpA, FpB);
+                layer(L + 1, FpB, FpA);
+            }
+            if (L < jc1)
+                layer(L, FpA, FpB);
+        } else {
+            // the sweeps that read D^-1 keep one layer per trip: two copies of
+            // the layer would not fit the register budget there
+            for (; L < jc1; ++L) {
+                layer(L, FpA, FpB);
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q)
+                        FpA[c][q] = FpB[c][q];
+            }
         }
-        if (L < jc1)
-            layer(L, FpA, FpB);
     }
     if (MODE == MODE_KUDOT || MODE == MODE_JACDOT) {
         const double s = block_sum<kTThreads>(part);
