@@ -144,7 +144,8 @@ int fantom_mesh_nodes_f32(int nelx, int nely, int nelz, double lx, double ly, do
  * free rows; explicit zeros of K0 stay in the pattern, as in the reference.
  * csr_row_nnz writes the length of every row (n_dof ints); the caller forms
  * indptr[0] = 0, indptr[r + 1] = indptr[r] + counts[r] (n_dof + 1 int64) and
- * sizes indices (int32) and data to indptr[n_dof].  Columns come out in
+ * sizes indices (int32) and data to indptr[n_dof]; assemble expects exactly
+ * that indptr for the same grid and mask.  Columns come out in
  * ascending order within each row (the reference keeps the order of first
  * visit: the same matrix); values are summed in a fixed order. */
 int fantom_csr_row_nnz(int nelx, int nely, int nelz, const uint8_t *mask, int *counts, void *stream);

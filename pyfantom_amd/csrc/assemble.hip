@@ -95,6 +95,7 @@ __global__ __launch_bounds__(448) void csr_fill_k(
     constexpr int NB = DIM == 3 ? 27 : 9;     // neighbour nodes
     constexpr int ITEMS = kCsrRows * NB;      // (row, neighbour) pairs per block
     constexpr unsigned kAll = (1u << DIM) - 1;
+    constexpr int kStage = kCsrRows * NB * DIM;   // staging entries: the longest possible rows
     __shared__ double K0[NL * NL];
     __shared__ double sdata[kCsrRows * NB * DIM];
     __shared__ int sidx[kCsrRows * NB * DIM];
@@ -140,7 +141,7 @@ __global__ __launch_bounds__(448) void csr_fill_k(
             // place of this pair in the block's range
             long long off = indptr[row] - base;
             if (!row_free) {
-                if (nb == NB / 2) {
+                if (nb == NB / 2 && (unsigned long long)off < (unsigned long long)kStage) {
                     sidx[off] = (int)row;
                     sdata[off] = 1.0;
                 }
@@ -175,7 +176,7 @@ __global__ __launch_bounds__(448) void csr_fill_k(
                 }
 #pragma unroll
                 for (int c = 0; c < DIM; ++c) {
-                    if (!((mq >> c) & 1u)) {
+                    if (!((mq >> c) & 1u) && (unsigned long long)off < (unsigned long long)kStage) {
                         sidx[off] = (int)(DIM * q + c);
                         sdata[off] = v[c];
                         ++off;
@@ -184,7 +185,9 @@ __global__ __launch_bounds__(448) void csr_fill_k(
             }
         }
         __syncthreads();
-        const int total = (int)(indptr[row_end] - base);
+        // a row pointer array that does not match the mask cannot push the
+        // staging past its end
+        const int total = (int)min(indptr[row_end] - base, (long long)kStage);
         for (int t = tid; t < total; t += blockDim.x) {
             __builtin_nontemporal_store(sidx[t], indices + base + t);
             __builtin_nontemporal_store(sdata[t], data + base + t);
