@@ -238,17 +238,67 @@ __global__ __launch_bounds__(kMmaBlock) void mma_dir_k(MmaDev d, MmaScal sc, Mma
     block_reduce(&mx, 1, true, d.partials);
 }
 
-// line-search trial point (n part) and its residual
+// line-search trial point with its residual and, in the same pass, the Newton
+// sums of mma_newton_k at that point (same arithmetic, same block order): when
+// the trial is accepted and the barrier parameter stays, the next Newton step
+// takes them from here instead of another pass over the n arrays.  Partials:
+// the residual record (sums of squares and gvec per block, then the block
+// maxima), then the Newton record at gridDim.x * (2 + M).
 template <int M>
-__global__ __launch_bounds__(kMmaBlock) void mma_trial_k(MmaDev d, MmaScal sc, double steg, double epsi)
+__global__ __launch_bounds__(kMmaBlock) void mma_trial_newton_k(MmaDev d, MmaScal sc, double steg, double epsi)
 {
+    constexpr int nv = 2 * M + M * M;
+    double sums[1 + M], v[nv];
+#pragma unroll
+    for (int i = 0; i < 1 + M; ++i)
+        sums[i] = 0.0;
+#pragma unroll
+    for (int k = 0; k < nv; ++k)
+        v[k] = 0.0;
+    double mx = 0.0;
     for (long j = (long)blockIdx.x * kMmaBlock + threadIdx.x; j < d.n; j += (long)gridDim.x * kMmaBlock) {
-        d.xt[j] = d.xs[j] + steg * d.dx[j];
-        d.xsit[j] = d.xsi[j] + steg * d.dxsi[j];
-        d.etat[j] = d.eta[j] + steg * d.deta[j];
+        const double x = d.xs[j] + steg * d.dx[j];
+        const double xsi = d.xsi[j] + steg * d.dxsi[j];
+        const double eta = d.eta[j] + steg * d.deta[j];
+        d.xt[j] = x;
+        d.xsit[j] = xsi;
+        d.etat[j] = eta;
+        const double alfa = d.alfa[j], beta = d.beta[j];
+        const double ux = d.upp[j] - x, xl = x - d.low[j];
+        const double ux2 = ux * ux, xl2 = xl * xl;
+        double plam = d.p0[j], qlam = d.q0[j];
+        double GG[M];
+#pragma unroll
+        for (int i = 0; i < M; ++i) {
+            const double P = d.P[(long)i * d.n + j], Q = d.Q[(long)i * d.n + j];
+            plam += P * sc.lam[i];
+            qlam += Q * sc.lam[i];
+            const double g = P / ux + Q / xl;
+            sums[1 + i] += g;
+            v[i] += g;
+            GG[i] = P / ux2 - Q / xl2;
+        }
+        const double rex = plam / ux2 - qlam / xl2 - xsi + eta;
+        const double rexsi = xsi * (x - alfa) - epsi;
+        const double reeta = eta * (beta - x) - epsi;
+        sums[0] += rex * rex + rexsi * rexsi + reeta * reeta;
+        mx = fmax(mx, fmax(fabs(rex), fmax(fabs(rexsi), fabs(reeta))));
+        const double delx = plam / ux2 - qlam / xl2 - epsi / (x - alfa) + epsi / (beta - x);
+        const double diagx = 2.0 * (plam / (ux2 * ux) + qlam / (xl2 * xl)) + xsi / (x - alfa) + eta / (beta - x);
+        const double dd = delx / diagx;
+#pragma unroll
+        for (int i = 0; i < M; ++i) {
+            v[M + i] += GG[i] * dd;
+#pragma unroll
+            for (int k = 0; k < M; ++k)
+                v[2 * M + i * M + k] += GG[i] * GG[k] / diagx;
+        }
     }
-    // each thread reads back only the entries it wrote: no barrier needed
-    resid_point<M>(d, sc, epsi, d.xt, d.xsit, d.etat, d.partials);
+    block_reduce(sums, 1 + M, false, d.partials);
+    __syncthreads();   // block_reduce's LDS is reused
+    block_reduce(&mx, 1, true, d.partials + (long)gridDim.x * (1 + M));
+    __syncthreads();
+    block_reduce(v, nv, false, d.partials + (long)gridDim.x * (2 + M));
 }
 
 // launch a kernel template instantiated for the constraint count m
@@ -424,7 +474,7 @@ extern "C" int fantom_mma_update_f64(long n, int m, int iter, const double *x, c
         for (int i = 0; i < m; ++i)
             sc.lam[i] = lamv[i];
         if (trial) {
-            MMA_DISPATCH(m, mma_trial_k, dim3(nb), dv, sc, steg, epsi);
+            MMA_DISPATCH(m, mma_trial_newton_k, dim3(nb), dv, sc, steg, epsi);
         } else {
             MMA_DISPATCH(m, mma_resid_k, dim3(nb), dv, sc, epsi);
         }
@@ -463,16 +513,23 @@ extern "C" int fantom_mma_update_f64(long n, int m, int iter, const double *x, c
         if (st)
             return st;
         int ittt = 0;
+        // the accepted trial of the previous step left the Newton sums of
+        // this point (same barrier parameter) behind its residual record
+        bool have_sums = false;
         while (rmax > 0.9 * epsi && ittt < 200) {
             ++ittt;
             ++steps;
             MmaScal sc;
             for (int i = 0; i < m; ++i)
                 sc.lam[i] = lam[i];
-            MMA_DISPATCH(m, mma_newton_k, dim3(nb), dv, sc, epsi);
-            FANTOM_LAUNCH_CHECK("mma_newton");
             const int nrec = 2 * m + m * m;
-            st = rd.fetch(dv.partials, (size_t)nb * nrec);
+            if (have_sums) {
+                st = rd.fetch(dv.partials + (size_t)nb * (2 + m), (size_t)nb * nrec);
+            } else {
+                MMA_DISPATCH(m, mma_newton_k, dim3(nb), dv, sc, epsi);
+                FANTOM_LAUNCH_CHECK("mma_newton");
+                st = rd.fetch(dv.partials, (size_t)nb * nrec);
+            }
             if (st)
                 return st;
             double gv[kMmaMaxM], sgd[kMmaMaxM], AA[(kMmaMaxM + 1) * (kMmaMaxM + 1)], rhs[kMmaMaxM + 1];
@@ -555,6 +612,7 @@ extern "C" int fantom_mma_update_f64(long n, int m, int iter, const double *x, c
             std::swap(dv.eta, dv.etat);
             rnorm = resinew;
             rmax = rmx;
+            have_sums = true;
         }
         epsi *= 0.1;
     }
