@@ -34,7 +34,7 @@
 
 namespace fantom {
 
-enum { D_DONE = 8, D_ITER = 9, D_RES = 10, D_TOL = 11, D_NB = 12, D_K = 13, D_OMEGA = 14 };
+enum { D_DONE = 8, D_ITER = 9, D_RES = 10, D_TOL = 11, D_NB = 12, D_K = 13, D_OMEGA = 14, D_XUPD = 15 };
 
 
 struct MgLevel {
@@ -99,8 +99,11 @@ __global__ void mg_conv_k(const double *__restrict__ partials, long np, double *
     }
 }
 
-__global__ void mg_xr_k(long n, double *__restrict__ x, double *__restrict__ r, const double *__restrict__ p,
-                        const double *__restrict__ q, const double *__restrict__ scal, double *__restrict__ partials)
+// r -= alpha q (frozen after convergence) and the partial sums of r.r;
+// scal[D_XUPD] records whether this iteration updated, for mg_px_k, which
+// applies the matching x += alpha p together with the next direction
+__global__ void mg_r_k(long n, double *__restrict__ r, const double *__restrict__ q, double *__restrict__ scal,
+                       double *__restrict__ partials)
 {
     const bool frozen = scal[D_DONE] != 0.0;
     const double alpha = scal[S_ALPHA];
@@ -108,15 +111,32 @@ __global__ void mg_xr_k(long n, double *__restrict__ x, double *__restrict__ r, 
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
         double ri = r[i];
         if (!frozen) {
-            x[i] += alpha * p[i];
             ri -= alpha * q[i];
             r[i] = ri;
         }
         s += ri * ri;
     }
     s = block_sum<256>(s);
-    if (threadIdx.x == 0)
+    if (threadIdx.x == 0) {
         partials[blockIdx.x] = s;
+        if (blockIdx.x == 0)
+            scal[D_XUPD] = frozen ? 0.0 : 1.0;
+    }
+}
+
+// x += alpha p (when mg_r_k updated r in this iteration), then p = z + beta p:
+// the iterate update rides on the direction update's pass over p
+__global__ __launch_bounds__(256) void mg_px_k(long n, double *__restrict__ x, double *__restrict__ p,
+                                               const double *__restrict__ z, const double *__restrict__ scal)
+{
+    const bool upd = scal[D_XUPD] != 0.0;
+    const double alpha = scal[S_ALPHA], beta = scal[S_BETA];
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const double pi = p[i];
+        if (upd)
+            x[i] += alpha * pi;
+        p[i] = z[i] + beta * pi;
+    }
 }
 
 __global__ void mg_init_scal_k(double *__restrict__ scal, double tol, double omega0)
@@ -280,14 +300,13 @@ static int mg_iteration(MgHandle *h)
                                                                 : ku_node_partials(h->lev[0].g);
     MG_TRY(lev_ku(h, 0, MODE_KUDOT, h->p, h->q, nullptr, nullptr, 0.0, h->partials));
     MG_TRY(launch_finalize(h->partials, npk, h->scal, FIN_ALPHA, 0, h->stream));
-    hipLaunchKernelGGL(mg_xr_k, dim3(vec_grid(n)), dim3(256), 0, h->stream, n, h->xs, h->r, h->p, h->q, h->scal,
-                       h->partials);
+    hipLaunchKernelGGL(mg_r_k, dim3(vec_grid(n)), dim3(256), 0, h->stream, n, h->r, h->q, h->scal, h->partials);
     hipLaunchKernelGGL(mg_conv_k, dim3(1), dim3(1024), 0, h->stream, h->partials, (long)vec_grid(n), h->scal,
                        h->sched);
     long nrz = 0;
     MG_TRY(mg_precond(h, h->r, h->z, &nrz));
     MG_TRY(launch_finalize(h->partials, nrz, h->scal, FIN_BETA, 0, h->stream));
-    MG_TRY(launch_pcg_p(n, h->p, h->z, h->scal, 0, h->stream));
+    hipLaunchKernelGGL(mg_px_k, dim3(vec_grid(n)), dim3(256), 0, h->stream, n, h->xs, h->p, h->z, h->scal);
     FANTOM_LAUNCH_CHECK("mg iteration");
     return kStatusOk;
 }

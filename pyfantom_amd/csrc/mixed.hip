@@ -29,7 +29,7 @@
 
 namespace fantom {
 
-enum { X_DONE = 8, X_ITER = 9, X_RES = 10, X_TOL = 11, X_NB = 12, X_K = 13, X_OMEGA = 14 };
+enum { X_DONE = 8, X_ITER = 9, X_RES = 10, X_TOL = 11, X_NB = 12, X_K = 13, X_OMEGA = 14, X_XUPD = 15 };
 
 struct MxLevel {
     Grid g;
@@ -190,12 +190,12 @@ __global__ void mx_zero_f32(long n, float *__restrict__ a)
 }
 
 // ---- FP64 PCG with the FP32 V-cycle as preconditioner ---------------------------
-// x += alpha p, r -= alpha q (frozen after convergence), r32 = (float) r, and
-// the partial sums of r.r
-__global__ __launch_bounds__(256) void mxd_xr_k(long n, double *__restrict__ x, double *__restrict__ r,
-                                               float *__restrict__ r32, const double *__restrict__ p,
-                                               const double *__restrict__ q, const double *__restrict__ scal,
-                                               double *__restrict__ partials)
+// r -= alpha q (frozen after convergence), r32 = (float) r, and the partial
+// sums of r.r; scal[X_XUPD] records whether this iteration updated, for
+// mxd_px_k, which applies the matching x += alpha p with the next direction
+__global__ __launch_bounds__(256) void mxd_r_k(long n, double *__restrict__ r, float *__restrict__ r32,
+                                              const double *__restrict__ q, double *__restrict__ scal,
+                                              double *__restrict__ partials)
 {
     const bool frozen = scal[X_DONE] != 0.0;
     const double alpha = scal[S_ALPHA];
@@ -203,7 +203,6 @@ __global__ __launch_bounds__(256) void mxd_xr_k(long n, double *__restrict__ x, 
     for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
         double ri = r[i];
         if (!frozen) {
-            x[i] += alpha * p[i];
             ri -= alpha * q[i];
             r[i] = ri;
         }
@@ -211,8 +210,25 @@ __global__ __launch_bounds__(256) void mxd_xr_k(long n, double *__restrict__ x, 
         s += ri * ri;
     }
     s = block_sum<256>(s);
-    if (threadIdx.x == 0)
+    if (threadIdx.x == 0) {
         partials[blockIdx.x] = s;
+        if (blockIdx.x == 0)
+            scal[X_XUPD] = frozen ? 0.0 : 1.0;
+    }
+}
+
+// x += alpha p (when mxd_r_k updated r in this iteration), then p = z + beta p
+__global__ __launch_bounds__(256) void mxd_px_k(long n, double *__restrict__ x, double *__restrict__ p,
+                                               const float *__restrict__ z, const double *__restrict__ scal)
+{
+    const bool upd = scal[X_XUPD] != 0.0;
+    const double alpha = scal[S_ALPHA], beta = scal[S_BETA];
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const double pi = p[i];
+        if (upd)
+            x[i] += alpha * pi;
+        p[i] = (double)z[i] + beta * pi;
+    }
 }
 
 // convergence test on ||r|| / ||f|| (scal[X_NB] = ||f||)
@@ -384,13 +400,12 @@ static int mxd_iteration(MxHandle *h)
     MX_TRY(launch_ku_tile(MODE_KUDOT, h->g, h->Kb64, h->E64, h->mask, h->p64, h->q64, nullptr, nullptr, 0.0,
                           h->partials, s));
     MX_TRY(launch_finalize(h->partials, ku_tile_partials(h->g), h->scal, FIN_ALPHA, 0, s));
-    hipLaunchKernelGGL(mxd_xr_k, dim3(G), dim3(256), 0, s, n, h->x64, h->r64, h->r, h->p64, h->q64, h->scal,
-                       h->partials);
+    hipLaunchKernelGGL(mxd_r_k, dim3(G), dim3(256), 0, s, n, h->r64, h->r, h->q64, h->scal, h->partials);
     hipLaunchKernelGGL(mxd_conv_k, dim3(1), dim3(1024), 0, s, h->partials, (long)G, h->scal, h->sched);
     MX_TRY(mx_precond(h, h->r, h->z));
     hipLaunchKernelGGL(mxd_dot_k, dim3(G), dim3(256), 0, s, n, h->r64, h->z, h->partials);
     MX_TRY(launch_finalize(h->partials, (long)G, h->scal, FIN_BETA, 0, s));
-    hipLaunchKernelGGL(mxd_p_k, dim3(G), dim3(256), 0, s, n, h->p64, h->z, h->scal, 0);
+    hipLaunchKernelGGL(mxd_px_k, dim3(G), dim3(256), 0, s, n, h->x64, h->p64, h->z, h->scal);
     FANTOM_LAUNCH_CHECK("mxd iteration");
     return kStatusOk;
 }
