@@ -36,7 +36,6 @@ struct MmaDev {
     double *low, *upp, *alfa, *beta, *p0, *q0, *P, *Q;
     double *xs, *xsi, *eta;      // current subproblem point (n part)
     double *xt, *xsit, *etat;    // line-search trial point
-    double *dx, *dxsi, *deta;    // Newton direction (n part)
     double *partials;
 };
 
@@ -206,7 +205,32 @@ __global__ __launch_bounds__(kMmaBlock) void mma_newton_k(MmaDev d, MmaScal sc, 
     block_reduce(v, nv, false, d.partials);
 }
 
-// search direction (n part) and the largest step ratio over it
+// search direction (n part) at element j of the current point (x, xsi, eta)
+template <int M>
+__device__ __forceinline__ void mma_direction(const MmaDev &d, const MmaScal &sc, const MmaScal &dl, double epsi,
+                                              long j, double x, double xsi, double eta, double alfa, double beta,
+                                              double &dx, double &dxsi, double &deta)
+{
+    const double ux = d.upp[j] - x, xl = x - d.low[j];
+    const double ux2 = ux * ux, xl2 = xl * xl;
+    double plam = d.p0[j], qlam = d.q0[j], gdl = 0.0;
+#pragma unroll
+    for (int i = 0; i < M; ++i) {
+        const double P = d.P[(long)i * d.n + j], Q = d.Q[(long)i * d.n + j];
+        plam += P * sc.lam[i];
+        qlam += Q * sc.lam[i];
+        gdl += (P / ux2 - Q / xl2) * dl.lam[i];
+    }
+    const double delx = plam / ux2 - qlam / xl2 - epsi / (x - alfa) + epsi / (beta - x);
+    const double diagx = 2.0 * (plam / (ux2 * ux) + qlam / (xl2 * xl)) + xsi / (x - alfa) + eta / (beta - x);
+    dx = -delx / diagx - gdl / diagx;
+    dxsi = -xsi + epsi / (x - alfa) - (xsi * dx) / (x - alfa);
+    deta = -eta + epsi / (beta - x) + (eta * dx) / (beta - x);
+}
+
+// largest step ratio over the search direction; the direction itself is not
+// stored: the line-search trial forms it again where it needs it, which
+// costs arithmetic instead of writing and reading back three n-vectors
 template <int M>
 __global__ __launch_bounds__(kMmaBlock) void mma_dir_k(MmaDev d, MmaScal sc, MmaScal dl, double epsi)
 {
@@ -214,24 +238,8 @@ __global__ __launch_bounds__(kMmaBlock) void mma_dir_k(MmaDev d, MmaScal sc, Mma
     for (long j = (long)blockIdx.x * kMmaBlock + threadIdx.x; j < d.n; j += (long)gridDim.x * kMmaBlock) {
         const double x = d.xs[j], xsi = d.xsi[j], eta = d.eta[j];
         const double alfa = d.alfa[j], beta = d.beta[j];
-        const double ux = d.upp[j] - x, xl = x - d.low[j];
-        const double ux2 = ux * ux, xl2 = xl * xl;
-        double plam = d.p0[j], qlam = d.q0[j], gdl = 0.0;
-#pragma unroll
-        for (int i = 0; i < M; ++i) {
-            const double P = d.P[(long)i * d.n + j], Q = d.Q[(long)i * d.n + j];
-            plam += P * sc.lam[i];
-            qlam += Q * sc.lam[i];
-            gdl += (P / ux2 - Q / xl2) * dl.lam[i];
-        }
-        const double delx = plam / ux2 - qlam / xl2 - epsi / (x - alfa) + epsi / (beta - x);
-        const double diagx = 2.0 * (plam / (ux2 * ux) + qlam / (xl2 * xl)) + xsi / (x - alfa) + eta / (beta - x);
-        const double dx = -delx / diagx - gdl / diagx;
-        const double dxsi = -xsi + epsi / (x - alfa) - (xsi * dx) / (x - alfa);
-        const double deta = -eta + epsi / (beta - x) + (eta * dx) / (beta - x);
-        d.dx[j] = dx;
-        d.dxsi[j] = dxsi;
-        d.deta[j] = deta;
+        double dx, dxsi, deta;
+        mma_direction<M>(d, sc, dl, epsi, j, x, xsi, eta, alfa, beta, dx, dxsi, deta);
         mx = fmax(mx, fmax(fmax(-1.01 * dxsi / xsi, -1.01 * deta / eta),
                            fmax(-1.01 * dx / (x - alfa), 1.01 * dx / (beta - x))));
     }
@@ -245,7 +253,8 @@ __global__ __launch_bounds__(kMmaBlock) void mma_dir_k(MmaDev d, MmaScal sc, Mma
 // the residual record (sums of squares and gvec per block, then the block
 // maxima), then the Newton record at gridDim.x * (2 + M).
 template <int M>
-__global__ __launch_bounds__(kMmaBlock) void mma_trial_newton_k(MmaDev d, MmaScal sc, double steg, double epsi)
+__global__ __launch_bounds__(kMmaBlock) void mma_trial_newton_k(MmaDev d, MmaScal sc, MmaScal base, MmaScal dl,
+                                                               double steg, double epsi)
 {
     constexpr int nv = 2 * M + M * M;
     double sums[1 + M], v[nv];
@@ -257,13 +266,16 @@ __global__ __launch_bounds__(kMmaBlock) void mma_trial_newton_k(MmaDev d, MmaSca
         v[k] = 0.0;
     double mx = 0.0;
     for (long j = (long)blockIdx.x * kMmaBlock + threadIdx.x; j < d.n; j += (long)gridDim.x * kMmaBlock) {
-        const double x = d.xs[j] + steg * d.dx[j];
-        const double xsi = d.xsi[j] + steg * d.dxsi[j];
-        const double eta = d.eta[j] + steg * d.deta[j];
+        const double alfa = d.alfa[j], beta = d.beta[j];
+        const double x0 = d.xs[j], xsi0 = d.xsi[j], eta0 = d.eta[j];
+        double dx, dxsi, deta;   // the direction of mma_dir_k at the current point
+        mma_direction<M>(d, base, dl, epsi, j, x0, xsi0, eta0, alfa, beta, dx, dxsi, deta);
+        const double x = x0 + steg * dx;
+        const double xsi = xsi0 + steg * dxsi;
+        const double eta = eta0 + steg * deta;
         d.xt[j] = x;
         d.xsit[j] = xsi;
         d.etat[j] = eta;
-        const double alfa = d.alfa[j], beta = d.beta[j];
         const double ux = d.upp[j] - x, xl = x - d.low[j];
         const double ux2 = ux * ux, xl2 = xl * xl;
         double plam = d.p0[j], qlam = d.q0[j];
@@ -394,7 +406,7 @@ extern "C" long fantom_mma_workspace_size(long n, int m)
     if (n <= 0 || m < 1 || m > kMmaMaxM)
         return -1;
     const long rec = 2 * m + m * m + 2 * (1 + m);
-    return (long)sizeof(double) * ((long)(14 + 2 * m) * n + (long)kMmaMaxBlocks * (rec + 2) + 64);
+    return (long)sizeof(double) * ((long)(11 + 2 * m) * n + (long)kMmaMaxBlocks * (rec + 2) + 64);
 }
 
 extern "C" int fantom_mma_update_f64(long n, int m, int iter, const double *x, const double *xold1,
@@ -441,9 +453,6 @@ extern "C" int fantom_mma_update_f64(long n, int m, int iter, const double *x, c
     dv.xt = take(n);
     dv.xsit = take(n);
     dv.etat = take(n);
-    dv.dx = take(n);
-    dv.dxsi = take(n);
-    dv.deta = take(n);
     dv.partials = take(0);
     const unsigned nb = mma_blocks(n);
     Reader rd{s, {}};
@@ -467,6 +476,9 @@ extern "C" int fantom_mma_update_f64(long n, int m, int iter, const double *x, c
         sl[i] = 1.0;
     }
     const int rrec = 1 + m;   // residual record: sumsq, gvec[m]; then nb maxabs
+    // the current point's multipliers and the dual direction of the Newton
+    // step whose line search is running (the trial forms the n direction)
+    MmaScal trial_base, trial_dl;
     auto residual = [&](double epsi, bool trial, double steg, double *gvec, double &rnorm, double &rmax,
                         const double *yv, double zv, const double *lamv, const double *muv, double zetv,
                         const double *sv) -> int {
@@ -474,7 +486,7 @@ extern "C" int fantom_mma_update_f64(long n, int m, int iter, const double *x, c
         for (int i = 0; i < m; ++i)
             sc.lam[i] = lamv[i];
         if (trial) {
-            MMA_DISPATCH(m, mma_trial_newton_k, dim3(nb), dv, sc, steg, epsi);
+            MMA_DISPATCH(m, mma_trial_newton_k, dim3(nb), dv, sc, trial_base, trial_dl, steg, epsi);
         } else {
             MMA_DISPATCH(m, mma_resid_k, dim3(nb), dv, sc, epsi);
         }
@@ -563,6 +575,8 @@ extern "C" int fantom_mma_update_f64(long n, int m, int iter, const double *x, c
             MmaScal dl;
             for (int i = 0; i < m; ++i)
                 dl.lam[i] = rhs[i];
+            trial_base = sc;
+            trial_dl = dl;
             const double dz = rhs[m];
             MMA_DISPATCH(m, mma_dir_k, dim3(nb), dv, sc, dl, epsi);
             FANTOM_LAUNCH_CHECK("mma_dir");
