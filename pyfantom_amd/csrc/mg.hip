@@ -111,30 +111,24 @@ __global__ __launch_bounds__(kBlock) void restrict2d(Grid gf, int dof, int norma
 
 // Fine value = average of its parent coarse values (each divided by its TW
 // when normalize); accumulate: xf += P ec, else xf = P ec; mask_f (optional):
-// constrained fine DOFs receive 0.  One thread per coarse node c (blockIdx.y
-// is the coarse plane): it loads the 2x2x2 coarse nodes c + {0,1}^3 once and
-// writes the up to 8 fine nodes 2c + o, o in {0,1}^3, whose parents are
-// c + o' with o' <= o -- 3 coarse loads per fine node instead of 24, and
-// each wave writes contiguous runs of z-node pairs.  Every fine value sums
-// its parents in the order x outer, y, z inner, as one thread per fine node
-// would, so results are unchanged bit for bit.
-template <typename T, int DOF>
-__global__ __launch_bounds__(kBlock) void prolong3d(Grid gf, int normalize, const uint8_t *__restrict__ mask_f,
-                                                  const T *__restrict__ ec, T *__restrict__ xf, int accumulate)
+// constrained fine DOFs receive 0.  prolong_row: the fine nodes 2c + (ox, OY,
+// oz) of coarse node c, whose parents are c + o' with o' <= o, from the
+// coarse nodes loaded once per thread; each wave writes contiguous runs of
+// z-node pairs.  Every fine value sums its parents in the order x outer, y,
+// z inner, as one thread per fine node would, so results are unchanged bit
+// for bit.
+template <typename T, int DOF, int OY>
+__device__ __forceinline__ void prolong_row(const Grid &gf, int normalize, const uint8_t *__restrict__ mask_f,
+                                            const T *__restrict__ ec, T *__restrict__ xf, int accumulate, int ic,
+                                            int jc, int kc)
 {
     const int nxc = gf.nelx / 2 + 1, nyc = gf.nely / 2 + 1, nzc = gf.nelz / 2 + 1;
-    const int pc = blockIdx.x * kBlock + threadIdx.x;
-    if (pc >= nxc * nzc)
-        return;
-    const int jc = blockIdx.y;
-    const int ic = pc / nzc;
-    const int kc = pc - ic * nzc;
     // parent values, clamped to the grid (a clamped copy is never used)
-    double ev[2][2][2][DOF];
+    double ev[2][OY + 1][2][DOF];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b)
+        for (int b = 0; b <= OY; ++b)
 #pragma unroll
             for (int d = 0; d < 2; ++d) {
                 const int ci = min(ic + a, nxc - 1), cj = min(jc + b, nyc - 1), ck = min(kc + d, nzc - 1);
@@ -151,28 +145,25 @@ __global__ __launch_bounds__(kBlock) void prolong3d(Grid gf, int normalize, cons
     // before the first store (stores to xf would otherwise order every later
     // load of xf behind them)
     const long plane = (long)gf.nx * gf.nz;
-    T xo[2][2][2][DOF];
-    int mo[2][2][2];
+    constexpr int oy = OY;
+    T xo[2][2][DOF];
+    int mo[2][2];
 #pragma unroll
     for (int ox = 0; ox < 2; ++ox)
 #pragma unroll
-        for (int oy = 0; oy < 2; ++oy)
+        for (int oz = 0; oz < 2; ++oz) {
+            const int i = min(2 * ic + ox, gf.nx - 1), j = min(2 * jc + oy, gf.ny - 1), k = min(2 * kc + oz, gf.nz - 1);
+            const long f = k + (long)gf.nz * i + plane * j;
 #pragma unroll
-            for (int oz = 0; oz < 2; ++oz) {
-                const int i = min(2 * ic + ox, gf.nx - 1), j = min(2 * jc + oy, gf.ny - 1),
-                          k = min(2 * kc + oz, gf.nz - 1);
-                const long f = k + (long)gf.nz * i + plane * j;
-#pragma unroll
-                for (int q = 0; q < DOF; ++q)
-                    xo[ox][oy][oz][q] = accumulate ? xf[f * DOF + q] : T(0);
-                mo[ox][oy][oz] = mask_f ? mask_f[f] : 0;
-            }
+            for (int q = 0; q < DOF; ++q)
+                xo[ox][oz][q] = accumulate ? xf[f * DOF + q] : T(0);
+            mo[ox][oz] = mask_f ? mask_f[f] : 0;
+        }
 #pragma unroll
     for (int ox = 0; ox < 2; ++ox)
 #pragma unroll
-        for (int oy = 0; oy < 2; ++oy)
-#pragma unroll
-            for (int oz = 0; oz < 2; ++oz) {
+        for (int oz = 0; oz < 2; ++oz) {
+            {
                 const int i = 2 * ic + ox, j = 2 * jc + oy, k = 2 * kc + oz;
                 if (i >= gf.nx || j >= gf.ny || k >= gf.nz)
                     continue;
@@ -200,15 +191,37 @@ __global__ __launch_bounds__(kBlock) void prolong3d(Grid gf, int normalize, cons
                 // product equals the reference's division
                 const double inv_count =
                     count == 8 ? 0.125 : count == 4 ? 0.25 : count == 2 ? 0.5 : count == 1 ? 1.0 : 0.0;
-                const int mf = mo[ox][oy][oz];
+                const int mf = mo[ox][oz];
 #pragma unroll
                 for (int q = 0; q < DOF; ++q) {
                     double v = acc[q] * inv_count;
                     if ((mf >> q) & 1)
                         v = 0.0;
-                    xf[f * DOF + q] = accumulate ? (T)((double)xo[ox][oy][oz][q] + v) : (T)v;
+                    xf[f * DOF + q] = accumulate ? (T)((double)xo[ox][oz][q] + v) : (T)v;
                 }
             }
+}
+}
+
+// One thread per coarse node c and fine plane 2 c_y + oy (blockIdx.y = 2 c_y
+// + oy): the plane's up to 4 fine nodes 2c + (ox, oy, oz) from the coarse
+// nodes they need (2x1x2 for oy = 0, 2x2x2 for oy = 1); half the registers of
+// one thread per whole 2x2x2 block, so more waves cover the memory latency.
+template <typename T, int DOF>
+__global__ __launch_bounds__(kBlock) void prolong3d(Grid gf, int normalize, const uint8_t *__restrict__ mask_f,
+                                                  const T *__restrict__ ec, T *__restrict__ xf, int accumulate)
+{
+    const int nxc = gf.nelx / 2 + 1, nzc = gf.nelz / 2 + 1;
+    const int pc = blockIdx.x * kBlock + threadIdx.x;
+    const int jc = blockIdx.y >> 1, oy = blockIdx.y & 1;
+    if (pc >= nxc * nzc || 2 * jc + oy >= gf.ny)
+        return;
+    const int ic = pc / nzc;
+    const int kc = pc - ic * nzc;
+    if (oy == 0)
+        prolong_row<T, DOF, 0>(gf, normalize, mask_f, ec, xf, accumulate, ic, jc, kc);
+    else
+        prolong_row<T, DOF, 1>(gf, normalize, mask_f, ec, xf, accumulate, ic, jc, kc);
 }
 
 template <typename T>
@@ -498,7 +511,7 @@ static unsigned nblk(long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 // the 3D transfers launch one workgroup row per node plane (grid y)
 static int check_planes(const Grid &gf)
 {
-    if (gf.ny > 65535 || (long)gf.nx * gf.nz > (1L << 30)) {
+    if (gf.ny > 65534 || (long)gf.nx * gf.nz > (1L << 30)) {   // grid.y: two per coarse plane
         set_error("3D transfer: %d node planes of %d x %d nodes exceed the launch limits", gf.ny, gf.nx, gf.nz);
         return kStatusBadArg;
     }
@@ -536,7 +549,7 @@ static int prolong_t(const Grid &gf, int dim, int dof, int normalize, const uint
     if (dim == 3) {
         if (int st = check_planes(gf))
             return st;
-        const dim3 grid(nblk((long)(gf.nelx / 2 + 1) * (gf.nelz / 2 + 1)), gf.nely / 2 + 1);
+        const dim3 grid(nblk((long)(gf.nelx / 2 + 1) * (gf.nelz / 2 + 1)), 2 * (gf.nely / 2 + 1));
         if (dof == 3)
             hipLaunchKernelGGL((prolong3d<T, 3>), grid, dim3(kBlock), 0, s, gf, normalize, mask_f, ec, xf, accumulate);
         else if (dof == 1)
