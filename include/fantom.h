@@ -215,6 +215,16 @@ int fantom_filter_adj_f64(int nelx, int nely, int nelz, int ntap, const int32_t 
 long fantom_oc_partials(long n);
 int fantom_oc_update_f64(long n, const double *x, const double *df, const double *dg, double vf, double move,
                          double *ocp, double *state, double *partials, double *x_new, void *stream);
+/* Allocation-free OC update.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same kernels and
+ * results as fantom_oc_update_f64, with ocp, state and partials held in a
+ * library-owned device array (33.7 MB per device, kept until the process
+ * ends); n <= 4,194,304, larger n returns status 1.  result: device double[2]
+ * = {final Lagrange multiplier, sum (x - x_new)^2}.  Calls that overlap on
+ * one device share that array and race: order them on one stream. */
+int fantom_oc_update_noalloc_f64(long n, const double *x, const double *df, const double *dg, double vf,
+                                 double move, double *x_new, double *result, void *stream);
 
 /* ---- A17: MMA update ----------------------------------------------------------
  * Replaces mmasub/subsolv (mma/CPU/_mma.py:32-165, 288-528) as driven by

@@ -113,6 +113,7 @@ SIGNATURES = {
                                        _vp]),
     "fantom_stencil_partials": (_c_long, [_c_int, _c_int, _c_int]),
     "fantom_oc_update_f64": (_c_int, [_c_long, _vp, _vp, _vp, _c_double, _c_double, _vp, _vp, _vp, _vp, _vp]),
+    "fantom_oc_update_noalloc_f64": (_c_int, [_c_long, _vp, _vp, _vp, _c_double, _c_double, _vp, _vp, _vp]),
     "fantom_mma_workspace_size": (_c_long, [_c_long, _c_int]),
     "fantom_mma_update_f64": (_c_int, [_c_long, _c_int, _c_int, _vp, _vp, _vp, _c_double, _c_double, _vp, _vp, _vp,
                                        _vp, _vp, _c_double, _c_double, _vp, _vp, _vp, _c_double, _c_int, _vp, _vp,

@@ -15,7 +15,9 @@
 // comparisons), so the sequence of l_mid values is the reference's while
 // taking L times fewer passes and no host synchronisation.
 #include <cfloat>
+#include <mutex>
 
+#include "../../include/fantom.h"
 #include "common.hpp"
 #include "fantom_internal.hpp"
 #include "vec.hpp"
@@ -222,3 +224,91 @@ int launch_oc_apply(long n, const double *x, const double *ocp, double move, con
 }
 
 }  // namespace fantom
+
+// ---- allocation-free variant -------------------------------------------------
+// fantom_oc_update_noalloc_f64 runs the same kernels as fantom_oc_update_f64
+// with its three scratch buffers (bisection state, per-block partial sums,
+// the ocP vector) in one file-scope device array instead of caller memory.
+// Each device holds its own copy of the array for the life of the process;
+// host code only ever uses the address hipGetSymbolAddress reports for the
+// current device, never the array's name.  Every call on a device shares that
+// device's array, so calls that overlap on one device (two streams, two host
+// threads) race and must be ordered by the caller.
+namespace fantom {
+
+constexpr long kOcNoAllocMaxN = 1L << 22;                 // 4,194,304 design variables
+constexpr long kOcScrState = 0;                           // double[16]
+constexpr long kOcScrPartials = 16;                       // double[16 * 1024]
+constexpr long kOcScrOcp = kOcScrPartials + 16 * 1024;    // double[kOcNoAllocMaxN]
+constexpr long kOcScrDoubles = kOcScrOcp + kOcNoAllocMaxN;  // 33.7 MB in all
+
+static __device__ __align__(256) double scratch[kOcScrDoubles];
+
+constexpr int kOcMaxDevices = 64;
+static double *g_oc_scratch[kOcMaxDevices];
+static std::mutex g_oc_scratch_mu;
+
+// Device address of this device's scratch array, looked up once per device
+// with that device current (the caller's current device).
+static int oc_scratch_ptr(double **out)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) {
+        set_error("oc_update_noalloc: hipGetDevice: %s", hipGetErrorString(e));
+        return kStatusHipError;
+    }
+    FANTOM_CHECK_ARG(dev >= 0 && dev < kOcMaxDevices, "oc_update_noalloc: device %d out of range", dev);
+    std::lock_guard<std::mutex> lock(g_oc_scratch_mu);
+    if (!g_oc_scratch[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_error("oc_update_noalloc: hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+            return kStatusHipError;
+        }
+        g_oc_scratch[dev] = static_cast<double *>(p);
+    }
+    *out = g_oc_scratch[dev];
+    return kStatusOk;
+}
+
+__global__ void oc_lambda_k(const double *__restrict__ state, double *__restrict__ result)
+{
+    if (threadIdx.x == 0)
+        result[0] = state[OC_LMID];
+}
+
+}  // namespace fantom
+
+extern "C" int fantom_oc_update_noalloc_f64(long n, const double *x, const double *df, const double *dg, double vf,
+                                            double move, double *x_new, double *result, void *stream)
+{
+    using namespace fantom;
+    FANTOM_CHECK_ARG(n > 0 && x && df && dg && x_new && result, "bad operands");
+    FANTOM_CHECK_ARG(n <= kOcNoAllocMaxN, "oc_update_noalloc: n = %ld exceeds the scratch capacity of %ld", n,
+                     kOcNoAllocMaxN);
+    FANTOM_CHECK_ARG(x != x_new, "x and x_new must not alias");
+    double *scr = nullptr;
+    int st = oc_scratch_ptr(&scr);
+    if (st)
+        return st;
+    double *state = scr + kOcScrState, *partials = scr + kOcScrPartials, *ocp = scr + kOcScrOcp;
+    hipStream_t s = as_stream(stream);
+    // oc_prepare/oc_init write every state slot and partial they later read
+    st = launch_oc_prepare(n, x, df, dg, ocp, state, partials, s);
+    if (st)
+        return st;
+    st = launch_oc_bisect(n, x, ocp, dg, move, vf, state, partials, 24, s);
+    if (st)
+        return st;
+    st = launch_oc_apply(n, x, ocp, move, state, x_new, partials, s);
+    if (st)
+        return st;
+    st = launch_finalize(partials, oc_partials(n), result, FIN_STORE, 1, s);
+    if (st)
+        return st;
+    hipLaunchKernelGGL(oc_lambda_k, dim3(1), dim3(64), 0, s, state, result);
+    FANTOM_LAUNCH_CHECK("oc_update_noalloc");
+    return kStatusOk;
+}
