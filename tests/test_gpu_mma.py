@@ -16,7 +16,11 @@ def _torch():
     return torch
 
 
-@pytest.mark.parametrize("n,m,it,seed", [(1000, 1, 1, 0), (5000, 1, 4, 1), (3000, 2, 3, 2), (200000, 1, 5, 3)])
+# n > 1024 blocks x 256 threads = 262,144 puts several elements on every
+# thread (the grid-stride regime of the 14M-variable bridge); m = 3 and m = 8
+# cover the larger block-reduction records of the dual system
+@pytest.mark.parametrize("n,m,it,seed", [(1000, 1, 1, 0), (5000, 1, 4, 1), (3000, 2, 3, 2), (200000, 1, 5, 3),
+                                         (600000, 1, 3, 4), (600000, 3, 4, 5), (400000, 8, 2, 6)])
 def test_mma_update_vs_oracle(n, m, it, seed):
     from oracle import mma_oracle as M
     from pyfantom_amd import _lib
