@@ -603,6 +603,12 @@ extern "C" int fantom_mma_update_f64(long n, int m, int iter, const double *x, c
                 so[i] = sl[i];
             }
             const double zo = z, zeto = zet;
+            // a non-finite residual would skip the line search below and leave
+            // the trial arrays unwritten: stop with an error instead
+            if (!std::isfinite(rnorm) || !std::isfinite(rmax)) {
+                set_error("MMA subproblem residual is not finite (%g)", rnorm);
+                return kStatusHipError;
+            }
             double resinew = 2.0 * rnorm, rmx = rmax;
             int itto = 0;
             while (resinew > rnorm && itto < 50) {
@@ -620,13 +626,16 @@ extern "C" int fantom_mma_update_f64(long n, int m, int iter, const double *x, c
                     return st;
                 steg /= 2.0;
             }
-            // accept the last trial point (as the reference does, also after 50 halvings)
-            std::swap(dv.xs, dv.xt);
-            std::swap(dv.xsi, dv.xsit);
-            std::swap(dv.eta, dv.etat);
-            rnorm = resinew;
-            rmax = rmx;
-            have_sums = true;
+            // accept the last trial point (as the reference does, also after 50
+            // halvings); the trial arrays hold a point only once a trial ran
+            if (itto > 0) {
+                std::swap(dv.xs, dv.xt);
+                std::swap(dv.xsi, dv.xsit);
+                std::swap(dv.eta, dv.etat);
+                rnorm = resinew;
+                rmax = rmx;
+                have_sums = true;
+            }
         }
         epsi *= 0.1;
     }

@@ -446,7 +446,9 @@ def fpcg_solve(kernel, f, precond, tol=1e-8, maxiter=500, x0=None, fallback=None
         vec.ku(p, q)
         pq = vec.dot(p, q)
         breakdown = not ok(pq)
-        if not breakdown:
+        if breakdown:
+            hist.append(hist[-1])   # one history entry per iteration keeps the stall test aligned
+        else:
             alpha = rz / pq
             vec.axpby(alpha, p, 1.0, x)
             vec.axpby(-alpha, q, 1.0, r)

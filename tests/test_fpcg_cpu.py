@@ -37,6 +37,8 @@ class _Precond:
         self.calls += 1
         if self.kind == "jacobi":
             z.copy_(r / self.d)
+        elif self.kind == "identity":
+            z.copy_(r)
         else:
             z.zero_()
 
@@ -82,3 +84,18 @@ def test_fpcg_breakdown_hands_over_to_fallback():
     x, info = fpcg_solve(None, f, _Precond(A, "zero"), tol=1e-8, maxiter=50, fallback=fb)
     assert info["fallback"]
     assert torch.allclose(x, torch.linalg.solve(A, f), atol=1e-10)
+
+
+def test_fpcg_pkp_breakdown_keeps_history_aligned():
+    """An indefinite operator with the right-hand side on its negative
+    eigenvector makes p.Kp < 0 on every iteration; without a fallback each
+    one restarts, and the residual history keeps one entry per iteration so
+    the stall test (which looks stall_window entries back) stays in range."""
+    n = 30
+    A = torch.diag(torch.cat([torch.tensor([-10.0]), torch.linspace(1.0, 5.0, n - 1, dtype=torch.float64)]))
+    f = torch.full((n,), 0.01, dtype=torch.float64)
+    f[0] = 1.0
+    x, info = fpcg_solve(None, f, _Precond(A, "identity"), tol=1e-12, maxiter=12, stall_window=5)
+    assert info["iterations"] == 12 and info["restarts"] == 12
+    assert len(info["history"]) == info["iterations"] + 1
+    assert np.isfinite(x.numpy()).all()
