@@ -35,7 +35,7 @@ METRIC = "s/SIMP iter to 1e-8 rel res, mixed+learned vs FP64 GMG-PCG; GDOF/s K·
 
 CONFIGS = {
     "c2": dict(nel=(128, 64, 64), n_level=4, name="cantilever_128x64x64"),
-    "c3": dict(nel=(256, 128, 128), n_level=5, name="cantilever_256x128x128"),
+    "c3": dict(nel=(256, 128, 128), n_level=4, name="cantilever_256x128x128"),
     "small": dict(nel=(32, 16, 16), n_level=3, name="cantilever_32x16x16"),
     # combined run: bridge, MMA, filter r = 2, volume fraction 0.15
     "c5": dict(nel=(384, 192, 192), n_level=6, name="bridge_384x192x192", kind="bridge", optimizer="mma",

@@ -7,6 +7,11 @@
 // exactly as the reference does; normalize=0 gives the plain trilinear pair
 // P_lin^T / P_lin used by the rediscretised hierarchy (the V-cycle with an
 // exact coarse solve is invariant to that diagonal scaling).
+#include <rocblas/rocblas.h>
+
+#include <algorithm>
+#include <mutex>
+
 #include "common.hpp"
 #include "fantom_internal.hpp"
 
@@ -645,9 +650,216 @@ int launch_dense_assemble_f32E(const Grid &g, const double *K0, const float *E, 
     return dense_assemble_t<float>(g, 3, K0, E, mask, A, s);
 }
 
+// ---- large-n inverse: 128-wide pivot blocks, trailing update as one GEMM ----------
+// With K the pivot block and P = A_KK^-1, one Gauss-Jordan block step is
+//   A <- Ahat - Xhat Rhat,
+// Ahat = A with block row K and block column K set to zero, Xhat (n x b) =
+// A_{:,K} with rows K replaced by -I, Rhat (b x n) = P A_{K,:} with columns K
+// replaced by P.  Rows K then hold P A_KR (and P), column K holds -A_RK P and
+// the rest A_RR - A_RK P A_KR, the same step as gj_pivot/gj_update above; the
+// product is a plain dense GEMM, left to rocBLAS.  Each step streams the
+// matrix once, so 128-wide blocks cost a quarter of the 32-wide passes.
+constexpr int kGJB = 128;
+
+// One 1024-thread workgroup inverts the b x b pivot block (b <= 128) in
+// registers: thread t holds row t / 8, columns c = t % 8 + 8 j (j < 16; the
+// interleave keeps the LDS reads of a pivot row free of bank conflicts).
+// Each step eliminates four pivots K4 at once: with Rt the pivot rows
+// A_{K4,:} whose K4 columns are replaced by I, D = A_{K4,K4} (SPD: a
+// principal block of a Schur complement of an SPD matrix) and the working
+// values of the pivot rows and columns set to zero, every entry takes
+// v += g . Rt_c (four FMAs), with g = (D^-1)_{r - k, :} in a pivot row and
+// -A_{r,K4} D^-1 elsewhere.  Every thread inverts the 4 x 4 block itself; Rt
+// and A_{:,K4} go through double-buffered LDS, one barrier per step.
+constexpr int kPivThreads = 1024;
+constexpr int kPivGroups = 8;                          // threads per row
+constexpr int kPivCols = kGJB / kPivGroups;            // 16 columns per thread
+
+__global__ __launch_bounds__(kPivThreads) void gj_pivot128(int n, int k0, int b, const double *__restrict__ A,
+                                                          double *__restrict__ P)
+{
+    __shared__ double rowb[2][4][kGJB];
+    __shared__ double colb[2][kGJB][4];
+    const int t = threadIdx.x, r = t / kPivGroups, cg = t % kPivGroups;
+    double v[kPivCols];
+#pragma unroll
+    for (int j = 0; j < kPivCols; ++j) {
+        const int c = cg + kPivGroups * j;
+        v[j] = (r < b && c < b) ? A[(long)(k0 + r) * n + k0 + c] : (r == c ? 1.0 : 0.0);
+    }
+#pragma unroll 1
+    for (int q = 0; q < kGJB / 4; ++q) {
+        const int k = 4 * q, buf = q & 1;
+        // pivot column k + i is column cg = k % 8 + i at slot jq of its threads
+        const int jq = k / kPivGroups, ic = cg - k % kPivGroups;
+        const bool pcol = ic >= 0 && ic < 4;
+        const bool prow = (r >> 2) == q;
+        if (pcol) {
+            double x = 0.0;
+#pragma unroll
+            for (int j = 0; j < kPivCols; ++j)
+                x = j == jq ? v[j] : x;
+            colb[buf][r][ic] = x;
+#pragma unroll
+            for (int j = 0; j < kPivCols; ++j)
+                v[j] = j == jq ? 0.0 : v[j];
+        }
+        if (prow) {
+            const int i = r - k;
+#pragma unroll
+            for (int j = 0; j < kPivCols; ++j) {
+                // the K4 columns of Rt are the identity
+                rowb[buf][i][cg + kPivGroups * j] = v[j] + ((pcol && j == jq && ic == i) ? 1.0 : 0.0);
+                v[j] = 0.0;
+            }
+        }
+        __syncthreads();
+        double d[4][4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int i2 = 0; i2 < 4; ++i2)
+                d[i][i2] = colb[buf][k + i][i2];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {   // in-place Gauss-Jordan inverse of D
+            const double pm = 1.0 / d[m][m];
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                if (j != m)
+                    d[m][j] *= pm;
+            d[m][m] = pm;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                if (i == m)
+                    continue;
+                const double f = d[i][m];
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    if (j != m)
+                        d[i][j] -= f * d[m][j];
+                d[i][m] = -f * pm;
+            }
+        }
+        double g[4];
+        if (prow) {
+#pragma unroll
+            for (int i2 = 0; i2 < 4; ++i2)
+                g[i2] = (r - k == 0) ? d[0][i2] : (r - k == 1) ? d[1][i2] : (r - k == 2) ? d[2][i2] : d[3][i2];
+        } else {
+            double a[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                a[i] = colb[buf][r][i];
+#pragma unroll
+            for (int i2 = 0; i2 < 4; ++i2)
+                g[i2] = -(a[0] * d[0][i2] + a[1] * d[1][i2] + a[2] * d[2][i2] + a[3] * d[3][i2]);
+        }
+        // compiler-only fences keep the row loads of one rank-1 part at a
+        // time in registers (hoisting all of them spills)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            asm volatile("" ::: "memory");
+#pragma unroll
+            for (int j = 0; j < kPivCols; ++j)
+                v[j] = fma(g[i], rowb[buf][i][cg + kPivGroups * j], v[j]);
+        }
+        asm volatile("" ::: "memory");
+    }
+    if (r < b) {
+#pragma unroll
+        for (int j = 0; j < kPivCols; ++j) {
+            const int c = cg + kPivGroups * j;
+            if (c < b)
+                P[r * kGJB + c] = v[j];
+        }
+    }
+}
+
+// Xhat (n x b, row stride b) and the P columns of Rhat (after the GEMM that
+// forms P A_{K,:})
+__global__ __launch_bounds__(256) void gj_panel_x(int n, int k0, int b, const double *__restrict__ A,
+                                                 const double *__restrict__ P, double *__restrict__ X,
+                                                 double *__restrict__ R)
+{
+    const long q = (long)blockIdx.x * 256 + threadIdx.x;
+    if (q < (long)n * b) {
+        const int i = (int)(q / b), c = (int)(q % b);
+        const bool inK = i >= k0 && i < k0 + b;
+        X[q] = inK ? (i - k0 == c ? -1.0 : 0.0) : A[(long)i * n + k0 + c];
+    }
+    if (q < (long)b * b) {   // Rhat[t][k0 + c] = P[t][c]
+        const int t = (int)(q / b), c = (int)(q % b);
+        R[(long)t * n + k0 + c] = P[t * kGJB + c];
+    }
+}
+
+// Ahat: block row K and block column K of A set to zero
+__global__ __launch_bounds__(256) void gj_zero_k(int n, int k0, int b, double *__restrict__ A)
+{
+    const long q = (long)blockIdx.x * 256 + threadIdx.x;
+    if (q < (long)n * b) {
+        A[(q / b) * n + k0 + q % b] = 0.0;   // column block K: b consecutive entries per row
+        A[(long)k0 * n + q] = 0.0;           // row block K: b contiguous rows
+    }
+}
+
+static std::mutex g_blas_mu;
+static rocblas_handle g_blas[64];
+
+static int spd_inverse_blocked(int n, double *A, double *work, hipStream_t s)
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) {
+        set_error("spd_inverse: no current device");
+        return kStatusHipError;
+    }
+    std::lock_guard<std::mutex> lock(g_blas_mu);
+    if (!g_blas[dev] && rocblas_create_handle(&g_blas[dev]) != rocblas_status_success) {
+        g_blas[dev] = nullptr;
+        set_error("spd_inverse: rocblas_create_handle failed");
+        return kStatusHipError;
+    }
+    rocblas_handle hb = g_blas[dev];
+    if (rocblas_set_stream(hb, s) != rocblas_status_success ||
+        rocblas_set_pointer_mode(hb, rocblas_pointer_mode_host) != rocblas_status_success) {
+        set_error("spd_inverse: rocblas stream / pointer mode");
+        return kStatusHipError;
+    }
+    double *X = work, *R = work + (size_t)n * kGJB, *P = R + (size_t)n * kGJB;
+    const double one = 1.0, minus_one = -1.0;
+    for (int k0 = 0; k0 < n; k0 += kGJB) {
+        const int b = std::min(kGJB, n - k0);
+        const long nb = (long)n * b;
+        hipLaunchKernelGGL(gj_pivot128, dim3(1), dim3(kPivThreads), 0, s, n, k0, b, A, P);
+        FANTOM_LAUNCH_CHECK("spd_inverse pivot");
+        // Rhat = P A_{K,:}: row-major b x n, i.e. Rhat^T = A_{K,:}^T P^T with
+        // A_{K,:}^T (n x b, ld n) and P^T (b x b, ld kGJB) column-major
+        static const double zero = 0.0;
+        if (rocblas_dgemm(hb, rocblas_operation_none, rocblas_operation_none, n, b, b, &one, A + (size_t)k0 * n, n,
+                          P, kGJB, &zero, R, n) != rocblas_status_success) {
+            set_error("spd_inverse: rocblas_dgemm (panel) failed");
+            return kStatusHipError;
+        }
+        hipLaunchKernelGGL(gj_panel_x, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, s, n, k0, b, A, P, X, R);
+        hipLaunchKernelGGL(gj_zero_k, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, s, n, k0, b, A);
+        FANTOM_LAUNCH_CHECK("spd_inverse block step");
+        // row-major A (n x n) -= Xhat (n x b) Rhat (b x n), i.e. in rocBLAS's
+        // column-major terms A^T -= Rhat^T Xhat^T with Rhat^T (n x b, ld n)
+        // and Xhat^T (b x n, ld b)
+        if (rocblas_dgemm(hb, rocblas_operation_none, rocblas_operation_none, n, n, b, &minus_one, R, n, X, b, &one,
+                          A, n) != rocblas_status_success) {
+            set_error("spd_inverse: rocblas_dgemm failed");
+            return kStatusHipError;
+        }
+    }
+    return kStatusOk;
+}
+
 int launch_spd_inverse(int n, double *A, double *work, hipStream_t s)
 {
-    // work: n*n doubles (the second buffer) + kGJ*kGJ (the pivot-block inverse)
+    // work: n*n doubles + kGJ*kGJ; the blocked path uses 2 n kGJB + kGJB^2 of it
+    if (n > 4 * kGJB)
+        return spd_inverse_blocked(n, A, work, s);
     const unsigned nt = (unsigned)((n + kGJ - 1) / kGJ);
     double *src = A, *dst = work, *pinv = work + (size_t)n * n;
     for (unsigned kb = 0; kb < nt; ++kb) {

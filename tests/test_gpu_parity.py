@@ -238,7 +238,10 @@ def test_coarsen(oracle):
         assert np.array_equal(_np(mc), ref)
 
 
-@pytest.mark.parametrize("nel", [(4, 2, 2), (8, 4, 4), (8, 4)])
+# (10, 6, 6) and (16, 8, 8): 1,617 and 4,131 DOF take the 128-wide blocked path
+# with its GEMM update (ragged last block), the latter being the coarsest grid
+# of the 256x128x128 cantilever after four coarsenings
+@pytest.mark.parametrize("nel", [(4, 2, 2), (8, 4, 4), (8, 4), (10, 6, 6), (16, 8, 8)])
 def test_dense_coarse_solve(oracle, nel):
     from pyfantom_amd import _lib
     torch = _torch()
