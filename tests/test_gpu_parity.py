@@ -319,21 +319,24 @@ def test_oc_update(oracle, n, seed):
 def test_simp(oracle):
     from pyfantom_amd import _lib
     torch = _torch()
-    rho = np.random.default_rng(0).uniform(0, 1, 10000)
+    # random densities plus the special points 0, 0.5, 1 and an overshoot
+    rho = np.concatenate([np.random.default_rng(0).uniform(0, 1, 10000), [0.0, 0.5, 1.0, 1.2, 0.1]])
     rd = _dev(rho)
     E = torch.empty_like(rd)
     dE = torch.empty_like(rd)
-    for hv in (0, 1):
-        _lib.call("fantom_simp_f64", len(rho), rd.data_ptr(), 0, 3.0, 1e-6, 1.0, hv, 2.0, 0.5, E.data_ptr(),
-                  dE.data_ptr())
-        assert _rel(_np(E), oracle.penalize_reference(rho, 3.0, 1e-6, 1.0, bool(hv))) <= 1e-14
-        assert _rel(_np(dE), oracle.penalize_grad_reference(rho, 3.0, 1.0, bool(hv))) <= 1e-14
+    # (void, E_mul): the clips below and above, E_mul > 1 and E_mul < void
+    for void, emul in ((1e-6, 1.0), (0.01, 2.0), (0.01, 0.001)):
+        for hv in (0, 1):
+            _lib.call("fantom_simp_f64", len(rho), rd.data_ptr(), 0, 3.0, void, emul, hv, 2.0, 0.5, E.data_ptr(),
+                      dE.data_ptr())
+            assert _rel(_np(E), oracle.penalize_reference(rho, 3.0, void, emul, bool(hv))) <= 1e-14
+            assert _rel(_np(dE), oracle.penalize_grad_reference(rho, 3.0, emul, bool(hv))) <= 1e-14
     _lib.call("fantom_simp_f64", len(rho), rd.data_ptr(), 1, 3.0, 1e-9, 1.0, 0, 2.0, 0.5, E.data_ptr(), dE.data_ptr())
     assert _rel(_np(E), oracle.penalize_modified(rho, 3.0, 1.0, 1e-9)) <= 1e-14
     assert _rel(_np(dE), oracle.penalize_grad_modified(rho, 3.0, 1.0, 1e-9)) <= 1e-14
 
 
-def _galerkin_ref(oracle, nel, E, con):
+def _galerkin_ref(oracle, nel, E, con, levels=2):
     """scipy Galerkin hierarchy: A_{l+1}^U = P^T A_l^U P (unconstrained), and
     the coarse masks by the coincident-node rule."""
     import scipy.sparse as sps
@@ -341,7 +344,7 @@ def _galerkin_ref(oracle, nel, E, con):
     AU = [oracle.assemble(K0, oracle.elements(nel), E, int(np.prod([n + 1 for n in nel])) * 3)]
     cons = [con]
     nels = [tuple(nel)]
-    for l in range(2):
+    for l in range(levels):
         cur = nels[-1]
         P = oracle.transfer_matrix(cur, 3, False).T.tocsr()
         AU.append((P.T @ AU[-1] @ P).tocsr())
@@ -376,7 +379,7 @@ def test_galerkin_levels(oracle, nel):
     rng = np.random.default_rng(5)
     E = rng.uniform(0.01, 1, int(np.prod(nel))) ** 3
     _, el, _, _, con = _case(oracle, nel, bc="clamp")
-    apply, nels, cons = _galerkin_ref(oracle, nel, E, con)
+    apply, nels, cons = _galerkin_ref(oracle, nel, E, con, levels=3 if all(n % 8 == 0 for n in nel) else 2)
     G = np.zeros(8 * 576)
     B = np.zeros(576)
     Gd = np.zeros(8 * 24)
@@ -424,10 +427,14 @@ def test_galerkin_levels(oracle, nel):
         g3 = tuple(n // 2 for n in g2)
         Ke3 = torch.empty(int(np.prod(g3)) * 576, dtype=torch.float64, device="cuda")
         _lib.call("fantom_gal_build_f64", *g3, 1, Gd_d.data_ptr(), None, Ke.data_ptr(), Ke3.data_ptr())
-        import scipy.sparse as sps
-        # A3 = P^T A2^U P directly from explicit element matrices: compare energies on a random vector
-        torch.cuda.synchronize()
-        assert torch.isfinite(Ke3).all()
+        # A3 = P^T A2^U P of the scipy hierarchy, with the level-3 Dirichlet rows
+        n3 = int(np.prod([n + 1 for n in g3])) * 3
+        u3 = rng.standard_normal(n3)
+        m3 = _dev(_mask_from_con(cons[3], 3))
+        u3d = _dev(u3)
+        y3 = torch.empty_like(u3d)
+        _lib.call("fantom_elem_ku_f64", *g3, Ke3.data_ptr(), m3.data_ptr(), u3d.data_ptr(), y3.data_ptr())
+        assert _rel(_np(y3), apply(3, u3)) <= 1e-12
 
 
 @pytest.mark.parametrize("nel", [(16, 8, 12), (32, 16, 16)])

@@ -120,6 +120,76 @@ def test_k0_2d_matches_top88(oracle):
     assert found
 
 
+def _top3d_ke(nu):
+    """The published unit-cube hex8 KE of top3d (Liu & Tovey 2014, lk_H8):
+    14 coefficients k = A^T [1, nu] / 144 arranged in six 6x6 blocks, times
+    1 / ((1 + nu)(1 - 2 nu))."""
+    A = np.array([[32, 6, -8, 6, -6, 4, 3, -6, -10, 3, -3, -3, -4, -8],
+                  [-48, 0, 0, -24, 24, 0, 0, 0, 12, -12, 0, 12, 12, 12]], float)
+    k = np.concatenate([[0.0], A.T @ np.array([1.0, nu]) / 144.0])   # 1-based as published
+    B = lambda rows: np.array([[k[i] for i in r] for r in rows])
+    K1 = B([[1, 2, 2, 3, 5, 5], [2, 1, 2, 4, 6, 7], [2, 2, 1, 4, 7, 6], [3, 4, 4, 1, 8, 8], [5, 6, 7, 8, 1, 2],
+            [5, 7, 6, 8, 2, 1]])
+    K2 = B([[9, 8, 12, 6, 4, 7], [8, 9, 12, 5, 3, 5], [10, 10, 13, 7, 4, 6], [6, 5, 11, 9, 2, 10],
+            [4, 3, 5, 2, 9, 12], [11, 4, 6, 12, 10, 13]])
+    K3 = B([[6, 7, 4, 9, 12, 8], [7, 6, 4, 10, 13, 10], [5, 5, 3, 8, 12, 9], [9, 10, 2, 6, 11, 5],
+            [12, 13, 10, 11, 6, 4], [2, 12, 9, 4, 5, 3]])
+    K4 = B([[14, 11, 11, 13, 10, 10], [11, 14, 11, 12, 9, 8], [11, 11, 14, 12, 8, 9], [13, 12, 12, 14, 7, 7],
+            [10, 9, 8, 7, 14, 11], [10, 8, 9, 7, 11, 14]])
+    K5 = B([[1, 2, 8, 3, 5, 4], [2, 1, 8, 4, 6, 11], [8, 8, 1, 5, 11, 6], [3, 4, 5, 1, 8, 2], [5, 6, 11, 8, 1, 8],
+            [4, 11, 6, 2, 8, 1]])
+    K6 = B([[14, 11, 7, 13, 10, 12], [11, 14, 7, 12, 9, 2], [7, 7, 14, 10, 2, 9], [13, 12, 10, 14, 7, 11],
+            [10, 9, 2, 7, 14, 7], [12, 2, 9, 11, 7, 14]])
+    KE = np.block([[K1, K2, K3, K4], [K2.T, K5, K6, K3.T], [K3.T, K6, K5.T, K2.T], [K4, K3, K2, K1.T]])
+    return KE / ((nu + 1) * (1 - 2 * nu))
+
+
+@pytest.mark.parametrize("nu", [0.3, 0.25, 0.45])
+def test_k0_3d_matches_top3d(oracle, nu):
+    """3D K0 (exact rational Gauss integration) equals top3d's published KE
+    entry for entry: top3d's local node order is the reference's
+    (physics/LinearElasticity.py:700-834 with Gauss points at +-1/sqrt(3))."""
+    K = oracle.k0_exact(3, nu)
+    T = _top3d_ke(nu)
+    assert np.abs(K - T).max() <= 1e-15 * np.abs(T).max()
+
+
+def test_simp_law_special_values(oracle):
+    """Hand values of both SIMP laws (Problem/CPU/MinimumCompliance.py:358-377):
+    reference law E = max(clip(H(rho)^p, void, 1) * E_mul, void) without and
+    with the Heaviside projection (H(0) = 0, H(eta) = eta for eta = 0.5,
+    H(1) = 1), and the modified law Emin + rho^p (E0 - Emin)."""
+    rho = np.array([0.0, 0.5, 1.0, 0.1, 1.2])
+    # p = 3, void 1e-6, E_mul 1
+    np.testing.assert_allclose(oracle.penalize_reference(rho, 3.0, 1e-6, 1.0, heavyside=False),
+                               [1e-6, 0.125, 1.0, 1e-3, 1.0], rtol=1e-15)
+    # E_mul = 2, void = 0.01: rho = 0.1 -> 1e-3 clipped up to void before the scaling
+    np.testing.assert_allclose(oracle.penalize_reference(rho, 3.0, 0.01, 2.0, heavyside=False),
+                               [0.02, 0.25, 2.0, 0.02, 2.0], rtol=1e-15)
+    # E_mul below void: the second clip raises everything to void
+    np.testing.assert_allclose(oracle.penalize_reference(rho, 3.0, 0.01, 0.001, heavyside=False),
+                               [0.01] * 5, rtol=1e-15)
+    # p = 1 is the plain clip
+    np.testing.assert_allclose(oracle.penalize_reference(rho, 1.0, 0.05, 1.0, heavyside=False),
+                               [0.05, 0.5, 1.0, 0.1, 1.0], rtol=1e-15)
+    # Heaviside, beta = 2 and 8, eta = 0.5
+    for beta in (2.0, 8.0):
+        e = oracle.penalize_reference(np.array([0.0, 0.5, 1.0]), 3.0, 1e-6, 1.0, heavyside=True, beta=beta)
+        np.testing.assert_allclose(e, [1e-6, 0.125, 1.0], rtol=1e-14)
+    np.testing.assert_allclose(oracle.penalize_modified(np.array([0.0, 0.5, 1.0]), 3.0, 1.0, 1e-9),
+                               [1e-9, 1e-9 + (1 - 1e-9) / 8, 1.0], rtol=1e-15)
+    np.testing.assert_allclose(oracle.penalize_modified(np.array([0.0, 0.5, 1.0]), 2.0, 10.0, 0.1),
+                               [0.1, 0.1 + 9.9 / 4, 10.0], rtol=1e-15)
+    # gradients: p H^(p-1) H' E_mul (the clip is ignored); H' = beta (1 - tanh^2(beta (rho - eta))) / den
+    np.testing.assert_allclose(oracle.penalize_grad_reference(np.array([0.5, 1.0]), 3.0, 2.0, heavyside=False),
+                               [1.5, 6.0], rtol=1e-15)
+    den = 2 * np.tanh(1.0)
+    np.testing.assert_allclose(oracle.penalize_grad_reference(np.array([0.5]), 3.0, 1.0, heavyside=True, beta=2.0),
+                               [3 * 0.25 * 2.0 / den], rtol=1e-14)
+    np.testing.assert_allclose(oracle.penalize_grad_modified(np.array([0.5]), 3.0, 1.0, 1e-9),
+                               [0.75 * (1 - 1e-9)], rtol=1e-15)
+
+
 def test_product_k0_matches_exact(oracle):
     """Product host K0 (Gauss quadrature) equals exact integration, 3D and 2D,
     unit and anisotropic spacing."""
