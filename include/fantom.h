@@ -324,6 +324,21 @@ int fantom_mg_destroy(void *handle);
  * times level_boost^level on each level.  The weight lives in a device
  * scalar, so batched and graph-replayed iterations follow it. */
 int fantom_mg_set_omega_schedule(void *handle, int adaptive, double min_omega, double level_boost);
+/* Smoother of the V-cycle: kind 0 = the weighted Jacobi sweeps above; kind 1
+ * = the fourth-kind Chebyshev smoother of the reference's CPU MultiGrid
+ * (_chebyshev_smoother / _estimate_rho, solvers/CPU/_solvers.py:884-954,
+ * left commented out there at :1045, :1066, :1088): n_smooth Jacobi sweeps
+ * with the weights fantom_chebyshev4_weights(n_smooth) / rho, rho the
+ * spectral radius of D^-1 A on each level from power iterations at setup
+ * (iters_first at the first setup, iters warm-started after; 0 keeps the
+ * previous estimate), multiplied by rho_scale.  Replaces the weight
+ * schedule; setting a schedule switches back to Jacobi.  smoother_rho reads
+ * the current estimates of the n_coarse smoothed levels. */
+int fantom_mg_set_smoother(void *handle, int kind, double rho_scale, int iters_first, int iters);
+int fantom_mg_smoother_rho(void *handle, double *rho, int n);
+/* Host helper: the n weights 1 / t_i (ascending) of the fourth-kind
+ * Chebyshev smoother of degree n, t_i = (1 - cos(2 i pi / (2n + 1))) / 2. */
+int fantom_chebyshev4_weights(int n, double *w);
 int fantom_mg_setup(void *handle, const double *E, const uint8_t *mask, void *stream);
 int fantom_mg_precondition(void *handle, const double *r, double *z, void *stream);
 int fantom_mg_solve(void *handle, const double *f, double *x, double tol, int maxiter, int use_x0, int use_graph,
@@ -347,6 +362,10 @@ int fantom_mx_destroy(void *handle);
  * (the FP64 PCG of fantom_mx_solve_pcg, the inner FP32 PCG of
  * fantom_mx_solve, restarted at min_omega on each refinement step). */
 int fantom_mx_set_omega_schedule(void *handle, int adaptive, double min_omega, double level_boost);
+/* As fantom_mg_set_smoother / fantom_mg_smoother_rho, for the FP32 V-cycle
+ * (power iterations on FP32 vectors with FP64 sums). */
+int fantom_mx_set_smoother(void *handle, int kind, double rho_scale, int iters_first, int iters);
+int fantom_mx_smoother_rho(void *handle, double *rho, int n);
 int fantom_mx_setup(void *handle, const double *E, const uint8_t *mask, void *stream);
 int fantom_mx_solve(void *handle, const double *f, double *x, double tol, double inner_tol, int max_outer,
                     int max_inner, int use_x0, int use_graph, int *outer_iters, int *inner_iters, double *rel_res,

@@ -586,3 +586,36 @@ PINNED = {
     "csr_assemble": "closed-form nonzero counts (free grid 9 prod(3n-2), clamped face); dense matrix equals the "
                     "scipy assembly; K u equals the C matvec with constraints; symmetric pattern",
 }
+
+
+# --------------------------------------------------------------------------
+# Chebyshev smoothing (solvers/CPU/_solvers.py:884-954, the CPU MultiGrid's
+# _estimate_rho and _chebyshev_smoother, whose calls the reference leaves
+# commented out at :1045/:1066/:1088)
+# --------------------------------------------------------------------------
+def chebyshev4_smoother(x, b, A, dinv, n_steps, rho):
+    """Fourth-kind Chebyshev smoother, the reference's three-term form
+    (:913-954): z_1 = 4/3 rho^-1 D^-1 r_0, z_k = (2k-3)/(2k+1) z_{k-1} +
+    (8k-4)/(2k+1) rho^-1 D^-1 r_{k-1}, x_k = x_{k-1} + z_k.  A: callable u -> A u."""
+    x = np.array(x, dtype=np.float64, copy=True)
+    z = np.zeros_like(x)
+    for k in range(1, n_steps + 1):
+        r = b - A(x)
+        f2 = (8 * k - 4) / (2 * k + 1)
+        if k == 1:
+            z = f2 / rho * (dinv * r)
+        else:
+            z = (2 * k - 3) / (2 * k + 1) * z + f2 / rho * (dinv * r)
+        x = x + z
+    return x
+
+
+def estimate_rho(A, dinv, x0, num_iterations=10):
+    """Power method on D^-1 A (:884-911): x <- D^-1 A x / |x|, then the
+    quotient x . D^-1 A x.  The reference starts from np.random.rand; here the
+    start vector is an argument (its body returns 0 before running, :885)."""
+    x = np.asarray(x0, dtype=np.float64) / np.linalg.norm(x0)
+    for _ in range(num_iterations):
+        x = dinv * A(x)
+        x /= np.linalg.norm(x)
+    return float(x @ (dinv * A(x)))

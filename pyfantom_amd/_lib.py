@@ -71,6 +71,9 @@ SIGNATURES = {
                                   _c_long, ctypes.POINTER(_vp)]),
     "fantom_mg_destroy": (_c_int, [_vp]),
     "fantom_mg_set_omega_schedule": (_c_int, [_vp, _c_int, _c_double, _c_double]),
+    "fantom_mg_set_smoother": (_c_int, [_vp, _c_int, _c_double, _c_int, _c_int]),
+    "fantom_mg_smoother_rho": (_c_int, [_vp, _vp, _c_int]),
+    "fantom_chebyshev4_weights": (_c_int, [_c_int, _vp]),
     "fantom_mg_setup": (_c_int, [_vp, _vp, _vp, _vp]),
     "fantom_mg_precondition": (_c_int, [_vp, _vp, _vp, _vp]),
     "fantom_mg_solve": (_c_int, [_vp, _vp, _vp, _c_double, _c_int, _c_int, _c_int, ctypes.POINTER(_c_int),
@@ -80,6 +83,8 @@ SIGNATURES = {
                                   ctypes.POINTER(_vp)]),
     "fantom_mx_destroy": (_c_int, [_vp]),
     "fantom_mx_set_omega_schedule": (_c_int, [_vp, _c_int, _c_double, _c_double]),
+    "fantom_mx_set_smoother": (_c_int, [_vp, _c_int, _c_double, _c_int, _c_int]),
+    "fantom_mx_smoother_rho": (_c_int, [_vp, _vp, _c_int]),
     "fantom_mx_setup": (_c_int, [_vp, _vp, _vp, _vp]),
     "fantom_mx_solve": (_c_int, [_vp, _vp, _vp, _c_double, _c_double, _c_int, _c_int, _c_int, _c_int,
                                  ctypes.POINTER(_c_int), ctypes.POINTER(_c_int), ctypes.POINTER(_c_double), _vp]),

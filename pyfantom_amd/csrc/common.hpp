@@ -44,6 +44,9 @@ struct Grid {
     int nx, ny, nz;  // nodes per axis (nz = 1 in 2D)
     long n_nodes, n_el;
     const double *omega_scale = nullptr;
+    // fused zero-guess sweep (MODE_JAC0*): x0 = w D^-1 b followed by one
+    // sweep of weight w * w2_ratio (1: both sweeps share the weight)
+    double w2_ratio = 1.0;
 };
 
 // Jacobi weight of a smoothing kernel: the argument, times the device scalar

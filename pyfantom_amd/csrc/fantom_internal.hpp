@@ -43,6 +43,18 @@ __device__ __forceinline__ double schedule_omega(const OmegaSchedule &o, double 
 }
 
 
+// cheb.hip (fourth-kind Chebyshev smoothing).  st: device double[4] per
+// level = {|v|_D^2, rho, 1 / (rho_scale rho), -}; partials: 2 * kVecMaxBlocks
+void chebyshev4_weights(int n, double *w);
+int launch_ch_init(long n, double *v, const double *dinv, int fill, int salt, double *partials, double *st,
+                   hipStream_t s);
+int launch_ch_init_f32(long n, float *v, const float *dinv, int fill, int salt, double *partials, double *st,
+                       hipStream_t s);
+int launch_ch_pass(long n, double *v, const double *y, const double *dinv, double rho_scale, double *partials,
+                   double *st, hipStream_t s);
+int launch_ch_pass_f32(long n, float *v, const float *y, const float *dinv, double rho_scale, double *partials,
+                       double *st, hipStream_t s);
+
 // ku_node.hip
 int launch_ku_node(int mode, const Grid &g, int dim, const double *K0, const double *E, const uint8_t *mask,
                    const double *u, double *y, const double *b, const double *dinv, double omega,

@@ -51,6 +51,7 @@ struct MgLevel {
     double *S;             // assembled stencil of an element-matrix level that is smoothed
     int kind;              // LK_* operator of this level
     double om;             // weight argument of the smoothing kernels (see OmegaSchedule)
+    double *ev;            // Chebyshev: power-iteration vector (kept between setups)
 };
 
 // operator kind of a level
@@ -74,6 +75,13 @@ struct MgHandle {
     hipGraphExec_t iter_exec;
     bool have_graph;
     int last_iters;
+    // fourth-kind Chebyshev smoothing (off: weighted Jacobi / the schedule)
+    int cheb;
+    double cw[16];         // sweep weights in units of 1 / rho, ascending
+    double cheb_scale;     // rho is multiplied by this before use
+    int cheb_iters_first, cheb_iters;
+    int cheb_setups;
+    double *chs;           // device double[4 nlev]: per level {|v|_D^2, rho, 1 / (scale rho), -}
 };
 
 // ---- small device kernels specific to the driver --------------------------------
@@ -218,24 +226,27 @@ static int mg_smooth(MgHandle *h, int l, const double *b, double **px, bool from
     MgLevel &L = h->lev[l];
     double *x = *px;
     double *other = (x == L.x) ? L.x2 : L.x;
+    // weight argument of sweep s: Jacobi's omega, or the Chebyshev weight
+    // (the kernels multiply it by 1 / rho through the level's omega_scale)
+    auto w = [&](int s) { return h->cheb ? h->cw[s] : L.om; };
     int s0 = 0;
     if (from_zero && h->n_smooth >= 2 && lev_has_jac0(h, l)) {
-        // x = omega D^-1 b followed by one sweep, in one pass
-        MG_TRY(lev_ku(h, l, MODE_JAC0, nullptr, x, b, L.dinv, L.om, nullptr));
+        // x = w0 D^-1 b followed by one sweep of weight w1 (L.g.w2_ratio), in one pass
+        MG_TRY(lev_ku(h, l, MODE_JAC0, nullptr, x, b, L.dinv, w(0), nullptr));
         s0 = 2;
     }
     for (int s = s0; s < h->n_smooth; ++s) {
         if (s == 0 && from_zero) {
-            MG_TRY(launch_scale(L.n_dof, L.om, L.dinv, b, x, h->stream, L.g.omega_scale));
+            MG_TRY(launch_scale(L.n_dof, w(0), L.dinv, b, x, h->stream, L.g.omega_scale));
             continue;
         }
         const bool last = final && s == h->n_smooth - 1;
         double *dst = last ? final : other;
         if (last && rz && l == 0 && h->kernel == FANTOM_KU_TILE && L.dim == 3) {
-            MG_TRY(lev_ku(h, l, MODE_JACDOT, x, dst, b, L.dinv, L.om, h->partials));
+            MG_TRY(lev_ku(h, l, MODE_JACDOT, x, dst, b, L.dinv, w(s), h->partials));
             *rz = true;
         } else {
-            MG_TRY(lev_ku(h, l, MODE_JAC, x, dst, b, L.dinv, L.om, nullptr));
+            MG_TRY(lev_ku(h, l, MODE_JAC, x, dst, b, L.dinv, w(s), nullptr));
         }
         other = x;
         x = dst;
@@ -349,6 +360,7 @@ static void layout(MgHandle *h, int nelx, int nely, int nelz, int n_coarse, int 
         L.E = cv.take<double>(L.g.n_el);
         L.mask = cv.take<uint8_t>(L.g.n_nodes);
         L.K0dense_d = cv.take<double>(576);
+        L.ev = cv.take<double>(L.n_dof);
         L.S = nullptr;
         if (L.kind == LK_ELEM) {
             L.Ke = cv.take<double>((size_t)L.g.n_el * 576);
@@ -372,7 +384,7 @@ static void layout(MgHandle *h, int nelx, int nely, int nelz, int n_coarse, int 
     h->p = cv.take<double>(n);
     h->q = cv.take<double>(n);
     h->f = cv.take<double>(n);
-    long np = kVecMaxBlocks;
+    long np = 2 * kVecMaxBlocks;   // the Chebyshev power pass keeps two sums per block
     np = std::max(np, ku_node_partials(h->lev[0].g));
     if (h->dim == 3)
         np = std::max(np, ku_tile_partials(h->lev[0].g));
@@ -386,6 +398,7 @@ static void layout(MgHandle *h, int nelx, int nely, int nelz, int n_coarse, int 
     h->n_partials = np;
     h->partials = cv.take<double>(np);
     h->scal = cv.take<double>(FANTOM_SCALARS);
+    h->chs = cv.take<double>(4 * (size_t)h->nlev);
     h->nc = (int)h->lev[h->nlev - 1].n_dof;
     h->Ainv = cv.take<double>((size_t)h->nc * h->nc);
     h->gj_work = cv.take<double>((size_t)h->nc * h->nc + 1024);
@@ -445,6 +458,11 @@ int fantom_mg_create(int nelx, int nely, int nelz, int n_coarse, int n_smooth, d
     h->kernel = (h->dim == 3) ? kernel : FANTOM_KU_NODE;
     h->have_graph = false;
     h->last_iters = 0;
+    h->cheb = 0;
+    h->cheb_scale = 1.0;
+    h->cheb_iters_first = 12;
+    h->cheb_iters = 4;
+    h->cheb_setups = 0;
     const int nl = h->dim == 3 ? 24 : 8;
     for (int l = 0; l < h->nlev; ++l) {
         MgLevel &L = h->lev[l];
@@ -507,10 +525,12 @@ int fantom_mg_set_omega_schedule(void *handle, int adaptive, double min_omega, d
     FANTOM_CHECK_ARG(!adaptive || (min_omega > 0.0 && level_boost > 0.0),
                      "adaptive schedule needs min_omega > 0 and level_boost > 0");
     h->sched = adaptive ? OmegaSchedule{1, min_omega, h->omega - min_omega} : OmegaSchedule{0, h->omega, 0.0};
+    h->cheb = 0;   // the weight schedule replaces a Chebyshev smoother
     double boost = 1.0;
     for (auto &L : h->lev) {
         L.om = adaptive ? boost : h->omega;
         L.g.omega_scale = adaptive ? h->scal + D_OMEGA : nullptr;
+        L.g.w2_ratio = 1.0;
         boost *= level_boost;
     }
     if (adaptive) {   // the weight of a V-cycle applied before any solve
@@ -524,6 +544,53 @@ int fantom_mg_set_omega_schedule(void *handle, int adaptive, double min_omega, d
         hipGraphExecDestroy(h->iter_exec);
         h->have_graph = false;
     }
+    return kStatusOk;
+}
+
+int fantom_mg_set_smoother(void *handle, int kind, double rho_scale, int iters_first, int iters)
+{
+    MgHandle *h = reinterpret_cast<MgHandle *>(handle);
+    FANTOM_CHECK_ARG(h, "null handle");
+    FANTOM_CHECK_ARG(kind == 0 || kind == 1, "smoother kind must be 0 (Jacobi) or 1 (Chebyshev)");
+    FANTOM_CHECK_ARG(kind == 0 || (rho_scale >= 0.5 && rho_scale <= 4.0 && iters_first >= 1 &&
+                                   iters_first <= 100 && iters >= 0 && iters <= 100),
+                     "Chebyshev needs rho_scale in [0.5, 4], iters_first in [1, 100], iters in [0, 100]");
+    h->cheb = kind;
+    h->sched = OmegaSchedule{0, h->omega, 0.0};
+    if (kind == 1) {
+        chebyshev4_weights(h->n_smooth, h->cw);
+        h->cheb_scale = rho_scale;
+        h->cheb_iters_first = iters_first;
+        h->cheb_iters = iters;
+        h->cheb_setups = 0;
+    }
+    for (int l = 0; l < h->nlev; ++l) {
+        MgLevel &L = h->lev[l];
+        L.om = h->omega;
+        L.g.omega_scale = kind == 1 ? h->chs + 4 * l + 2 : nullptr;
+        L.g.w2_ratio = (kind == 1 && h->n_smooth >= 2) ? h->cw[1] / h->cw[0] : 1.0;
+    }
+    if (h->have_graph) {   // the captured iteration holds the old weights
+        hipGraphExecDestroy(h->iter_exec);
+        h->have_graph = false;
+    }
+    return kStatusOk;
+}
+
+int fantom_mg_smoother_rho(void *handle, double *rho, int n)
+{
+    MgHandle *h = reinterpret_cast<MgHandle *>(handle);
+    FANTOM_CHECK_ARG(h && rho && n >= h->nlev - 1, "null operand or n below the number of smoothed levels");
+    std::vector<double> st(4 * (size_t)h->nlev);
+    hipError_t e = hipMemcpyAsync(st.data(), h->chs, sizeof(double) * st.size(), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) {
+        set_error("smoother rho read: %s", hipGetErrorString(e));
+        return kStatusHipError;
+    }
+    for (int l = 0; l < h->nlev - 1; ++l)
+        rho[l] = st[4 * l + 1];
     return kStatusOk;
 }
 
@@ -603,6 +670,22 @@ int fantom_mg_setup(void *handle, const double *E, const uint8_t *mask, void *st
             MG_TRY(launch_dinv_elem(L.g, L.Ke, L.mask, L.dinv, h->stream));
         else
             MG_TRY(launch_diag(L.g, h->dim, L.K0dense_d, L.E, L.mask, L.dinv, 1, h->stream));
+    }
+    if (h->cheb) {
+        // rho of D^-1 A per smoothed level; the first setup starts from a
+        // fixed pseudo-random vector, later ones from the previous vector
+        const bool first = h->cheb_setups == 0;
+        const int iters = first ? h->cheb_iters_first : h->cheb_iters;
+        for (int l = 0; l < h->nlev - 1 && iters > 0; ++l) {
+            MgLevel &L = h->lev[l];
+            double *st = h->chs + 4 * l;
+            MG_TRY(launch_ch_init(L.n_dof, L.ev, L.dinv, first ? 1 : 0, l, h->partials, st, h->stream));
+            for (int it = 0; it < iters; ++it) {
+                MG_TRY(lev_ku(h, l, MODE_KU, L.ev, L.x2, nullptr, nullptr, 0.0, nullptr));
+                MG_TRY(launch_ch_pass(L.n_dof, L.ev, L.x2, L.dinv, h->cheb_scale, h->partials, st, h->stream));
+            }
+        }
+        ++h->cheb_setups;
     }
     MgLevel &C = h->lev[h->nlev - 1];
     if (C.kind == LK_ELEM)

@@ -44,6 +44,7 @@ struct MxLevel {
     float *S;     // assembled FP32 stencil of an element-matrix level that is smoothed
     int kind;     // 0 fine, 1 rediscretised, 2 Galerkin level 1, 3 element matrices
     double om;    // weight argument of the smoothing kernels (see OmegaSchedule)
+    float *ev;    // Chebyshev: power-iteration vector (kept between setups)
 };
 
 struct MxHandle {
@@ -70,6 +71,12 @@ struct MxHandle {
     hipGraphExec_t iter_exec, pc_exec;
     bool have_graph, have_pc_graph;
     int last_inner, last_pc;
+    // fourth-kind Chebyshev smoothing (see fantom_mg_set_smoother)
+    int cheb;
+    double cw[16];
+    double cheb_scale;
+    int cheb_iters_first, cheb_iters, cheb_setups;
+    double *chs;   // device double[4 nlev]
 };
 
 __global__ void mx_copy_f32(long n, const float *__restrict__ a, float *__restrict__ b)
@@ -299,19 +306,20 @@ __global__ void mxd_norm_k(const double *__restrict__ partials, long np, double 
             return _s;                                                                                     \
     } while (0)
 
+// om: the weight argument of a smoothing mode (Jacobi's omega or a Chebyshev weight)
 static int mx_ku(MxHandle *h, int l, int mode, const float *u, float *y, const float *b, const float *dinv,
-                 double *partials)
+                 double *partials, double om = 0.0)
 {
     MxLevel &L = h->lev[l];
     if (L.kind == 2)
-        return launch_gal1_f32(mode, L.g, h->Btab32, h->lev[0].E, L.mask, u, y, b, dinv, L.om, partials, h->stream);
+        return launch_gal1_f32(mode, L.g, h->Btab32, h->lev[0].E, L.mask, u, y, b, dinv, om, partials, h->stream);
     if (L.kind == 3)
-        return L.S ? launch_stencil_ku_f32(mode, L.g, L.S, u, y, b, dinv, L.om, partials, h->stream)
-                   : launch_elem_f32(mode, L.g, L.Ke, L.mask, u, y, b, dinv, L.om, partials, h->stream);
+        return L.S ? launch_stencil_ku_f32(mode, L.g, L.S, u, y, b, dinv, om, partials, h->stream)
+                   : launch_elem_f32(mode, L.g, L.Ke, L.mask, u, y, b, dinv, om, partials, h->stream);
     // the hex8 tile sweeps form D^-1 from the moduli (dinv is the level's
     // own D^-1: the zero-guess sweep reads it once, for x0)
     return launch_ku_tile_f32(mode == MODE_JAC0 ? MODE_JAC0OWN : mode, L.g, L.Kb, L.E, L.mask, u, y, b,
-                              mode == MODE_JAC ? nullptr : dinv, L.om, partials, h->stream);
+                              mode == MODE_JAC ? nullptr : dinv, om, partials, h->stream);
 }
 
 // n_smooth sweeps on x (*px); final (optional) receives the last sweep
@@ -320,21 +328,24 @@ static int mx_smooth(MxHandle *h, int l, const float *b, float **px, bool from_z
     MxLevel &L = h->lev[l];
     float *x = *px;
     float *other = (x == L.x) ? L.x2 : L.x;
+    // weight argument of sweep s: Jacobi's omega, or the Chebyshev weight
+    // (the kernels multiply it by 1 / rho through the level's omega_scale)
+    auto w = [&](int s) { return h->cheb ? h->cw[s] : L.om; };
     int s0 = 0;
     // every kind of the FP32 hierarchy fuses the zero-guess scaling except
     // element-matrix levels without a stencil
     if (from_zero && h->n_smooth >= 2 && (L.kind != 3 || L.S != nullptr)) {
-        MX_TRY(mx_ku(h, l, MODE_JAC0, nullptr, x, b, L.dinv, nullptr));
+        MX_TRY(mx_ku(h, l, MODE_JAC0, nullptr, x, b, L.dinv, nullptr, w(0)));
         s0 = 2;
     }
     for (int s = s0; s < h->n_smooth; ++s) {
         if (s == 0 && from_zero) {
             hipLaunchKernelGGL(mx_scale_k, dim3(vec_grid(L.n_dof)), dim3(256), 0, h->stream, L.n_dof,
-                               (float)L.om, L.g.omega_scale, L.dinv, b, x);
+                               (float)w(0), L.g.omega_scale, L.dinv, b, x);
             continue;
         }
         float *dst = (final && s == h->n_smooth - 1) ? final : other;
-        MX_TRY(mx_ku(h, l, MODE_JAC, x, dst, b, L.dinv, nullptr));
+        MX_TRY(mx_ku(h, l, MODE_JAC, x, dst, b, L.dinv, nullptr, w(s)));
         other = x;
         x = dst;
     }
@@ -443,6 +454,7 @@ static void mx_layout(MxHandle *h, int nelx, int nely, int nelz, int n_coarse, i
         L.E = cv.take<float>(L.g.n_el);
         L.mask = cv.take<uint8_t>(L.g.n_nodes);
         L.K0dense_d = cv.take<double>(576);
+        L.ev = cv.take<float>(L.n_dof);
         L.kind = l == 0 ? 0 : (!h->galerkin ? 1 : (l == 1 ? 2 : 3));
         L.Ke = L.kind == 3 ? cv.take<double>((size_t)L.g.n_el * 576) : nullptr;
         L.S = (L.kind == 3 && l < h->nlev - 1) ? cv.take<float>((size_t)L.g.n_nodes * 243) : nullptr;
@@ -470,12 +482,14 @@ static void mx_layout(MxHandle *h, int nelx, int nely, int nelz, int n_coarse, i
     h->q = cv.take<float>(h->n);
     h->p64 = cv.take<double>(h->n);
     h->q64 = cv.take<double>(h->n);
-    long np = std::max((long)kVecMaxBlocks, std::max(ku_tile_partials(h->g), ku_tile_partials_f32(h->g)));
+    // at least two sums per block for the Chebyshev power pass
+    long np = std::max(2 * (long)kVecMaxBlocks, std::max(ku_tile_partials(h->g), ku_tile_partials_f32(h->g)));
     for (int l = 1; l < h->nlev; ++l)
         np = std::max(np, std::max(std::max(elem_partials(h->lev[l].g), gal1_partials(h->lev[l].g)),
                                    stencil_partials(h->lev[l].g)));
     h->partials = cv.take<double>(np);
     h->scal = cv.take<double>(FANTOM_SCALARS);
+    h->chs = cv.take<double>(4 * (size_t)h->nlev);
     h->nc = (int)h->lev[h->nlev - 1].n_dof;
     h->Ainv = cv.take<double>((size_t)h->nc * h->nc);
     h->gj_work = cv.take<double>((size_t)h->nc * h->nc + 1024);
@@ -557,6 +571,11 @@ int fantom_mx_create(int nelx, int nely, int nelz, int n_coarse, int n_smooth, d
     h->have_pc_graph = false;
     h->last_inner = 0;
     h->last_pc = 0;
+    h->cheb = 0;
+    h->cheb_scale = 1.0;
+    h->cheb_iters_first = 12;
+    h->cheb_iters = 4;
+    h->cheb_setups = 0;
     int st = fantom_hex8_blocks(K0_host, h->Kb64, nullptr);
     if (st) {
         delete h;
@@ -613,10 +632,12 @@ int fantom_mx_set_omega_schedule(void *handle, int adaptive, double min_omega, d
     FANTOM_CHECK_ARG(!adaptive || (min_omega > 0.0 && level_boost > 0.0),
                      "adaptive schedule needs min_omega > 0 and level_boost > 0");
     h->sched = adaptive ? OmegaSchedule{1, min_omega, h->omega - min_omega} : OmegaSchedule{0, h->omega, 0.0};
+    h->cheb = 0;   // the weight schedule replaces a Chebyshev smoother
     double boost = 1.0;
     for (auto &L : h->lev) {
         L.om = adaptive ? boost : h->omega;
         L.g.omega_scale = adaptive ? h->scal + X_OMEGA : nullptr;
+        L.g.w2_ratio = 1.0;
         boost *= level_boost;
     }
     if (adaptive) {   // the weight of a V-cycle applied before any solve
@@ -634,6 +655,57 @@ int fantom_mx_set_omega_schedule(void *handle, int adaptive, double min_omega, d
         hipGraphExecDestroy(h->iter_exec);
         h->have_graph = false;
     }
+    return kStatusOk;
+}
+
+int fantom_mx_set_smoother(void *handle, int kind, double rho_scale, int iters_first, int iters)
+{
+    MxHandle *h = reinterpret_cast<MxHandle *>(handle);
+    FANTOM_CHECK_ARG(h, "null handle");
+    FANTOM_CHECK_ARG(kind == 0 || kind == 1, "smoother kind must be 0 (Jacobi) or 1 (Chebyshev)");
+    FANTOM_CHECK_ARG(kind == 0 || (rho_scale >= 0.5 && rho_scale <= 4.0 && iters_first >= 1 &&
+                                   iters_first <= 100 && iters >= 0 && iters <= 100),
+                     "Chebyshev needs rho_scale in [0.5, 4], iters_first in [1, 100], iters in [0, 100]");
+    h->cheb = kind;
+    h->sched = OmegaSchedule{0, h->omega, 0.0};
+    if (kind == 1) {
+        chebyshev4_weights(h->n_smooth, h->cw);
+        h->cheb_scale = rho_scale;
+        h->cheb_iters_first = iters_first;
+        h->cheb_iters = iters;
+        h->cheb_setups = 0;
+    }
+    for (int l = 0; l < h->nlev; ++l) {
+        MxLevel &L = h->lev[l];
+        L.om = h->omega;
+        L.g.omega_scale = kind == 1 ? h->chs + 4 * l + 2 : nullptr;
+        L.g.w2_ratio = (kind == 1 && h->n_smooth >= 2) ? h->cw[1] / h->cw[0] : 1.0;
+    }
+    if (h->have_pc_graph) {   // the captured iterations hold the old weights
+        hipGraphExecDestroy(h->pc_exec);
+        h->have_pc_graph = false;
+    }
+    if (h->have_graph) {
+        hipGraphExecDestroy(h->iter_exec);
+        h->have_graph = false;
+    }
+    return kStatusOk;
+}
+
+int fantom_mx_smoother_rho(void *handle, double *rho, int n)
+{
+    MxHandle *h = reinterpret_cast<MxHandle *>(handle);
+    FANTOM_CHECK_ARG(h && rho && n >= h->nlev - 1, "null operand or n below the number of smoothed levels");
+    std::vector<double> st(4 * (size_t)h->nlev);
+    hipError_t e = hipMemcpyAsync(st.data(), h->chs, sizeof(double) * st.size(), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) {
+        set_error("smoother rho read: %s", hipGetErrorString(e));
+        return kStatusHipError;
+    }
+    for (int l = 0; l < h->nlev - 1; ++l)
+        rho[l] = st[4 * l + 1];
     return kStatusOk;
 }
 
@@ -701,6 +773,20 @@ int fantom_mx_setup(void *handle, const double *E, const uint8_t *mask, void *st
             MX_TRY(launch_dinv_elem_f32(L.g, L.Ke, L.mask, L.dinv, s));
         else
             MX_TRY(launch_diag_f32(L.g, L.K0dense_d, L.E, L.mask, L.dinv, s));
+    }
+    if (h->cheb) {   // rho of D^-1 A per smoothed level (see fantom_mg_setup)
+        const bool first = h->cheb_setups == 0;
+        const int iters = first ? h->cheb_iters_first : h->cheb_iters;
+        for (int l = 0; l < h->nlev - 1 && iters > 0; ++l) {
+            MxLevel &L = h->lev[l];
+            double *st = h->chs + 4 * l;
+            MX_TRY(launch_ch_init_f32(L.n_dof, L.ev, L.dinv, first ? 1 : 0, l, h->partials, st, s));
+            for (int it = 0; it < iters; ++it) {
+                MX_TRY(mx_ku(h, l, MODE_KU, L.ev, L.x2, nullptr, nullptr, nullptr));
+                MX_TRY(launch_ch_pass_f32(L.n_dof, L.ev, L.x2, L.dinv, h->cheb_scale, h->partials, st, s));
+            }
+        }
+        ++h->cheb_setups;
     }
     // the coarsest operator and its inverse stay FP64
     MxLevel &C = h->lev[h->nlev - 1];

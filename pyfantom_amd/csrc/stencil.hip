@@ -134,9 +134,9 @@ __global__ __launch_bounds__(kStBlock) void ku_stencil_k(Grid g, const T *__rest
             T o;
             if (MODE == MODE_JAC)
                 o = u[r] + omega * dinv[r] * (bvec[r] - kuv);
-            else if (MODE == MODE_JAC0) {
+            else if (MODE == MODE_JAC0) {   // x0 = omega D^-1 b, then one sweep of weight omega * w2_ratio
                 const T x0 = omega * dinv[r] * bvec[r];
-                o = x0 + omega * dinv[r] * (bvec[r] - kuv);
+                o = x0 + (omega * (T)g.w2_ratio) * dinv[r] * (bvec[r] - kuv);
             }
             else if (MODE == MODE_RES)
                 o = bvec[r] - kuv;

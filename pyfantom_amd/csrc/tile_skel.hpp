@@ -114,6 +114,9 @@ __global__ __launch_bounds__(kTThreads) __attribute__((amdgpu_waves_per_eu(Op::k
     constexpr bool kEpiB = (kJacobi || MODE == MODE_RES) && MODE != MODE_JAC0OWN;
     if (kJacobi)
         omega = jacobi_weight(g, omega);
+    // weight of the sweep that follows x0 in the zero-guess modes
+    const T ratio = (T)g.w2_ratio;
+    const T omega2 = omega * ratio;
     // double-buffered by layer parity, so that one barrier per layer orders
     // both exchanges
     __shared__ T xf[2][6][kTThreads];   // z-butterflies, exchanged with column lx - 1
@@ -320,8 +323,10 @@ __global__ __launch_bounds__(kTThreads) __attribute__((amdgpu_waves_per_eu(Op::k
                 for (int c = 0; c < 3; ++c) {
                     const T kuv = ((mcur >> c) & 1) ? ucur[c] : val[c] + xr[rb][c][nb_dn];
                     T o;
-                    if (MODE == MODE_JAC0OWN)   // x0 = omega D^-1 b, so x0 + omega D^-1 (b - K x0) is:
-                        o = (ucur[c] + ucur[c]) - omega * dgcur[c] * kuv;
+                    if (MODE == MODE_JAC0OWN)   // x0 = omega D^-1 b, so x0 + omega2 D^-1 (b - K x0) is:
+                        o = (ucur[c] + ratio * ucur[c]) - omega2 * dgcur[c] * kuv;
+                    else if (MODE == MODE_JAC0)
+                        o = ucur[c] + omega2 * dgcur[c] * (bcur[c] - kuv);
                     else if (kJacobi)
                         o = ucur[c] + omega * dgcur[c] * (bcur[c] - kuv);
                     else if (MODE == MODE_RES)

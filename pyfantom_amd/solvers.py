@@ -20,7 +20,12 @@ Deliberate differences from the reference (each keeps the solution K^-1 f):
     times 1.06^level); ``omega_schedule='fixed'`` keeps omega on every level;
   * cycle='W' runs a V-cycle and cycle='w' a W-cycle at ``w_level`` -- this
     is the reference's observed behaviour (it compares with 'w');
-  * the returned residual is the true ||f - K u|| / ||f||.
+  * the returned residual is the true ||f - K u|| / ||f||;
+  * ``smoother='chebyshev'`` (native driver) runs the fourth-kind Chebyshev
+    smoother that the reference's CPU MultiGrid carries but leaves commented
+    out (solvers/CPU/_solvers.py:884-954): n_smooth sweeps with the weights
+    of fantom_chebyshev4_weights over rho(D^-1 A) from power iterations at
+    each setup (``cheb_power_iters`` = (first, later), warm-started).
 """
 from __future__ import annotations
 
@@ -130,7 +135,8 @@ class MultiGrid(_DeviceKrylov):
                  w_level=1, coarse_solver="cholmod", matrix_free=False, low_level_tol=1e-8,
                  low_level_maxiter=5000, min_omega=0.4, omega_boost=1.06, omega_schedule="reference",
                  check_every=1, native=True, use_graph=True, precision="fp64", inner_tol=1e-4,
-                 max_outer=20, coarse=None, mixed_scheme="pcg", fallback=True, fallback_maxiter=1000):
+                 max_outer=20, coarse=None, mixed_scheme="pcg", fallback=True, fallback_maxiter=1000,
+                 smoother="jacobi", cheb_rho_scale=1.0, cheb_power_iters=(12, 4)):
         if coarse_solver not in ["cg", "gmres", "splu", "spsolve", "cholmod"]:
             raise ValueError("Coarse solver not recognized.")
         if coarse_solver in ["splu", "spsolve"] and matrix_free:
@@ -170,6 +176,13 @@ class MultiGrid(_DeviceKrylov):
         if precision == "mixed" and not self.native:
             raise ValueError("the mixed-precision solve runs in the native driver (V-cycle)")
         self.precision = precision
+        if smoother not in ("jacobi", "chebyshev"):
+            raise ValueError("smoother must be 'jacobi' or 'chebyshev'")
+        if smoother == "chebyshev" and not self.native:
+            raise ValueError("the Chebyshev smoother runs in the native driver (V-cycle)")
+        self.smoother = smoother
+        self.cheb_rho_scale = float(cheb_rho_scale)
+        self.cheb_power_iters = tuple(int(v) for v in cheb_power_iters)
         # mixed precision: "pcg" = FP64 PCG preconditioned by an FP32 V-cycle;
         # "ir" = FP64 iterative refinement around an FP32 GMG-PCG
         if mixed_scheme not in ("pcg", "ir"):
@@ -246,9 +259,22 @@ class MultiGrid(_DeviceKrylov):
         return z
 
     def _native_schedule(self, L, pre, h):
-        adaptive = self.omega_schedule == "reference"
+        adaptive = self.omega_schedule == "reference" and self.smoother == "jacobi"
         _lib.check(getattr(L, pre + "set_omega_schedule")(h, int(adaptive), self.min_omega if adaptive else 0.0,
                                                               self.omega_boost if adaptive else 1.0))
+        if self.smoother == "chebyshev":
+            _lib.check(getattr(L, pre + "set_smoother")(h, 1, self.cheb_rho_scale, *self.cheb_power_iters))
+
+    def smoother_rho(self):
+        """Current rho(D^-1 A) estimates of the smoothed levels (Chebyshev), fine first."""
+        import ctypes
+        if self._nh is None or self.smoother != "chebyshev":
+            return None
+        n = self._n_levels - 1
+        out = (ctypes.c_double * max(n, 1))()
+        pre = "fantom_mx_" if self._nh_key[-1] == "mixed" else "fantom_mg_"
+        _lib.check(getattr(_lib.load(), pre + "smoother_rho")(self._nh, out, n))
+        return list(out)[:n]
 
     def _native_prepare(self):
         import ctypes
@@ -256,7 +282,8 @@ class MultiGrid(_DeviceKrylov):
         L = _lib.load()
         gal = int(self.coarse == "galerkin")
         key = (k.grid, self.n_level, self.n_smooth, self.omega, k.kernel_kind, gal, self.omega_schedule,
-               self.min_omega, self.omega_boost, self.precision)
+               self.min_omega, self.omega_boost, self.smoother, self.cheb_rho_scale, self.cheb_power_iters,
+               self.precision)
         if self._nh is None or self._nh_key != key:
             self._free_native()
             n_coarse = self._n_coarse()
@@ -268,6 +295,7 @@ class MultiGrid(_DeviceKrylov):
             _lib.check(L.fantom_mg_create(*k.grid, n_coarse, self.n_smooth, self.omega, kid, gal, K0.ctypes.data,
                                           self._ws.data_ptr(), nbytes, ctypes.byref(h)))
             self._nh, self._nh_key, self._nh_state = h, key, None
+            self._n_levels = n_coarse + 1
             _LIVE.add(self)
             self._native_schedule(L, "fantom_mg_", h)
         state = (k.rho_version, k.con_version, k.rho.data_ptr())
@@ -284,7 +312,8 @@ class MultiGrid(_DeviceKrylov):
             raise NotImplementedError("the mixed-precision solve is implemented for hex8 grids")
         gal = int(self.coarse == "galerkin")
         key = (k.grid, self.n_level, self.n_smooth, self.omega, k.kernel_kind, gal, self.omega_schedule,
-               self.min_omega, self.omega_boost, self.precision)
+               self.min_omega, self.omega_boost, self.smoother, self.cheb_rho_scale, self.cheb_power_iters,
+               self.precision)
         pre = "fantom_mx_" if mixed else "fantom_mg_"
         if self._nh is None or self._nh_key != key:
             self._free_native()
@@ -338,7 +367,8 @@ class MultiGrid(_DeviceKrylov):
             if self._fb is None:
                 self._fb = MultiGrid(self.mesh, k, maxiter=self.fallback_maxiter, tol=self.tol, n_smooth=self.n_smooth,
                                      omega=self.omega, n_level=self.n_level, cycle=self.cycle,
-                                     use_graph=self.use_graph, coarse=self.coarse)
+                                     use_graph=self.use_graph, coarse=self.coarse, smoother=self.smoother,
+                                     cheb_rho_scale=self.cheb_rho_scale, cheb_power_iters=self.cheb_power_iters)
             self._fb.tol = self.tol
             self._fb.last_x0 = x
             x, r = self._fb.solve(rhs, use_last=True)

@@ -521,3 +521,45 @@ def test_csr_assemble_pattern_and_values(oracle, nel):
     rows_c = np.flatnonzero(con)
     assert np.array_equal(np.diff(indptr_c)[rows_c], np.ones(len(rows_c), np.int64))
     assert np.array_equal(idx_c[indptr_c[rows_c]], rows_c) and np.all(data_c[indptr_c[rows_c]] == 1.0)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 4])
+def test_chebyshev4_weights_match_reference_recurrence(oracle, n):
+    """The product form the drivers run (n Jacobi sweeps with weights
+    fantom_chebyshev4_weights(n) / rho) is the reference's fourth-kind
+    three-term smoother: on the scalar problem a = t, D = 1, b = 0 both give
+    the error p_n(t) e_0 for every t; p_n(1) = (-1)^n / (2n + 1) and |p_n| <= 1
+    on [0, 1]."""
+    import ctypes
+    from pyfantom_amd import _lib
+    w = (ctypes.c_double * n)()
+    _lib.check(_lib.load().fantom_chebyshev4_weights(n, w))
+    w = np.array(list(w))
+    assert np.all(np.diff(w) > 0)
+    t = np.linspace(0.0, 1.3, 131)
+    ref = oracle.chebyshev4_smoother(np.ones_like(t), np.zeros_like(t), lambda u: t * u, np.ones_like(t), n, 1.0)
+    prod = np.ones_like(t)
+    for wi in w:
+        prod = prod - wi * t * prod      # one sweep: e <- (1 - w t) e
+    assert np.abs(prod - ref).max() <= 1e-13
+    assert abs(ref[100] - (-1) ** n / (2 * n + 1)) <= 1e-14
+    assert np.abs(ref[:101]).max() <= 1.0 + 1e-14
+
+
+def test_estimate_rho_power_method(oracle):
+    """The power method of the reference converges to the top eigenvalue of
+    D^-1 A (a generalised symmetric eigenproblem, checked with scipy) on a
+    small clamped hex8 block."""
+    import scipy.linalg as sla
+    nel = (4, 2, 2)
+    K0 = oracle.k0_exact(3, 0.3)
+    el = oracle.elements(nel)
+    E = np.random.default_rng(2).uniform(0.01, 1.0, len(el)) ** 3
+    n = int(np.prod([v + 1 for v in nel])) * 3
+    con = np.zeros(n, bool)
+    con[:9] = True
+    A = oracle.assemble(K0, el, E, n, con).toarray()
+    d = np.diag(A)
+    lam = sla.eigh(A, np.diag(d), eigvals_only=True)[-1]
+    est = oracle.estimate_rho(lambda u: A @ u, 1.0 / d, np.random.default_rng(0).random(n), 200)
+    assert abs(est - lam) <= 1e-6 * lam
