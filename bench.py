@@ -45,34 +45,26 @@ CONFIGS = {
 }
 
 
+# SIMP legs: name -> MultiGrid options.  "fp64" is the plain FP64 GMG-PCG of
+# the specification (fixed-weight Jacobi); the others change only the solver.
+LEGS = {
+    "fp64": dict(precision="fp64", smoother="jacobi"),
+    "mixed": dict(precision="mixed", smoother="jacobi"),
+    "fp64_cheb": dict(precision="fp64", smoother="chebyshev"),
+    "mixed_cheb": dict(precision="mixed", smoother="chebyshev"),
+}
+
+
 def build_problem(pf, nel, n_level, tol=1e-8, precision="fp64", kind="cantilever", vf=0.3, r=1.5, omega=0.6,
-                  omega_schedule="reference"):
+                  omega_schedule="reference", smoother="jacobi"):
     """cantilever: x = 0 face clamped, total load -1 in y over the nodes of
     the bottom edge (y = 0) of the x = L face.  bridge: the four bottom
     corner nodes (y = 0, x in {0, L}, z in {0, W}) pinned in all DOFs, total
     load -1 in y spread evenly over the nodes of the top surface y = H."""
-    mesh = pf.StructuredMesh3D(*nel, *nel, physics=pf.LinearElasticity(E=1.0, nu=0.3))
-    K = pf.StructuredStiffnessKernel(mesh)
-    solver = pf.MultiGrid(mesh, K, maxiter=1000, tol=tol, n_smooth=2, omega=omega, n_level=n_level, cycle="V",
-                          precision=precision, omega_schedule=omega_schedule)
-    FE = pf.FiniteElement(mesh, K, solver)
-    nx, ny, nz = (n + 1 for n in nel)
-    if kind == "cantilever":
-        j, k = np.meshgrid(np.arange(ny), np.arange(nz), indexing="ij")
-        FE.add_dirichlet_boundary_condition(node_ids=(k + nz * nx * j).ravel(), rhs=0)
-        FE.add_point_forces(forces=np.array([[0.0, -1.0 / nz, 0.0]]), node_ids=np.arange(nz) + nz * (nx - 1))
-    elif kind == "bridge":
-        corners = np.array([k + nz * i for i in (0, nx - 1) for k in (0, nz - 1)])   # y = 0 plane
-        FE.add_dirichlet_boundary_condition(node_ids=corners, rhs=0)
-        i, k = np.meshgrid(np.arange(nx), np.arange(nz), indexing="ij")
-        top = (k + nz * i + nz * nx * (ny - 1)).ravel()
-        FE.add_point_forces(forces=np.array([[0.0, -1.0 / top.size, 0.0]]), node_ids=top)
-    else:
-        raise ValueError(kind)
-    filt = pf.StructuredFilter3D(mesh, r)
-    prob = pf.MinimumCompliance(FE, filt, E_mul=[1.0], void=1e-9, penalty=3.0, volume_fraction=[vf],
-                                heavyside=False, law="modified")
-    return mesh, K, solver, FE, filt, prob
+    from pyfantom_amd import workloads
+    build = workloads.cantilever if kind == "cantilever" else workloads.bridge
+    return build(nel, vf=vf, r=r, maxiter=1000, tol=tol, n_smooth=2, omega=omega, n_level=n_level, cycle="V",
+                 precision=precision, omega_schedule=omega_schedule, smoother=smoother)
 
 
 def time_ku(torch, pf, K, mesh, reps=100):
@@ -155,7 +147,18 @@ def time_construct(torch, K, mesh, reps):
             "achieved_gbs": alg / t / 1e9, "hbm_fraction": alg / t / 8e12, "kernel": "csr_fill_k<3>"}
 
 
-def precision_errors(torch, pf, mesh, FE_m, FE, n_level):
+def headline(legs):
+    """The fastest leg that reached the FP64 compliance to 1e-6: its time
+    per SIMP iteration and its ratio to the plain FP64 GMG-PCG."""
+    ok = {k: v for k, v in legs.items() if v["compliance_rel_diff_vs_fp64"] <= 1e-6}
+    if not ok:
+        return None
+    k = max(ok, key=lambda n: ok[n]["speedup_vs_fp64"])
+    return {"leg": k, "s_per_iter": ok[k]["s_per_iter"], "speedup_vs_fp64": ok[k]["speedup_vs_fp64"],
+            "compliance_rel_diff_vs_fp64": ok[k]["compliance_rel_diff_vs_fp64"]}
+
+
+def precision_errors(torch, pf, mesh, FE_m, FE, n_level, smoother="jacobi"):
     """One cold solve of the seeded field (x ~ U[0.01, 1], default_rng(0),
     E = Emin + x^3) with the mixed-precision solver and with the FP64 one:
     relative errors of u (L2), compliance and the element sensitivity
@@ -168,7 +171,7 @@ def precision_errors(torch, pf, mesh, FE_m, FE, n_level):
     out = {}
     for tag, fe, prec in (("mixed", FE_m, "mixed"), ("fp64", FE, "fp64")):
         s = pf.MultiGrid(mesh, fe.kernel, maxiter=1000, tol=1e-8, n_smooth=2, omega=0.4, n_level=n_level,
-                         cycle="V", precision=prec, omega_schedule="fixed", fallback=False)
+                         cycle="V", precision=prec, omega_schedule="fixed", fallback=False, smoother=smoother)
         fe.kernel.set_rho(E)
         U, res = s.solve(fe.rhs, use_last=False)
         out[tag] = (U.clone(), float(torch.dot(fe.rhs, U)), fe.kernel.process_grad(U).clone(), res,
@@ -179,7 +182,8 @@ def precision_errors(torch, pf, mesh, FE_m, FE, n_level):
     return {"u_rel_l2": float(nrm(um - u6) / nrm(u6)), "compliance_rel": abs(cm - c6) / abs(c6),
             "dc_rel_l2": float(nrm(dm - d6) / nrm(d6)), "rel_residual_mixed": float(rm),
             "rel_residual_fp64": float(r6), "pcg_iterations_mixed": im, "pcg_iterations_fp64": i6,
-            "solver": "GMG-PCG, 2+2 Jacobi sweeps, fixed omega 0.4, cold start, tol 1e-8"}
+            "solver": ("GMG-PCG, 2+2 Jacobi sweeps, fixed omega 0.4" if smoother == "jacobi" else
+                       "GMG-PCG, fourth-kind Chebyshev 2+2") + ", cold start, tol 1e-8"}
 
 
 def measured_traffic(workload):
@@ -235,7 +239,9 @@ def main():
     ap.add_argument("--config", default="c3", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--ku-reps", type=int, default=100)
-    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision SIMP run")
+    ap.add_argument("--legs", default="fp64,mixed,mixed_cheb",
+                    help="SIMP legs to time after the FP64 baseline: " + ", ".join(sorted(LEGS)))
+    ap.add_argument("--no-seeded", action="store_true", help="skip the seeded-field precision comparison")
     ap.add_argument("--progress", action="store_true", help="print compliance and PCG iterations every step")
     ap.add_argument("--omega-schedule", default=None, choices=["reference", "fixed"],
                     help="Jacobi weight: the reference's adaptive schedule (omega is its maximum) or fixed omega "
@@ -287,17 +293,19 @@ def main():
 
     fallbacks = []
 
-    def simp_run(precision):
-        """W untimed + K timed SIMP iterations; returns (seconds, iterations, problem objects)."""
-        mesh, K, solver, FE, filt, prob = build_problem(pf, nel, n_level, precision=precision,
-                                                        kind=cfg.get("kind", "cantilever"), vf=cfg.get("vf", 0.3),
-                                                        r=cfg.get("r", 1.5), omega=cfg.get("omega", 0.6),
-                                                        omega_schedule=args.omega_schedule)
-        print(f"# {precision}: setup done: {cfg['name']} dofs={K.shape[0]}", file=sys.stderr, flush=True)
+    def simp_run(leg):
+        """W untimed + K timed SIMP iterations of one leg; returns (seconds,
+        PCG iterations, outer iterations, problem objects)."""
+        opts = LEGS[leg]
+        mesh, K, solver, FE, filt, prob = build_problem(pf, nel, n_level, kind=cfg.get("kind", "cantilever"),
+                                                        vf=cfg.get("vf", 0.3), r=cfg.get("r", 1.5),
+                                                        omega=cfg.get("omega", 0.6),
+                                                        omega_schedule=args.omega_schedule, **opts)
+        print(f"# {leg}: setup done: {cfg['name']} dofs={K.shape[0]}", file=sys.stderr, flush=True)
         opt = pf.MMA(prob) if cfg.get("optimizer") == "mma" else pf.OC(prob, move=0.2)
         for w in range(args.warmup):
             opt.iter()
-            print(f"# {precision}: warmup {w}: c={prob.f():.6e} iters={solver.last_iterations}", file=sys.stderr,
+            print(f"# {leg}: warmup {w}: c={prob.f():.6e} iters={solver.last_iterations}", file=sys.stderr,
                   flush=True)
         torch.cuda.synchronize()
         iters, outer = [], []
@@ -310,38 +318,38 @@ def main():
             if solver.last_fallback:
                 fallbacks.append((s, solver.last_fallback_iterations))
             if args.progress:
-                print(f"# {precision}: step {s}: c={prob.f():.6e} iters={solver.last_iterations}", file=sys.stderr,
+                print(f"# {leg}: step {s}: c={prob.f():.6e} iters={solver.last_iterations}", file=sys.stderr,
                       flush=True)
         torch.cuda.synchronize()
         t = time.perf_counter() - t0
-        print(f"# {precision}: timed {args.steps} steps: {t / args.steps * 1e3:.1f} ms/step, pcg iters {iters}",
+        print(f"# {leg}: timed {args.steps} steps: {t / args.steps * 1e3:.1f} ms/step, pcg iters {iters}",
               file=sys.stderr, flush=True)
         return t, iters, outer, (mesh, K, solver, FE, filt, prob)
 
+    def rel(a, b):
+        return float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))
+
     t, iters, _, (mesh, K, solver, FE, filt, prob) = simp_run("fp64")
     ms = t / args.steps * 1e3
-    mixed = None
-    if not args.no_mixed:
-        tm, it_m, out_m, objs_m = simp_run("mixed")
-        fb_m = list(fallbacks)
-        prob_m = objs_m[5]
-
-        def rel(a, b):
-            return float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))
-
-        mixed = {"s_per_iter": tm / args.steps, "inner_iterations_mean": float(np.mean(it_m)),
-                 "outer_iterations_mean": float(np.mean([o for o in out_m if o is not None] or [0])),
-                 "speedup_vs_fp64": t / tm, "compliance": prob_m.f(),
-                 "compliance_rel_diff_vs_fp64": abs(prob_m.f() - prob.f()) / abs(prob.f()),
-                 "solve_rel_residual": float(prob_m._residual),
-                 # end state of the same number of SIMP steps from the same start
-                 "u_rel_l2_vs_fp64": rel(prob_m._U, prob._U),
-                 "dc_rel_l2_vs_fp64": rel(prob_m.nabla_f(), prob.nabla_f()),
-                 "seeded_field": precision_errors(torch, pf, mesh, objs_m[3], FE, n_level),
-                 "fp64_fallbacks": [{"step": s, "fp64_iterations": n} for s, n in fb_m],
-                 "note": "FP64 PCG (FP64 K.u, dots, step lengths, true residual) preconditioned by an FP32 "
-                         "V-cycle (FP32 WHT K.u, Galerkin level 1, stencils, transfers); same SIMP steps"}
-        del objs_m, prob_m
+    legs = {}
+    for leg in [l for l in args.legs.split(",") if l and l != "fp64"]:
+        tl, it_l, out_l, objs_l = simp_run(leg)
+        fb_l = list(fallbacks)
+        prob_l = objs_l[5]
+        legs[leg] = {"options": LEGS[leg], "s_per_iter": tl / args.steps, "speedup_vs_fp64": t / tl,
+                     "pcg_iterations_mean": float(np.mean(it_l)), "pcg_iterations": it_l,
+                     "compliance": prob_l.f(),
+                     # end state of the same number of SIMP steps from the same start
+                     "compliance_rel_diff_vs_fp64": abs(prob_l.f() - prob.f()) / abs(prob.f()),
+                     "solve_rel_residual": float(prob_l._residual),
+                     "u_rel_l2_vs_fp64": rel(prob_l._U, prob._U),
+                     "dc_rel_l2_vs_fp64": rel(prob_l.nabla_f(), prob.nabla_f()),
+                     "fp64_fallbacks": [{"step": s, "fp64_iterations": n} for s, n in fb_l]}
+        if LEGS[leg]["precision"] == "mixed" and not args.no_seeded:
+            legs[leg]["seeded_field"] = precision_errors(torch, pf, mesh, objs_l[3], FE, n_level,
+                                                         smoother=LEGS[leg]["smoother"])
+        del objs_l, prob_l
+        torch.cuda.empty_cache()
     ku = time_ku(torch, pf, K, mesh, args.ku_reps)
     print(f"# K.u: {ku['seconds']*1e6:.1f} us, {ku['achieved_gbs']:.0f} GB/s", file=sys.stderr, flush=True)
     out = {
@@ -367,7 +375,8 @@ def main():
         "solve_rel_residual": float(prob._residual),
         "ku": {"gdof_s": ku["gdof_s"], "seconds": ku["seconds"], "kernel": ku["kernel"],
                "buffer_sets": ku["buffer_sets"]},
-        "mixed_precision": mixed,
+        "legs": legs,
+        "headline": headline(legs),
         "roofline": {"bound": "hbm", "achieved": ku["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ku["achieved_gbs"] / HBM_PEAK_GBS, "traffic": measured_traffic(cfg["name"]),
                      "kernel": ku["kernel"], "algorithmic_bytes": ku["bytes"],
