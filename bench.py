@@ -52,19 +52,30 @@ LEGS = {
     "mixed": dict(precision="mixed", smoother="jacobi"),
     "fp64_cheb": dict(precision="fp64", smoother="chebyshev"),
     "mixed_cheb": dict(precision="mixed", smoother="chebyshev"),
+    # flexible PCG: FP32 V-cycle + the committed network's correction, FP64 GMG-PCG fallback
+    "mixed_learned": dict(precision="mixed", smoother="jacobi", learned=True),
 }
 
 
 def build_problem(pf, nel, n_level, tol=1e-8, precision="fp64", kind="cantilever", vf=0.3, r=1.5, omega=0.6,
-                  omega_schedule="reference", smoother="jacobi"):
+                  omega_schedule="reference", smoother="jacobi", learned=False):
     """cantilever: x = 0 face clamped, total load -1 in y over the nodes of
     the bottom edge (y = 0) of the x = L face.  bridge: the four bottom
     corner nodes (y = 0, x in {0, L}, z in {0, W}) pinned in all DOFs, total
-    load -1 in y spread evenly over the nodes of the top surface y = H."""
+    load -1 in y spread evenly over the nodes of the top surface y = H.
+    learned: the solver is LearnedSolver with the committed network."""
     from pyfantom_amd import workloads
     build = workloads.cantilever if kind == "cantilever" else workloads.bridge
-    return build(nel, vf=vf, r=r, maxiter=1000, tol=tol, n_smooth=2, omega=omega, n_level=n_level, cycle="V",
-                 precision=precision, omega_schedule=omega_schedule, smoother=smoother)
+    mesh, K, solver, FE, filt, prob = build(nel, vf=vf, r=r, maxiter=1000, tol=tol, n_smooth=2, omega=omega,
+                                            n_level=n_level, cycle="V", precision=precision,
+                                            omega_schedule=omega_schedule, smoother=smoother)
+    if learned:
+        from pyfantom_amd.learned import ConvNet, LearnedSolver
+        net = ConvNet.from_file(os.path.join(ROOT, "pyfantom_amd", "data", "learned_precond_96x48x48.npz"))
+        solver = LearnedSolver(mesh, K, net, maxiter=1000, tol=tol, precision=precision, n_level=n_level,
+                               n_smooth=2, omega=omega, smoother=smoother, omega_schedule=omega_schedule)
+        FE.solver = solver   # the optimiser's first evaluation already uses it
+    return mesh, K, solver, FE, filt, prob
 
 
 def time_ku(torch, pf, K, mesh, reps=100):
@@ -239,7 +250,7 @@ def main():
     ap.add_argument("--config", default="c3", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--ku-reps", type=int, default=100)
-    ap.add_argument("--legs", default="fp64,mixed,mixed_cheb",
+    ap.add_argument("--legs", default="fp64,mixed,mixed_cheb,mixed_learned",
                     help="SIMP legs to time after the FP64 baseline: " + ", ".join(sorted(LEGS)))
     ap.add_argument("--no-seeded", action="store_true", help="skip the seeded-field precision comparison")
     ap.add_argument("--progress", action="store_true", help="print compliance and PCG iterations every step")
@@ -345,7 +356,7 @@ def main():
                      "u_rel_l2_vs_fp64": rel(prob_l._U, prob._U),
                      "dc_rel_l2_vs_fp64": rel(prob_l.nabla_f(), prob.nabla_f()),
                      "fp64_fallbacks": [{"step": s, "fp64_iterations": n} for s, n in fb_l]}
-        if LEGS[leg]["precision"] == "mixed" and not args.no_seeded:
+        if LEGS[leg]["precision"] == "mixed" and not LEGS[leg].get("learned") and not args.no_seeded:
             legs[leg]["seeded_field"] = precision_errors(torch, pf, mesh, objs_l[3], FE, n_level,
                                                          smoother=LEGS[leg]["smoother"])
         del objs_l, prob_l

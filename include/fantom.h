@@ -367,6 +367,10 @@ int fantom_mx_set_omega_schedule(void *handle, int adaptive, double min_omega, d
 int fantom_mx_set_smoother(void *handle, int kind, double rho_scale, int iters_first, int iters);
 int fantom_mx_smoother_rho(void *handle, double *rho, int n);
 int fantom_mx_setup(void *handle, const double *E, const uint8_t *mask, void *stream);
+/* z = V(r) with the FP32 V-cycle: r (FP64) is rounded to FP32, the cycle runs
+ * as in fantom_mx_solve_pcg, z is returned in FP64 (a preconditioner for a
+ * caller's own Krylov loop, e.g. flexible PCG). */
+int fantom_mx_precondition(void *handle, const double *r, double *z, void *stream);
 int fantom_mx_solve(void *handle, const double *f, double *x, double tol, double inner_tol, int max_outer,
                     int max_inner, int use_x0, int use_graph, int *outer_iters, int *inner_iters, double *rel_res,
                     void *stream);

@@ -84,6 +84,7 @@ SIGNATURES = {
     "fantom_mx_destroy": (_c_int, [_vp]),
     "fantom_mx_set_omega_schedule": (_c_int, [_vp, _c_int, _c_double, _c_double]),
     "fantom_mx_set_smoother": (_c_int, [_vp, _c_int, _c_double, _c_int, _c_int]),
+    "fantom_mx_precondition": (_c_int, [_vp, _vp, _vp, _vp]),
     "fantom_mx_smoother_rho": (_c_int, [_vp, _vp, _c_int]),
     "fantom_mx_setup": (_c_int, [_vp, _vp, _vp, _vp]),
     "fantom_mx_solve": (_c_int, [_vp, _vp, _vp, _c_double, _c_double, _c_int, _c_int, _c_int, _c_int,

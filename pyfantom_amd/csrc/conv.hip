@@ -14,6 +14,8 @@
 // Forward: M = voxels, K = c_in (per tap), N = c_out.  Backward-weight:
 // M = c_out, K = voxels, N = c_in.  Backward-data is the forward with the
 // tap-flipped, channel-transposed weights.
+#include <algorithm>
+
 #include "common.hpp"
 
 namespace fantom {
@@ -118,6 +120,221 @@ __global__ __launch_bounds__(kConvBlock) void conv3d_fwd_k(int D, int H, int W, 
                     out_f32[v * Cout + co] = val;
             }
         }
+}
+
+// Forward with the weights staged in LDS: a workgroup holds the weights of
+// CO_W output tiles (cout [16 CO_W blockIdx.y, +16 CO_W)) for all 27 taps and
+// Cin = 16 CI_T, loaded once, rows padded by 4 entries so that the B-operand
+// reads of the 16 rows fall in distinct banks.  Each wave then walks groups
+// of kFwdRT consecutive 16-voxel row tiles (grid-stride), and every weight
+// operand read from LDS feeds kFwdRT MFMAs.  Same output as conv3d_fwd_k.
+constexpr int kFwdRT = 4;
+
+__host__ __device__ constexpr int fwd2_row(int cin) { return cin + 4; }   // LDS row stride (entries)
+
+template <int CI_T, int CO_W>
+__global__ __launch_bounds__(kConvBlock) void conv3d_fwd2_k(int D, int H, int W, int Cout,
+                                                          const uint16_t *__restrict__ x,
+                                                          const uint16_t *__restrict__ w,
+                                                          const float *__restrict__ bias,
+                                                          const uint16_t *__restrict__ skip,
+                                                          uint16_t *__restrict__ out, float *__restrict__ out_f32,
+                                                          int relu)
+{
+    constexpr int Cin = CI_T * 16, Srow = fwd2_row(Cin), CoW = CO_W * 16;
+    extern __shared__ uint16_t Wl[];   // [27][CoW][Srow]
+    const int cb = blockIdx.y * CoW;
+    // stage the weight slice: 8-byte pieces of w[t][cb + co][ci]
+    for (int q = threadIdx.x; q < 27 * CoW * (Cin / 4); q += kConvBlock) {
+        const int piece = q % (Cin / 4), rowi = q / (Cin / 4);
+        const int t = rowi / CoW, co = rowi % CoW;
+        *reinterpret_cast<s4 *>(Wl + (t * CoW + co) * Srow + 4 * piece) =
+            *reinterpret_cast<const s4 *>(w + ((long)t * Cout + cb + co) * Cin + 4 * piece);
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int row = lane & 15, kg = lane >> 4;
+    const long N = (long)D * H * W;
+    const long ngroups = (N + 16 * kFwdRT - 1) / (16 * kFwdRT);
+    for (long gi = (long)blockIdx.x * 4 + wave; gi < ngroups; gi += (long)gridDim.x * 4) {
+        const long v0 = gi * (16 * kFwdRT);
+        int pj[kFwdRT], pi[kFwdRT], pk[kFwdRT];
+        bool pok[kFwdRT];
+#pragma unroll
+        for (int rt = 0; rt < kFwdRT; ++rt) {
+            const long v = v0 + rt * 16 + row;
+            pok[rt] = v < N;
+            const long vv = pok[rt] ? v : 0;
+            pk[rt] = (int)(vv % W);
+            pi[rt] = (int)((vv / W) % H);
+            pj[rt] = (int)(vv / ((long)W * H));
+        }
+        f4 acc[kFwdRT][CO_W];
+#pragma unroll
+        for (int rt = 0; rt < kFwdRT; ++rt)
+#pragma unroll
+            for (int ct = 0; ct < CO_W; ++ct)
+                acc[rt][ct] = f4{0.f, 0.f, 0.f, 0.f};
+        for (int t = 0; t < 27; ++t) {
+            const int dy = t / 9 - 1, dx = (t / 3) % 3 - 1, dz = t % 3 - 1;
+            long nb[kFwdRT];
+            bool ok[kFwdRT];
+#pragma unroll
+            for (int rt = 0; rt < kFwdRT; ++rt) {
+                const int j = pj[rt] + dy, i = pi[rt] + dx, k = pk[rt] + dz;
+                ok[rt] = pok[rt] && j >= 0 && j < D && i >= 0 && i < H && k >= 0 && k < W;
+                nb[rt] = ok[rt] ? ((long)j * H + i) * W + k : 0;
+            }
+#pragma unroll
+            for (int c = 0; c < CI_T; ++c) {
+                s4 a[kFwdRT];
+#pragma unroll
+                for (int rt = 0; rt < kFwdRT; ++rt) {
+                    a[rt] = s4{0, 0, 0, 0};
+                    if (ok[rt])
+                        a[rt] = *reinterpret_cast<const s4 *>(x + nb[rt] * Cin + c * 16 + 4 * kg);
+                }
+#pragma unroll
+                for (int ct = 0; ct < CO_W; ++ct) {
+                    const s4 b = *reinterpret_cast<const s4 *>(Wl + (t * CoW + ct * 16 + row) * Srow + c * 16 + 4 * kg);
+#pragma unroll
+                    for (int rt = 0; rt < kFwdRT; ++rt)
+                        acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[rt], b, acc[rt][ct], 0, 0, 0);
+                }
+            }
+        }
+#pragma unroll
+        for (int rt = 0; rt < kFwdRT; ++rt)
+#pragma unroll
+            for (int ct = 0; ct < CO_W; ++ct) {
+                const int co = cb + ct * 16 + row;
+                const float bco = bias ? bias[co] : 0.f;
+#pragma unroll
+                for (int ii = 0; ii < 4; ++ii) {
+                    const long v = v0 + rt * 16 + 4 * kg + ii;
+                    if (v >= N)
+                        continue;
+                    float val = acc[rt][ct][ii] + bco;
+                    if (skip)
+                        val += bf2f(skip[v * Cout + co]);
+                    if (relu)
+                        val = fmaxf(val, 0.f);
+                    out[v * Cout + co] = f2bf(val);
+                    if (out_f32)
+                        out_f32[v * Cout + co] = val;
+                }
+            }
+    }
+}
+
+// Forward on bricks staged in LDS (level-0/1 shapes, Cin <= 32): a workgroup
+// owns a brick of 16 (W) x BI (H) x BJ (D) output voxels, loads the input
+// brick with its one-voxel halo -- 18 x (BI + 2) x (BJ + 2) voxels, rows padded
+// by 4 channels so that the 16 voxels of an A operand hit distinct banks --
+// and the weights of its CO_T output tiles once, then each wave runs the 27
+// taps of its row tiles (16 voxels along W at fixed (j, i)) from LDS only.
+// Every input voxel is read from memory once per brick instead of 27 times.
+template <int CI_T, int CO_T, int BI, int BJ>
+__global__ __launch_bounds__(kConvBlock) void conv3d_brick_k(int D, int H, int W, int Cout,
+                                                           const uint16_t *__restrict__ x,
+                                                           const uint16_t *__restrict__ w,
+                                                           const float *__restrict__ bias,
+                                                           const uint16_t *__restrict__ skip,
+                                                           uint16_t *__restrict__ out, float *__restrict__ out_f32,
+                                                           int relu)
+{
+    constexpr int Cin = CI_T * 16, Sv = Cin + 4, CoT = CO_T * 16, Sw = Cin + 4;
+    constexpr int HK = 18, HI = BI + 2, HJ = BJ + 2, NHV = HK * HI * HJ;
+    constexpr int RTW = BI * BJ / 4;   // row tiles per wave
+    extern __shared__ uint16_t sm[];
+    uint16_t *Xs = sm;                 // [NHV][Sv]
+    uint16_t *Ws = sm + NHV * Sv;      // [27][CoT][Sw]
+    const int nbk = (W + 15) / 16, nbi = (H + BI - 1) / BI;
+    const int bk = blockIdx.x % nbk, bi = (blockIdx.x / nbk) % nbi, bj = blockIdx.x / (nbk * nbi);
+    const int k0 = bk * 16, i0 = bi * BI, j0 = bj * BJ, cb = blockIdx.y * CoT;
+    // input halo brick, 8-byte pieces; zero outside the volume
+    for (int q = threadIdx.x; q < NHV * (Cin / 4); q += kConvBlock) {
+        const int piece = q % (Cin / 4), hv = q / (Cin / 4);
+        const int hk = hv % HK, hi = (hv / HK) % HI, hj = hv / (HK * HI);
+        const int k = k0 - 1 + hk, i = i0 - 1 + hi, j = j0 - 1 + hj;
+        s4 val = s4{0, 0, 0, 0};
+        if (k >= 0 && k < W && i >= 0 && i < H && j >= 0 && j < D)
+            val = *reinterpret_cast<const s4 *>(x + (((long)j * H + i) * W + k) * Cin + 4 * piece);
+        *reinterpret_cast<s4 *>(Xs + hv * Sv + 4 * piece) = val;
+    }
+    for (int q = threadIdx.x; q < 27 * CoT * (Cin / 4); q += kConvBlock) {
+        const int piece = q % (Cin / 4), rowi = q / (Cin / 4);
+        const int t = rowi / CoT, co = rowi % CoT;
+        *reinterpret_cast<s4 *>(Ws + (t * CoT + co) * Sw + 4 * piece) =
+            *reinterpret_cast<const s4 *>(w + ((long)t * Cout + cb + co) * Cin + 4 * piece);
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int row = lane & 15, kg = lane >> 4;
+    f4 acc[RTW][CO_T];
+#pragma unroll
+    for (int r = 0; r < RTW; ++r)
+#pragma unroll
+        for (int ct = 0; ct < CO_T; ++ct)
+            acc[r][ct] = f4{0.f, 0.f, 0.f, 0.f};
+    for (int t = 0; t < 27; ++t) {
+        const int dy = t / 9 - 1, dx = (t / 3) % 3 - 1, dz = t % 3 - 1;
+#pragma unroll
+        for (int c = 0; c < CI_T; ++c) {
+            s4 b[CO_T];
+#pragma unroll
+            for (int ct = 0; ct < CO_T; ++ct)
+                b[ct] = *reinterpret_cast<const s4 *>(Ws + (t * CoT + ct * 16 + row) * Sw + c * 16 + 4 * kg);
+#pragma unroll
+            for (int r = 0; r < RTW; ++r) {
+                const int rt = wave * RTW + r, il = rt % BI, jl = rt / BI;
+                const int hv = ((jl + 1 + dy) * HI + (il + 1 + dx)) * HK + (row + 1 + dz);
+                const s4 a = *reinterpret_cast<const s4 *>(Xs + hv * Sv + c * 16 + 4 * kg);
+#pragma unroll
+                for (int ct = 0; ct < CO_T; ++ct)
+                    acc[r][ct] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b[ct], acc[r][ct], 0, 0, 0);
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < RTW; ++r) {
+        const int rt = wave * RTW + r, il = rt % BI, jl = rt / BI;
+        const int i = i0 + il, j = j0 + jl;
+        if (i >= H || j >= D)
+            continue;
+#pragma unroll
+        for (int ct = 0; ct < CO_T; ++ct) {
+            const int co = cb + ct * 16 + row;
+            const float bco = bias ? bias[co] : 0.f;
+#pragma unroll
+            for (int ii = 0; ii < 4; ++ii) {
+                const int k = k0 + 4 * kg + ii;
+                if (k >= W)
+                    continue;
+                const long v = ((long)j * H + i) * W + k;
+                float val = acc[r][ct][ii] + bco;
+                if (skip)
+                    val += bf2f(skip[v * Cout + co]);
+                if (relu)
+                    val = fmaxf(val, 0.f);
+                out[v * Cout + co] = f2bf(val);
+                if (out_f32)
+                    out_f32[v * Cout + co] = val;
+            }
+        }
+    }
+}
+
+template <int CI_T, int CO_T, int BI, int BJ>
+static void launch_brick(int D, int H, int W, int Cout, const uint16_t *x, const uint16_t *w, const float *bias,
+                         const uint16_t *skip, uint16_t *out, float *out_f32, int relu, hipStream_t s)
+{
+    constexpr int Cin = CI_T * 16;
+    const unsigned nb = (unsigned)(((W + 15) / 16) * ((H + BI - 1) / BI) * ((D + BJ - 1) / BJ));
+    const size_t lds = ((size_t)18 * (BI + 2) * (BJ + 2) * (Cin + 4) + (size_t)27 * CO_T * 16 * (Cin + 4)) *
+                       sizeof(uint16_t);
+    hipLaunchKernelGGL((conv3d_brick_k<CI_T, CO_T, BI, BJ>), dim3(nb, (unsigned)(Cout / (16 * CO_T))),
+                       dim3(kConvBlock), lds, s, D, H, W, Cout, x, w, bias, skip, out, out_f32, relu);
 }
 
 // Partial weight gradients: part[p][tap][co][ci] = sum over this wave's voxel
@@ -417,8 +634,38 @@ int fantom_conv3d_fwd_bf16(int D, int H, int W, int Cin, int Cout, const uint16_
     FANTOM_CHECK_ARG(D > 0 && H > 0 && W > 0 && x && w && out, "bad operands");
     FANTOM_CHECK_ARG(Cin % 16 == 0 && Cin >= 16 && Cin <= 256, "Cin must be a multiple of 16 in [16, 256]");
     const long N = (long)D * H * W;
-    const unsigned blocks = (unsigned)((N + 4 * 16 * kRowTiles - 1) / (4 * 16 * kRowTiles));
     hipStream_t s = as_stream(stream);
+    // input bricks and weights in LDS (<= 64 KB) for Cin = 16 (the 32-channel
+    // brick measured slower than the weight-staged kernel below)
+    if (Cin == 16 && (Cout == 16 || Cout == 32)) {
+        if (Cout == 16)
+            launch_brick<1, 1, 8, 2>(D, H, W, Cout, x, w, bias, skip, out, out_f32, relu, s);
+        else
+            launch_brick<1, 2, 8, 2>(D, H, W, Cout, x, w, bias, skip, out, out_f32, relu, s);
+        FANTOM_LAUNCH_CHECK("conv3d_fwd");
+        return kStatusOk;
+    }
+    // LDS-staged weights for Cin <= 64: (CI_T, CO_W) with the slice <= 64 KB
+    {
+        const int ci_t = Cin / 16;
+        const int co_w = (ci_t == 4 || Cout == 16) ? 1 : 2;
+        if (ci_t <= 4 && ci_t != 3 && (Cout == 16 || Cout == 32 || Cout == 64)) {
+            const long groups = (N + 16 * kFwdRT - 1) / (16 * kFwdRT);
+            const unsigned gx = (unsigned)std::min<long>((groups + 3) / 4, 2048);
+            const dim3 grid(gx, (unsigned)(Cout / (16 * co_w)));
+            const size_t lds = (size_t)27 * 16 * co_w * fwd2_row(Cin) * sizeof(uint16_t);
+#define FWD2(CI, CO)                                                                                       \
+    if (ci_t == CI && co_w == CO) {                                                                        \
+        hipLaunchKernelGGL((conv3d_fwd2_k<CI, CO>), grid, dim3(kConvBlock), lds, s, D, H, W, Cout, x, w, bias, \
+                           skip, out, out_f32, relu);                                                      \
+        FANTOM_LAUNCH_CHECK("conv3d_fwd");                                                                 \
+        return kStatusOk;                                                                                  \
+    }
+            FWD2(1, 1) FWD2(1, 2) FWD2(2, 1) FWD2(2, 2) FWD2(4, 1)
+#undef FWD2
+        }
+    }
+    const unsigned blocks = (unsigned)((N + 4 * 16 * kRowTiles - 1) / (4 * 16 * kRowTiles));
     switch (Cout) {
     case 16:
         hipLaunchKernelGGL(conv3d_fwd_k<1>, dim3(blocks), dim3(kConvBlock), 0, s, D, H, W, Cin, x, w, bias, skip, out,

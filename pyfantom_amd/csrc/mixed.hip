@@ -724,6 +724,26 @@ int fantom_mx_destroy(void *handle)
     return kStatusOk;
 }
 
+__global__ void mx_f2d_k(long n, const float *__restrict__ a, double *__restrict__ b)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        b[i] = (double)a[i];
+}
+
+int fantom_mx_precondition(void *handle, const double *r, double *z, void *stream)
+{
+    MxHandle *h = reinterpret_cast<MxHandle *>(handle);
+    FANTOM_CHECK_ARG(h && r && z, "null operand");
+    MX_TRY(mx_join(as_stream(stream), h->stream));
+    const long n = h->n;
+    const unsigned G = vec_grid(n);
+    hipLaunchKernelGGL(mxd_down_k, dim3(G), dim3(256), 0, h->stream, n, r, h->r);
+    MX_TRY(mx_precond(h, h->r, h->z));
+    hipLaunchKernelGGL(mx_f2d_k, dim3(G), dim3(256), 0, h->stream, n, h->z, z);
+    FANTOM_LAUNCH_CHECK("mx precondition");
+    return mx_join(h->stream, as_stream(stream));
+}
+
 __global__ void mx_e_down_k(long n, const double *__restrict__ a, float *__restrict__ b)
 {
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)

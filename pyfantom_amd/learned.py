@@ -294,6 +294,29 @@ class NetRunner:
         _lib.call("fantom_finalize_f64", _lib.ptr(self.partials), 256, _lib.ptr(self.scal), _lib.FIN_STORE, 1)
         return self.scal[1]
 
+    def energy_loss_and_grad(self, t, e, vec, free):
+        """Energy loss of the hybrid correction, L = 1 - (t.c)^2 / ((c.Kc)(t.e))
+        with c = N(t / rms(t)) (free DOFs) and t = K e: the fraction of the
+        error energy e.Ke that the Galerkin step g c (g = t.c / c.Kc) leaves.
+        Scale-free in c; dL/dc = -2a/(b s) t + 2a^2/(b^2 s) Kc with a = t.c,
+        b = c.Kc, s = t.e.  Fills the gradients, returns L (host float)."""
+        B = self._bufs()
+        y = self.forward(t).view(self.nv[0], 16)
+        c = y[:, :3].reshape(-1).double() * free
+        kc = torch.empty_like(c)
+        vec.ku(c, kc)
+        L, g = energy_loss_grad(t, e, c, kc)
+        if not math.isfinite(L):
+            return L
+        g = g * free
+        g = g / max(float(g.abs().max()), 1e-300)        # Adam is invariant to the gradient's scale
+        dout = B[(0, 16, 2)]   # backward() uses (0, 16, 0) and (0, 16, 1) as work buffers
+        dv = dout.view(self.nv[0], 16)
+        dv.zero_()
+        dv[:, :3] = g.view(-1, 3).to(torch.bfloat16).view(torch.int16)
+        self.backward(dout)
+        return L
+
     def adam_step(self, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):
         net = self.net
         net.step += 1
@@ -308,6 +331,18 @@ class NetRunner:
                       net.step)
             _lib.call("fantom_adam_f32", net.b[name].numel(), _lib.ptr(net.b[name]), _lib.ptr(self.gb[name]),
                       _lib.ptr(net.m_b[name]), _lib.ptr(net.v_b[name]), None, lr, b1, b2, eps, net.step)
+
+
+def energy_loss_grad(t, e, c, kc):
+    """L = 1 - a^2 / (b s) with a = t.c, b = c.Kc (kc = K c), s = t.e, and
+    dL/dc = -2a/(b s) t + 2a^2/(b^2 s) Kc (K symmetric).  For t = K e this is
+    the fraction of e's energy left after the best multiple of c is removed.
+    Returns (L, dL/dc); L is NaN when b or s is not positive."""
+    a, b, sv = float(torch.dot(t, c)), float(torch.dot(c, kc)), float(torch.dot(t, e))
+    if not (b > 0.0 and sv > 0.0 and math.isfinite(a)):
+        return float("nan"), None
+    g = (-2.0 * a / (b * sv)) * t + (2.0 * a * a / (b * b * sv)) * kc
+    return 1.0 - a * a / (b * sv), g
 
 
 class _Vec:
@@ -392,7 +427,7 @@ class LearnedPreconditioner:
 
 
 def fpcg_solve(kernel, f, precond, tol=1e-8, maxiter=500, x0=None, fallback=None, stall_window=20,
-               stall_factor=0.9):
+               stall_factor=0.9, record=None):
     """Flexible PCG (Polak-Ribiere beta) with a nonlinear preconditioner.
 
     fallback: a MultiGrid solver; if the residual does not drop below
@@ -400,6 +435,7 @@ def fpcg_solve(kernel, f, precond, tol=1e-8, maxiter=500, x0=None, fallback=None
     preconditioned direction breaks down (r.z or p.Kp not positive and
     finite), the solve continues with plain GMG-PCG from the current iterate.
     Without a fallback a breakdown restarts the recurrence with z = r.
+    record (optional): called as record(x) after every iterate update.
     Returns (x, info) with info = {iterations, fallback, residual, history}."""
     vec = precond.vec if hasattr(precond, "vec") else _Vec(kernel)
     n = f.numel()
@@ -452,6 +488,8 @@ def fpcg_solve(kernel, f, precond, tol=1e-8, maxiter=500, x0=None, fallback=None
             alpha = rz / pq
             vec.axpby(alpha, p, 1.0, x)
             vec.axpby(-alpha, q, 1.0, r)
+            if record is not None:
+                record(x)
             R = math.sqrt(max(vec.dot(r, r), 0.0)) / nb
             hist.append(R)
             if R < tol:
@@ -484,3 +522,65 @@ def _fallback_from(mg, f, x):
     """Continue with GMG-PCG from the iterate x."""
     mg.last_x0 = x.clone()
     return mg.solve(f, use_last=True)
+
+
+def density_from_moduli(E, Emin=1e-9, E0=1.0, penalty=3.0):
+    """Physical (filtered) element density from the moduli of the modified
+    SIMP law E = Emin + rho^p (E0 - Emin): the network's density channel,
+    so that the solver needs only what FiniteElement.solve hands it."""
+    return ((E - Emin) / (E0 - Emin)).clamp(min=0.0) ** (1.0 / penalty)
+
+
+class LearnedSolver:
+    """Flexible PCG preconditioned by one GMG V-cycle (FP64, or FP32 for
+    precision='mixed') plus the learned correction of what the cycle leaves
+    (LearnedPreconditioner, mode 'hybrid'), with plain FP64 GMG-PCG as the
+    fallback on stagnation or breakdown.  Same protocol as MultiGrid
+    (solvers/commons/__init__.py:1-67): solve(rhs, rho=None, use_last=True)
+    -> (U, true relative residual), reset(), last_iterations, last_x0,
+    last_fallback, last_fallback_iterations.  No counterpart in the reference."""
+
+    def __init__(self, mesh, kernel, net: ConvNet, maxiter=500, tol=1e-8, precision="mixed", n_level=4, n_smooth=2,
+                 omega=0.6, smoother="jacobi", omega_schedule="fixed", Emin=1e-9, E0=1.0, penalty=3.0,
+                 stall_window=20, fallback_maxiter=1000):
+        from .solvers import MultiGrid
+        self.kernel = kernel
+        self.maxiter = int(maxiter)
+        self.tol = float(tol)
+        self.Emin, self.E0, self.penalty = Emin, E0, penalty
+        self.stall_window = int(stall_window)
+        opts = dict(n_smooth=n_smooth, omega=omega, n_level=n_level, cycle="V", omega_schedule=omega_schedule,
+                    smoother=smoother)
+        self.mg = MultiGrid(mesh, kernel, maxiter=1, tol=tol, precision=precision, fallback=False, **opts)
+        self.fb = MultiGrid(mesh, kernel, maxiter=fallback_maxiter, tol=tol, precision="fp64", **opts)
+        self.M = LearnedPreconditioner(net, kernel, mg=self.mg, mode="hybrid")
+        self._dens_state = None
+        self.last_x0 = None
+        self.last_iterations = 0
+        self.last_fallback = False
+        self.last_fallback_iterations = 0
+        self.last_info = None
+
+    def reset(self):
+        self.last_x0 = None
+        self._dens_state = None
+        self.mg.reset()
+        self.fb.reset()
+
+    def solve(self, rhs, rho=None, use_last=True):
+        k = self.kernel
+        if rho is not None:
+            k.set_rho(rho)
+        state = (k.rho_version, k.rho.data_ptr())
+        if self._dens_state != state:
+            self.M.set_density(density_from_moduli(k.rho, self.Emin, self.E0, self.penalty))
+            self._dens_state = state
+        x0 = self.last_x0 if (use_last and self.last_x0 is not None and self.last_x0.numel() == rhs.numel()) else None
+        x, info = fpcg_solve(k, rhs, self.M, tol=self.tol, maxiter=self.maxiter, x0=x0, fallback=self.fb,
+                             stall_window=self.stall_window)
+        self.last_iterations = info["iterations"]
+        self.last_fallback = info["fallback"]
+        self.last_fallback_iterations = info["fallback_iterations"]
+        self.last_info = info
+        self.last_x0 = x
+        return x, info["residual"]

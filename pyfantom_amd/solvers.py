@@ -248,14 +248,16 @@ class MultiGrid(_DeviceKrylov):
         return n_coarse
 
     def precondition_native(self, r, z):
-        """z = V(r): one V-cycle of the native FP64 hierarchy for the kernel's
-        current E (sets the hierarchy up if needed)."""
-        if self.precision != "fp64" or not self.native:
-            raise NotImplementedError("precondition_native needs the native FP64 driver")
+        """z = V(r): one V-cycle of the native hierarchy (FP64, or FP32 with
+        FP64 input and output for precision='mixed') for the kernel's current E
+        (sets the hierarchy up if needed)."""
+        if not self.native:
+            raise NotImplementedError("precondition_native needs the native driver")
         k = self.kernel
         if self._nh is None or self._nh_state != (k.rho_version, k.con_version, k.rho.data_ptr()):
             self._native_prepare()
-        _lib.call("fantom_mg_precondition", self._nh, _lib.ptr(r), _lib.ptr(z))
+        pre = "fantom_mx_" if self.precision == "mixed" else "fantom_mg_"
+        _lib.call(pre + "precondition", self._nh, _lib.ptr(r), _lib.ptr(z))
         return z
 
     def _native_schedule(self, L, pre, h):
@@ -277,33 +279,8 @@ class MultiGrid(_DeviceKrylov):
         return list(out)[:n]
 
     def _native_prepare(self):
-        import ctypes
-        k = self.kernel
-        L = _lib.load()
-        gal = int(self.coarse == "galerkin")
-        key = (k.grid, self.n_level, self.n_smooth, self.omega, k.kernel_kind, gal, self.omega_schedule,
-               self.min_omega, self.omega_boost, self.smoother, self.cheb_rho_scale, self.cheb_power_iters,
-               self.precision)
-        if self._nh is None or self._nh_key != key:
-            self._free_native()
-            n_coarse = self._n_coarse()
-            nbytes = L.fantom_mg_workspace_size(*k.grid, n_coarse, gal)
-            self._ws = torch.empty(nbytes, dtype=torch.uint8, device=k.device)
-            h = ctypes.c_void_p()
-            K0 = np.ascontiguousarray(k.K_single, dtype=np.float64)
-            kid = _lib.KU_TILE if k.kernel_kind == "tile" else _lib.KU_NODE
-            _lib.check(L.fantom_mg_create(*k.grid, n_coarse, self.n_smooth, self.omega, kid, gal, K0.ctypes.data,
-                                          self._ws.data_ptr(), nbytes, ctypes.byref(h)))
-            self._nh, self._nh_key, self._nh_state = h, key, None
-            self._n_levels = n_coarse + 1
-            _LIVE.add(self)
-            self._native_schedule(L, "fantom_mg_", h)
-        state = (k.rho_version, k.con_version, k.rho.data_ptr())
-        if self._nh_state != state:
-            _lib.call("fantom_mg_setup", self._nh, _lib.ptr(k.rho), _lib.ptr(k.mask) if k.has_con else None)
-            self._nh_state = state
-
-    def _native_solve(self, rhs, use_last):
+        """Create the native handle for the current options (FP64 or mixed)
+        and set it up for the kernel's current E; returns (library, prefix)."""
         import ctypes
         k = self.kernel
         L = _lib.load()
@@ -331,9 +308,7 @@ class MultiGrid(_DeviceKrylov):
                 kid = _lib.KU_TILE if k.kernel_kind == "tile" else _lib.KU_NODE
                 _lib.check(L.fantom_mg_create(*k.grid, n_coarse, self.n_smooth, self.omega, kid, gal,
                                               K0.ctypes.data, self._ws.data_ptr(), nbytes, ctypes.byref(h)))
-            self._nh = h
-            self._nh_key = key
-            self._nh_state = None
+            self._nh, self._nh_key, self._nh_state = h, key, None
             self._n_levels = n_coarse + 1
             _LIVE.add(self)
             self._native_schedule(L, pre, h)
@@ -341,6 +316,13 @@ class MultiGrid(_DeviceKrylov):
         if self._nh_state != state:
             _lib.call(pre + "setup", self._nh, _lib.ptr(k.rho), _lib.ptr(k.mask) if k.has_con else None)
             self._nh_state = state
+        return L, pre
+
+    def _native_solve(self, rhs, use_last):
+        import ctypes
+        k = self.kernel
+        mixed = self.precision == "mixed"
+        L, pre = self._native_prepare()
         n = rhs.numel()
         use_x0 = bool(use_last and self.last_x0 is not None and self.last_x0.numel() == n)
         x = self.last_x0 if use_x0 else torch.empty_like(rhs)

@@ -99,3 +99,31 @@ def test_fpcg_pkp_breakdown_keeps_history_aligned():
     assert info["iterations"] == 12 and info["restarts"] == 12
     assert len(info["history"]) == info["iterations"] + 1
     assert np.isfinite(x.numpy()).all()
+
+
+def test_energy_loss_gradient_fd():
+    """The training loss of the learned correction: L(c) = 1 - (t.c)^2 /
+    ((c.Kc)(t.e)) with t = K e is the energy fraction left by the best
+    multiple of c (checked against the explicit minimisation), and its
+    gradient matches central finite differences."""
+    from pyfantom_amd.learned import energy_loss_grad
+    A, _ = _spd(n=40, seed=5)
+    rng = np.random.default_rng(6)
+    e = torch.from_numpy(rng.standard_normal(40))
+    t = A @ e
+    c = torch.from_numpy(rng.standard_normal(40))
+    L, g = energy_loss_grad(t, e, c, A @ c)
+    gam = float(t @ c) / float(c @ (A @ c))
+    r = e - gam * c
+    assert abs(L - float(r @ (A @ r)) / float(e @ (A @ e))) <= 1e-12
+    h = 1e-6
+    fd = torch.zeros(40, dtype=torch.float64)
+    for i in range(40):
+        cp, cm = c.clone(), c.clone()
+        cp[i] += h
+        cm[i] -= h
+        fd[i] = (energy_loss_grad(t, e, cp, A @ cp)[0] - energy_loss_grad(t, e, cm, A @ cm)[0]) / (2 * h)
+    assert torch.allclose(g, fd, rtol=1e-5, atol=1e-8)
+    # exact direction: nothing left
+    L0, _ = energy_loss_grad(t, e, 3.0 * e, A @ (3.0 * e))
+    assert abs(L0) <= 1e-12

@@ -129,55 +129,145 @@ def test_training_and_fpcg(oracle):
         assert abs(c - c0) <= 1e-6 * abs(c0)
 
 
-def test_trained_network_ab(oracle):
-    """A/B of the committed trained network (pyfantom_amd/data, trained on the
-    GPU by pyfantom_amd/train_learned.py on 96x48x48 SIMP runs) on a held-out
-    problem: a 64x32x32 cantilever after 8 OC iterations.  Compares flexible
-    PCG with the V-cycle + learned correction against the same with an
-    untrained network and against plain GMG-PCG, all to 1e-8; prints the
-    iteration counts and times (the gain, or loss, of the learned part) and
-    checks that every variant reaches the FP64 solution."""
+def _mid_design(nel, n_level=4, simp_iters=8):
+    """An edge-loaded cantilever after a few OC iterations (held out: the
+    network was trained on point loads at 96x48x48)."""
+    import pyfantom_amd as pf
+    from pyfantom_amd.train_learned import cantilever_problem
+    mesh, K, solver, FE, filt, prob = cantilever_problem(nel, vf=0.3, load="edge", n_level=n_level)
+    opt = pf.OC(prob, move=0.2)
+    for _ in range(simp_iters):
+        opt.iter()
+    return mesh, K, solver, FE
+
+
+def _committed_net():
     import os
+    import pyfantom_amd as pf
+    from pyfantom_amd.learned import ConvNet
+    path = os.path.join(os.path.dirname(pf.__file__), "data", "learned_precond_96x48x48.npz")
+    with np.load(path, allow_pickle=False) as z:
+        meta = {k: z[k] for k in ("pairs", "grid", "loss_kind", "pair_mode") if k in z.files}
+    return ConvNet.from_file(path), meta
+
+
+def test_trained_network_gains_energy(oracle):
+    """What training optimised, on held-out data: on post-V-cycle pairs of an
+    edge-loaded 64x32x32 design (the network saw point-loaded 96x48x48 runs),
+    one application of the committed network removes more of the remaining
+    error energy than an untrained network of the same architecture."""
+    import torch
+    import pyfantom_amd as pf
+    from pyfantom_amd.learned import ConvNet, NetRunner, _Vec, density_from_moduli
+    from pyfantom_amd.train_learned import design_pairs
+    _torch()
+    net, meta = _committed_net()
+    print(f"trained network energy gain (trained on {int(meta['pairs'])} {meta['pair_mode']} pairs, "
+          f"{meta['loss_kind']} loss, grid {list(meta['grid'])})", flush=True)
+    assert int(meta["pairs"]) >= 20000
+    mesh, K, solver, FE = _mid_design((64, 32, 32))
+    mg = pf.MultiGrid(mesh, K, maxiter=1, tol=1e-8, n_smooth=2, omega=0.6, n_level=4, cycle="V",
+                      omega_schedule="fixed", precision="mixed", fallback=False)
+    pairs = design_pairs(K, solver, FE, mg, n_pairs=12)
+    vec = _Vec(K)
+    free = (~K.constraints).to(torch.float64)
+    out = {}
+    for label, nn in (("trained", net), ("untrained", ConvNet(seed=12345))):
+        run = NetRunner(nn, K.grid)
+        run.set_density(density_from_moduli(K.rho))
+        ls = [run.energy_loss_and_grad(t, e, vec, free) for t, e in pairs]
+        out[label] = float(np.mean(ls))
+    print(f"  energy left after one correction: trained {out['trained']:.3f}, untrained {out['untrained']:.3f}",
+          flush=True)
+    assert out["trained"] < out["untrained"]
+
+
+def test_trained_network_ab(oracle):
+    """A/B on the held-out 64x32x32 design, all to 1e-8: plain FP64 GMG-PCG,
+    and flexible PCG with the FP32 V-cycle plus the learned correction for the
+    committed and for an untrained network (LearnedSolver).  Checks that every
+    variant reaches the FP64 solution; prints iterations and times, the gain
+    (or loss) of the learned part."""
     import time
     import torch
     import pyfantom_amd as pf
-    from pyfantom_amd.learned import ConvNet, LearnedPreconditioner, fpcg_solve
-    from pyfantom_amd.train_learned import cantilever_problem
+    from pyfantom_amd.learned import ConvNet, LearnedSolver
     _torch()
-    path = os.path.join(os.path.dirname(pf.__file__), "data", "learned_precond_96x48x48.npz")
     print("trained network A/B", flush=True)
-    nel = (64, 32, 32)
-    mesh, K, solver, FE, filt, prob = cantilever_problem(nel, vf=0.3, load="edge", n_level=4)
-    opt = pf.OC(prob, move=0.2)
-    for _ in range(8):
-        opt.iter()
-    x = prob.get_desvars()
-    plain = pf.MultiGrid(mesh, K, maxiter=1000, tol=1e-8, n_smooth=2, omega=0.6, n_level=4, cycle="V")
+    net, _ = _committed_net()
+    mesh, K, solver, FE = _mid_design((64, 32, 32))
+    plain = pf.MultiGrid(mesh, K, maxiter=1000, tol=1e-8, n_smooth=2, omega=0.6, n_level=4, cycle="V",
+                         omega_schedule="fixed")
     plain.solve(FE.rhs, use_last=False)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     u0, r0 = plain.solve(FE.rhs, use_last=False)
     torch.cuda.synchronize()
-    t_plain = time.perf_counter() - t0
+    results = {"plain": (plain.last_iterations, time.perf_counter() - t0)}
     u0 = u0.clone()
     c0 = float(torch.dot(FE.rhs, u0))
-    results = {"plain": (plain.last_iterations, t_plain)}
-    for label, net in (("trained", ConvNet.from_file(path)), ("untrained", ConvNet(seed=0))):
-        mg = pf.MultiGrid(mesh, K, maxiter=1000, tol=1e-8, n_smooth=2, omega=0.6, n_level=4, cycle="V")
-        fb = pf.MultiGrid(mesh, K, maxiter=1000, tol=1e-8, n_smooth=2, omega=0.6, n_level=4, cycle="V")
-        M = LearnedPreconditioner(net, K, mg=mg, mode="hybrid")
-        M.set_density(x)
-        fpcg_solve(K, FE.rhs, M, tol=1e-8, maxiter=3, fallback=None)   # warm-up
+    for label, nn in (("trained", net), ("untrained", ConvNet(seed=12345))):
+        ls = LearnedSolver(mesh, K, nn, maxiter=400, tol=1e-8, precision="mixed", n_level=4)
+        ls.solve(FE.rhs, use_last=False)   # warm-up
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        u, info = fpcg_solve(K, FE.rhs, M, tol=1e-8, maxiter=300, fallback=fb)
+        u, res = ls.solve(FE.rhs, use_last=False)
         torch.cuda.synchronize()
-        t = time.perf_counter() - t0
         c = float(torch.dot(FE.rhs, u))
-        results[label] = (info["iterations"] + info["fallback_iterations"], t)
-        assert info["residual"] < 1.5e-8
+        results[label] = (ls.last_iterations + ls.last_fallback_iterations, time.perf_counter() - t0)
+        assert res < 1.5e-8
         assert abs(c - c0) <= 1e-6 * abs(c0)
     print("A/B (iterations, seconds): " + ", ".join(f"{k} {v[0]} / {v[1] * 1e3:.1f} ms" for k, v in results.items()),
           flush=True)
-    print(f"trained vs untrained iterations: {results['trained'][0]} vs {results['untrained'][0]}; "
-          f"trained vs plain GMG-PCG time ratio {results['trained'][1] / results['plain'][1]:.2f}", flush=True)
+    test_trained_network_ab.results = results
+
+
+@pytest.mark.xfail(strict=False, reason="measured: the committed network wins its training objective (energy left "
+                                        "after one correction) but not the PCG iteration count -- 51-78 trained "
+                                        "against 48 untrained and 44 plain at 64x32x32, 52 / 47 / 46 at "
+                                        "192x96x96: a stronger nonlinear correction disturbs the conjugacy of the "
+                                        "Krylov recurrence more")
+def test_trained_network_fewer_iterations(oracle):
+    """The claim the A/B is for: the trained network takes fewer flexible-PCG
+    iterations than the untrained one (held-out 64x32x32 design)."""
+    res = getattr(test_trained_network_ab, "results", None)
+    if res is None:
+        test_trained_network_ab(oracle)
+        res = test_trained_network_ab.results
+    assert res["trained"][0] < res["untrained"][0]
+
+
+def test_c4_learned_solver(oracle):
+    """192x96x96 (5.4M DOF) edge-loaded cantilever after 10 OC iterations:
+    LearnedSolver (FP32 V-cycle + learned correction in flexible PCG, FP64
+    GMG-PCG fallback) reaches 1e-8 with compliance within 1e-6 of the plain
+    FP64 GMG-PCG solve; iterations and times of plain, mixed and learned are
+    printed."""
+    import time
+    import torch
+    import pyfantom_amd as pf
+    from pyfantom_amd.learned import LearnedSolver
+    _torch()
+    print("C4 learned solver 192x96x96", flush=True)
+    net, _ = _committed_net()
+    mesh, K, solver, FE = _mid_design((192, 96, 96), simp_iters=10)
+    out = {}
+    for label in ("plain", "mixed", "learned"):
+        if label == "learned":
+            s = LearnedSolver(mesh, K, net, maxiter=400, tol=1e-8, precision="mixed", n_level=4)
+        else:
+            s = pf.MultiGrid(mesh, K, maxiter=1000, tol=1e-8, n_smooth=2, omega=0.6, n_level=4, cycle="V",
+                             omega_schedule="fixed", precision="fp64" if label == "plain" else "mixed")
+        s.solve(FE.rhs, use_last=False)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        u, res = s.solve(FE.rhs, use_last=False)
+        torch.cuda.synchronize()
+        out[label] = (s.last_iterations, time.perf_counter() - t0, float(torch.dot(FE.rhs, u)), res,
+                      getattr(s, "last_fallback", False))
+        assert res < 1.5e-8, label
+    c0 = out["plain"][2]
+    for label, (it, t, c, res, fb) in out.items():
+        print(f"  {label}: {it} iterations, {t * 1e3:.1f} ms, residual {res:.2e}, fallback {fb}, "
+              f"|c - c_fp64|/c {abs(c - c0) / abs(c0):.2e}", flush=True)
+        assert abs(c - c0) <= 1e-6 * abs(c0), label
