@@ -207,17 +207,35 @@ def measured_traffic(workload):
         return None
 
 
-def cpu_baseline(nel, n_level, pcg_iters_per_step, max_iter=4):
-    """FP64 oracle (C/OpenMP) on the host: K.u and a bounded GMG-PCG sample."""
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(nel=(128, 64, 64), n_level=4, steps=3):
+    """Whole SIMP iterations of the FP64 oracle (C/OpenMP restatement of the
+    reference's CPU path) on the host: filter -> modified SIMP -> GMG-PCG to
+    1e-8 (warm-started from the previous solution, hierarchy rebuilt every
+    solve as the reference does) -> compliance -> dc -> chain rule -> filter
+    adjoint -> OC, on the 128x64x64 cantilever from x0 = 0.3.  One untimed
+    step, then the median of `steps` timed steps, all host threads; K.u and one
+    PCG iteration (with its setup) also at 1 thread.  The oracle's coarse
+    operators are rediscretised (2^3-averaged moduli), where the reference's
+    CPU MultiGrid builds Galerkin CSR levels (solvers/CPU/_solvers.py:956-1147)."""
     from oracle import oracle as O
     O.build()
-    n_el = int(np.prod(nel))
-    x = np.random.default_rng(0).uniform(0.01, 1.0, n_el)
-    E = 1e-9 + x ** 3 * (1 - 1e-9)
+    threads = O.num_threads()
+    nx, ny, nz = (n + 1 for n in nel)
+    n_el, n_dof = int(np.prod(nel)), nx * ny * nz * 3
     K0 = O.k0_exact(3, 0.3)
     el = O.elements(nel)
-    nx, ny, nz = (n + 1 for n in nel)
-    n_dof = nx * ny * nz * 3
+    offs, w = O.filter_taps(3, 1.5)
     con = np.zeros(n_dof, bool)
     j, k = np.meshgrid(np.arange(ny), np.arange(nz), indexing="ij")
     clamp = (k + nz * nx * j).ravel()
@@ -225,21 +243,49 @@ def cpu_baseline(nel, n_level, pcg_iters_per_step, max_iter=4):
         con[clamp * 3 + d] = True
     f = np.zeros(n_dof)
     f[(np.arange(nz) + nz * (nx - 1)) * 3 + 1] = -1.0 / nz
-    u = np.random.default_rng(1).standard_normal(n_dof)
-    O.matvec(K0, el, E, u, con)
-    t0 = time.perf_counter()
-    O.matvec(K0, el, E, u, con)
-    t_ku = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    _, it, _ = O.mgpcg_solve(nel, K0, E, con, f, n_level=n_level, n_smooth=2, omega=0.6, tol=1e-30,
-                             maxiter=max_iter)
-    t_solve = time.perf_counter() - t0
-    per_it = t_solve / max(it, 1)
-    return dict(value=per_it * pcg_iters_per_step, unit="s/SIMP-iter", cores=O.num_threads(), kind="port",
-                sample=(f"oracle GMG-PCG on the same cantilever ({nel[0]}x{nel[1]}x{nel[2]}, seeded E), {it} PCG "
-                        f"iterations incl. hierarchy setup ({t_solve:.2f} s) scaled to the GPU run's mean "
-                        f"{pcg_iters_per_step:.1f} PCG iterations per SIMP step; filter/OC excluded"),
-                ku_seconds=t_ku, ku_gdof_s=n_dof / t_ku / 1e9)
+    dg = np.full(n_el, 1.0 / n_el)
+
+    def step(x, u):
+        rho = O.filter_fwd(nel, offs, w, x)
+        E = O.penalize_modified(rho, 3.0, 1.0, 1e-9)
+        u, it, res = O.mgpcg_solve(nel, K0, E, con, f, n_level=n_level, n_smooth=2, omega=0.6, tol=1e-8, x0=u)
+        c = float(f @ u)
+        dx = O.filter_adj(nel, offs, w, O.process_dk(K0, el, u) * O.penalize_grad_modified(rho, 3.0, 1.0, 1e-9))
+        xn, _ = O.oc_update(x, dx, dg, 0.3, 0.2)
+        return xn, u, it, c, E
+
+    x, u = np.full(n_el, 0.3), None
+    x, u, it0, c, E = step(x, u)          # untimed
+    times, its = [], []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        x, u, it, c, E = step(x, u)
+        times.append(time.perf_counter() - t0)
+        its.append(it)
+    t_ku = {}
+    t_it = {}
+    for nth in (threads, 1):
+        O.set_num_threads(nth)
+        v = np.random.default_rng(1).standard_normal(n_dof)
+        O.matvec(K0, el, E, v, con)
+        t0 = time.perf_counter()
+        O.matvec(K0, el, E, v, con)
+        t_ku[nth] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        O.mgpcg_solve(nel, K0, E, con, f, n_level=n_level, n_smooth=2, omega=0.6, tol=1e-30, maxiter=1)
+        t_it[nth] = time.perf_counter() - t0
+    O.set_num_threads(threads)
+    med = float(np.median(times))
+    return dict(value=med, unit="s/SIMP-iter", cores=threads, kind="port",
+                workload=f"cantilever_{nel[0]}x{nel[1]}x{nel[2]}_simp_oc_gmgpcg_1e-8",
+                sample=(f"oracle SIMP iterations on the {nel[0]}x{nel[1]}x{nel[2]} cantilever (filter, SIMP law, "
+                        f"GMG-PCG to 1e-8 warm-started, {n_level} coarsenings, rediscretised coarse operators, "
+                        f"2+2 Jacobi omega 0.6, dc, filter adjoint, OC): median of {steps} after 1 untimed, "
+                        f"{threads} threads"),
+                step_seconds=times, pcg_iterations=its, compliance=c, cpu_model=_cpu_model(),
+                ku_seconds={str(kk): vv for kk, vv in t_ku.items()},
+                pcg_iteration_with_setup_seconds={str(kk): vv for kk, vv in t_it.items()},
+                ku_gdof_s={str(kk): n_dof / vv / 1e9 for kk, vv in t_ku.items()})
 
 
 def main():
@@ -395,7 +441,24 @@ def main():
     }
     if not args.no_cpu and cfg.get("kind", "cantilever") == "cantilever":
         print("# cpu baseline ...", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(nel, n_level, float(np.mean(iters)))
+        cb = cpu_baseline()
+        # the GPU's FP64 leg on the same 128x64x64 workload, same start
+        cfg2 = CONFIGS["c2"]
+        _, K2, solver2, _, _, prob2 = build_problem(pf, cfg2["nel"], cfg2["n_level"], omega_schedule="fixed")
+        opt2 = pf.OC(prob2, move=0.2)
+        opt2.iter()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(cb["pcg_iterations"].__len__()):
+            opt2.iter()
+        torch.cuda.synchronize()
+        g2 = (time.perf_counter() - t0) / len(cb["pcg_iterations"])
+        cb["gpu_fp64_same_workload_s_per_iter"] = g2
+        cb["gpu_speedup_same_workload"] = cb["value"] / g2
+        print(f"# cpu baseline: {cb['value']:.2f} s/SIMP-iter on {cb['cores']} threads ({cb['cpu_model']}), "
+              f"GPU {g2 * 1e3:.1f} ms", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cb
+        del K2, solver2, prob2, opt2
     print(json.dumps(out), flush=True)
 
 

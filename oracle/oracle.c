@@ -640,7 +640,7 @@ static double or_dot(const double *a, const double *b, long n)
  */
 int or_mgpcg_solve(int dim, const int *nel, const double *K0f, const double *E, const uint8_t *con,
                    const double *f, double *u, int nlev_coarse, int n_smooth, double omega, double tol,
-                   int max_iter, double *rel_res)
+                   int max_iter, double *rel_res, int use_x0)
 {
     const int npe = dim == 3 ? 8 : 4, dof = dim == 3 ? 3 : 2, nl = npe * dof;
     or_mg mg;
@@ -713,8 +713,9 @@ int or_mgpcg_solve(int dim, const int *nel, const double *K0f, const double *E, 
         double *z = (double *)malloc(sizeof(double) * (size_t)n);
         double *p = (double *)malloc(sizeof(double) * (size_t)n);
         double *q = (double *)malloc(sizeof(double) * (size_t)n);
-        for (long i = 0; i < n; ++i)
-            u[i] = 0.0;
+        if (!use_x0)   /* warm start: u holds the initial guess */
+            for (long i = 0; i < n; ++i)
+                u[i] = 0.0;
         or_level_matvec(F, dof, u, q);
         for (long i = 0; i < n; ++i)
             r[i] = f[i] - q[i];
@@ -772,6 +773,8 @@ int or_mgpcg_solve(int dim, const int *nel, const double *K0f, const double *E, 
     free(mg.Lc);
     return it;
 }
+
+void or_set_num_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
 
 int or_num_threads(void)
 {

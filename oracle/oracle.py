@@ -68,9 +68,12 @@ def lib():
         L.or_filter_adj.argtypes = [c_int, _I, c_int, _I, _D, _D, _D, _D]
         L.or_oc_update.argtypes = [c_long, _D, _D, _D, c_double, c_double, _D]
         L.or_oc_update.restype = c_double
-        L.or_mgpcg_solve.argtypes = [c_int, _I, _D, _D, _U8, _D, _D, c_int, c_int, c_double, c_double, c_int, _D]
+        L.or_mgpcg_solve.argtypes = [c_int, _I, _D, _D, _U8, _D, _D, c_int, c_int, c_double, c_double, c_int, _D,
+                                     c_int]
         L.or_mgpcg_solve.restype = c_int
         L.or_num_threads.restype = c_int
+        L.or_set_num_threads.argtypes = [c_int]
+        L.or_set_num_threads.restype = None
         _lib = L
     return _lib
 
@@ -483,20 +486,25 @@ def penalize_grad_modified(rho, p=3.0, E0=1.0, Emin=1e-9):
 # --------------------------------------------------------------------------
 # FP64 GMG-PCG (the timed CPU baseline)
 # --------------------------------------------------------------------------
-def mgpcg_solve(nel, K0, E, con, f, n_level=4, n_smooth=2, omega=0.6, tol=1e-8, maxiter=1000):
+def mgpcg_solve(nel, K0, E, con, f, n_level=4, n_smooth=2, omega=0.6, tol=1e-8, maxiter=1000, x0=None):
     nel_a = np.ascontiguousarray(nel, np.int32)
     f = _f64(f)
-    u = np.zeros_like(f)
+    u = np.zeros_like(f) if x0 is None else np.array(x0, dtype=np.float64, copy=True)
     m = _mask(con, f.shape[0])
     res = ctypes.c_double(0.0)
     it = lib().or_mgpcg_solve(len(nel), _p(nel_a, _I), _p(_f64(K0), _D), _p(_f64(E), _D),
                               _p(m, _U8) if m is not None else None, _p(f, _D), _p(u, _D), int(n_level),
-                              int(n_smooth), float(omega), float(tol), int(maxiter), ctypes.byref(res))
+                              int(n_smooth), float(omega), float(tol), int(maxiter), ctypes.byref(res),
+                              int(x0 is not None))
     return u, it, res.value
 
 
 def num_threads():
     return lib().or_num_threads()
+
+
+def set_num_threads(n):
+    lib().or_set_num_threads(int(n))
 
 
 # Which check pins each oracle function (see tests/test_oracle.py).
