@@ -70,6 +70,12 @@ int fantom_stream_sync(void *stream);
  */
 int fantom_ku_f64(int kernel, int nelx, int nely, int nelz, const double *K0, const double *E,
                   const uint8_t *mask, const double *u, double *y, void *stream);
+/* FP32 y = K(E) u on hex8 grids (the dtype = float32 dispatch of
+ * core/CUDA/Kernels/_matvec.py:25-31 and core/CPU/_ops.py:212-245): the
+ * tiled Walsh-Hadamard operator in FP32; Kb is the HOST 72-double block form
+ * (fantom_hex8_blocks).  2D grids return status 3. */
+int fantom_ku_f32(int nelx, int nely, int nelz, const double *Kb, const float *E, const uint8_t *mask,
+                  const float *u, float *y, void *stream);
 /* Jacobi sweep fused into the operator: x_out = x_in + omega*dinv*(b - K x_in)
  * (solvers/CUDA/_solvers.py:437-440). x_out must not alias x_in.  With the
  * tiled kernel dinv may be NULL: D^-1 is then formed in the sweep from E

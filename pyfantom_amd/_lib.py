@@ -30,6 +30,7 @@ SIGNATURES = {
     "fantom_last_error": (ctypes.c_char_p, []),
     "fantom_stream_sync": (_c_int, [_vp]),
     "fantom_ku_f64": (_c_int, [_c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "fantom_ku_f32": (_c_int, [_c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
     "fantom_ku_jacobi_f64": (_c_int, [_c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _c_double, _vp,
                                       _vp, _vp]),
     "fantom_ku_jacobi0_f64": (_c_int, [_c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _c_double, _vp, _vp]),

@@ -79,6 +79,20 @@ int fantom_ku_f64(int kernel, int nelx, int nely, int nelz, const double *K0, co
     return ku_dispatch(kernel, MODE_KU, nelx, nely, nelz, K0, E, mask, u, y, nullptr, nullptr, 0.0, nullptr, stream);
 }
 
+int fantom_ku_f32(int nelx, int nely, int nelz, const double *Kb, const float *E, const uint8_t *mask,
+                  const float *u, float *y, void *stream)
+{
+    GRID_OR_RETURN(nelx, nely, nelz);
+    if (nelz <= 0) {
+        set_error("the FP32 operator is implemented for hex8 grids (nelz > 0)");
+        return kStatusNotSupported;
+    }
+    FANTOM_CHECK_ARG(Kb && E && u && y, "null operand");
+    FANTOM_CHECK_ARG(u != y, "u and y must not alias");
+    return launch_ku_tile_f32(MODE_KU, make_grid(nelx, nely, nelz), Kb, E, mask, u, y, nullptr, nullptr, 0.0, nullptr,
+                              as_stream(stream));
+}
+
 int fantom_ku_jacobi_f64(int kernel, int nelx, int nely, int nelz, const double *K0, const double *E,
                          const uint8_t *mask, const double *b, const double *dinv, double omega, const double *x_in,
                          double *x_out, void *stream)

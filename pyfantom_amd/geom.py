@@ -37,7 +37,11 @@ class _StructuredMesh:
         h = np.array(self.spacing, dtype=np.float64)
         bits = _HEX_BITS if len(self.nel) == 3 else _QUAD_BITS
         ref = bits * h[None, :]
-        self.K_single = np.asarray(physics.K(ref), dtype=dtype)
+        # K_single in the mesh dtype (the reference's attribute); the kernels
+        # take their element matrix from the FP64 one, so a float32 mesh
+        # differs from a float64 one only in its vector storage
+        self.K_single64 = np.asarray(physics.K(ref), dtype=np.float64)
+        self.K_single = self.K_single64.astype(dtype)
         self.locals = [np.asarray(a, dtype=dtype) for a in physics.locals(ref)]
         self.A_single = np.array([physics.volume(ref)], dtype=dtype)
         self.As = self.A_single

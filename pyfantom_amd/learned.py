@@ -368,13 +368,13 @@ class _Vec:
     def ku(self, u, y):
         k = self.k
         kid, k0 = k.op_args()
-        _lib.call("fantom_ku_f64", kid, *k.grid, k.k0_ptr(k0), _lib.ptr(k.rho),
+        _lib.call("fantom_ku_f64", kid, *k.grid, k.k0_ptr(k0), _lib.ptr(k.rho64),
                   _lib.ptr(k.mask) if k.has_con else None, _lib.ptr(u), _lib.ptr(y))
 
     def residual(self, f, x, r):
         k = self.k
         kid, k0 = k.op_args()
-        _lib.call("fantom_ku_residual_f64", kid, *k.grid, k.k0_ptr(k0), _lib.ptr(k.rho),
+        _lib.call("fantom_ku_residual_f64", kid, *k.grid, k.k0_ptr(k0), _lib.ptr(k.rho64),
                   _lib.ptr(k.mask) if k.has_con else None, _lib.ptr(f), _lib.ptr(x), _lib.ptr(r))
 
 
@@ -571,9 +571,9 @@ class LearnedSolver:
         k = self.kernel
         if rho is not None:
             k.set_rho(rho)
-        state = (k.rho_version, k.rho.data_ptr())
+        state = (k.rho_version, k.rho64.data_ptr())
         if self._dens_state != state:
-            self.M.set_density(density_from_moduli(k.rho, self.Emin, self.E0, self.penalty))
+            self.M.set_density(density_from_moduli(k.rho64, self.Emin, self.E0, self.penalty))
             self._dens_state = state
         x0 = self.last_x0 if (use_last and self.last_x0 is not None and self.last_x0.numel() == rhs.numel()) else None
         x, info = fpcg_solve(k, rhs, self.M, tol=self.tol, maxiter=self.maxiter, x0=x0, fallback=self.fb,

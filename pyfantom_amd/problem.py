@@ -134,6 +134,8 @@ class MinimumCompliance:
         U, residual = self.FE.solve(self._E)
         compliance = self._dot(self.FE.rhs, U)
         df = self.FE.kernel.process_grad(U)
+        if df.dtype != torch.float64:   # float32 meshes: the chain rule stays FP64
+            df = df.double()
         _lib.call("fantom_mul_f64", self.nel, _lib.ptr(self._dE), _lib.ptr(df), _lib.ptr(self._dr))
         self._nabla_f = self.filter._rmatvec(self._dr)
         self._f = float(compliance.item())
@@ -198,13 +200,13 @@ class MinimumCompliance:
                   "and more iteration from prior solve will be applied).")
         mesh, k = self.FE.mesh, self.FE.kernel
         D, B = (torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(self.device) for a in mesh.locals)
-        K0 = torch.from_numpy(np.ascontiguousarray(mesh.K_single, dtype=np.float64)).to(self.device)
+        K0 = k.K0
         ns = D.shape[0]
         strain = torch.empty((self.nel, ns), dtype=torch.float64, device=self.device)
         stress = torch.empty_like(strain)
         energy = torch.empty(self.nel, dtype=torch.float64, device=self.device)
         vm = torch.empty_like(energy)
-        U = k._as_vec(U)
+        U = k._as_vec(U, torch.float64)
         _lib.call("fantom_fea_locals_f64", *k.grid, _lib.ptr(K0), _lib.ptr(D), _lib.ptr(B), _lib.ptr(rho),
                   _lib.ptr(U), _lib.ptr(strain), _lib.ptr(stress), _lib.ptr(energy), _lib.ptr(vm))
         return {"strain": strain, "stress": stress, "strain_energy": energy, "von_mises": vm,

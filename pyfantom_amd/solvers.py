@@ -113,7 +113,7 @@ class _DeviceKrylov(Solver):
         """q = K p and scal[PQ/ALPHA] from p.q."""
         k = self.kernel
         kid, k0 = k.op_args()
-        _lib.call("fantom_ku_dot_f64", kid, *k.grid, k.k0_ptr(k0), _lib.ptr(k.rho),
+        _lib.call("fantom_ku_dot_f64", kid, *k.grid, k.k0_ptr(k0), _lib.ptr(k.rho64),
                   _lib.ptr(k.mask) if k.has_con else None, _lib.ptr(p), _lib.ptr(q), _lib.ptr(self._partials))
         npart = _lib.load().fantom_ku_partials(kid, *k.grid)
         _lib.call("fantom_finalize_f64", _lib.ptr(self._partials), npart, _lib.ptr(self._scal), _lib.FIN_ALPHA, 0)
@@ -121,7 +121,7 @@ class _DeviceKrylov(Solver):
     def _true_residual(self, x, rhs, out):
         k = self.kernel
         kid, k0 = k.op_args()
-        _lib.call("fantom_ku_residual_f64", kid, *k.grid, k.k0_ptr(k0), _lib.ptr(k.rho),
+        _lib.call("fantom_ku_residual_f64", kid, *k.grid, k.k0_ptr(k0), _lib.ptr(k.rho64),
                   _lib.ptr(k.mask) if k.has_con else None, _lib.ptr(rhs), _lib.ptr(x), _lib.ptr(out))
         return self._norm(out)
 
@@ -254,7 +254,7 @@ class MultiGrid(_DeviceKrylov):
         if not self.native:
             raise NotImplementedError("precondition_native needs the native driver")
         k = self.kernel
-        if self._nh is None or self._nh_state != (k.rho_version, k.con_version, k.rho.data_ptr()):
+        if self._nh is None or self._nh_state != (k.rho_version, k.con_version, k.rho64.data_ptr()):
             self._native_prepare()
         pre = "fantom_mx_" if self.precision == "mixed" else "fantom_mg_"
         _lib.call(pre + "precondition", self._nh, _lib.ptr(r), _lib.ptr(z))
@@ -300,7 +300,7 @@ class MultiGrid(_DeviceKrylov):
                 raise ValueError("grid does not support the requested hierarchy")
             self._ws = torch.empty(nbytes, dtype=torch.uint8, device=k.device)
             h = ctypes.c_void_p()
-            K0 = np.ascontiguousarray(k.K_single, dtype=np.float64)
+            K0 = np.ascontiguousarray(k.K0.cpu().numpy())
             if mixed:
                 _lib.check(L.fantom_mx_create(*k.grid, n_coarse, self.n_smooth, self.omega, gal, K0.ctypes.data,
                                               self._ws.data_ptr(), nbytes, ctypes.byref(h)))
@@ -312,9 +312,9 @@ class MultiGrid(_DeviceKrylov):
             self._n_levels = n_coarse + 1
             _LIVE.add(self)
             self._native_schedule(L, pre, h)
-        state = (k.rho_version, k.con_version, k.rho.data_ptr())
+        state = (k.rho_version, k.con_version, k.rho64.data_ptr())
         if self._nh_state != state:
-            _lib.call(pre + "setup", self._nh, _lib.ptr(k.rho), _lib.ptr(k.mask) if k.has_con else None)
+            _lib.call(pre + "setup", self._nh, _lib.ptr(k.rho64), _lib.ptr(k.mask) if k.has_con else None)
             self._nh_state = state
         return L, pre
 
@@ -394,7 +394,7 @@ class MultiGrid(_DeviceKrylov):
         if self.levels is None or self.levels[0].n_dof != k.shape[0]:
             self._make_levels()
         L0 = self.levels[0]
-        L0.E = k.rho
+        L0.E = k.rho64
         L0.mask = k.mask if k.has_con else torch.zeros_like(L0.mask)
         for l in range(1, len(self.levels)):
             f, c = self.levels[l - 1], self.levels[l]
@@ -455,7 +455,7 @@ class MultiGrid(_DeviceKrylov):
             raise ValueError("Solver requires a density vector to be passed or set on the kernel.")
         if rho is not None:
             k.set_rho(rho)
-        rhs = k._as_vec(rhs)
+        rhs = k._as_vec(rhs, torch.float64)
         if self.native:
             return self._native_solve(rhs, use_last)
         n = rhs.numel()
@@ -472,7 +472,7 @@ class MultiGrid(_DeviceKrylov):
         if norm_b == 0.0:
             norm_b = 1.0
         kid, k0 = k.op_args()
-        _lib.call("fantom_ku_residual_f64", kid, *k.grid, k.k0_ptr(k0), _lib.ptr(k.rho),
+        _lib.call("fantom_ku_residual_f64", kid, *k.grid, k.k0_ptr(k0), _lib.ptr(k.rho64),
                   _lib.ptr(k.mask) if k.has_con else None, _lib.ptr(rhs), _lib.ptr(x), _lib.ptr(r))
         self.precondition(r, z)
         self._dot(r, z, 0, _lib.FIN_RZ0)
@@ -528,7 +528,7 @@ class CG(_DeviceKrylov):
             raise ValueError("Solver requires a density vector to be passed or set on the kernel.")
         if rho is not None:
             k.set_rho(rho)
-        rhs = k._as_vec(rhs)
+        rhs = k._as_vec(rhs, torch.float64)
         n = rhs.numel()
         self._alloc(n)
         x = self.last_x0 if (use_last and self.last_x0 is not None and self.last_x0.numel() == n) \
@@ -536,7 +536,7 @@ class CG(_DeviceKrylov):
         r, p, q = self._r, self._p, self._q
         norm_b = self._norm(rhs) or 1.0
         kid, k0 = k.op_args()
-        _lib.call("fantom_ku_residual_f64", kid, *k.grid, k.k0_ptr(k0), _lib.ptr(k.rho),
+        _lib.call("fantom_ku_residual_f64", kid, *k.grid, k.k0_ptr(k0), _lib.ptr(k.rho64),
                   _lib.ptr(k.mask) if k.has_con else None, _lib.ptr(rhs), _lib.ptr(x), _lib.ptr(r))
         self._dot(r, r, 0, _lib.FIN_RZ0)
         _lib.call("fantom_pcg_p_f64", n, _lib.ptr(p), _lib.ptr(r), _lib.ptr(self._scal), 1)

@@ -48,7 +48,9 @@ class StructuredStiffnessKernel:
         self.shape = (n_dof, n_dof)
         dev = _lib.device()
         self.device = dev
-        self.K0 = torch.from_numpy(np.ascontiguousarray(self.K_single, dtype=np.float64)).to(dev)
+        K64 = getattr(mesh, "K_single64", None)
+        K64 = np.asarray(self.K_single, dtype=np.float64) if K64 is None else K64
+        self.K0 = torch.from_numpy(np.ascontiguousarray(K64)).to(dev)
         if kernel is None:
             kernel = "tile" if self.dim == 3 else "node"
         if kernel not in ("tile", "node"):
@@ -56,7 +58,7 @@ class StructuredStiffnessKernel:
         if kernel == "tile" and self.dim != 3:
             raise ValueError("the tiled kernel is for hex8 meshes only")
         self.kernel_kind = kernel
-        self.blocks = hex8_blocks(self.K_single) if self.dim == 3 else None
+        self.blocks = hex8_blocks(K64) if self.dim == 3 else None
         self.constraints = torch.zeros(n_dof, dtype=torch.bool, device=dev)
         self.mask = torch.zeros(self.n_nodes, dtype=torch.uint8, device=dev)
         self.has_con = False
@@ -64,6 +66,11 @@ class StructuredStiffnessKernel:
         self.non_con_map = self.idx_map
         self.has_rho = False
         self.rho = None
+        self.rho64 = None   # FP64 moduli read by the solvers (the same tensor for FP64 meshes)
+        self.f32 = np.dtype(self.dtype) == np.float32
+        if self.f32 and self.dim != 3:
+            raise NotImplementedError("dtype=float32 is implemented for hex8 meshes (the FP32 tiled operator)")
+        self.vdtype = torch.float32 if self.f32 else torch.float64
         self.rho_version = 0
         self.con_version = 0
         self.mat_vec = None
@@ -119,18 +126,24 @@ class StructuredStiffnessKernel:
         rho = rho.to(device=self.device, dtype=torch.float64).contiguous()
         if rho.shape[0] != self.n_elements:
             raise ValueError("rho must have one entry per element.")
-        self.rho = rho
+        self.rho64 = rho
+        self.rho = rho.to(torch.float32) if self.f32 else rho
         self.has_rho = True
         self.rho_version += 1
 
-    def _as_vec(self, v):
+    def _as_vec(self, v, dtype=None):
         if not isinstance(v, torch.Tensor):
             raise NotImplementedError("Only device tensors as vectors are supported.")
-        return v.to(device=self.device, dtype=torch.float64).contiguous()
+        return v.to(device=self.device, dtype=dtype or self.vdtype).contiguous()
 
     def _matvec(self, rho, vec):
-        if self.mat_vec is None or self.mat_vec.shape != vec.shape:
+        if self.mat_vec is None or self.mat_vec.shape != vec.shape or self.mat_vec.dtype != vec.dtype:
             self.mat_vec = torch.empty_like(vec)
+        if vec.dtype == torch.float32:
+            # dtype float32: the FP32 tiled operator, as the reference dispatches f4 kernels
+            _lib.call("fantom_ku_f32", *self.grid, self.blocks.ctypes.data, _lib.ptr(self.rho),
+                      _lib.ptr(self.mask) if self.has_con else None, _lib.ptr(vec), _lib.ptr(self.mat_vec))
+            return self.mat_vec
         kid, k0 = self.op_args()
         _lib.call("fantom_ku_f64", kid, *self.grid, self.k0_ptr(k0), _lib.ptr(rho),
                   _lib.ptr(self.mask) if self.has_con else None, _lib.ptr(vec), _lib.ptr(self.mat_vec))
@@ -167,12 +180,13 @@ class StructuredStiffnessKernel:
         """diag(K(rho)), constrained DOFs -> 1 (stiffness/CUDA/_FEA.py:368-420)."""
         if rho is None and not self.has_rho:
             raise ValueError("Rho has not been set. diagonal works only after setting rho or if rho is provided.")
-        rho = self.rho if rho is None else rho.to(device=self.device, dtype=torch.float64).contiguous()
+        rho = self.rho64 if rho is None else rho.to(device=self.device, dtype=torch.float64).contiguous()
         if self.diag is None:
             self.diag = torch.empty(self.shape[0], dtype=torch.float64, device=self.device)
         _lib.call("fantom_diag_f64", *self.grid, _lib.ptr(self.K0), _lib.ptr(rho),
                   _lib.ptr(self.mask) if self.has_con else None, _lib.ptr(self.diag), 0)
-        return self.diag
+        # float32 meshes get the FP64 diagonal rounded to their dtype
+        return self.diag.to(torch.float32) if self.f32 else self.diag
 
     def set_constraints(self, constraints):
         """Replace the Dirichlet set (stiffness/CUDA/_FEA.py:422-439)."""
@@ -195,11 +209,12 @@ class StructuredStiffnessKernel:
 
     def process_grad(self, U):
         """dk_e = -u_e^T K0 u_e (stiffness/CUDA/_FEA.py:458-482)."""
-        U = self._as_vec(U)
+        U = self._as_vec(U, torch.float64)
         if self.dk is None:
             self.dk = torch.empty(self.n_elements, dtype=torch.float64, device=self.device)
         _lib.call("fantom_dc_f64", *self.grid, _lib.ptr(self.K0), _lib.ptr(U), _lib.ptr(self.dk))
-        return self.dk
+        # float32 meshes: the sensitivity of the FP64 contraction in their dtype
+        return self.dk.to(torch.float32) if self.f32 else self.dk
 
     def construct(self, rho):
         """Explicit K(rho) as a torch sparse CSR tensor on the device
@@ -239,6 +254,7 @@ class StructuredStiffnessKernel:
     def reset(self):
         self.has_rho = False
         self.rho = None
+        self.rho64 = None
         self.constraints[:] = False
         self.has_con = False
         self._update_mask()

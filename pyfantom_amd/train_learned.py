@@ -92,7 +92,7 @@ def trajectory_pairs(K, solver, FE, mg, net, n_pairs=8):
     ustar = ustar.clone()
     solver.tol = old
     M = LearnedPreconditioner(net, K, mg=mg, mode="hybrid")
-    M.set_density(density_from_moduli(K.rho))
+    M.set_density(density_from_moduli(K.rho64))
     xs = []
     fpcg_solve(K, FE.rhs, M, tol=1e-30, maxiter=n_pairs, fallback=None, stall_window=10 ** 6,
                record=lambda x: xs.append(x.clone()))
@@ -133,7 +133,7 @@ def train(nel, target_pairs, simp_iters=10, pairs_per_design=20, lr=5e-4, seed=0
         free = (~K.constraints).to(torch.float64)
         for it in range(simp_iters):
             opt.iter()
-            runner.set_density(density_from_moduli(K.rho))
+            runner.set_density(density_from_moduli(K.rho64))
             pairs = (trajectory_pairs(K, solver, FE, mg, net, pairs_per_design) if mode == "trajectory" else
                      design_pairs(K, solver, FE, mg, pairs_per_design, mode=mode))
             for t, e in pairs:
@@ -166,7 +166,7 @@ def evaluate(net, nel, n_level=4, simp_iters=10, log=print):
     opt = pf.OC(prob, move=0.2)
     for _ in range(simp_iters):
         opt.iter()
-    E = K.rho.clone()
+    E = K.rho64.clone()
     res = {"grid": list(nel), "simp_iters": simp_iters}
 
     def timed(fn):
