@@ -5,14 +5,22 @@
 //
 // Every per-variable quantity (asymptotes, the convex approximation, the
 // Newton right-hand sides, the search direction, the line-search trial point
-// and its residual) is computed by fused kernels over the n design variables;
-// the m x m dual system, the scalar variables (y, z, lam, mu, zet, s) and the
-// interior-point control flow stay on the host, which reads back one record of
-// block partial sums per kernel and adds them in block order (deterministic).
-// The subproblem takes the m < n branch of the Newton system (topology
-// problems have n >> m).
+// and its residual) is computed by fused passes over the n design variables.
+// The m x m dual system, the scalar variables (y, z, lam, mu, zet, s) and the
+// interior-point control flow run on the device too: subsolv's loops become a
+// state machine whose state (MmaCtl) lives in the workspace.  One unit of
+// work is two launches: mma_pass_k runs the n pass the current phase needs
+// (prep, residual, Newton sums, direction, line-search trial), and
+// mma_ctl_k adds its block partials in block order (deterministic) and takes
+// the next control step (the dual solve, step lengths, acceptance, barrier
+// updates), choosing the next phase.  The host enqueues units in batches and
+// reads one small status record per batch; the first batch is sized from the
+// previous update's unit count, so an update normally needs a single read, at
+// its end.  Units after the end are no-ops.  The subproblem takes the m < n
+// branch of the Newton system (topology problems have n >> m).
 #include <cmath>
-#include <vector>
+#include <algorithm>
+#include <atomic>
 
 #include "../../include/fantom.h"
 #include "common.hpp"
@@ -70,7 +78,7 @@ __device__ inline void block_reduce(double *vals, int nv, bool is_max, double *o
 // asymptotes, move bounds, approximation terms, the starting point of the
 // subproblem, and the sums b_i = sum_j P_ij / ux_j + Q_ij / xl_j
 template <int M>
-__global__ __launch_bounds__(kMmaBlock) void mma_prep_k(MmaDev d, int iter, double move)
+__device__ inline void mma_prep(const MmaDev &d, int iter, double move)
 {
     double bsum[M];
 #pragma unroll
@@ -161,15 +169,9 @@ __device__ inline void resid_point(const MmaDev &d, const MmaScal &sc, double ep
     block_reduce(&mx, 1, true, out + (long)gridDim.x * (1 + M));
 }
 
-template <int M>
-__global__ __launch_bounds__(kMmaBlock) void mma_resid_k(MmaDev d, MmaScal sc, double epsi)
-{
-    resid_point<M>(d, sc, epsi, d.xs, d.xsi, d.eta, d.partials);
-}
-
 // Newton right-hand side pieces: gvec, GG (delx / diagx), GG diag(1/diagx) GG^T
 template <int M>
-__global__ __launch_bounds__(kMmaBlock) void mma_newton_k(MmaDev d, MmaScal sc, double epsi)
+__device__ inline void mma_newton(const MmaDev &d, const MmaScal &sc, double epsi)
 {
     constexpr int nv = 2 * M + M * M;
     double v[nv];
@@ -232,7 +234,7 @@ __device__ __forceinline__ void mma_direction(const MmaDev &d, const MmaScal &sc
 // stored: the line-search trial forms it again where it needs it, which
 // costs arithmetic instead of writing and reading back three n-vectors
 template <int M>
-__global__ __launch_bounds__(kMmaBlock) void mma_dir_k(MmaDev d, MmaScal sc, MmaScal dl, double epsi)
+__device__ inline void mma_dir(const MmaDev &d, const MmaScal &sc, const MmaScal &dl, double epsi)
 {
     double mx = -INFINITY;
     for (long j = (long)blockIdx.x * kMmaBlock + threadIdx.x; j < d.n; j += (long)gridDim.x * kMmaBlock) {
@@ -247,14 +249,14 @@ __global__ __launch_bounds__(kMmaBlock) void mma_dir_k(MmaDev d, MmaScal sc, Mma
 }
 
 // line-search trial point with its residual and, in the same pass, the Newton
-// sums of mma_newton_k at that point (same arithmetic, same block order): when
+// sums of mma_newton at that point (same arithmetic, same block order): when
 // the trial is accepted and the barrier parameter stays, the next Newton step
 // takes them from here instead of another pass over the n arrays.  Partials:
 // the residual record (sums of squares and gvec per block, then the block
 // maxima), then the Newton record at gridDim.x * (2 + M).
 template <int M>
-__global__ __launch_bounds__(kMmaBlock) void mma_trial_newton_k(MmaDev d, MmaScal sc, MmaScal base, MmaScal dl,
-                                                               double steg, double epsi)
+__device__ inline void mma_trial_newton(const MmaDev &d, const MmaScal &sc, const MmaScal &base, const MmaScal &dl,
+                                        double steg, double epsi)
 {
     constexpr int nv = 2 * M + M * M;
     double sums[1 + M], v[nv];
@@ -268,7 +270,7 @@ __global__ __launch_bounds__(kMmaBlock) void mma_trial_newton_k(MmaDev d, MmaSca
     for (long j = (long)blockIdx.x * kMmaBlock + threadIdx.x; j < d.n; j += (long)gridDim.x * kMmaBlock) {
         const double alfa = d.alfa[j], beta = d.beta[j];
         const double x0 = d.xs[j], xsi0 = d.xsi[j], eta0 = d.eta[j];
-        double dx, dxsi, deta;   // the direction of mma_dir_k at the current point
+        double dx, dxsi, deta;   // the direction of mma_dir at the current point
         mma_direction<M>(d, base, dl, epsi, j, x0, xsi0, eta0, alfa, beta, dx, dxsi, deta);
         const double x = x0 + steg * dx;
         const double xsi = xsi0 + steg * dxsi;
@@ -313,7 +315,374 @@ __global__ __launch_bounds__(kMmaBlock) void mma_trial_newton_k(MmaDev d, MmaSca
     block_reduce(v, nv, false, d.partials + (long)gridDim.x * (2 + M));
 }
 
-// launch a kernel template instantiated for the constraint count m
+
+// ---------------------------------------------------------------------------
+// Device control of the subproblem solve
+// ---------------------------------------------------------------------------
+
+// which n pass runs next
+enum MmaPhase : int { kPhPrep = 0, kPhResid = 1, kPhNewton = 2, kPhDir = 3, kPhTrial = 4, kPhDone = 5 };
+enum MmaCtlStatus : int { kCtlOk = 0, kCtlSingular = 1, kCtlNonFinite = 2 };
+
+// the m part of subsolv's state and its loop counters; the first ints are the
+// status record the host reads
+struct MmaCtl {
+    int phase, status, steps, units;
+    int par;         // 1: the current n point sits in the trial arrays
+    int outer, ittt, itto, have_sums, pad;
+    double epsi, steg, rnorm, rmax, z, zet, zo, zeto, dz, dzet;
+    double b[kMmaMaxM], y[kMmaMaxM], lam[kMmaMaxM], mu[kMmaMaxM], sl[kMmaMaxM];
+    double yo[kMmaMaxM], lamo[kMmaMaxM], muo[kMmaMaxM], so[kMmaMaxM];
+    double dy[kMmaMaxM], dl[kMmaMaxM], dmu[kMmaMaxM], ds[kMmaMaxM], base[kMmaMaxM];
+};
+constexpr int kMmaCtlHeaderInts = 4;
+
+// the problem's m-part constants (host arrays of the entry point)
+struct MmaConst {
+    int m, sub_maxiter;
+    double a0, sub_tol;
+    double a[kMmaMaxM], c[kMmaMaxM], d[kMmaMaxM], fval[kMmaMaxM];
+};
+
+// the n pass of the current phase; the current point is (xs, xsi, eta), or
+// the trial arrays when par is set (an accepted trial swaps the two sets)
+template <int M>
+__global__ __launch_bounds__(kMmaBlock) void mma_pass_k(MmaDev d, const MmaCtl *__restrict__ ctl, int iter,
+                                                       double move)
+{
+    const int phase = ctl->phase;
+    if (phase == kPhDone)
+        return;
+    if (phase == kPhPrep) {
+        mma_prep<M>(d, iter, move);
+        return;
+    }
+    MmaDev cur = d;
+    if (ctl->par) {
+        cur.xs = d.xt;
+        cur.xsi = d.xsit;
+        cur.eta = d.etat;
+        cur.xt = d.xs;
+        cur.xsit = d.xsi;
+        cur.etat = d.eta;
+    }
+    MmaScal sc, base, dl;
+#pragma unroll
+    for (int i = 0; i < M; ++i) {
+        sc.lam[i] = ctl->lam[i];
+        base.lam[i] = ctl->base[i];
+        dl.lam[i] = ctl->dl[i];
+    }
+    const double epsi = ctl->epsi;
+    switch (phase) {
+    case kPhResid: resid_point<M>(cur, sc, epsi, cur.xs, cur.xsi, cur.eta, cur.partials); break;
+    case kPhNewton: mma_newton<M>(cur, sc, epsi); break;
+    case kPhDir: mma_dir<M>(cur, base, dl, epsi); break;
+    default: mma_trial_newton<M>(cur, sc, base, dl, ctl->steg, epsi); break;
+    }
+}
+
+// small dense solve (partial pivoting), n <= kMmaMaxM + 1
+__device__ bool mma_solve_small(int n, double *A, double *b)
+{
+#pragma clang fp contract(off)
+    for (int k = 0; k < n; ++k) {
+        int p = k;
+        for (int i = k + 1; i < n; ++i)
+            if (fabs(A[i * n + k]) > fabs(A[p * n + k]))
+                p = i;
+        if (A[p * n + k] == 0.0)
+            return false;
+        if (p != k) {
+            for (int j = 0; j < n; ++j) {
+                const double t = A[k * n + j];
+                A[k * n + j] = A[p * n + j];
+                A[p * n + j] = t;
+            }
+            const double t = b[k];
+            b[k] = b[p];
+            b[p] = t;
+        }
+        for (int i = k + 1; i < n; ++i) {
+            const double f = A[i * n + k] / A[k * n + k];
+            for (int j = k; j < n; ++j)
+                A[i * n + j] -= f * A[k * n + j];
+            b[i] -= f * b[k];
+        }
+    }
+    for (int k = n - 1; k >= 0; --k) {
+        double v = b[k];
+        for (int j = k + 1; j < n; ++j)
+            v -= A[k * n + j] * b[j];
+        b[k] = v / A[k * n + k];
+    }
+    return true;
+}
+
+// the full residual norm and max from the n part's sums (red[0] = sum of
+// squares, red[1 + i] = gvec_i, red[1 + m] = max) and the m part at (y, z,
+// lam, mu, zet, s)
+__device__ void mma_m_residual(const MmaConst &k, const MmaCtl &c, const double *red, double epsi, double &rnorm,
+                               double &rmax)
+{
+#pragma clang fp contract(off)
+    const int m = k.m;
+    double ss = red[0];
+    rmax = red[1 + m];
+    double alam = 0.0;
+    for (int i = 0; i < m; ++i)
+        alam += k.a[i] * c.lam[i];
+    for (int i = 0; i < m; ++i) {
+        const double gvec = red[1 + i];
+        const double rey = k.c[i] + k.d[i] * c.y[i] - c.mu[i] - c.lam[i];
+        const double relam = gvec - k.a[i] * c.z - c.y[i] + c.sl[i] - c.b[i];
+        const double remu = c.mu[i] * c.y[i] - epsi;
+        const double res = c.lam[i] * c.sl[i] - epsi;
+        ss += rey * rey + relam * relam + remu * remu + res * res;
+        rmax = fmax(rmax, fmax(fmax(fabs(rey), fabs(relam)), fmax(fabs(remu), fabs(res))));
+    }
+    const double rez = k.a0 - c.zet - alam, rezet = c.zet * c.z - epsi;
+    ss += rez * rez + rezet * rezet;
+    rmax = fmax(rmax, fmax(fabs(rez), fabs(rezet)));
+    rnorm = sqrt(ss);
+}
+
+// one Newton step's dual system from the Newton record nr (gvec m, GG.dd m,
+// the m x m product); sets the dual direction and the next phase (DIR)
+__device__ void mma_newton_solve(const MmaConst &k, MmaCtl &c, const double *nr)
+{
+#pragma clang fp contract(off)
+    const int m = k.m, M1 = m + 1;
+    double AA[(kMmaMaxM + 1) * (kMmaMaxM + 1)], rhs[kMmaMaxM + 1];
+    const double epsi = c.epsi;
+    double alam = 0.0;
+    for (int i = 0; i < m; ++i)
+        alam += k.a[i] * c.lam[i];
+    const double delz = k.a0 - alam - epsi / c.z;
+    double dely[kMmaMaxM], diagy[kMmaMaxM];
+    for (int i = 0; i < m; ++i) {
+        dely[i] = k.c[i] + k.d[i] * c.y[i] - c.lam[i] - epsi / c.y[i];
+        const double dellam = nr[i] - k.a[i] * c.z - c.y[i] - c.b[i] + epsi / c.lam[i];
+        diagy[i] = k.d[i] + c.mu[i] / c.y[i];
+        const double diaglamyi = c.sl[i] / c.lam[i] + 1.0 / diagy[i];
+        rhs[i] = dellam + dely[i] / diagy[i] - nr[m + i];
+        for (int q = 0; q < m; ++q)
+            AA[i * M1 + q] = nr[2 * m + i * m + q] + (i == q ? diaglamyi : 0.0);
+        AA[i * M1 + m] = k.a[i];
+        AA[m * M1 + i] = k.a[i];
+    }
+    AA[m * M1 + m] = -c.zet / c.z;
+    rhs[m] = delz;
+    if (!mma_solve_small(M1, AA, rhs)) {
+        c.status = kCtlSingular;
+        c.phase = kPhDone;
+        return;
+    }
+    for (int i = 0; i < m; ++i) {
+        c.dl[i] = rhs[i];
+        c.base[i] = c.lam[i];
+        c.dy[i] = -dely[i] / diagy[i] + rhs[i] / diagy[i];
+    }
+    c.dz = rhs[m];
+    c.phase = kPhDir;
+}
+
+// subsolv's inner-loop test: another Newton step at this barrier level, or
+// the next level (or the end)
+__device__ void mma_next_step(const MmaConst &k, MmaCtl &c, const double *trial_newton)
+{
+    if (c.rmax > 0.9 * c.epsi && c.ittt < 200) {
+        ++c.ittt;
+        ++c.steps;
+        if (c.have_sums) {
+            mma_newton_solve(k, c, trial_newton);
+        } else {
+            c.phase = kPhNewton;
+        }
+        return;
+    }
+    c.epsi *= 0.1;
+    if (c.epsi > k.sub_tol && c.outer < k.sub_maxiter) {
+        ++c.outer;
+        c.phase = kPhResid;
+    } else {
+        c.phase = kPhDone;
+    }
+}
+
+// set the trial m part for step length steg and request the trial pass
+__device__ void mma_trial_point(const MmaConst &k, MmaCtl &c)
+{
+#pragma clang fp contract(off)
+    ++c.itto;
+    const double steg = c.steg;
+    for (int i = 0; i < k.m; ++i) {
+        c.y[i] = c.yo[i] + steg * c.dy[i];
+        c.lam[i] = c.lamo[i] + steg * c.dl[i];
+        c.mu[i] = c.muo[i] + steg * c.dmu[i];
+        c.sl[i] = c.so[i] + steg * c.ds[i];
+    }
+    c.z = c.zo + steg * c.dz;
+    c.zet = c.zeto + steg * c.dzet;
+    c.phase = kPhTrial;
+}
+
+constexpr int kCtlThreads = 128;
+
+// reduce the record the last pass left (one thread per record entry, block
+// order) and take the next control step
+__global__ __launch_bounds__(kCtlThreads) void mma_ctl_k(MmaCtl *__restrict__ ctl,
+                                                         const double *__restrict__ partials, unsigned nb,
+                                                         MmaConst k)
+{
+#pragma clang fp contract(off)
+    __shared__ double red[4 + 3 * kMmaMaxM + kMmaMaxM * kMmaMaxM];
+    const int phase = ctl->phase;
+    if (phase == kPhDone)
+        return;
+    const int m = k.m, nrec = 2 * m + m * m, t = threadIdx.x;
+    // entries: [0, 1 + m) residual sums, 1 + m residual max, then the Newton
+    // record (newton pass: at 0; trial pass: behind the residual record)
+    auto sum = [&](long off, int rec, int e) {
+        double v = 0.0;
+        for (unsigned b = 0; b < nb; ++b)
+            v += partials[off + (long)b * rec + e];
+        return v;
+    };
+    auto mx = [&](long off) {
+        double v = -INFINITY;
+        for (unsigned b = 0; b < nb; ++b)
+            v = fmax(v, partials[off + b]);
+        return v;
+    };
+    if (phase == kPhPrep) {
+        if (t < m)
+            red[t] = sum(0, m, t);
+    } else if (phase == kPhNewton) {
+        if (t < nrec)
+            red[2 + m + t] = sum(0, nrec, t);
+    } else if (phase == kPhDir) {
+        if (t == 0)
+            red[0] = mx(0);
+    } else {   // residual or trial
+        if (t < 1 + m)
+            red[t] = sum(0, 1 + m, t);
+        else if (t == 1 + m)
+            red[t] = mx((long)nb * (1 + m));
+        else if (phase == kPhTrial && t < 2 + m + nrec)
+            red[t] = sum((long)nb * (2 + m), nrec, t - 2 - m);
+    }
+    __syncthreads();
+    if (t != 0)
+        return;
+    MmaCtl c = *ctl;
+    ++c.units;
+    const double *newton_rec = red + 2 + m;
+    switch (phase) {
+    case kPhPrep: {
+        for (int i = 0; i < m; ++i) {
+            c.b[i] = red[i] - k.fval[i];
+            c.y[i] = 1.0;
+            c.lam[i] = 1.0;
+            c.mu[i] = fmax(1.0, 0.5 * k.c[i]);
+            c.sl[i] = 1.0;
+        }
+        c.z = 1.0;
+        c.zet = 1.0;
+        c.epsi = 1.0;
+        c.outer = 0;
+        if (c.epsi > k.sub_tol && c.outer < k.sub_maxiter) {
+            ++c.outer;
+            c.phase = kPhResid;
+        } else {
+            c.phase = kPhDone;
+        }
+        break;
+    }
+    case kPhResid: {
+        mma_m_residual(k, c, red, c.epsi, c.rnorm, c.rmax);
+        c.ittt = 0;
+        c.have_sums = 0;
+        mma_next_step(k, c, newton_rec);
+        break;
+    }
+    case kPhNewton: mma_newton_solve(k, c, newton_rec); break;
+    case kPhDir: {
+        const double epsi = c.epsi;
+        double stm = red[0];
+        for (int i = 0; i < m; ++i) {
+            c.dmu[i] = -c.mu[i] + epsi / c.y[i] - (c.mu[i] * c.dy[i]) / c.y[i];
+            c.ds[i] = -c.sl[i] + epsi / c.lam[i] - (c.sl[i] * c.dl[i]) / c.lam[i];
+            stm = fmax(stm, fmax(fmax(-1.01 * c.dy[i] / c.y[i], -1.01 * c.dl[i] / c.lam[i]),
+                                 fmax(-1.01 * c.dmu[i] / c.mu[i], -1.01 * c.ds[i] / c.sl[i])));
+        }
+        c.dzet = -c.zet + epsi / c.z - c.zet * c.dz / c.z;
+        stm = fmax(stm, fmax(-1.01 * c.dz / c.z, -1.01 * c.dzet / c.zet));
+        c.steg = 1.0 / fmax(stm, 1.0);
+        for (int i = 0; i < m; ++i) {
+            c.yo[i] = c.y[i];
+            c.lamo[i] = c.lam[i];
+            c.muo[i] = c.mu[i];
+            c.so[i] = c.sl[i];
+        }
+        c.zo = c.z;
+        c.zeto = c.zet;
+        // a non-finite residual would end the line search before any trial
+        if (!isfinite(c.rnorm) || !isfinite(c.rmax)) {
+            c.status = kCtlNonFinite;
+            c.phase = kPhDone;
+            break;
+        }
+        c.itto = 0;
+        if (2.0 * c.rnorm > c.rnorm) {   // resinew = 2 rnorm > rnorm
+            mma_trial_point(k, c);
+        } else {   // rnorm == 0: no trial, keep the point; its Newton sums
+                   // are not in this record, so the next step recomputes them
+            c.have_sums = 0;
+            mma_next_step(k, c, newton_rec);
+        }
+        break;
+    }
+    default: {   // trial
+        double resinew, rmx;
+        mma_m_residual(k, c, red, c.epsi, resinew, rmx);
+        c.steg /= 2.0;
+        if (resinew > c.rnorm && c.itto < 50) {
+            mma_trial_point(k, c);
+            break;
+        }
+        // accept the last trial point (as the reference does, also after 50
+        // halvings)
+        c.par ^= 1;
+        c.rnorm = resinew;
+        c.rmax = rmx;
+        c.have_sums = 1;
+        mma_next_step(k, c, newton_rec);
+        break;
+    }
+    }
+    *ctl = c;
+}
+
+__global__ void mma_ctl_init_k(MmaCtl *ctl)
+{
+    if (threadIdx.x == 0) {
+        MmaCtl c = {};
+        c.phase = kPhPrep;
+        *ctl = c;
+    }
+}
+
+// x_new = the current point's x
+__global__ __launch_bounds__(kMmaBlock) void mma_result_k(const MmaCtl *__restrict__ ctl, long n,
+                                                         const double *__restrict__ xs,
+                                                         const double *__restrict__ xt, double *__restrict__ x_new)
+{
+    const double *src = ctl->par ? xt : xs;
+    for (long j = (long)blockIdx.x * kMmaBlock + threadIdx.x; j < n; j += (long)gridDim.x * kMmaBlock)
+        x_new[j] = src[j];
+}
+
 #define MMA_DISPATCH(m, KERNEL, grid, ...)                                                                 \
     switch (m) {                                                                                          \
     case 1: hipLaunchKernelGGL(KERNEL<1>, grid, dim3(kMmaBlock), 0, s, __VA_ARGS__); break;               \
@@ -332,81 +701,25 @@ static unsigned mma_blocks(long n)
     return (unsigned)(b < kMmaMaxBlocks ? (b > 0 ? b : 1) : kMmaMaxBlocks);
 }
 
+// units of the previous update (first batch size of the next one)
+static std::atomic<int> g_mma_last_units{0};
+
 }  // namespace fantom
 
 using namespace fantom;
 
-namespace {
-
-struct Reader {
-    hipStream_t s;
-    std::vector<double> buf;
-    int fetch(const double *dev, size_t count)
-    {
-        buf.resize(count);
-        if (hipMemcpyAsync(buf.data(), dev, count * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
-            hipStreamSynchronize(s) != hipSuccess) {
-            set_error("mma readback: %s", hipGetErrorString(hipGetLastError()));
-            return kStatusHipError;
-        }
-        return kStatusOk;
-    }
-    // sum of record entry k over blocks, in block order
-    double sum(unsigned nb, int rec, int k) const
-    {
-        double v = 0.0;
-        for (unsigned b = 0; b < nb; ++b)
-            v += buf[(size_t)b * rec + k];
-        return v;
-    }
-    double max(unsigned nb, size_t off, int rec, int k) const
-    {
-        double v = -INFINITY;
-        for (unsigned b = 0; b < nb; ++b)
-            v = fmax(v, buf[off + (size_t)b * rec + k]);
-        return v;
-    }
-};
-
-// small dense solve (partial pivoting), n <= kMmaMaxM + 1
-bool solve_small(int n, double *A, double *b)
+static long mma_ctl_offset(long n, int m)
 {
-    for (int k = 0; k < n; ++k) {
-        int p = k;
-        for (int i = k + 1; i < n; ++i)
-            if (fabs(A[i * n + k]) > fabs(A[p * n + k]))
-                p = i;
-        if (A[p * n + k] == 0.0)
-            return false;
-        if (p != k) {
-            for (int j = 0; j < n; ++j)
-                std::swap(A[k * n + j], A[p * n + j]);
-            std::swap(b[k], b[p]);
-        }
-        for (int i = k + 1; i < n; ++i) {
-            const double f = A[i * n + k] / A[k * n + k];
-            for (int j = k; j < n; ++j)
-                A[i * n + j] -= f * A[k * n + j];
-            b[i] -= f * b[k];
-        }
-    }
-    for (int k = n - 1; k >= 0; --k) {
-        double v = b[k];
-        for (int j = k + 1; j < n; ++j)
-            v -= A[k * n + j] * b[j];
-        b[k] = v / A[k * n + k];
-    }
-    return true;
+    const long rec = 2 * m + m * m + 2 * (1 + m);
+    return (long)(11 + 2 * m) * n + (long)kMmaMaxBlocks * (rec + 2);
 }
-
-}  // namespace
 
 extern "C" long fantom_mma_workspace_size(long n, int m)
 {
     if (n <= 0 || m < 1 || m > kMmaMaxM)
         return -1;
-    const long rec = 2 * m + m * m + 2 * (1 + m);
-    return (long)sizeof(double) * ((long)(11 + 2 * m) * n + (long)kMmaMaxBlocks * (rec + 2) + 64);
+    const long ctl_doubles = ((long)sizeof(MmaCtl) + 7) / 8;
+    return (long)sizeof(double) * (mma_ctl_offset(n, m) + ctl_doubles + 64);
 }
 
 extern "C" int fantom_mma_update_f64(long n, int m, int iter, const double *x, const double *xold1,
@@ -454,196 +767,51 @@ extern "C" int fantom_mma_update_f64(long n, int m, int iter, const double *x, c
     dv.xsit = take(n);
     dv.etat = take(n);
     dv.partials = take(0);
+    MmaCtl *ctl = reinterpret_cast<MmaCtl *>(w + mma_ctl_offset(n, m));
     const unsigned nb = mma_blocks(n);
-    Reader rd{s, {}};
-
-    MMA_DISPATCH(m, mma_prep_k, dim3(nb), dv, iter, move);
-    FANTOM_LAUNCH_CHECK("mma_prep");
-    int st = rd.fetch(dv.partials, (size_t)nb * m);
-    if (st)
-        return st;
-    double b[kMmaMaxM];
-    for (int i = 0; i < m; ++i)
-        b[i] = rd.sum(nb, m, i) - fval[i];
-
-    // subproblem state (m part) on the host, mirroring subsolv's initial point
-    double y[kMmaMaxM], lam[kMmaMaxM], mu[kMmaMaxM], sl[kMmaMaxM];
-    double z = 1.0, zet = 1.0;
+    MmaConst kc = {};
+    kc.m = m;
+    kc.sub_maxiter = sub_maxiter;
+    kc.a0 = a0;
+    kc.sub_tol = sub_tol;
     for (int i = 0; i < m; ++i) {
-        y[i] = 1.0;
-        lam[i] = 1.0;
-        mu[i] = fmax(1.0, 0.5 * c[i]);
-        sl[i] = 1.0;
+        kc.a[i] = a[i];
+        kc.c[i] = c[i];
+        kc.d[i] = d[i];
+        kc.fval[i] = fval[i];
     }
-    const int rrec = 1 + m;   // residual record: sumsq, gvec[m]; then nb maxabs
-    // the current point's multipliers and the dual direction of the Newton
-    // step whose line search is running (the trial forms the n direction)
-    MmaScal trial_base, trial_dl;
-    auto residual = [&](double epsi, bool trial, double steg, double *gvec, double &rnorm, double &rmax,
-                        const double *yv, double zv, const double *lamv, const double *muv, double zetv,
-                        const double *sv) -> int {
-        MmaScal sc;
-        for (int i = 0; i < m; ++i)
-            sc.lam[i] = lamv[i];
-        if (trial) {
-            MMA_DISPATCH(m, mma_trial_newton_k, dim3(nb), dv, sc, trial_base, trial_dl, steg, epsi);
-        } else {
-            MMA_DISPATCH(m, mma_resid_k, dim3(nb), dv, sc, epsi);
-        }
-        FANTOM_LAUNCH_CHECK("mma_resid");
-        int r = rd.fetch(dv.partials, (size_t)nb * (rrec + 1));
-        if (r)
-            return r;
-        double ss = rd.sum(nb, rrec, 0);
-        rmax = rd.max(nb, (size_t)nb * rrec, 1, 0);
-        double alam = 0.0;
-        for (int i = 0; i < m; ++i)
-            alam += a[i] * lamv[i];
-        for (int i = 0; i < m; ++i) {
-            gvec[i] = rd.sum(nb, rrec, 1 + i);
-            const double rey = c[i] + d[i] * yv[i] - muv[i] - lamv[i];
-            const double relam = gvec[i] - a[i] * zv - yv[i] + sv[i] - b[i];
-            const double remu = muv[i] * yv[i] - epsi;
-            const double res = lamv[i] * sv[i] - epsi;
-            ss += rey * rey + relam * relam + remu * remu + res * res;
-            rmax = fmax(rmax, fmax(fmax(fabs(rey), fabs(relam)), fmax(fabs(remu), fabs(res))));
-        }
-        const double rez = a0 - zetv - alam, rezet = zetv * zv - epsi;
-        ss += rez * rez + rezet * rezet;
-        rmax = fmax(rmax, fmax(fabs(rez), fabs(rezet)));
-        rnorm = sqrt(ss);
-        return kStatusOk;
-    };
 
-    double epsi = 1.0;
-    int outer = 0, steps = 0;
-    double gvec[kMmaMaxM];
-    while (epsi > sub_tol && outer < sub_maxiter) {
-        ++outer;
-        double rnorm, rmax;
-        st = residual(epsi, false, 0.0, gvec, rnorm, rmax, y, z, lam, mu, zet, sl);
-        if (st)
-            return st;
-        int ittt = 0;
-        // the accepted trial of the previous step left the Newton sums of
-        // this point (same barrier parameter) behind its residual record
-        bool have_sums = false;
-        while (rmax > 0.9 * epsi && ittt < 200) {
-            ++ittt;
-            ++steps;
-            MmaScal sc;
-            for (int i = 0; i < m; ++i)
-                sc.lam[i] = lam[i];
-            const int nrec = 2 * m + m * m;
-            if (have_sums) {
-                st = rd.fetch(dv.partials + (size_t)nb * (2 + m), (size_t)nb * nrec);
-            } else {
-                MMA_DISPATCH(m, mma_newton_k, dim3(nb), dv, sc, epsi);
-                FANTOM_LAUNCH_CHECK("mma_newton");
-                st = rd.fetch(dv.partials, (size_t)nb * nrec);
-            }
-            if (st)
-                return st;
-            double gv[kMmaMaxM], sgd[kMmaMaxM], AA[(kMmaMaxM + 1) * (kMmaMaxM + 1)], rhs[kMmaMaxM + 1];
-            for (int i = 0; i < m; ++i) {
-                gv[i] = rd.sum(nb, nrec, i);
-                sgd[i] = rd.sum(nb, nrec, m + i);
-            }
-            double dely[kMmaMaxM], diagy[kMmaMaxM];
-            double alam = 0.0;
-            for (int i = 0; i < m; ++i)
-                alam += a[i] * lam[i];
-            const double delz = a0 - alam - epsi / z;
-            const int M1 = m + 1;
-            for (int i = 0; i < m; ++i) {
-                dely[i] = c[i] + d[i] * y[i] - lam[i] - epsi / y[i];
-                const double dellam = gv[i] - a[i] * z - y[i] - b[i] + epsi / lam[i];
-                diagy[i] = d[i] + mu[i] / y[i];
-                const double diaglamyi = sl[i] / lam[i] + 1.0 / diagy[i];
-                rhs[i] = dellam + dely[i] / diagy[i] - sgd[i];
-                for (int k = 0; k < m; ++k)
-                    AA[i * M1 + k] = rd.sum(nb, nrec, 2 * m + i * m + k) + (i == k ? diaglamyi : 0.0);
-                AA[i * M1 + m] = a[i];
-                AA[m * M1 + i] = a[i];
-            }
-            AA[m * M1 + m] = -zet / z;
-            rhs[m] = delz;
-            if (!solve_small(M1, AA, rhs)) {
-                set_error("MMA Newton system is singular");
-                return kStatusHipError;
-            }
-            MmaScal dl;
-            for (int i = 0; i < m; ++i)
-                dl.lam[i] = rhs[i];
-            trial_base = sc;
-            trial_dl = dl;
-            const double dz = rhs[m];
-            MMA_DISPATCH(m, mma_dir_k, dim3(nb), dv, sc, dl, epsi);
-            FANTOM_LAUNCH_CHECK("mma_dir");
-            st = rd.fetch(dv.partials, (size_t)nb);
-            if (st)
-                return st;
-            double stm = rd.max(nb, 0, 1, 0);
-            double dy[kMmaMaxM], dmu[kMmaMaxM], ds[kMmaMaxM];
-            for (int i = 0; i < m; ++i) {
-                dy[i] = -dely[i] / diagy[i] + dl.lam[i] / diagy[i];
-                dmu[i] = -mu[i] + epsi / y[i] - (mu[i] * dy[i]) / y[i];
-                ds[i] = -sl[i] + epsi / lam[i] - (sl[i] * dl.lam[i]) / lam[i];
-                stm = fmax(stm, fmax(fmax(-1.01 * dy[i] / y[i], -1.01 * dl.lam[i] / lam[i]),
-                                     fmax(-1.01 * dmu[i] / mu[i], -1.01 * ds[i] / sl[i])));
-            }
-            const double dzet = -zet + epsi / z - zet * dz / z;
-            stm = fmax(stm, fmax(-1.01 * dz / z, -1.01 * dzet / zet));
-            double steg = 1.0 / fmax(stm, 1.0);
-            double yo[kMmaMaxM], lamo[kMmaMaxM], muo[kMmaMaxM], so[kMmaMaxM];
-            for (int i = 0; i < m; ++i) {
-                yo[i] = y[i];
-                lamo[i] = lam[i];
-                muo[i] = mu[i];
-                so[i] = sl[i];
-            }
-            const double zo = z, zeto = zet;
-            // a non-finite residual would skip the line search below and leave
-            // the trial arrays unwritten: stop with an error instead
-            if (!std::isfinite(rnorm) || !std::isfinite(rmax)) {
-                set_error("MMA subproblem residual is not finite (%g)", rnorm);
-                return kStatusHipError;
-            }
-            double resinew = 2.0 * rnorm, rmx = rmax;
-            int itto = 0;
-            while (resinew > rnorm && itto < 50) {
-                ++itto;
-                for (int i = 0; i < m; ++i) {
-                    y[i] = yo[i] + steg * dy[i];
-                    lam[i] = lamo[i] + steg * dl.lam[i];
-                    mu[i] = muo[i] + steg * dmu[i];
-                    sl[i] = so[i] + steg * ds[i];
-                }
-                z = zo + steg * dz;
-                zet = zeto + steg * dzet;
-                st = residual(epsi, true, steg, gvec, resinew, rmx, y, z, lam, mu, zet, sl);
-                if (st)
-                    return st;
-                steg /= 2.0;
-            }
-            // accept the last trial point (as the reference does, also after 50
-            // halvings); the trial arrays hold a point only once a trial ran
-            if (itto > 0) {
-                std::swap(dv.xs, dv.xt);
-                std::swap(dv.xsi, dv.xsit);
-                std::swap(dv.eta, dv.etat);
-                rnorm = resinew;
-                rmax = rmx;
-                have_sums = true;
-            }
+    hipLaunchKernelGGL(mma_ctl_init_k, dim3(1), dim3(64), 0, s, ctl);
+    FANTOM_LAUNCH_CHECK("mma_ctl_init");
+    int hdr[kMmaCtlHeaderInts];
+    int batch = std::max(8, g_mma_last_units.load() + 2);
+    for (;;) {
+        for (int u = 0; u < batch; ++u) {
+            MMA_DISPATCH(m, mma_pass_k, dim3(nb), dv, ctl, iter, move);
+            hipLaunchKernelGGL(mma_ctl_k, dim3(1), dim3(kCtlThreads), 0, s, ctl, dv.partials, nb, kc);
         }
-        epsi *= 0.1;
+        FANTOM_LAUNCH_CHECK("mma units");
+        if (hipMemcpyAsync(hdr, ctl, sizeof(hdr), hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess) {
+            set_error("mma status read: %s", hipGetErrorString(hipGetLastError()));
+            return kStatusHipError;
+        }
+        if (hdr[0] == kPhDone)
+            break;
+        batch = 16;
     }
-    if (hipMemcpyAsync(x_new, dv.xs, sizeof(double) * n, hipMemcpyDeviceToDevice, s) != hipSuccess) {
-        set_error("mma copy: %s", hipGetErrorString(hipGetLastError()));
+    g_mma_last_units.store(hdr[3]);
+    if (hdr[1] == kCtlSingular) {
+        set_error("MMA Newton system is singular");
         return kStatusHipError;
     }
+    if (hdr[1] == kCtlNonFinite) {
+        set_error("MMA subproblem residual is not finite");
+        return kStatusHipError;
+    }
+    hipLaunchKernelGGL(mma_result_k, dim3(nb), dim3(kMmaBlock), 0, s, ctl, n, dv.xs, dv.xt, x_new);
+    FANTOM_LAUNCH_CHECK("mma_result");
     if (newton_steps)
-        *newton_steps = steps;
+        *newton_steps = hdr[2];
     return kStatusOk;
 }

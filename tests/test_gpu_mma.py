@@ -37,7 +37,7 @@ def test_mma_update_vs_oracle(n, m, it, seed):
     fval = rng.uniform(-0.02, 0.02, m)
     xmin, xmax = np.zeros(n), np.ones(n)
     a, c, d = np.zeros(m), np.full(m, 1e5), np.zeros(m)
-    xr, lowr, uppr, _ = M.mmasub(it, x, xmin, xmax, xold1, xold2, df0, fval, dfdx, low.copy(), upp.copy(), a0=1.0,
+    xr, lowr, uppr, extra = M.mmasub(it, x, xmin, xmax, xold1, xold2, df0, fval, dfdx, low.copy(), upp.copy(), a0=1.0,
                                  a=a, c=c, d=d, move=0.5)
     dev = lambda v: torch.from_numpy(np.ascontiguousarray(v)).cuda()
     xd, x1d, x2d, dfd, dgd = dev(x), dev(xold1), dev(xold2), dev(df0), dev(dfdx)
@@ -55,8 +55,60 @@ def test_mma_update_vs_oracle(n, m, it, seed):
     assert np.abs(lowd.cpu().numpy() - lowr).max() <= 1e-14
     assert np.abs(uppd.cpu().numpy() - uppr).max() <= 1e-14
     err = np.abs(xn.cpu().numpy() - xr).max()
-    print(f"  newton steps {steps.value}, max |x - x_oracle| = {err:.2e}", flush=True)
+    ref_steps = extra[-1][-1]
+    print(f"  newton steps {steps.value} (oracle {ref_steps}), max |x - x_oracle| = {err:.2e}", flush=True)
     assert err <= 1e-8
+    # the device control flow takes the oracle's path: the same number of
+    # Newton steps up to a borderline stopping test
+    assert abs(steps.value - ref_steps) <= 2
+
+
+def test_mma_device_control_repeatable():
+    """Two updates on the same data give bit-identical points and step
+    counts although their first status reads come after different numbers
+    of queued units (the first batch is sized from the previous update)."""
+    from pyfantom_amd import _lib
+    torch = _torch()
+    n, m = 300000, 2
+    print(f"mma repeatable n={n} m={m}", flush=True)
+    rng = np.random.default_rng(11)
+    x = rng.uniform(0.05, 0.95, n)
+    dev = lambda v: torch.from_numpy(np.ascontiguousarray(v)).cuda()
+    xd, dfd = dev(x), dev(-rng.uniform(0.0, 2.0, n))
+    dgd = dev(rng.uniform(0.5, 1.5, (m, n)) / n)
+    fval = (ctypes.c_double * m)(0.01, -0.01)
+    zeros = (ctypes.c_double * m)(0.0, 0.0)
+    cc = (ctypes.c_double * m)(1e5, 1e5)
+    nbytes = _lib.load().fantom_mma_workspace_size(n, m)
+    out = []
+    for trial in range(3):
+        lowd, uppd = dev(x - 0.3), dev(x + 0.3)
+        xn = torch.empty_like(xd)
+        work = torch.empty((nbytes + 7) // 8, dtype=torch.float64, device="cuda")
+        steps = ctypes.c_int(0)
+        _lib.call("fantom_mma_update_f64", n, m, 1, xd.data_ptr(), None, None, 0.0, 1.0, dfd.data_ptr(), fval,
+                  dgd.data_ptr(), lowd.data_ptr(), uppd.data_ptr(), 0.2, 1.0, zeros, cc, zeros, 1e-7, 100,
+                  xn.data_ptr(), work.data_ptr(), work.numel() * 8, ctypes.byref(steps))
+        torch.cuda.synchronize()
+        out.append((xn.cpu(), steps.value))
+        if trial == 0:
+            # a tiny update in between resets the first-batch size
+            t = torch.full((16,), 0.5, dtype=torch.float64, device="cuda")
+            w2 = torch.empty(_lib.load().fantom_mma_workspace_size(16, 1) // 8 + 1, dtype=torch.float64,
+                             device="cuda")
+            lo, up, xx = t - 0.5, t + 0.5, torch.empty_like(t)
+            one = (ctypes.c_double * 1)(1e5)
+            z1 = (ctypes.c_double * 1)(0.0)
+            f1 = (ctypes.c_double * 1)(0.0)
+            _lib.call("fantom_mma_update_f64", 16, 1, 1, t.data_ptr(), None, None, 0.0, 1.0,
+                      (-t).contiguous().data_ptr(), f1, (t / 16).contiguous().data_ptr(), lo.data_ptr(),
+                      up.data_ptr(), 0.2, 1.0, z1, one, z1, 1e-7, 100, xx.data_ptr(), w2.data_ptr(),
+                      w2.numel() * 8, None)
+            torch.cuda.synchronize()
+    print(f"  steps {[o[1] for o in out]}", flush=True)
+    for xo, so in out[1:]:
+        assert so == out[0][1]
+        assert torch.equal(xo, out[0][0])
 
 
 def test_mma_rejects_bad_sizes():
