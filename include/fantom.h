@@ -372,6 +372,41 @@ int fantom_mx_set_omega_schedule(void *handle, int adaptive, double min_omega, d
  * (power iterations on FP32 vectors with FP64 sums). */
 int fantom_mx_set_smoother(void *handle, int kind, double rho_scale, int iters_first, int iters);
 int fantom_mx_smoother_rho(void *handle, double *rho, int n);
+/* Operator of the V-cycle's fine and level-1 sweeps (replaces, inside the
+ * preconditioner, the element contraction of core/CUDA/Kernels/matvec.cu:1-98
+ * and the Galerkin product of core/CPU/_mgm.py:308-942): kind 0 = FP32 in
+ * the Walsh-Hadamard form (default); bit 0 / bit 1: the fine / level-1
+ * sweeps with BF16 operands (node values and the element / child matrices)
+ * and FP32 accumulation on the matrix cores (v_mfma_f32_16x16x16_bf16),
+ * moduli and vectors FP32; bits 2-3: the split forms of fantom_sweep_bf16.
+ * The outer PCG, the coarser levels and the coarsest solve are unchanged. */
+int fantom_mx_set_operator(void *handle, int kind);
+/* Flexible PCG in fantom_mx_solve_pcg (on != 0): beta = r_new.(z_new -
+ * z_old) / r_old.z_old (Polak-Ribiere) instead of the reference's
+ * r_new.z_new / r_old.z_old (solvers/CUDA/_solvers.py:646-652), which keeps
+ * the PCG converging when the preconditioner is not an exactly fixed linear
+ * operator (rounding to BF16 in the V-cycle); one more FP32 vector. */
+int fantom_mx_set_flexible(void *handle, int on);
+/* The BF16 matrix-core sweep of fantom_mx_set_operator kind 1, on its own
+ * (same element contraction as core/CUDA/Kernels/matvec.cu:1-98, BF16
+ * operands, FP32 sums and vectors).  sweep_bf16_tables (host): the operand
+ * table (table_size uint16 values) of nterm 1 = K0 (24 x 24, reference corner
+ * order) or nterm 8 = the 8 level-1 child matrices G of galerkin_tables.
+ * sweep_bf16 on the grid nelx x nely x nelz (the level-1 grid for nterm 8,
+ * whose moduli E are then the fine ones, grid 2x): mode 0 y = A u, 1 one
+ * Jacobi sweep y = u + omega D^-1 (b - A u), 2 residual y = b - A u, 4 two
+ * sweeps from x0 = omega dinv b (u unused).  nterm 1 forms D^-1 from the
+ * moduli when rd (host float[3], 1 / K0 diagonal per component) is given
+ * (mode 1 with dinv NULL, and the second sweep of mode 4); Dirichlet nodes
+ * (mask bit c) keep identity rows.  split bit 0: the node values enter as
+ * hi + lo BF16 pairs (x - bf16(x) rounded again, two products), bit 1
+ * (nterm 1): the element matrix likewise; the table holds both parts.
+ * Device pointers except tab's source and rd. */
+long fantom_sweep_bf16_table_size(int nterm);
+int fantom_sweep_bf16_tables(int nterm, const double *M, uint16_t *out);
+int fantom_sweep_bf16(int mode, int nterm, int split, int nelx, int nely, int nelz, const uint16_t *tab, const float *E,
+                      const uint8_t *mask, const float *u, float *y, const float *b, const float *dinv, double omega,
+                      const float *rd, void *stream);
 int fantom_mx_setup(void *handle, const double *E, const uint8_t *mask, void *stream);
 /* z = V(r) with the FP32 V-cycle: r (FP64) is rounded to FP32, the cycle runs
  * as in fantom_mx_solve_pcg, z is returned in FP64 (a preconditioner for a

@@ -85,6 +85,12 @@ SIGNATURES = {
     "fantom_mx_destroy": (_c_int, [_vp]),
     "fantom_mx_set_omega_schedule": (_c_int, [_vp, _c_int, _c_double, _c_double]),
     "fantom_mx_set_smoother": (_c_int, [_vp, _c_int, _c_double, _c_int, _c_int]),
+    "fantom_mx_set_operator": (_c_int, [_vp, _c_int]),
+    "fantom_mx_set_flexible": (_c_int, [_vp, _c_int]),
+    "fantom_sweep_bf16_table_size": (_c_long, [_c_int]),
+    "fantom_sweep_bf16_tables": (_c_int, [_c_int, _vp, _vp]),
+    "fantom_sweep_bf16": (_c_int, [_c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp,
+                                   _vp, _c_double, _vp, _vp]),
     "fantom_mx_precondition": (_c_int, [_vp, _vp, _vp, _vp]),
     "fantom_mx_smoother_rho": (_c_int, [_vp, _vp, _c_int]),
     "fantom_mx_setup": (_c_int, [_vp, _vp, _vp, _vp]),

@@ -46,6 +46,17 @@ __device__ __forceinline__ double schedule_omega(const OmegaSchedule &o, double 
 // cheb.hip (fourth-kind Chebyshev smoothing).  st: device double[4] per
 // level = {|v|_D^2, rho, 1 / (rho_scale rho), -}; partials: 2 * kVecMaxBlocks
 void chebyshev4_weights(int n, double *w);
+// BF16 matrix-core sweeps (mfma_sweep.hip): nterm 1 = K0 scaled by the
+// level's moduli, 8 = the level-1 Galerkin operator from the fine moduli
+// split bit 0: node values as hi + lo BF16 pairs; bit 1 (nterm 1): the
+// element matrix as hi + lo pairs (mf_tables writes both tables)
+void mf_tables(const double *M, int nterm, uint16_t *out);
+long mf_table_halves(int nterm);
+long mf_table_size(int nterm);
+int launch_mf_sweep(int mode, int nterm, const Grid &g, const uint16_t *Atab, const float *E, const uint8_t *mask,
+                    const float *u, float *y, const float *b, const float *dinv, double omega, const float *rd,
+                    hipStream_t s, int split = 0);
+
 int launch_ch_init(long n, double *v, const double *dinv, int fill, int salt, double *partials, double *st,
                    hipStream_t s);
 int launch_ch_init_f32(long n, float *v, const float *dinv, int fill, int salt, double *partials, double *st,

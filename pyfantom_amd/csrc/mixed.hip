@@ -77,6 +77,16 @@ struct MxHandle {
     double cheb_scale;
     int cheb_iters_first, cheb_iters, cheb_setups;
     double *chs;   // device double[4 nlev]
+    // operator of the fine and level-1 sweeps (fantom_mx_set_operator): bit
+    // 0 / 1 the fine / level-1 sweeps on the matrix cores (BF16 operands,
+    // else FP32 in the WHT form), bits 2-3 the split forms of the operands
+    int op_kind;
+    uint16_t *Amf1, *Amf8;   // MFMA operand tables: K0, the 8 level-1 child matrices
+    float rd32[3];           // 1 / K0 diagonal per component
+    // flexible PCG (fantom_mx_set_flexible): Polak-Ribiere beta with the
+    // previous preconditioned residual
+    int flexible;
+    float *zprev;
 };
 
 __global__ void mx_copy_f32(long n, const float *__restrict__ a, float *__restrict__ b)
@@ -224,9 +234,11 @@ __global__ __launch_bounds__(256) void mxd_r_k(long n, double *__restrict__ r, f
     }
 }
 
-// x += alpha p (when mxd_r_k updated r in this iteration), then p = z + beta p
+// x += alpha p (when mxd_r_k updated r in this iteration), then p = z + beta p;
+// zprev (flexible PCG, optional) = z
 __global__ __launch_bounds__(256) void mxd_px_k(long n, double *__restrict__ x, double *__restrict__ p,
-                                               const float *__restrict__ z, const double *__restrict__ scal)
+                                               const float *__restrict__ z, const double *__restrict__ scal,
+                                               float *__restrict__ zprev)
 {
     const bool upd = scal[X_XUPD] != 0.0;
     const double alpha = scal[S_ALPHA], beta = scal[S_BETA];
@@ -234,7 +246,10 @@ __global__ __launch_bounds__(256) void mxd_px_k(long n, double *__restrict__ x, 
         const double pi = p[i];
         if (upd)
             x[i] += alpha * pi;
-        p[i] = (double)z[i] + beta * pi;
+        const float zi = z[i];
+        p[i] = (double)zi + beta * pi;
+        if (zprev)
+            zprev[i] = zi;
     }
 }
 
@@ -259,16 +274,43 @@ __global__ void mxd_conv_k(const double *__restrict__ partials, long np, double 
     }
 }
 
-// partial sums of r . z with z the FP32 preconditioned residual
+// partial sums of r . z with z the FP32 preconditioned residual, and (flexible
+// PCG, zprev given) of r . zprev behind them
 __global__ __launch_bounds__(256) void mxd_dot_k(long n, const double *__restrict__ r, const float *__restrict__ z,
-                                                double *__restrict__ partials)
+                                                const float *__restrict__ zprev, double *__restrict__ partials)
 {
-    double s = 0.0;
-    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256)
-        s += r[i] * (double)z[i];
+    double s = 0.0, s2 = 0.0;
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+        const double ri = r[i];
+        s += ri * (double)z[i];
+        if (zprev)
+            s2 += ri * (double)zprev[i];
+    }
     s = block_sum<256>(s);
-    if (threadIdx.x == 0)
+    if (zprev)
+        s2 = block_sum<256>(s2);
+    if (threadIdx.x == 0) {
         partials[blockIdx.x] = s;
+        if (zprev)
+            partials[gridDim.x + blockIdx.x] = s2;
+    }
+}
+
+// beta of the flexible PCG (Polak-Ribiere): (r.z - r.zprev) / rz_old
+__global__ __launch_bounds__(1024) void mxd_beta_pr_k(const double *__restrict__ partials, long np,
+                                                     double *__restrict__ scal)
+{
+    double a = 0.0, c = 0.0;
+    for (long i = threadIdx.x; i < np; i += 1024) {
+        a += partials[i];
+        c += partials[np + i];
+    }
+    a = block_sum<1024>(a);
+    c = block_sum<1024>(c);
+    if (threadIdx.x == 0) {
+        scal[S_BETA] = (a - c) / scal[S_RZ];
+        scal[S_RZ] = a;
+    }
 }
 
 // p = z + beta p (first: p = z)
@@ -311,6 +353,15 @@ static int mx_ku(MxHandle *h, int l, int mode, const float *u, float *y, const f
                  double *partials, double om = 0.0)
 {
     MxLevel &L = h->lev[l];
+    if (h->op_kind && mode != MODE_KUDOT && mode != MODE_JACDOT) {
+        const int split = (h->op_kind >> 2) & 3;
+        if (L.kind == 0 && (h->op_kind & 1))   // the fine sweeps form D^-1 from the moduli (x0 reads it)
+            return launch_mf_sweep(mode, 1, L.g, h->Amf1, L.E, L.mask, u, y, b, mode == MODE_JAC ? nullptr : dinv,
+                                   om, h->rd32, h->stream, split);
+        if (L.kind == 2 && (h->op_kind & 2))
+            return launch_mf_sweep(mode, 8, L.g, h->Amf8, h->lev[0].E, L.mask, u, y, b, dinv, om, nullptr,
+                                   h->stream, split & 1);
+    }
     if (L.kind == 2)
         return launch_gal1_f32(mode, L.g, h->Btab32, h->lev[0].E, L.mask, u, y, b, dinv, om, partials, h->stream);
     if (L.kind == 3)
@@ -414,9 +465,13 @@ static int mxd_iteration(MxHandle *h)
     hipLaunchKernelGGL(mxd_r_k, dim3(G), dim3(256), 0, s, n, h->r64, h->r, h->q64, h->scal, h->partials);
     hipLaunchKernelGGL(mxd_conv_k, dim3(1), dim3(1024), 0, s, h->partials, (long)G, h->scal, h->sched);
     MX_TRY(mx_precond(h, h->r, h->z));
-    hipLaunchKernelGGL(mxd_dot_k, dim3(G), dim3(256), 0, s, n, h->r64, h->z, h->partials);
-    MX_TRY(launch_finalize(h->partials, (long)G, h->scal, FIN_BETA, 0, s));
-    hipLaunchKernelGGL(mxd_px_k, dim3(G), dim3(256), 0, s, n, h->x64, h->p64, h->z, h->scal);
+    float *zp = h->flexible ? h->zprev : nullptr;
+    hipLaunchKernelGGL(mxd_dot_k, dim3(G), dim3(256), 0, s, n, h->r64, h->z, zp, h->partials);
+    if (zp)
+        hipLaunchKernelGGL(mxd_beta_pr_k, dim3(1), dim3(1024), 0, s, h->partials, (long)G, h->scal);
+    else
+        MX_TRY(launch_finalize(h->partials, (long)G, h->scal, FIN_BETA, 0, s));
+    hipLaunchKernelGGL(mxd_px_k, dim3(G), dim3(256), 0, s, n, h->x64, h->p64, h->z, h->scal, zp);
     FANTOM_LAUNCH_CHECK("mxd iteration");
     return kStatusOk;
 }
@@ -470,6 +525,9 @@ static void mx_layout(MxHandle *h, int nelx, int nely, int nelz, int n_coarse, i
         if (h->nlev == 2)
             h->Ke1 = cv.take<double>((size_t)h->lev[1].g.n_el * 576);
     }
+    h->zprev = cv.take<float>(h->n);
+    h->Amf1 = cv.take<uint16_t>(mf_table_size(1));
+    h->Amf8 = h->galerkin ? cv.take<uint16_t>(mf_table_size(8)) : nullptr;
     h->E64 = cv.take<double>(h->g.n_el);
     h->x64 = cv.take<double>(h->n);
     h->r64 = cv.take<double>(h->n);
@@ -614,6 +672,27 @@ int fantom_mx_create(int nelx, int nely, int nelz, int n_coarse, int n_smooth, d
         e = hipMemcpyAsync(h->Btab32, tB32.data(), sizeof(float) * tB32.size(), hipMemcpyHostToDevice, h->stream);
     if (h->galerkin && e == hipSuccess)
         e = hipMemcpyAsync(h->Gdiag, tD.data(), sizeof(double) * tD.size(), hipMemcpyHostToDevice, h->stream);
+    // matrix-core operand tables: K0 in natural corner order (b = bx + 2 by +
+    // 4 bz), the level-1 child matrices G_ch (already natural)
+    static const int loc[8] = {0, 1, 3, 2, 4, 5, 7, 6};   // natural corner -> reference local corner
+    std::vector<double> Kn(576);
+    for (int b = 0; b < 8; ++b)
+        for (int c = 0; c < 3; ++c)
+            for (int b2 = 0; b2 < 8; ++b2)
+                for (int d = 0; d < 3; ++d)
+                    Kn[(b * 3 + c) * 24 + b2 * 3 + d] = K0_host[(loc[b] * 3 + c) * 24 + loc[b2] * 3 + d];
+    for (int c = 0; c < 3; ++c)
+        h->rd32[c] = (float)(1.0 / Kn[c * 24 + c]);
+    h->op_kind = 0;
+    h->flexible = 0;
+    std::vector<uint16_t> tA1(mf_table_size(1)), tA8(h->galerkin ? mf_table_size(8) : 0);
+    mf_tables(Kn.data(), 1, tA1.data());
+    if (h->galerkin)
+        mf_tables(tG.data(), 8, tA8.data());
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(h->Amf1, tA1.data(), sizeof(uint16_t) * tA1.size(), hipMemcpyHostToDevice, h->stream);
+    if (h->galerkin && e == hipSuccess)
+        e = hipMemcpyAsync(h->Amf8, tA8.data(), sizeof(uint16_t) * tA8.size(), hipMemcpyHostToDevice, h->stream);
     if (e == hipSuccess)
         e = hipStreamSynchronize(h->stream);
     if (e != hipSuccess) {
@@ -622,6 +701,35 @@ int fantom_mx_create(int nelx, int nely, int nelz, int n_coarse, int n_smooth, d
         return kStatusHipError;
     }
     *handle = h;
+    return kStatusOk;
+}
+
+int fantom_mx_set_operator(void *handle, int kind)
+{
+    MxHandle *h = reinterpret_cast<MxHandle *>(handle);
+    FANTOM_CHECK_ARG(h, "null handle");
+    FANTOM_CHECK_ARG(kind >= 0 && kind <= 15, "operator kind must be in [0, 15]");
+    h->op_kind = kind;
+    if (h->have_pc_graph) {   // the captured iterations hold the old kernels
+        hipGraphExecDestroy(h->pc_exec);
+        h->have_pc_graph = false;
+    }
+    if (h->have_graph) {
+        hipGraphExecDestroy(h->iter_exec);
+        h->have_graph = false;
+    }
+    return kStatusOk;
+}
+
+int fantom_mx_set_flexible(void *handle, int on)
+{
+    MxHandle *h = reinterpret_cast<MxHandle *>(handle);
+    FANTOM_CHECK_ARG(h, "null handle");
+    h->flexible = on ? 1 : 0;
+    if (h->have_pc_graph) {   // the captured iterations hold the old beta
+        hipGraphExecDestroy(h->pc_exec);
+        h->have_pc_graph = false;
+    }
     return kStatusOk;
 }
 
@@ -973,9 +1081,11 @@ int fantom_mx_solve_pcg(void *handle, const double *f, double *x, double tol, in
     if (sqrt(fmax(hs[S_TMP], 0.0)) / hs[X_NB] >= tol) {
         hipLaunchKernelGGL(mxd_down_k, dim3(G), dim3(256), 0, s, n, h->r64, h->r);
         MX_TRY(mx_precond(h, h->r, h->z));
-        hipLaunchKernelGGL(mxd_dot_k, dim3(G), dim3(256), 0, s, n, h->r64, h->z, h->partials);
+        hipLaunchKernelGGL(mxd_dot_k, dim3(G), dim3(256), 0, s, n, h->r64, h->z, nullptr, h->partials);
         MX_TRY(launch_finalize(h->partials, (long)G, h->scal, FIN_RZ0, 0, s));
         hipLaunchKernelGGL(mxd_p_k, dim3(G), dim3(256), 0, s, n, h->p64, h->z, h->scal, 1);
+        if (h->flexible)
+            hipLaunchKernelGGL(mx_copy_f32, dim3(G), dim3(256), 0, s, n, h->z, h->zprev);
         FANTOM_LAUNCH_CHECK("mxd init");
         if (use_graph && !h->have_pc_graph)
             MX_TRY(mx_capture(h, true));

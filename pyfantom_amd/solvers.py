@@ -136,7 +136,7 @@ class MultiGrid(_DeviceKrylov):
                  low_level_maxiter=5000, min_omega=0.4, omega_boost=1.06, omega_schedule="reference",
                  check_every=1, native=True, use_graph=True, precision="fp64", inner_tol=1e-4,
                  max_outer=20, coarse=None, mixed_scheme="pcg", fallback=True, fallback_maxiter=1000,
-                 smoother="jacobi", cheb_rho_scale=1.0, cheb_power_iters=(12, 4)):
+                 smoother="jacobi", cheb_rho_scale=1.0, cheb_power_iters=(12, 4), flexible=None):
         if coarse_solver not in ["cg", "gmres", "splu", "spsolve", "cholmod"]:
             raise ValueError("Coarse solver not recognized.")
         if coarse_solver in ["splu", "spsolve"] and matrix_free:
@@ -171,11 +171,22 @@ class MultiGrid(_DeviceKrylov):
         # Python driver below also covers the reference's live 'w' cycle
         self.native = bool(native) and cycle != "w"
         self.use_graph = bool(use_graph)
-        if precision not in ("fp64", "mixed"):
-            raise ValueError("precision must be 'fp64' or 'mixed'")
-        if precision == "mixed" and not self.native:
+        if precision not in ("fp64", "mixed", "bf16"):
+            raise ValueError("precision must be 'fp64', 'mixed' or 'bf16'")
+        # 'bf16': the mixed-precision solve with the V-cycle's fine and
+        # level-1 sweeps on the matrix cores (BF16 operands, FP32 sums)
+        self.mixed = precision in ("mixed", "bf16")
+        if self.mixed and not self.native:
             raise ValueError("the mixed-precision solve runs in the native driver (V-cycle)")
         self.precision = precision
+        # flexible (Polak-Ribiere) PCG around the mixed-precision V-cycle;
+        # default on for the BF16 V-cycle, whose rounding makes the
+        # preconditioner vary from one application to the next
+        self.flexible = (precision == "bf16") if flexible is None else bool(flexible)
+        # operator kind of the BF16 V-cycle (fantom_mx_set_operator): fine
+        # and level-1 sweeps on the matrix cores with the node values split
+        # into hi + lo BF16 pairs (single BF16 node values stall the PCG)
+        self.bf16_operator = 1 | 2 | 4
         if smoother not in ("jacobi", "chebyshev"):
             raise ValueError("smoother must be 'jacobi' or 'chebyshev'")
         if smoother == "chebyshev" and not self.native:
@@ -229,7 +240,7 @@ class MultiGrid(_DeviceKrylov):
 
     def _free_native(self):
         if getattr(self, "_nh", None) is not None:
-            if self._nh_key is not None and self._nh_key[-1] == "mixed":
+            if self._nh_key is not None and self._nh_key[-1] in ("mixed", "bf16"):
                 _lib.load().fantom_mx_destroy(self._nh)
             else:
                 _lib.load().fantom_mg_destroy(self._nh)
@@ -256,7 +267,7 @@ class MultiGrid(_DeviceKrylov):
         k = self.kernel
         if self._nh is None or self._nh_state != (k.rho_version, k.con_version, k.rho64.data_ptr()):
             self._native_prepare()
-        pre = "fantom_mx_" if self.precision == "mixed" else "fantom_mg_"
+        pre = "fantom_mx_" if self.mixed else "fantom_mg_"
         _lib.call(pre + "precondition", self._nh, _lib.ptr(r), _lib.ptr(z))
         return z
 
@@ -274,7 +285,7 @@ class MultiGrid(_DeviceKrylov):
             return None
         n = self._n_levels - 1
         out = (ctypes.c_double * max(n, 1))()
-        pre = "fantom_mx_" if self._nh_key[-1] == "mixed" else "fantom_mg_"
+        pre = "fantom_mx_" if self._nh_key[-1] in ("mixed", "bf16") else "fantom_mg_"
         _lib.check(getattr(_lib.load(), pre + "smoother_rho")(self._nh, out, n))
         return list(out)[:n]
 
@@ -284,13 +295,13 @@ class MultiGrid(_DeviceKrylov):
         import ctypes
         k = self.kernel
         L = _lib.load()
-        mixed = self.precision == "mixed"
+        mixed = self.mixed
         if mixed and k.dim != 3:
             raise NotImplementedError("the mixed-precision solve is implemented for hex8 grids")
         gal = int(self.coarse == "galerkin")
         key = (k.grid, self.n_level, self.n_smooth, self.omega, k.kernel_kind, gal, self.omega_schedule,
                self.min_omega, self.omega_boost, self.smoother, self.cheb_rho_scale, self.cheb_power_iters,
-               self.precision)
+               self.flexible, self.bf16_operator, self.precision)
         pre = "fantom_mx_" if mixed else "fantom_mg_"
         if self._nh is None or self._nh_key != key:
             self._free_native()
@@ -312,6 +323,9 @@ class MultiGrid(_DeviceKrylov):
             self._n_levels = n_coarse + 1
             _LIVE.add(self)
             self._native_schedule(L, pre, h)
+            if mixed:
+                _lib.check(L.fantom_mx_set_operator(h, self.bf16_operator if self.precision == "bf16" else 0))
+                _lib.check(L.fantom_mx_set_flexible(h, int(self.flexible)))
         state = (k.rho_version, k.con_version, k.rho64.data_ptr())
         if self._nh_state != state:
             _lib.call(pre + "setup", self._nh, _lib.ptr(k.rho64), _lib.ptr(k.mask) if k.has_con else None)
@@ -321,7 +335,7 @@ class MultiGrid(_DeviceKrylov):
     def _native_solve(self, rhs, use_last):
         import ctypes
         k = self.kernel
-        mixed = self.precision == "mixed"
+        mixed = self.mixed
         L, pre = self._native_prepare()
         n = rhs.numel()
         use_x0 = bool(use_last and self.last_x0 is not None and self.last_x0.numel() == n)

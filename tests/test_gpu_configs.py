@@ -3,8 +3,8 @@ against the FP64 oracle through size-independent properties:
 
 * 128x64x64 and 256x128x128: the fine K.u of the seeded field against the
   oracle's element-loop matvec (core/CPU/_ops.py:330-368) to 1e-13;
-* 256x128x128 precision form: FP64 and mixed-precision GMG-PCG to 1e-8 on
-  the seeded field; the oracle's K.u of each solution gives its true
+* 256x128x128 precision form: FP64 and mixed-precision (FP32 and BF16
+  V-cycles) GMG-PCG to 1e-8 on the seeded field; the oracle's K.u of each solution gives its true
   residual and a second-order estimate of the exact compliance
   (c* = f.u + u.r + r.K^-1 r, r = f - K u), which both solves must meet to
   1e-6; u and dc differences are printed;
@@ -73,7 +73,8 @@ def _oracle_checks(oracle, nel, E, con, f, u):
 
 def test_c3_precision_form_vs_fp64_and_oracle(oracle):
     """256x128x128 cantilever, seeded field, 1e-8: FP64 and mixed-precision
-    (FP64 PCG, FP32 V-cycle) solutions against each other and the oracle."""
+    (FP64 PCG with an FP32 V-cycle, and with the BF16 matrix-core V-cycle)
+    solutions against each other and the oracle."""
     import pyfantom_amd as pf
     from pyfantom_amd.workloads import cantilever, seeded_moduli
     torch = _torch()
@@ -85,7 +86,7 @@ def test_c3_precision_form_vs_fp64_and_oracle(oracle):
     E = seeded_moduli(mesh.n_elements)
     K.set_rho(torch.from_numpy(E).cuda())
     sol = {}
-    for prec in ("fp64", "mixed"):
+    for prec in ("fp64", "mixed", "bf16"):
         # the seeded field's D^-1 K reaches a spectral radius of about 4.4, so
         # a fixed Jacobi weight above 2 / 4.4 amplifies its highest modes
         s = pf.MultiGrid(mesh, K, maxiter=500, tol=1e-8, n_smooth=2, omega=0.4, n_level=5, cycle="V",
@@ -96,24 +97,27 @@ def test_c3_precision_form_vs_fp64_and_oracle(oracle):
         sol[prec] = (u, res, s.last_iterations, dc)
         print(f"  {prec}: {s.last_iterations} PCG iterations, residual {res:.2e}", flush=True)
         del s
-    (u6, r6, i6, d6), (um, rm, im, dm) = sol["fp64"], sol["mixed"]
+    u6, r6, i6, d6 = sol["fp64"]
     tr6, cstar, el, K0 = _oracle_checks(oracle, nel, E, con, f, u6)
-    trm, _, _, _ = _oracle_checks(oracle, nel, E, con, f, um)
-    c6, cm = float(f @ u6), float(f @ um)
-    dco = oracle.process_dk(K0, el, um)
-    du = np.linalg.norm(um - u6) / np.linalg.norm(u6)
-    ddc = np.linalg.norm(dm - d6) / np.linalg.norm(d6)
-    print(f"  oracle true residual fp64 {tr6:.2e} mixed {trm:.2e}; compliance fp64 {c6!r} mixed {cm!r} "
-          f"oracle estimate {cstar!r}", flush=True)
-    print(f"  mixed vs fp64: |du|/|u| {du:.2e} |dc|/c {abs(cm - c6) / abs(c6):.2e} |d(dc)|/|dc| {ddc:.2e}; "
-          f"mixed vs oracle: |c - c*|/c* {abs(cm - cstar) / abs(cstar):.2e}", flush=True)
-    assert r6 < 1e-8 and rm < 1e-8
-    assert tr6 < 1.5e-8 and trm < 1.5e-8
+    c6 = float(f @ u6)
+    print(f"  fp64: oracle true residual {tr6:.2e}, compliance {c6!r}, oracle estimate {cstar!r}", flush=True)
+    assert r6 < 1e-8 and tr6 < 1.5e-8
     assert abs(c6 - cstar) <= 1e-6 * abs(cstar)
-    assert abs(cm - cstar) <= 1e-6 * abs(cstar)
-    assert abs(cm - c6) <= 1e-6 * abs(c6)
-    # the device sensitivity of the mixed solution is the oracle's on the same u
-    assert np.linalg.norm(dm - dco) <= 1e-12 * np.linalg.norm(dco)
+    for prec in ("mixed", "bf16"):   # FP32 and BF16 V-cycles
+        um, rm, im, dm = sol[prec]
+        trm, _, _, _ = _oracle_checks(oracle, nel, E, con, f, um)
+        cm = float(f @ um)
+        dco = oracle.process_dk(K0, el, um)
+        du = np.linalg.norm(um - u6) / np.linalg.norm(u6)
+        ddc = np.linalg.norm(dm - d6) / np.linalg.norm(d6)
+        print(f"  {prec}: oracle true residual {trm:.2e}, compliance {cm!r}; vs fp64: |du|/|u| {du:.2e} "
+              f"|dc|/c {abs(cm - c6) / abs(c6):.2e} |d(dc)|/|dc| {ddc:.2e}; vs oracle: |c - c*|/c* "
+              f"{abs(cm - cstar) / abs(cstar):.2e}", flush=True)
+        assert rm < 1e-8 and trm < 1.5e-8
+        assert abs(cm - cstar) <= 1e-6 * abs(cstar)
+        assert abs(cm - c6) <= 1e-6 * abs(c6)
+        # the device sensitivity of the solution is the oracle's on the same u
+        assert np.linalg.norm(dm - dco) <= 1e-12 * np.linalg.norm(dco)
 
 
 def _bridge_con_f(nel):
@@ -139,7 +143,7 @@ def test_c5_bridge_mma_step(oracle):
     nel = (384, 192, 192)
     con, f = _bridge_con_f(nel)
     out = {}
-    for prec in ("fp64", "mixed"):
+    for prec in ("fp64", "mixed", "bf16"):
         print(f"C5 bridge MMA step, {prec}", flush=True)
         mesh, K, solver, FE, filt, prob = bridge(nel, n_level=6, omega=0.5, omega_schedule="reference",
                                                  precision=prec)
