@@ -52,13 +52,18 @@ LEGS = {
     "mixed": dict(precision="mixed", smoother="jacobi"),
     "fp64_cheb": dict(precision="fp64", smoother="chebyshev"),
     "mixed_cheb": dict(precision="mixed", smoother="chebyshev"),
+    # FP32 V-cycle under flexible (Polak-Ribiere) PCG
+    "mixed_flex": dict(precision="mixed", smoother="jacobi", flexible=True),
+    # V-cycle fine and level-1 sweeps on the matrix cores (BF16 operands,
+    # split node values), flexible PCG
+    "bf16": dict(precision="bf16", smoother="jacobi"),
     # flexible PCG: FP32 V-cycle + the committed network's correction, FP64 GMG-PCG fallback
     "mixed_learned": dict(precision="mixed", smoother="jacobi", learned=True),
 }
 
 
 def build_problem(pf, nel, n_level, tol=1e-8, precision="fp64", kind="cantilever", vf=0.3, r=1.5, omega=0.6,
-                  omega_schedule="reference", smoother="jacobi", learned=False):
+                  omega_schedule="reference", smoother="jacobi", learned=False, flexible=None):
     """cantilever: x = 0 face clamped, total load -1 in y over the nodes of
     the bottom edge (y = 0) of the x = L face.  bridge: the four bottom
     corner nodes (y = 0, x in {0, L}, z in {0, W}) pinned in all DOFs, total
@@ -68,7 +73,7 @@ def build_problem(pf, nel, n_level, tol=1e-8, precision="fp64", kind="cantilever
     build = workloads.cantilever if kind == "cantilever" else workloads.bridge
     mesh, K, solver, FE, filt, prob = build(nel, vf=vf, r=r, maxiter=1000, tol=tol, n_smooth=2, omega=omega,
                                             n_level=n_level, cycle="V", precision=precision,
-                                            omega_schedule=omega_schedule, smoother=smoother)
+                                            omega_schedule=omega_schedule, smoother=smoother, flexible=flexible)
     if learned:
         from pyfantom_amd.learned import ConvNet, LearnedSolver
         net = ConvNet.from_file(os.path.join(ROOT, "pyfantom_amd", "data", "learned_precond_96x48x48.npz"))
@@ -169,7 +174,7 @@ def headline(legs):
             "compliance_rel_diff_vs_fp64": ok[k]["compliance_rel_diff_vs_fp64"]}
 
 
-def precision_errors(torch, pf, mesh, FE_m, FE, n_level, smoother="jacobi"):
+def precision_errors(torch, pf, mesh, FE_m, FE, n_level, smoother="jacobi", precision="mixed", flexible=None):
     """One cold solve of the seeded field (x ~ U[0.01, 1], default_rng(0),
     E = Emin + x^3) with the mixed-precision solver and with the FP64 one:
     relative errors of u (L2), compliance and the element sensitivity
@@ -180,9 +185,10 @@ def precision_errors(torch, pf, mesh, FE_m, FE, n_level, smoother="jacobi"):
     x = np.random.default_rng(0).uniform(0.01, 1.0, mesh.n_elements)
     E = torch.from_numpy(1e-9 + x ** 3 * (1 - 1e-9)).cuda()
     out = {}
-    for tag, fe, prec in (("mixed", FE_m, "mixed"), ("fp64", FE, "fp64")):
+    for tag, fe, prec in (("mixed", FE_m, precision), ("fp64", FE, "fp64")):
         s = pf.MultiGrid(mesh, fe.kernel, maxiter=1000, tol=1e-8, n_smooth=2, omega=0.4, n_level=n_level,
-                         cycle="V", precision=prec, omega_schedule="fixed", fallback=False, smoother=smoother)
+                         cycle="V", precision=prec, omega_schedule="fixed", fallback=False, smoother=smoother,
+                         flexible=flexible if prec != "fp64" else None)
         fe.kernel.set_rho(E)
         U, res = s.solve(fe.rhs, use_last=False)
         out[tag] = (U.clone(), float(torch.dot(fe.rhs, U)), fe.kernel.process_grad(U).clone(), res,
@@ -296,7 +302,7 @@ def main():
     ap.add_argument("--config", default="c3", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--ku-reps", type=int, default=100)
-    ap.add_argument("--legs", default="fp64,mixed,mixed_cheb,mixed_learned",
+    ap.add_argument("--legs", default="fp64,mixed,mixed_flex,bf16,mixed_cheb,mixed_learned",
                     help="SIMP legs to time after the FP64 baseline: " + ", ".join(sorted(LEGS)))
     ap.add_argument("--no-seeded", action="store_true", help="skip the seeded-field precision comparison")
     ap.add_argument("--progress", action="store_true", help="print compliance and PCG iterations every step")
@@ -402,9 +408,11 @@ def main():
                      "u_rel_l2_vs_fp64": rel(prob_l._U, prob._U),
                      "dc_rel_l2_vs_fp64": rel(prob_l.nabla_f(), prob.nabla_f()),
                      "fp64_fallbacks": [{"step": s, "fp64_iterations": n} for s, n in fb_l]}
-        if LEGS[leg]["precision"] == "mixed" and not LEGS[leg].get("learned") and not args.no_seeded:
+        if LEGS[leg]["precision"] in ("mixed", "bf16") and not LEGS[leg].get("learned") and not args.no_seeded:
             legs[leg]["seeded_field"] = precision_errors(torch, pf, mesh, objs_l[3], FE, n_level,
-                                                         smoother=LEGS[leg]["smoother"])
+                                                         smoother=LEGS[leg]["smoother"],
+                                                         precision=LEGS[leg]["precision"],
+                                                         flexible=LEGS[leg].get("flexible"))
         del objs_l, prob_l
         torch.cuda.empty_cache()
     ku = time_ku(torch, pf, K, mesh, args.ku_reps)
