@@ -11,7 +11,7 @@
 // state machine whose state (MmaCtl) lives in the workspace.  One unit of
 // work is two launches: mma_pass_k runs the n pass the current phase needs
 // (prep, residual, Newton sums, direction, line-search trial), and
-// mma_ctl_k adds its block partials in block order (deterministic) and takes
+// mma_ctl_k adds its block partials in a fixed order (deterministic) and takes
 // the next control step (the dual solve, step lengths, acceptance, barrier
 // updates), choosing the next phase.  The host enqueues units in batches and
 // reads one small status record per batch; the first batch is sized from the
@@ -527,10 +527,14 @@ __device__ void mma_trial_point(const MmaConst &k, MmaCtl &c)
     c.phase = kPhTrial;
 }
 
-constexpr int kCtlThreads = 128;
+constexpr int kCtlThreads = 1024;
+constexpr int kCtlWaves = kCtlThreads / 64;
 
-// reduce the record the last pass left (one thread per record entry, block
-// order) and take the next control step
+// reduce the record the last pass left and take the next control step.
+// Each record entry is reduced by one wave: lane l adds blocks l, l + 64, ...
+// in that order, then a fixed butterfly combines the lanes, so the sums are
+// the same on every run (a single thread walking the blocks serialised 1024
+// dependent loads per entry).
 __global__ __launch_bounds__(kCtlThreads) void mma_ctl_k(MmaCtl *__restrict__ ctl,
                                                          const double *__restrict__ partials, unsigned nb,
                                                          MmaConst k)
@@ -540,37 +544,44 @@ __global__ __launch_bounds__(kCtlThreads) void mma_ctl_k(MmaCtl *__restrict__ ct
     const int phase = ctl->phase;
     if (phase == kPhDone)
         return;
-    const int m = k.m, nrec = 2 * m + m * m, t = threadIdx.x;
+    const int m = k.m, nrec = 2 * m + m * m, t = threadIdx.x, lane = t & 63, wv = t >> 6;
     // entries: [0, 1 + m) residual sums, 1 + m residual max, then the Newton
     // record (newton pass: at 0; trial pass: behind the residual record)
-    auto sum = [&](long off, int rec, int e) {
-        double v = 0.0;
-        for (unsigned b = 0; b < nb; ++b)
-            v += partials[off + (long)b * rec + e];
-        return v;
-    };
-    auto mx = [&](long off) {
-        double v = -INFINITY;
-        for (unsigned b = 0; b < nb; ++b)
-            v = fmax(v, partials[off + b]);
-        return v;
-    };
-    if (phase == kPhPrep) {
-        if (t < m)
-            red[t] = sum(0, m, t);
-    } else if (phase == kPhNewton) {
-        if (t < nrec)
-            red[2 + m + t] = sum(0, nrec, t);
-    } else if (phase == kPhDir) {
-        if (t == 0)
-            red[0] = mx(0);
-    } else {   // residual or trial
-        if (t < 1 + m)
-            red[t] = sum(0, 1 + m, t);
-        else if (t == 1 + m)
-            red[t] = mx((long)nb * (1 + m));
-        else if (phase == kPhTrial && t < 2 + m + nrec)
-            red[t] = sum((long)nb * (2 + m), nrec, t - 2 - m);
+    const int ne = phase == kPhPrep ? m : phase == kPhNewton ? nrec : phase == kPhDir ? 1
+                 : phase == kPhResid ? 2 + m : 2 + m + nrec;
+    for (int e = wv; e < ne; e += kCtlWaves) {
+        long off;
+        int rec, idx, slot;
+        bool is_max = false;
+        if (phase == kPhPrep) {
+            off = 0, rec = m, idx = e, slot = e;
+        } else if (phase == kPhNewton) {
+            off = 0, rec = nrec, idx = e, slot = 2 + m + e;
+        } else if (phase == kPhDir) {
+            off = 0, rec = 1, idx = 0, slot = 0, is_max = true;
+        } else if (e < 1 + m) {
+            off = 0, rec = 1 + m, idx = e, slot = e;
+        } else if (e == 1 + m) {
+            off = (long)nb * (1 + m), rec = 1, idx = 0, slot = e, is_max = true;
+        } else {
+            off = (long)nb * (2 + m), rec = nrec, idx = e - 2 - m, slot = e;
+        }
+        double v = is_max ? -INFINITY : 0.0;
+#pragma unroll
+        for (int q = 0; q < kMmaMaxBlocks / 64; ++q) {
+            const unsigned b = (unsigned)(lane + 64 * q);
+            if (b < nb) {
+                const double x = partials[off + (long)b * rec + idx];
+                v = is_max ? fmax(v, x) : v + x;
+            }
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const double w = __shfl_xor(v, o, 64);
+            v = is_max ? fmax(v, w) : v + w;
+        }
+        if (lane == 0)
+            red[slot] = v;
     }
     __syncthreads();
     if (t != 0)
