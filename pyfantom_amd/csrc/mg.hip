@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <mutex>
+#include <vector>
 
 #include "common.hpp"
 #include "fantom_internal.hpp"
@@ -793,6 +794,35 @@ __global__ __launch_bounds__(256) void gj_panel_x(int n, int k0, int b, const do
     }
 }
 
+// gj_panel_x, gj_zero_k and the copy of the next pivot block (rows and
+// columns K1, b1 wide, into T with row stride kGJB) in one pass.  Thread q
+// reads entry (q / b, k0 + q % b) of the column block for Xhat before it
+// zeroes that same entry, and the K1 block lies outside block row and column
+// K, so the three parts do not race.
+__global__ __launch_bounds__(256) void gj_panel_zero_copy(int n, int k0, int b, int k1, int b1,
+                                                         double *__restrict__ A, const double *__restrict__ P,
+                                                         double *__restrict__ X, double *__restrict__ R,
+                                                         double *__restrict__ T)
+{
+    const long q = (long)blockIdx.x * 256 + threadIdx.x;
+    if (q < (long)n * b) {
+        const int i = (int)(q / b), c = (int)(q % b);
+        const bool inK = i >= k0 && i < k0 + b;
+        const long ac = (long)i * n + k0 + c;
+        X[q] = inK ? (i - k0 == c ? -1.0 : 0.0) : A[ac];
+        A[ac] = 0.0;
+        A[(long)k0 * n + q] = 0.0;
+    }
+    if (q < (long)b * b) {
+        const int t = (int)(q / b), c = (int)(q % b);
+        R[(long)t * n + k0 + c] = P[t * kGJB + c];
+    }
+    if (q < (long)b1 * b1) {
+        const int r = (int)(q / b1), c = (int)(q % b1);
+        T[r * kGJB + c] = A[(long)(k1 + r) * n + k1 + c];
+    }
+}
+
 // Ahat: block row K and block column K of A set to zero
 __global__ __launch_bounds__(256) void gj_zero_k(int n, int k0, int b, double *__restrict__ A)
 {
@@ -805,6 +835,118 @@ __global__ __launch_bounds__(256) void gj_zero_k(int n, int k0, int b, double *_
 
 static std::mutex g_blas_mu;
 static rocblas_handle g_blas[64];
+
+// Look-ahead streams of the blocked inverse, per device: the pivot-block
+// inversions run on CUs of their own (a one-workgroup, latency-bound kernel
+// that slows to twice its time when it shares a CU with the GEMM's
+// workgroups), the GEMMs and the small kernels on all other CUs.  Default
+// priority, like every stream here.
+struct GjStreams {
+    bool tried = false, ok = false;
+    hipStream_t gemm = nullptr, piv = nullptr;
+    hipEvent_t fork = nullptr, block = nullptr, pivot = nullptr, done = nullptr;
+};
+static GjStreams g_gj[64];
+
+static bool gj_streams(int dev, GjStreams &gs)
+{
+    if (gs.tried)
+        return gs.ok;
+    gs.tried = true;
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 2)
+        return false;
+    // logical CUs 0-7 are one CU on each of the 8 XCDs (a queue needs a CU
+    // on every XCD, which dispatches round-robin); the pivot stream gets
+    // those, the GEMM stream the rest
+    constexpr int kPivCUs = 8;
+    if (ncu <= 2 * kPivCUs)
+        return false;
+    const int words = (ncu + 31) / 32;
+    std::vector<uint32_t> rest(words, 0u), piv(words, 0u);
+    for (int c = 0; c < ncu; ++c)
+        (c < kPivCUs ? piv : rest)[c / 32] |= 1u << (c % 32);
+    gs.ok = hipExtStreamCreateWithCUMask(&gs.gemm, (uint32_t)words, rest.data()) == hipSuccess &&
+            hipExtStreamCreateWithCUMask(&gs.piv, (uint32_t)words, piv.data()) == hipSuccess &&
+            hipEventCreateWithFlags(&gs.fork, hipEventDisableTiming) == hipSuccess &&
+            hipEventCreateWithFlags(&gs.block, hipEventDisableTiming) == hipSuccess &&
+            hipEventCreateWithFlags(&gs.pivot, hipEventDisableTiming) == hipSuccess &&
+            hipEventCreateWithFlags(&gs.done, hipEventDisableTiming) == hipSuccess;
+    (void)hipGetLastError();   // a refused CU mask leaves the single-stream path
+    return gs.ok;
+}
+
+#define GJ_HIP(expr, what)                                                                                 \
+    do {                                                                                                  \
+        const hipError_t _e = (expr);                                                                     \
+        if (_e != hipSuccess) {                                                                           \
+            set_error("spd_inverse: %s: %s", what, hipGetErrorString(_e));                                \
+            return kStatusHipError;                                                                       \
+        }                                                                                                 \
+    } while (0)
+#define GJ_BLAS(expr, what)                                                                                \
+    do {                                                                                                  \
+        if ((expr) != rocblas_status_success) {                                                           \
+            set_error("spd_inverse: %s failed", what);                                                    \
+            return kStatusHipError;                                                                       \
+        }                                                                                                 \
+    } while (0)
+
+// The blocked inverse with look-ahead.  The next pivot inversion reads only
+// the next pivot block, and A keeps no other use of that block (the next
+// step replaces it by its inverse), so each step copies that block to T
+// before the full rank-b update, and the pivot stream forms its update (a
+// b x b x b product) and inverts it while the full update runs on the other
+// CUs.
+static int spd_inverse_lookahead(int n, double *A, double *work, hipStream_t s, rocblas_handle hb,
+                                 const GjStreams &gs)
+{
+    double *X = work, *R = work + (size_t)n * kGJB, *P = R + (size_t)n * kGJB, *T = P + (size_t)kGJB * kGJB;
+    const double one = 1.0, minus_one = -1.0, zero = 0.0;
+    hipStream_t sg = gs.gemm, sp = gs.piv;
+    GJ_BLAS(rocblas_set_stream(hb, sg), "rocblas stream");
+    GJ_BLAS(rocblas_set_pointer_mode(hb, rocblas_pointer_mode_host), "rocblas pointer mode");
+    GJ_HIP(hipEventRecord(gs.fork, s), "fork");
+    GJ_HIP(hipStreamWaitEvent(sg, gs.fork, 0), "fork");
+    GJ_HIP(hipStreamWaitEvent(sp, gs.fork, 0), "fork");
+    hipLaunchKernelGGL(gj_pivot128, dim3(1), dim3(kPivThreads), 0, sp, n, 0, std::min(kGJB, n), A, P);
+    FANTOM_LAUNCH_CHECK("spd_inverse pivot");
+    GJ_HIP(hipEventRecord(gs.pivot, sp), "pivot event");
+    for (int k0 = 0; k0 < n; k0 += kGJB) {
+        const int b = std::min(kGJB, n - k0);
+        const long nb = (long)n * b;
+        GJ_HIP(hipStreamWaitEvent(sg, gs.pivot, 0), "wait for the pivot");
+        // Rhat = P A_{K,:} (see spd_inverse_blocked for the column-major mapping)
+        GJ_BLAS(rocblas_dgemm(hb, rocblas_operation_none, rocblas_operation_none, n, b, b, &one, A + (size_t)k0 * n, n,
+                              P, kGJB, &zero, R, n),
+                "rocblas_dgemm (panel)");
+        const int k1 = k0 + b, b1 = std::max(0, std::min(kGJB, n - k1));
+        hipLaunchKernelGGL(gj_panel_zero_copy, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, sg, n, k0, b, k1, b1,
+                           A, P, X, R, T);
+        FANTOM_LAUNCH_CHECK("spd_inverse block step");
+        if (b1 > 0) {
+            // on the pivot stream: T -= Xhat_{K1,:} Rhat_{:,K1} (T held
+            // A_{K1,K1}, copied before the full update below changes it), then
+            // the inversion of T
+            GJ_HIP(hipEventRecord(gs.block, sg), "block event");
+            GJ_HIP(hipStreamWaitEvent(sp, gs.block, 0), "wait for the block");
+            GJ_BLAS(rocblas_set_stream(hb, sp), "rocblas stream");
+            GJ_BLAS(rocblas_dgemm(hb, rocblas_operation_none, rocblas_operation_none, b1, b1, b, &minus_one, R + k1, n,
+                                  X + (size_t)k1 * b, b, &one, T, kGJB),
+                    "rocblas_dgemm (pivot block)");
+            GJ_BLAS(rocblas_set_stream(hb, sg), "rocblas stream");
+            hipLaunchKernelGGL(gj_pivot128, dim3(1), dim3(kPivThreads), 0, sp, kGJB, 0, b1, T, P);
+            FANTOM_LAUNCH_CHECK("spd_inverse pivot");
+            GJ_HIP(hipEventRecord(gs.pivot, sp), "pivot event");
+        }
+        GJ_BLAS(rocblas_dgemm(hb, rocblas_operation_none, rocblas_operation_none, n, n, b, &minus_one, R, n, X, b, &one,
+                              A, n),
+                "rocblas_dgemm");
+    }
+    GJ_HIP(hipEventRecord(gs.done, sg), "join");
+    GJ_HIP(hipStreamWaitEvent(s, gs.done, 0), "join");
+    return kStatusOk;
+}
 
 static int spd_inverse_blocked(int n, double *A, double *work, hipStream_t s)
 {
@@ -820,6 +962,10 @@ static int spd_inverse_blocked(int n, double *A, double *work, hipStream_t s)
         return kStatusHipError;
     }
     rocblas_handle hb = g_blas[dev];
+    GjStreams &gs = g_gj[dev];
+    // the look-ahead pays once a rank-128 update outlasts a pivot inversion
+    if (n > 16 * kGJB && gj_streams(dev, gs))
+        return spd_inverse_lookahead(n, A, work, s, hb, gs);
     if (rocblas_set_stream(hb, s) != rocblas_status_success ||
         rocblas_set_pointer_mode(hb, rocblas_pointer_mode_host) != rocblas_status_success) {
         set_error("spd_inverse: rocblas stream / pointer mode");
@@ -857,7 +1003,7 @@ static int spd_inverse_blocked(int n, double *A, double *work, hipStream_t s)
 
 int launch_spd_inverse(int n, double *A, double *work, hipStream_t s)
 {
-    // work: n*n doubles + kGJ*kGJ; the blocked path uses 2 n kGJB + kGJB^2 of it
+    // work: n*n doubles + kGJ*kGJ; the blocked path uses 2 n kGJB + 2 kGJB^2 of it
     if (n > 4 * kGJB)
         return spd_inverse_blocked(n, A, work, s);
     const unsigned nt = (unsigned)((n + kGJ - 1) / kGJ);
