@@ -373,8 +373,11 @@ static int mx_ku(MxHandle *h, int l, int mode, const float *u, float *y, const f
                               mode == MODE_JAC ? nullptr : dinv, om, partials, h->stream);
 }
 
-// n_smooth sweeps on x (*px); final (optional) receives the last sweep
-static int mx_smooth(MxHandle *h, int l, const float *b, float **px, bool from_zero, float *final = nullptr)
+// n_smooth sweeps on x (*px); final (optional) receives the last sweep.
+// rz (optional, fine level): the last sweep also leaves the partial sums of
+// b.x (the PCG's r.z, b being the FP32 residual) and *rz is set.
+static int mx_smooth(MxHandle *h, int l, const float *b, float **px, bool from_zero, float *final = nullptr,
+                     bool *rz = nullptr)
 {
     MxLevel &L = h->lev[l];
     float *x = *px;
@@ -395,8 +398,14 @@ static int mx_smooth(MxHandle *h, int l, const float *b, float **px, bool from_z
                                (float)w(0), L.g.omega_scale, L.dinv, b, x);
             continue;
         }
-        float *dst = (final && s == h->n_smooth - 1) ? final : other;
-        MX_TRY(mx_ku(h, l, MODE_JAC, x, dst, b, L.dinv, nullptr, w(s)));
+        const bool last = final && s == h->n_smooth - 1;
+        float *dst = last ? final : other;
+        if (last && rz && l == 0 && !(h->op_kind & 1)) {
+            MX_TRY(mx_ku(h, l, MODE_JACDOT, x, dst, b, L.dinv, h->partials, w(s)));
+            *rz = true;
+        } else {
+            MX_TRY(mx_ku(h, l, MODE_JAC, x, dst, b, L.dinv, nullptr, w(s)));
+        }
         other = x;
         x = dst;
     }
@@ -405,7 +414,7 @@ static int mx_smooth(MxHandle *h, int l, const float *b, float **px, bool from_z
     return kStatusOk;
 }
 
-static int mx_cycle(MxHandle *h, int l, const float *b, float **out, float *final = nullptr)
+static int mx_cycle(MxHandle *h, int l, const float *b, float **out, float *final = nullptr, bool *rz = nullptr)
 {
     MxLevel &L = h->lev[l];
     if (l == h->nlev - 1) {
@@ -422,15 +431,15 @@ static int mx_cycle(MxHandle *h, int l, const float *b, float **out, float *fina
     float *e = nullptr;
     MX_TRY(mx_cycle(h, l + 1, C.b, &e));
     MX_TRY(launch_prolong_f32(L.g, 3, 3, 0, L.mask, e, x, 1, h->stream));
-    MX_TRY(mx_smooth(h, l, b, &x, false, final));
+    MX_TRY(mx_smooth(h, l, b, &x, false, final, rz));
     *out = x;
     return kStatusOk;
 }
 
-static int mx_precond(MxHandle *h, const float *r, float *z)
+static int mx_precond(MxHandle *h, const float *r, float *z, bool *rz = nullptr)
 {
     float *out = nullptr;
-    MX_TRY(mx_cycle(h, 0, r, &out, z));
+    MX_TRY(mx_cycle(h, 0, r, &out, z, rz));
     if (out != z)   // no post-smoothing sweep
         hipLaunchKernelGGL(mx_copy_f32, dim3(vec_grid(h->n)), dim3(256), 0, h->stream, h->n, out, z);
     FANTOM_LAUNCH_CHECK("mx precond");
@@ -464,13 +473,20 @@ static int mxd_iteration(MxHandle *h)
     MX_TRY(launch_finalize(h->partials, ku_tile_partials(h->g), h->scal, FIN_ALPHA, 0, s));
     hipLaunchKernelGGL(mxd_r_k, dim3(G), dim3(256), 0, s, n, h->r64, h->r, h->q64, h->scal, h->partials);
     hipLaunchKernelGGL(mxd_conv_k, dim3(1), dim3(1024), 0, s, h->partials, (long)G, h->scal, h->sched);
-    MX_TRY(mx_precond(h, h->r, h->z));
     float *zp = h->flexible ? h->zprev : nullptr;
-    hipLaunchKernelGGL(mxd_dot_k, dim3(G), dim3(256), 0, s, n, h->r64, h->z, zp, h->partials);
-    if (zp)
-        hipLaunchKernelGGL(mxd_beta_pr_k, dim3(1), dim3(1024), 0, s, h->partials, (long)G, h->scal);
-    else
-        MX_TRY(launch_finalize(h->partials, (long)G, h->scal, FIN_BETA, 0, s));
+    // standard PCG: the last fine sweep of the V-cycle forms r.z (with r
+    // rounded to FP32, a relative change of about 1e-7 in beta)
+    bool rz = false;
+    MX_TRY(mx_precond(h, h->r, h->z, zp ? nullptr : &rz));
+    if (rz) {
+        MX_TRY(launch_finalize(h->partials, ku_tile_partials_f32(h->lev[0].g), h->scal, FIN_BETA, 0, s));
+    } else {
+        hipLaunchKernelGGL(mxd_dot_k, dim3(G), dim3(256), 0, s, n, h->r64, h->z, zp, h->partials);
+        if (zp)
+            hipLaunchKernelGGL(mxd_beta_pr_k, dim3(1), dim3(1024), 0, s, h->partials, (long)G, h->scal);
+        else
+            MX_TRY(launch_finalize(h->partials, (long)G, h->scal, FIN_BETA, 0, s));
+    }
     hipLaunchKernelGGL(mxd_px_k, dim3(G), dim3(256), 0, s, n, h->x64, h->p64, h->z, h->scal, zp);
     FANTOM_LAUNCH_CHECK("mxd iteration");
     return kStatusOk;
