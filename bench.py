@@ -163,6 +163,68 @@ def time_construct(torch, K, mesh, reps):
             "achieved_gbs": alg / t / 1e9, "hbm_fraction": alg / t / 8e12, "kernel": "csr_fill_k<3>"}
 
 
+def time_unstructured(torch, pf, reps=50, shape=(48, 32, 64), cpu=True):
+    """Unstructured K.u (UniformStiffnessKernel, fantom_ku_general_f64) on a
+    jittered tetrahedral mesh of the box [0,2] x [0,1] x [0,1], shape cells cut
+    into 6 tetrahedra each (default 589,824 tet4, 12 x 12 element matrices),
+    x = 0 face clamped, moduli U[0.1, 1.1] and u ~ N(0, 1) from
+    default_rng(0).  HIP events on the stream the kernels run on; the element
+    matrices (680 MB) exceed the 256 MB Infinity Cache, so every product reads
+    them from HBM.  Algorithmic bytes per product: the element matrices once
+    (8 m^2 per element), the moduli and connectivity (8 + 4 npe per element),
+    u read and y written (16 per DOF).  The per-element products written and
+    re-read by the gather (16 m per element) are implementation traffic, not
+    counted.  CPU: the oracle's node-basis product on the same mesh, on the
+    host's OpenMP threads."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from unstructured_meshes import tet_mesh
+    X, el = tet_mesh(*shape, jitter=0.1, seed=2)
+    mesh = pf.GeneralMesh(X, el)
+    K = pf.UniformStiffnessKernel(mesh)
+    K.set_constraints(np.concatenate([3 * np.flatnonzero(mesh.nodes[:, 0] < 1e-9) + d for d in range(3)]))
+    rng = np.random.default_rng(0)
+    E_h = rng.uniform(0.1, 1.1, mesh.n_elements)
+    u_h = rng.standard_normal(K.shape[0])
+    K.set_rho(torch.from_numpy(E_h).cuda())
+    u = torch.from_numpy(u_h).cuda()
+    for _ in range(5):
+        K.dot(u)
+    stream = torch.cuda.current_stream()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    for _ in range(reps):
+        K.dot(u)
+    ev1.record(stream)
+    ev1.synchronize()
+    t = ev0.elapsed_time(ev1) / 1e3 / reps
+    m = K.groups[0]["m"]
+    n_el, n_dof = mesh.n_elements, K.shape[0]
+    alg = n_el * (8.0 * m * m + 8.0 + 4.0 * m // 3) + 16.0 * n_dof
+    out = {"workload": f"tet4_{shape[0]}x{shape[1]}x{shape[2]}x6", "elements": n_el, "dofs": n_dof,
+           "seconds": t, "gdof_s": n_dof / t / 1e9, "algorithmic_bytes": alg,
+           "roofline": {"bound": "hbm", "achieved": alg / t / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": alg / t / 1e9 / HBM_PEAK_GBS},
+           "kernels": ["kg_elem_k<4, 3, 1>", "kg_gather_k<0>"]}
+    if cpu:
+        from oracle import oracle as O
+        O.build()
+        con = K.constraints.cpu().numpy()
+        args = (mesh.K_flat, mesh.K_ptr, mesh.elements_flat, mesh.elements_ptr, 3)
+        O.matvec_flat(*args, E_h, u_h, con)
+        n_cpu = 5
+        t0 = time.perf_counter()
+        for _ in range(n_cpu):
+            y_cpu = O.matvec_flat(*args, E_h, u_h, con)
+        tc = (time.perf_counter() - t0) / n_cpu
+        y_gpu = K.dot(u).cpu().numpy()
+        out["cpu_baseline"] = {"value": n_dof / tc / 1e9, "unit": "GDOF/s", "seconds": tc,
+                               "cores": int(O.lib().or_num_threads()), "kind": "port",
+                               "sample": f"{n_cpu} products on the same mesh"}
+        out["max_abs_diff_vs_oracle"] = float(np.abs(y_gpu - y_cpu).max())
+        out["max_abs_y"] = float(np.abs(y_cpu).max())
+    return out
+
+
 def headline(legs):
     """The fastest leg that reached the FP64 compliance to 1e-6: its time
     per SIMP iteration and its ratio to the plain FP64 GMG-PCG."""
@@ -313,6 +375,9 @@ def main():
                     help="only the isolated K.u of the workload and a 256 MiB calibration copy (for counter runs)")
     ap.add_argument("--construct-only", action="store_true",
                     help="only the explicit CSR assembly K(E) @ I of the workload (construct())")
+    ap.add_argument("--unstructured-only", action="store_true",
+                    help="only the unstructured tet4 K.u (UniformStiffnessKernel) and its CPU oracle")
+    ap.add_argument("--no-unstructured", action="store_true", help="skip the unstructured K.u measurement")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: there is no multi-GPU path (the sharding is a design, not code)",
@@ -334,6 +399,18 @@ def main():
         clamp = (k + (nel[2] + 1) * (nel[0] + 1) * j).ravel()
         K.set_constraints(np.concatenate([3 * clamp + d for d in range(3)]))
         print(json.dumps(time_construct(torch, K, mesh, args.ku_reps)), flush=True)
+        return
+
+    if args.unstructured_only:
+        res = time_unstructured(torch, pf, args.ku_reps, cpu=not args.no_cpu)
+        # one device copy of a known size: calibrates FETCH_SIZE / WRITE_SIZE in counter runs
+        src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        res["calibration_copy_bytes_each_way"] = src.numel() * 8
+        print(json.dumps(res), flush=True)
         return
 
     if args.ku_only:
@@ -447,6 +524,11 @@ def main():
                      "kernel": ku["kernel"], "algorithmic_bytes": ku["bytes"],
                      "traffic_source": "profiles/ku_tile_f64_traffic.md (FETCH_SIZE/WRITE_SIZE, copy-calibrated)"},
     }
+    if not args.no_unstructured:
+        torch.cuda.empty_cache()
+        out["unstructured_ku"] = time_unstructured(torch, pf, cpu=not args.no_cpu)
+        print(f"# unstructured K.u: {out['unstructured_ku']['seconds']*1e6:.1f} us, "
+              f"{out['unstructured_ku']['roofline']['achieved']:.0f} GB/s", file=sys.stderr, flush=True)
     if not args.no_cpu and cfg.get("kind", "cantilever") == "cantilever":
         print("# cpu baseline ...", file=sys.stderr, flush=True)
         cb = cpu_baseline()

@@ -476,6 +476,47 @@ int fantom_simp_f64(long n, const double *rho, int law, double p, double lo, dou
                     double beta, double eta, double *E, double *dE, void *stream);
 int fantom_mul_f64(long n, const double *a, const double *b, double *out, void *stream);
 
+/* ---- Unstructured meshes: node-basis operators ----------------------------
+ * Replace the "full" and "flat" node-basis kernels of UniformStiffnessKernel /
+ * GeneralStiffnessKernel (core/CUDA/Kernels/matvec.cu:145-300 with their
+ * _wcon variants, diag.cu:98-147, grad.cu:18-52; CPU forms
+ * core/CPU/_ops.py:16-330; classes stiffness/CUDA/_FEA.py:675-1630).
+ * Elements are grouped by node count (at most 8 groups); a group's arrays are
+ * on the device, structure-of-arrays so that consecutive threads read
+ * consecutive addresses: node j of element l at nodes[j n_el + l], entry
+ * (i, k) of its m x m matrix (m = npe dof, DOF order (node, component)) at
+ * K[(i m + k) n_el + l]; el_ids[l] is the element's global id (the index into
+ * E and dk).  The node-sorted incidence list gives node n's pairs for j in
+ * [node_ptr[n], node_ptr[n+1]): inc_l[j] the element's index in its group,
+ * inc_ga[j] = group * 256 + local node index; each row is summed in that
+ * order (deterministic, no atomics).  con (n_nodes dof bytes, may be NULL):
+ * Dirichlet DOFs keep identity rows and are skipped as columns.
+ * ku: mode 0 y = K(E) u, 2 y = b - K(E) u, 3 y = K(E) u and the per-block
+ * partial sums of u.y in partials[0:ku_general_partials]; Y is a device
+ * workspace of ku_general_workspace doubles (per-element products E_e K_e u_e,
+ * laid out like K).
+ * diag: diag K(E), 1 on Dirichlet DOFs.  dc: dk[el_ids[l]] = -u_e^T K_e u_e.
+ * Limits: 1 <= dof <= 3, npe dof <= 96. */
+typedef struct fantom_elem_group {
+    int n_el;          /* elements in the group */
+    int npe;           /* nodes per element */
+    const int *el_ids; /* [n_el] global element ids */
+    const int *nodes;  /* [npe][n_el] node ids */
+    const double *K;   /* [m m][n_el] element matrices, m = npe dof */
+} fantom_elem_group;
+
+long fantom_ku_general_partials(int n_nodes, int dof);
+long fantom_ku_general_workspace(int dof, int n_groups, const fantom_elem_group *groups);
+int fantom_ku_general_f64(int mode, int n_nodes, int dof, int n_groups, const fantom_elem_group *groups,
+                          const int *node_ptr, const int *inc_l, const int *inc_ga, const double *E,
+                          const uint8_t *con, const double *u, double *y, const double *b, double *Y,
+                          double *partials, void *stream);
+int fantom_diag_general_f64(int n_nodes, int dof, int n_groups, const fantom_elem_group *groups,
+                            const int *node_ptr, const int *inc_l, const int *inc_ga, const double *E,
+                            const uint8_t *con, double *diag, void *stream);
+int fantom_dc_general_f64(int dof, int n_groups, const fantom_elem_group *groups, const double *u, double *dk,
+                          void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -197,6 +197,129 @@ void or_process_dk(const double *K0, const int32_t *elements, long n_el, int npe
 }
 
 /* ------------------------------------------------------------------------ */
+/* Unstructured ("flat") forms: per-element matrices of any size.            */
+/* Element e: nodes elements_flat[eptr[e] .. eptr[e+1]), matrix              */
+/* K_flat[K_ptr[e] ..] of m x m, m = size dof.  Node incidence grouped by    */
+/* node, element ids ascending (stiffness/CPU/_FEA.py GeneralStiffnessKernel */
+/* sorter / node_ids).                                                       */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    int32_t *start; /* n_nodes + 1 */
+    int32_t *slot;  /* flat position f in elements_flat, grouped by node */
+    int32_t *el;    /* element of flat position f */
+} or_flat_incidence;
+
+static or_flat_incidence or_make_flat_incidence(const int32_t *elements_flat, const int32_t *eptr, long n_el,
+                                                int n_nodes)
+{
+    or_flat_incidence inc;
+    const long nf = eptr[n_el];
+    inc.start = (int32_t *)calloc((size_t)n_nodes + 1, sizeof(int32_t));
+    inc.slot = (int32_t *)malloc(sizeof(int32_t) * (size_t)(nf > 0 ? nf : 1));
+    inc.el = (int32_t *)malloc(sizeof(int32_t) * (size_t)(nf > 0 ? nf : 1));
+    for (long e = 0; e < n_el; ++e)
+        for (long f = eptr[e]; f < eptr[e + 1]; ++f)
+            inc.el[f] = (int32_t)e;
+    for (long f = 0; f < nf; ++f)
+        inc.start[elements_flat[f] + 1]++;
+    for (int n = 0; n < n_nodes; ++n)
+        inc.start[n + 1] += inc.start[n];
+    int32_t *fill = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n_nodes > 0 ? n_nodes : 1));
+    memcpy(fill, inc.start, sizeof(int32_t) * (size_t)n_nodes);
+    for (long f = 0; f < nf; ++f)
+        inc.slot[fill[elements_flat[f]]++] = (int32_t)f;
+    free(fill);
+    return inc;
+}
+
+static void or_free_flat_incidence(or_flat_incidence *inc)
+{
+    free(inc->start);
+    free(inc->slot);
+    free(inc->el);
+}
+
+/* K(E)u, core/CPU/_ops.py:247-286 (mat_vec_node_basis_parallel_flat_wcon)  */
+/* and :145-176 (con == NULL).  Constrained row: out_i = u_i; constrained    */
+/* column: skipped.                                                          */
+void or_matvec_flat(const double *K_flat, const int64_t *K_ptr, const int32_t *elements_flat, const int32_t *eptr,
+                    long n_el, int dof, int n_nodes, const double *E, const double *u, const uint8_t *con,
+                    double *out)
+{
+    or_flat_incidence inc = or_make_flat_incidence(elements_flat, eptr, n_el, n_nodes);
+#pragma omp parallel for schedule(static)
+    for (int n = 0; n < n_nodes; ++n) {
+        for (int c = 0; c < dof; ++c) {
+            const long r = (long)n * dof + c;
+            out[r] = (con && con[r]) ? u[r] : 0.0;
+        }
+        for (int s = inc.start[n]; s < inc.start[n + 1]; ++s) {
+            const int f = inc.slot[s], e = inc.el[f];
+            const int a = f - eptr[e], size = eptr[e + 1] - eptr[e], nl = size * dof;
+            const int32_t *em = elements_flat + eptr[e];
+            for (int c = 0; c < dof; ++c) {
+                const long r = (long)n * dof + c;
+                if (con && con[r])
+                    continue;
+                double val = 0.0;
+                for (int l = 0; l < nl; ++l) {
+                    const long kk = (long)em[l / dof] * dof + l % dof;
+                    if (con && con[kk])
+                        continue;
+                    val += K_flat[K_ptr[e] + (long)(a * dof + c) * nl + l] * u[kk];
+                }
+                out[r] += val * E[e];
+            }
+        }
+    }
+    or_free_flat_incidence(&inc);
+}
+
+/* diag(K(E)); constrained DOF -> 1.  core/CPU/_ops.py:110-143. */
+void or_diag_flat(const double *K_flat, const int64_t *K_ptr, const int32_t *elements_flat, const int32_t *eptr,
+                  long n_el, int dof, int n_nodes, const double *E, const uint8_t *con, double *out)
+{
+    or_flat_incidence inc = or_make_flat_incidence(elements_flat, eptr, n_el, n_nodes);
+#pragma omp parallel for schedule(static)
+    for (int n = 0; n < n_nodes; ++n) {
+        for (int c = 0; c < dof; ++c) {
+            const long r = (long)n * dof + c;
+            out[r] = (con && con[r]) ? 1.0 : 0.0;
+        }
+        for (int s = inc.start[n]; s < inc.start[n + 1]; ++s) {
+            const int f = inc.slot[s], e = inc.el[f];
+            const int a = f - eptr[e], nl = (eptr[e + 1] - eptr[e]) * dof;
+            for (int c = 0; c < dof; ++c) {
+                const long r = (long)n * dof + c;
+                if (con && con[r])
+                    continue;
+                out[r] += K_flat[K_ptr[e] + (long)(a * dof + c) * nl + a * dof + c] * E[e];
+            }
+        }
+    }
+    or_free_flat_incidence(&inc);
+}
+
+/* dk_e = -u_e^T K_e u_e.  core/CPU/_ops.py:25-43 (process_dk_flat). */
+void or_process_dk_flat(const double *K_flat, const int64_t *K_ptr, const int32_t *elements_flat,
+                        const int32_t *eptr, long n_el, int dof, const double *u, double *out)
+{
+#pragma omp parallel for schedule(static)
+    for (long e = 0; e < n_el; ++e) {
+        const int nl = (eptr[e + 1] - eptr[e]) * dof;
+        const int32_t *em = elements_flat + eptr[e];
+        double acc = 0.0;
+        for (int i = 0; i < nl; ++i) {
+            double t = 0.0;
+            for (int l = 0; l < nl; ++l)
+                t += K_flat[K_ptr[e] + (long)i * nl + l] * u[(long)em[l / dof] * dof + l % dof];
+            acc += t * u[(long)em[i / dof] * dof + i % dof];
+        }
+        out[e] = -acc;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
 /* A7: restriction / prolongation.  core/CPU/_mgm.py:10-84 (3D restriction), */
 /* :86-148 (2D), :159-238 (3D prolongation), :240-297 (2D).                  */
 /* normalize = 1 reproduces the reference (division by the boundary-         */

@@ -59,6 +59,10 @@ def lib():
         L.or_matvec.argtypes = [_D, _I, c_long, c_int, c_int, c_int, _D, _D, _U8, _D]
         L.or_diag.argtypes = [_D, _I, c_long, c_int, c_int, c_int, _D, _U8, _D]
         L.or_process_dk.argtypes = [_D, _I, c_long, c_int, c_int, _D, _D]
+        _L64 = ctypes.POINTER(ctypes.c_int64)
+        L.or_matvec_flat.argtypes = [_D, _L64, _I, _I, c_long, c_int, c_int, _D, _D, _U8, _D]
+        L.or_diag_flat.argtypes = [_D, _L64, _I, _I, c_long, c_int, c_int, _D, _U8, _D]
+        L.or_process_dk_flat.argtypes = [_D, _L64, _I, _I, c_long, c_int, _D, _D]
         L.or_restrict_3d.argtypes = [c_int, c_int, c_int, c_int, c_int, _D, _D]
         L.or_restrict_2d.argtypes = [c_int, c_int, c_int, c_int, _D, _D]
         L.or_prolong_3d.argtypes = [c_int, c_int, c_int, c_int, c_int, _D, _D]
@@ -319,6 +323,72 @@ def process_dk(K0, elems, u):
     out = np.empty(len(elems))
     lib().or_process_dk(_p(K0, _D), _p(elems, _I), len(elems), npe, dof, _p(_f64(u), _D), _p(out, _D))
     return out
+
+
+# --------------------------------------------------------------------------
+# Unstructured ("flat") meshes: per-element matrices of varying size
+# --------------------------------------------------------------------------
+def _flat_args(K_flat, K_ptr, elements_flat, elements_ptr):
+    return (_f64(np.asarray(K_flat).reshape(-1)), np.ascontiguousarray(K_ptr, np.int64),
+            np.ascontiguousarray(elements_flat, np.int32), np.ascontiguousarray(elements_ptr, np.int32))
+
+
+def matvec_flat(K_flat, K_ptr, elements_flat, elements_ptr, dof, E, u, con=None):
+    """Node-basis K(E)u for mixed element sizes (core/CPU/_ops.py:145-176,
+    :247-286)."""
+    Kf, Kp, ef, ep = _flat_args(K_flat, K_ptr, elements_flat, elements_ptr)
+    u = _f64(u)
+    m = _mask(con, u.shape[0])
+    out = np.empty_like(u)
+    _L64 = ctypes.POINTER(ctypes.c_int64)
+    lib().or_matvec_flat(_p(Kf, _D), _p(Kp, _L64), _p(ef, _I), _p(ep, _I), len(ep) - 1, dof, u.shape[0] // dof,
+                         _p(_f64(E), _D), _p(u, _D), _p(m, _U8) if m is not None else None, _p(out, _D))
+    return out
+
+
+def diag_flat(K_flat, K_ptr, elements_flat, elements_ptr, dof, E, n_dof, con=None):
+    """diag K(E), constrained DOFs 1 (core/CPU/_ops.py:110-143)."""
+    Kf, Kp, ef, ep = _flat_args(K_flat, K_ptr, elements_flat, elements_ptr)
+    m = _mask(con, n_dof)
+    out = np.empty(n_dof)
+    _L64 = ctypes.POINTER(ctypes.c_int64)
+    lib().or_diag_flat(_p(Kf, _D), _p(Kp, _L64), _p(ef, _I), _p(ep, _I), len(ep) - 1, dof, n_dof // dof,
+                       _p(_f64(E), _D), _p(m, _U8) if m is not None else None, _p(out, _D))
+    return out
+
+
+def process_dk_flat(K_flat, K_ptr, elements_flat, elements_ptr, dof, u):
+    """-u_e^T K_e u_e per element (core/CPU/_ops.py:25-43)."""
+    Kf, Kp, ef, ep = _flat_args(K_flat, K_ptr, elements_flat, elements_ptr)
+    out = np.empty(len(ep) - 1)
+    _L64 = ctypes.POINTER(ctypes.c_int64)
+    lib().or_process_dk_flat(_p(Kf, _D), _p(Kp, _L64), _p(ef, _I), _p(ep, _I), len(ep) - 1, dof, _p(_f64(u), _D),
+                             _p(out, _D))
+    return out
+
+
+def assemble_flat(K_flat, K_ptr, elements_flat, elements_ptr, dof, E, n_dof, con=None):
+    """Brute-force scipy assembly of sum_e E_e P_e^T K_e P_e with the
+    Dirichlet convention of ``assemble`` (independent of the node gather)."""
+    import scipy.sparse as sps
+
+    Kf = np.asarray(K_flat, np.float64).reshape(-1)
+    rows, cols, vals = [], [], []
+    for e in range(len(elements_ptr) - 1):
+        nodes = np.asarray(elements_flat[elements_ptr[e]:elements_ptr[e + 1]], np.int64)
+        g = (nodes[:, None] * dof + np.arange(dof)[None, :]).ravel()
+        m = g.size
+        Ke = Kf[K_ptr[e]:K_ptr[e] + m * m].reshape(m, m)
+        rows.append(np.repeat(g, m))
+        cols.append(np.tile(g, m))
+        vals.append((E[e] * Ke).ravel())
+    K = sps.coo_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                       shape=(n_dof, n_dof)).tocsr()
+    if con is not None:
+        con = np.asarray(con, bool)
+        free = sps.diags((~con).astype(float))
+        K = (free @ K @ free + sps.diags(con.astype(float))).tocsr()
+    return K
 
 
 def _coarse_nodes(nel):
@@ -591,6 +661,9 @@ PINNED = {
     "b_center / d_iso": "patch test (linear field -> exact constant strain); constant-strain energy u^T K0 u = "
                         "V eps^T D eps against the exactly integrated K0",
     "fea_locals": "energy sum equals 1/2 u^T K u of the scipy assembly; von Mises of a uniaxial stress is |sigma|",
+    "matvec_flat / diag_flat": "scipy assembly (assemble_flat) on tri/quad/tet/hex and mixed meshes; equals the "
+                               "shared-K0 matvec/diag on a structured grid; symmetry",
+    "process_dk_flat": "equals the shared-K0 process_dk on a structured grid; dense u_e^T K_e u_e",
     "csr_assemble": "closed-form nonzero counts (free grid 9 prod(3n-2), clamped face); dense matrix equals the "
                     "scipy assembly; K u equals the C matvec with constraints; symmetric pattern",
 }

@@ -125,6 +125,11 @@ SIGNATURES = {
     "fantom_stencil_ku_f64": (_c_int, [_c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _c_double, _vp,
                                        _vp]),
     "fantom_stencil_partials": (_c_long, [_c_int, _c_int, _c_int]),
+    "fantom_ku_general_partials": (_c_long, [_c_int, _c_int]),
+    "fantom_ku_general_workspace": (_c_long, [_c_int, _c_int, _vp]),
+    "fantom_ku_general_f64": (_c_int, [_c_int, _c_int, _c_int, _c_int] + [_vp] * 12),
+    "fantom_diag_general_f64": (_c_int, [_c_int, _c_int, _c_int] + [_vp] * 8),
+    "fantom_dc_general_f64": (_c_int, [_c_int, _c_int] + [_vp] * 4),
     "fantom_oc_update_f64": (_c_int, [_c_long, _vp, _vp, _vp, _c_double, _c_double, _vp, _vp, _vp, _vp, _vp]),
     "fantom_oc_update_noalloc_f64": (_c_int, [_c_long, _vp, _vp, _vp, _c_double, _c_double, _vp, _vp, _vp]),
     "fantom_mma_workspace_size": (_c_long, [_c_long, _c_int]),
@@ -134,6 +139,12 @@ SIGNATURES = {
 }
 
 _lib = None
+
+
+class ElemGroup(ctypes.Structure):
+    """fantom_elem_group: one element size of an unstructured mesh (device arrays)."""
+    _fields_ = [("n_el", ctypes.c_int), ("npe", ctypes.c_int), ("el_ids", ctypes.c_void_p),
+                ("nodes", ctypes.c_void_p), ("K", ctypes.c_void_p)]
 
 
 class FantomLibraryMissing(RuntimeError):

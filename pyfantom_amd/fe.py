@@ -28,7 +28,8 @@ class FiniteElement:
         self.rhs = torch.zeros(n_dof, dtype=torch.float64, device=kernel.device)
         self.d_rhs = torch.full((n_dof,), float("nan"), dtype=torch.float64, device=kernel.device)
         self.nel = mesh.n_elements
-        self.is_3D = len(mesh.nel) == 3
+        self.is_3D = (len(mesh.nel) if hasattr(mesh, "nel") else np.asarray(mesh.nodes).shape[1]) == 3
+        self._nodes = None
         self.dof = mesh.dof
 
     def _dev(self, a, dtype):
@@ -40,7 +41,16 @@ class FiniteElement:
     def nearest_nodes(self, positions) -> torch.Tensor:
         """Node ids nearest to the given coordinates (KDTree query semantics,
         FiniteElement/CPU/FiniteElement.py:179-184): on a structured grid the
-        nearest node is the rounded grid index, computed on the device."""
+        nearest node is the rounded grid index, computed on the device; on an
+        unstructured mesh it is the argmin of the distances to every node
+        (lowest id on ties, as a KDTree query over the node array)."""
+        if not hasattr(self.mesh, "nel"):
+            if self._nodes is None:
+                self._nodes = self._dev(self.mesh.nodes, torch.float64)
+            X = self._nodes
+            P = self._dev(positions, torch.float64).reshape(-1, X.shape[1])
+            out = [torch.cdist(P[i:i + 1024], X).argmin(dim=1) for i in range(0, P.shape[0], 1024)]
+            return torch.cat(out) if out else torch.zeros(0, dtype=torch.int64, device=X.device)
         d = len(self.mesh.nel)
         P = self._dev(positions, torch.float64).reshape(-1, d)
         h = torch.tensor([float(v) for v in self.mesh.spacing], dtype=torch.float64, device=P.device)

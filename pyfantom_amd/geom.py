@@ -1,4 +1,4 @@
-"""Structured meshes and structured density filters.
+"""Structured and unstructured meshes, structured density filters.
 
 ``StructuredMesh2D/3D`` mirror the reference's structured meshes
 (geom/CPU/_mesh.py:10-210, geom/CUDA/_mesh.py:10-170): same constructor,
@@ -126,6 +126,81 @@ class StructuredMesh2D(_StructuredMesh):
         c = np.meshgrid(np.linspace(self.dx / 2, self.lx - self.dx / 2, self.nelx),
                         np.linspace(self.dy / 2, self.ly - self.dy / 2, self.nely), indexing="ij")
         return torch.from_numpy(np.stack(c, -1).reshape(-1, 2).astype(self.dtype)).to(_lib.device())
+
+
+class GeneralMesh:
+    """Unstructured mesh of triangles / quads (2D) or tetrahedra / hexes (3D)
+    (geom/CPU/_mesh.py GeneralMesh, geom/CUDA/_mesh.py:195-240).
+
+    ``elements`` is an (n_el x npe) array (uniform mesh) or a list of node
+    lists of varying length (mixed element types).  Attributes as the
+    reference: ``is_uniform``, ``elements_flat``, ``element_sizes``,
+    ``elements_ptr``, ``Ks`` (n_el x m x m, uniform) or ``K_flat`` / ``K_ptr``
+    (mixed), ``As``, ``volume``, ``dof``, ``centeroids`` (and ``centroids``),
+    ``locals`` / ``locals_flat``.  Nodes that no element uses are dropped and
+    the connectivity renumbered (the reference drops the nodes but computes
+    the element matrices from the old numbering; here both use the new one).
+    Host arrays; the stiffness kernels upload what they need.
+    """
+
+    def __init__(self, nodes, elements, dtype=np.float64, physics=None):
+        physics = physics or LinearElasticity(E=1.0, nu=1.0 / 3.0)
+        nodes = np.asarray(nodes, dtype=np.float64)
+        if isinstance(elements, np.ndarray) and elements.ndim == 2:
+            # one element type: no per-element Python work
+            if elements.shape[0] == 0:
+                raise ValueError("the mesh has no elements")
+            sizes = np.full(elements.shape[0], elements.shape[1], dtype=np.int32)
+            flat = np.asarray(elements, dtype=np.int64).ravel()
+        else:
+            elems = [np.asarray(e, dtype=np.int64).ravel() for e in elements]
+            if not elems:
+                raise ValueError("the mesh has no elements")
+            sizes = np.array([e.size for e in elems], dtype=np.int32)
+            flat = np.concatenate(elems)
+        if flat.min() < 0 or flat.max() >= nodes.shape[0]:
+            raise ValueError("element connectivity refers to missing nodes")
+        used = np.unique(flat)
+        remap = np.full(nodes.shape[0], -1, dtype=np.int64)
+        remap[used] = np.arange(used.size)
+        flat = remap[flat]
+        self.dtype = dtype
+        self.physics = physics
+        self.nodes = nodes[used]
+        self.element_sizes = sizes
+        self.elements_ptr = np.concatenate(([0], np.cumsum(sizes))).astype(np.int32)
+        self.elements_flat = flat.astype(np.int32)
+        self.is_uniform = bool(np.all(sizes == sizes[0]))
+        n_el = sizes.shape[0]
+        if self.is_uniform:
+            self.elements = flat.reshape(n_el, sizes[0]).astype(np.int32)
+            X = self.nodes[self.elements]
+            self.centeroids = X.mean(axis=1).astype(dtype)
+            self.Ks = np.asarray(physics.K(X), dtype=dtype)
+            self.locals = [np.asarray(a, dtype=dtype) for a in physics.locals(X)]
+            self.As = np.asarray(physics.volume(X), dtype=dtype).reshape(-1)
+            m = self.Ks.shape[1]
+            self.K_flat = self.Ks.reshape(-1)
+            self.K_ptr = (np.arange(n_el + 1, dtype=np.int64) * m * m)
+        else:
+            per = [flat[a:b] for a, b in zip(self.elements_ptr[:-1], self.elements_ptr[1:])]
+            self.centeroids = np.stack([self.nodes[e].mean(axis=0) for e in per]).astype(dtype)
+            self.elements = [e.astype(np.int32) for e in per]
+            Ks = [np.asarray(physics.K(self.nodes[e]), dtype=dtype) for e in per]
+            self.K_flat = np.concatenate([k.ravel() for k in Ks])
+            self.K_ptr = np.concatenate(([0], np.cumsum([k.size for k in Ks]))).astype(np.int64)
+            loc = [physics.locals(self.nodes[e]) for e in per]
+            self.locals_flat = [np.concatenate([np.asarray(l[j], dtype=dtype).ravel() for l in loc])
+                                for j in range(len(loc[0]))]
+            self.locals_ptr = [np.concatenate(([0], np.cumsum([np.asarray(l[j]).size for l in loc])))
+                               .astype(np.int32) for j in range(len(loc[0]))]
+            self.As = np.array([physics.volume(self.nodes[e]) for e in per], dtype=dtype)
+        self.centroids = self.centeroids
+        m0 = int(np.sqrt(self.K_ptr[1] - self.K_ptr[0]))
+        self.dof = m0 // int(sizes[0])
+        self.volume = float(np.sum(self.As))
+        self.n_elements = n_el
+        self.n_nodes = self.nodes.shape[0]
 
 
 def filter_taps(dim: int, r_min: float, spacing):

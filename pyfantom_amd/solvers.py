@@ -95,10 +95,13 @@ class _DeviceKrylov(Solver):
         if getattr(self, "_n", None) != n:
             z = lambda: torch.zeros(n, dtype=torch.float64, device=dev)
             self._r, self._z, self._p, self._q = z(), z(), z(), z()
-            self._np = max(_lib.load().fantom_vec_partials(n),
-                           _lib.load().fantom_ku_partials(_lib.KU_TILE if self.kernel.dim == 3 else _lib.KU_NODE,
-                                                          *self.kernel.grid),
-                           _lib.load().fantom_ku_partials(_lib.KU_NODE, *self.kernel.grid))
+            k = self.kernel
+            if hasattr(k, "grid"):
+                ku = max(_lib.load().fantom_ku_partials(_lib.KU_TILE if k.dim == 3 else _lib.KU_NODE, *k.grid),
+                         _lib.load().fantom_ku_partials(_lib.KU_NODE, *k.grid))
+            else:
+                ku = k.n_partials()
+            self._np = max(_lib.load().fantom_vec_partials(n), ku)
             self._partials = torch.zeros(self._np, dtype=torch.float64, device=dev)
             self._scal = torch.zeros(_lib.N_SCALARS, dtype=torch.float64, device=dev)
             self._n = n
@@ -111,18 +114,11 @@ class _DeviceKrylov(Solver):
 
     def _fine_ku_dot(self, p, q):
         """q = K p and scal[PQ/ALPHA] from p.q."""
-        k = self.kernel
-        kid, k0 = k.op_args()
-        _lib.call("fantom_ku_dot_f64", kid, *k.grid, k.k0_ptr(k0), _lib.ptr(k.rho64),
-                  _lib.ptr(k.mask) if k.has_con else None, _lib.ptr(p), _lib.ptr(q), _lib.ptr(self._partials))
-        npart = _lib.load().fantom_ku_partials(kid, *k.grid)
+        npart = self.kernel.ku_dot(p, q, self._partials)
         _lib.call("fantom_finalize_f64", _lib.ptr(self._partials), npart, _lib.ptr(self._scal), _lib.FIN_ALPHA, 0)
 
     def _true_residual(self, x, rhs, out):
-        k = self.kernel
-        kid, k0 = k.op_args()
-        _lib.call("fantom_ku_residual_f64", kid, *k.grid, k.k0_ptr(k0), _lib.ptr(k.rho64),
-                  _lib.ptr(k.mask) if k.has_con else None, _lib.ptr(rhs), _lib.ptr(x), _lib.ptr(out))
+        self.kernel.ku_residual(x, rhs, out)
         return self._norm(out)
 
     def _norm(self, v):
@@ -143,6 +139,8 @@ class MultiGrid(_DeviceKrylov):
             raise ValueError("Matrix free is not supported with splu solver use cg instead.")
         if omega_schedule not in ("fixed", "reference"):
             raise ValueError("omega_schedule must be 'fixed' or 'reference'")
+        if not hasattr(kernel, "grid"):
+            raise ValueError("MultiGrid needs a structured mesh and a StructuredStiffnessKernel.")
         self.mesh = mesh
         self.kernel = kernel
         self.maxiter = int(maxiter)
@@ -549,9 +547,7 @@ class CG(_DeviceKrylov):
             else torch.zeros_like(rhs)
         r, p, q = self._r, self._p, self._q
         norm_b = self._norm(rhs) or 1.0
-        kid, k0 = k.op_args()
-        _lib.call("fantom_ku_residual_f64", kid, *k.grid, k.k0_ptr(k0), _lib.ptr(k.rho64),
-                  _lib.ptr(k.mask) if k.has_con else None, _lib.ptr(rhs), _lib.ptr(x), _lib.ptr(r))
+        k.ku_residual(x, rhs, r)
         self._dot(r, r, 0, _lib.FIN_RZ0)
         _lib.call("fantom_pcg_p_f64", n, _lib.ptr(p), _lib.ptr(r), _lib.ptr(self._scal), 1)
         it = 0

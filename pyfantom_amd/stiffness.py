@@ -109,6 +109,23 @@ class StructuredStiffnessKernel:
             return k0.ctypes.data
         return k0.data_ptr()
 
+    def n_partials(self):
+        kid, _ = self.op_args()
+        return _lib.load().fantom_ku_partials(kid, *self.grid)
+
+    def ku_residual(self, x, rhs, out):
+        """out = rhs - K(E) x (FP64 moduli)."""
+        kid, k0 = self.op_args()
+        _lib.call("fantom_ku_residual_f64", kid, *self.grid, self.k0_ptr(k0), _lib.ptr(self.rho64),
+                  _lib.ptr(self.mask) if self.has_con else None, _lib.ptr(rhs), _lib.ptr(x), _lib.ptr(out))
+
+    def ku_dot(self, p, q, partials):
+        """q = K(E) p and the block partial sums of p.q; returns their count."""
+        kid, k0 = self.op_args()
+        _lib.call("fantom_ku_dot_f64", kid, *self.grid, self.k0_ptr(k0), _lib.ptr(self.rho64),
+                  _lib.ptr(self.mask) if self.has_con else None, _lib.ptr(p), _lib.ptr(q), _lib.ptr(partials))
+        return _lib.load().fantom_ku_partials(kid, *self.grid)
+
     def _update_mask(self):
         c = self.constraints.view(self.n_nodes, self.dof).to(torch.uint8)
         bits = torch.zeros(self.n_nodes, dtype=torch.uint8, device=self.device)
@@ -261,3 +278,286 @@ class StructuredStiffnessKernel:
         self.mat_vec = None
         self.diag = None
         self.dk = None
+
+
+class _NodeBasisStiffnessKernel:
+    """Shared body of ``UniformStiffnessKernel`` and ``GeneralStiffnessKernel``
+    (stiffness/CUDA/_FEA.py:675-1105 and :1107-1630): unstructured meshes with
+    per-element matrices, applied matrix-free.
+
+    Device layout: the elements are grouped by node count; each group keeps
+    its element ids, its connectivity (npe x n_el) and its matrices
+    structure-of-arrays (m*m x n_el), so that the per-element product reads
+    the matrices coalesced.  A product is one pass per group computing
+    E_e K_e u_e into a workspace, then a per-DOF gather over the node-sorted
+    incidence (the reference's ``sorter`` / ``node_ids`` pair, here a stable
+    argsort of ``elements_flat``: each node visits its elements in ascending
+    id order, so every row is summed in a fixed order).  FP64 only."""
+
+    def __init__(self, mesh):
+        if np.dtype(mesh.dtype) != np.float64:
+            raise NotImplementedError("unstructured meshes are implemented in float64")
+        self.mesh = mesh
+        self.nodes, self.elements = mesh.nodes, mesh.elements
+        self.dof = int(mesh.dof)
+        self.dtype = mesh.dtype
+        self.n_nodes = int(mesh.n_nodes)
+        self.n_elements = int(mesh.n_elements)
+        n_dof = self.n_nodes * self.dof
+        self.shape = (n_dof, n_dof)
+        flat = np.asarray(mesh.elements_flat, dtype=np.int64)
+        eptr = np.asarray(mesh.elements_ptr, dtype=np.int64)
+        sizes = np.diff(eptr)
+        kp = np.asarray(mesh.K_ptr, dtype=np.int64)
+        K_flat = np.asarray(mesh.K_flat, dtype=np.float64).reshape(-1)
+        if kp[-1] != K_flat.size or np.any(np.diff(kp) != (sizes * self.dof) ** 2):
+            raise ValueError("K_ptr does not match the element sizes and dof")
+        npes = np.unique(sizes)
+        if npes.size > 8 or npes.max() > 255 or npes.max() * self.dof > 96:
+            raise ValueError("at most 8 element sizes of at most 96 DOF each are supported")
+        dev = _lib.device()
+        self.device = dev
+        up = lambda a, dt: torch.from_numpy(np.ascontiguousarray(a, dtype=dt)).to(dev)
+        # element groups
+        group = np.empty(self.n_elements, dtype=np.int64)
+        local = np.empty(self.n_elements, dtype=np.int64)
+        self.groups = []
+        for g, npe in enumerate(npes.tolist()):
+            ids = np.flatnonzero(sizes == npe)
+            group[ids] = g
+            local[ids] = np.arange(ids.size)
+            m = npe * self.dof
+            nodes = flat[eptr[ids][:, None] + np.arange(npe)[None, :]]            # n_g x npe
+            if ids.size == self.n_elements:
+                Kg = K_flat.reshape(ids.size, m * m)                                 # one size: a view
+            else:
+                Kg = K_flat[kp[ids][:, None] + np.arange(m * m)[None, :]]          # n_g x m*m
+            Kd = up(Kg, np.float64).t().contiguous()                               # m*m x n_g
+            self.groups.append({"npe": npe, "m": m, "n_el": int(ids.size), "el_ids": up(ids, np.int32),
+                                "nodes": up(nodes.T, np.int32), "K": Kd})
+        self._gtab = (_lib.ElemGroup * len(self.groups))(*[
+            _lib.ElemGroup(g["n_el"], g["npe"], g["el_ids"].data_ptr(), g["nodes"].data_ptr(), g["K"].data_ptr())
+            for g in self.groups])
+        # node-sorted incidence: (element index in its group, group * 256 + local node)
+        el_ids = np.repeat(np.arange(self.n_elements, dtype=np.int64), sizes)
+        order = np.argsort(flat, kind="stable")
+        inc_el = el_ids[order]
+        node_ptr = np.searchsorted(flat[order], np.arange(self.n_nodes + 1), side="left")
+        self.node_ptr = up(node_ptr, np.int32)
+        self.inc_l = up(local[inc_el], np.int32)
+        self.inc_ga = up(group[inc_el] * 256 + (order - eptr[inc_el]), np.int32)
+        nY = _lib.load().fantom_ku_general_workspace(self.dof, len(self.groups), ctypes.addressof(self._gtab))
+        if nY < 0:
+            raise ValueError("invalid element groups")
+        self._Y = torch.empty(max(int(nY), 1), dtype=torch.float64, device=dev)
+        self.elements_size = int(sizes.max())
+        self.max_con_count = int(sizes.max()) * self.dof * int(np.diff(node_ptr).max())
+        self.constraints = torch.zeros(n_dof, dtype=torch.bool, device=dev)
+        self._con_u8 = torch.zeros(n_dof, dtype=torch.uint8, device=dev)
+        self.has_con = False
+        self.idx_map = torch.arange(n_dof, dtype=torch.int32, device=dev)
+        self.non_con_map = self.idx_map
+        self.has_rho = False
+        self.rho = None
+        self.rho64 = None
+        self.f32 = False
+        self.vdtype = torch.float64
+        self.mat_vec = None
+        self.diag = None
+        self.dk = None
+        self.CSR = None
+        self.ptr = None
+        self._coo = None
+        self.has_been_constructed = False
+        self.matvec = self.dot
+        self.rmatvec = self.dot
+        self.matmat = self.dot
+        self.rmatmat = self.dot
+
+    @property
+    def has_cons(self):
+        return self.has_con
+
+    @has_cons.setter
+    def has_cons(self, v):
+        self.has_con = bool(v)
+
+    def _con(self):
+        return _lib.ptr(self._con_u8) if self.has_con else None
+
+    def _inc(self):
+        return (len(self.groups), ctypes.addressof(self._gtab), _lib.ptr(self.node_ptr), _lib.ptr(self.inc_l),
+                _lib.ptr(self.inc_ga))
+
+    def _ku(self, mode, E, u, y, b=None, partials=None):
+        _lib.call("fantom_ku_general_f64", mode, self.n_nodes, self.dof, *self._inc(), _lib.ptr(E), self._con(),
+                  _lib.ptr(u), _lib.ptr(y), _lib.ptr(b), _lib.ptr(self._Y), _lib.ptr(partials))
+
+    # ---- operator calls used by the solvers -----------------------------------------
+    def n_partials(self):
+        return _lib.load().fantom_ku_general_partials(self.n_nodes, self.dof)
+
+    def ku_residual(self, x, rhs, out):
+        """out = rhs - K(E) x."""
+        self._ku(2, self.rho64, x, out, b=rhs)
+
+    def ku_dot(self, p, q, partials):
+        """q = K(E) p and the block partial sums of p.q; returns their count."""
+        self._ku(3, self.rho64, p, q, partials=partials)
+        return self.n_partials()
+
+    # ---- reference API ---------------------------------------------------------------
+    def set_rho(self, rho):
+        """Set per-element moduli (stiffness/CUDA/_FEA.py:172-187)."""
+        if not isinstance(rho, torch.Tensor):
+            rho = torch.as_tensor(np.asarray(rho))
+        rho = rho.to(device=self.device, dtype=torch.float64).contiguous()
+        if rho.shape[0] != self.n_elements:
+            raise ValueError("rho must have one entry per element.")
+        self.rho = self.rho64 = rho
+        self.has_rho = True
+
+    def _as_vec(self, v, dtype=None):
+        if not isinstance(v, torch.Tensor):
+            raise NotImplementedError("Only device tensors as vectors are supported.")
+        return v.to(device=self.device, dtype=dtype or torch.float64).contiguous()
+
+    def _matvec(self, rho, vec):
+        if self.mat_vec is None or self.mat_vec.shape != vec.shape:
+            self.mat_vec = torch.empty_like(vec)
+        rho = rho.to(device=self.device, dtype=torch.float64).contiguous()
+        self._ku(0, rho, vec, self.mat_vec)
+        return self.mat_vec
+
+    def _rmatvec(self, rho, vec):
+        return self._matvec(rho, vec)
+
+    def dot(self, rhs):
+        """K(rho) @ rhs (stiffness/CUDA/_FEA.py:1051-1105, :1574-1628); returns a
+        reused buffer.  A sparse right-hand side goes through ``construct``."""
+        if not self.has_rho:
+            raise ValueError("Rho has not been set. dot works only after setting rho.")
+        if isinstance(rhs, torch.Tensor) and rhs.layout in (torch.sparse_csr, torch.sparse_coo):
+            if rhs.shape[0] != self.shape[0]:
+                raise ValueError("Shape of the input matrix does not match the number of nodes and dof.")
+            A = self.construct(self.rho)
+            B = rhs.to(device=self.device, dtype=torch.float64).to_sparse_csr()
+            it = A.crow_indices().dtype
+            if B.crow_indices().dtype != it:
+                B = torch.sparse_csr_tensor(B.crow_indices().to(it), B.col_indices().to(it), B.values(), size=B.shape)
+            return torch.sparse.mm(A, B)
+        if isinstance(rhs, torch.Tensor):
+            if rhs.ndim == 1 and rhs.shape[0] == self.shape[0]:
+                return self._matvec(self.rho, self._as_vec(rhs))
+            raise ValueError("Shape of the input vector does not match the number of nodes and dof.")
+        raise NotImplementedError("Only device tensors as vectors and sparse matrices are supported.")
+
+    def __matmul__(self, rhs):
+        return self.dot(rhs)
+
+    def diagonal(self, rho=None):
+        """diag(K(rho)), constrained DOFs -> 1 (stiffness/CUDA/_FEA.py:772-828)."""
+        if rho is None and not self.has_rho:
+            raise ValueError("Rho has not been set. diagonal works only after setting rho or if rho is provided.")
+        rho = self.rho64 if rho is None else rho.to(device=self.device, dtype=torch.float64).contiguous()
+        if self.diag is None:
+            self.diag = torch.empty(self.shape[0], dtype=torch.float64, device=self.device)
+        _lib.call("fantom_diag_general_f64", self.n_nodes, self.dof, *self._inc(), _lib.ptr(rho), self._con(),
+                  _lib.ptr(self.diag))
+        return self.diag
+
+    def _idx(self, constraints):
+        if isinstance(constraints, torch.Tensor):
+            return constraints.to(self.device).long()
+        return torch.as_tensor(np.asarray(constraints, dtype=np.int64), device=self.device)
+
+    def set_constraints(self, constraints):
+        """Replace the Dirichlet set (stiffness/CUDA/_FEA.py:830-847)."""
+        self.constraints[:] = False
+        self.add_constraints(constraints)
+
+    def add_constraints(self, constraints):
+        """Add Dirichlet DOFs (stiffness/CUDA/_FEA.py:849-864)."""
+        idx = self._idx(constraints)
+        if idx.numel():
+            self.constraints[idx] = True
+        self.has_con = True
+        self._con_u8 = self.constraints.to(torch.uint8)
+        self.non_con_map = self.idx_map[~self.constraints]
+
+    def process_grad(self, U):
+        """dk_e = -u_e^T K_e u_e (stiffness/CUDA/_FEA.py:866-889)."""
+        U = self._as_vec(U, torch.float64)
+        if self.dk is None:
+            self.dk = torch.empty(self.n_elements, dtype=torch.float64, device=self.device)
+        _lib.call("fantom_dc_general_f64", self.dof, len(self.groups), ctypes.addressof(self._gtab), _lib.ptr(U),
+                  _lib.ptr(self.dk))
+        return self.dk
+
+    def construct(self, rho):
+        """Explicit K(rho) as a torch sparse CSR tensor on the device
+        (stiffness/CUDA/_FEA.py:891-923): the element matrices scattered by
+        their global DOFs and summed, with the Dirichlet convention of ``dot``
+        (unit rows for constrained DOFs, their columns dropped elsewhere)."""
+        if not isinstance(rho, torch.Tensor):
+            rho = torch.as_tensor(np.asarray(rho))
+        rho = rho.to(device=self.device, dtype=torch.float64).contiguous()
+        if rho.shape[0] != self.n_elements:
+            raise ValueError("rho must have one entry per element.")
+        if self._coo is None:
+            # global (row, col) of every stored entry, in the groups' K order
+            parts = []
+            for g in self.groups:
+                m, n_g, dof = g["m"], g["n_el"], self.dof
+                i = torch.arange(m, device=self.device)
+                gdof = g["nodes"].long()[i // dof] * dof + (i % dof)[:, None]           # m x n_g
+                row = gdof[:, None, :].expand(m, m, n_g).reshape(-1)
+                col = gdof[None, :, :].expand(m, m, n_g).reshape(-1)
+                el = g["el_ids"].long()[None, :].expand(m * m, n_g).reshape(-1)
+                parts.append((el, row, col))
+            self._coo = tuple(torch.cat(t) for t in zip(*parts))
+        el, row, col = self._coo
+        val = torch.cat([g["K"].reshape(-1) for g in self.groups]) * rho[el]
+        if self.has_con:
+            keep = ~(self.constraints[row] | self.constraints[col])
+            cdofs = torch.nonzero(self.constraints).reshape(-1)
+            row = torch.cat([row[keep], cdofs])
+            col = torch.cat([col[keep], cdofs])
+            val = torch.cat([val[keep], torch.ones(cdofs.numel(), dtype=torch.float64, device=self.device)])
+        A = torch.sparse_coo_tensor(torch.stack([row, col]), val, size=self.shape).coalesce()
+        self.CSR = A.to_sparse_csr()
+        self.ptr = self.CSR.crow_indices()
+        self.has_been_constructed = True
+        return self.CSR
+
+    def reset(self):
+        self.has_rho = False
+        self.rho = None
+        self.rho64 = None
+        self.constraints[:] = False
+        self._con_u8.zero_()
+        self.has_con = False
+        self.non_con_map = self.idx_map
+        self.mat_vec = None
+        self.diag = None
+        self.dk = None
+
+
+class UniformStiffnessKernel(_NodeBasisStiffnessKernel):
+    """One element type (stiffness/CUDA/_FEA.py:675-1105): ``mesh.Ks`` holds
+    the n_el x m x m element matrices."""
+
+    def __init__(self, mesh):
+        if not mesh.is_uniform:
+            raise ValueError("Mesh is not uniform, you should use GeneralStiffnessKernel instead.")
+        super().__init__(mesh)
+
+
+class GeneralStiffnessKernel(_NodeBasisStiffnessKernel):
+    """Mixed element types (stiffness/CUDA/_FEA.py:1107-1630): element matrices
+    of varying size in ``mesh.K_flat`` at offsets ``mesh.K_ptr``."""
+
+    def __init__(self, mesh):
+        if mesh.is_uniform:
+            raise ValueError("Mesh is uniform, you should use UniformStiffnessKernel instead.")
+        super().__init__(mesh)
