@@ -80,6 +80,19 @@ inline bool grid_ok(int nelx, int nely, int nelz)
     return nn * 3 < (1L << 31);
 }
 
+// Workgroup barrier that orders LDS only.  __syncthreads() is a workgroup
+// fence on every address space, which on gfx9 makes each wave wait for all
+// of its outstanding global loads and stores (s_waitcnt vmcnt(0)) before the
+// barrier, so no prefetch survives it.  Fences restricted to the local
+// address space wait for the LDS counter only.  For kernels whose waves
+// exchange data through LDS and never through global memory.
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // Deterministic block sum: wave64 butterfly, then the first wave sums the
 // per-wave results in a fixed order.  Returns the sum in thread 0.
 template <int BLOCK>

@@ -325,6 +325,251 @@ __global__ __launch_bounds__(kConvBlock) void conv3d_brick_k(int D, int H, int W
     }
 }
 
+// Forward that walks planes along D (the level-0/1/2 shapes).  A workgroup
+// owns a tile of 16 (W) x BI (H) output voxels, CO_T tiles of 16 output
+// channels (blockIdx.y picks them when Cout is split), and a chunk of planes
+// along D.  Its weights stay in LDS for the whole walk, and the input planes
+// j - 1, j, j + 1 with their one-voxel halo (18 x (BI + 2) voxels) sit in a
+// ring of three LDS slots: every input voxel is read from memory about 1.3
+// times instead of 27.  Plane j + 2 is fetched into registers while plane j
+// is computed and is written to the slot of plane j - 1 between two LDS-only
+// barriers, so neither the fetch nor the output stores are drained by a
+// barrier.
+//
+// The products run on v_mfma_f32_16x16x32_bf16 with the weights as A (M =
+// c_out) and the voxels as B (N = 16 voxels along W).  The K axis is the
+// flattened (tap, c_in) axis, f = tap Cin + c_in, in blocks of 32: lane group
+// kg = lane / 16 supplies the 8 consecutive entries f = 32 b + 8 kg .. + 7 of
+// block b (one tap, 8 channels: a 16-byte LDS read for A and for B), so that
+// 16-channel layers pair two taps per instruction; entries past 27 Cin are
+// zero.  Each lane ends with four consecutive output channels of one voxel
+// and writes them as one 8-byte store (16 bytes for the FP32 copy).  LDS holds
+// both operands chunk-major, [8-channel chunk][voxel or c_out row][8], so the
+// 16 lanes of a lane group read 256 consecutive bytes: rows of 16-byte pieces
+// at a stride of 2 Cin bytes measured 1.4x slower per ds_read_b128 for every
+// stride that is not an odd multiple of 32 bytes, and 2.7x at 128 bytes.
+// Same output as conv3d_fwd_k up to the order of the FP32 sums.
+typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
+typedef unsigned short us8 __attribute__((ext_vector_type(8)));
+
+template <int CI_T, int CO_T, int BI>
+__global__ __launch_bounds__(kConvBlock) void conv3d_roll_k(int D, int H, int W, int Cout, int chunk,
+                                                          const uint16_t *__restrict__ x,
+                                                          const uint16_t *__restrict__ w,
+                                                          const float *__restrict__ bias,
+                                                          const uint16_t *__restrict__ skip,
+                                                          uint16_t *__restrict__ out, float *__restrict__ out_f32,
+                                                          int relu)
+{
+    constexpr int Cin = CI_T * 16, CoT = CO_T * 16;
+    constexpr int HK = 18, HI = BI + 2, PV = HK * HI;   // voxels of a staged plane
+    constexpr int PPV = Cin / 8;                        // 16-byte pieces per voxel
+    constexpr int NPIECE = PV * PPV;
+    constexpr int NP = (NPIECE + kConvBlock - 1) / kConvBlock;
+    constexpr int RTW = BI / 4;                         // row tiles (values of i) per wave
+    constexpr int NKB = (27 * Cin + 31) / 32;           // K blocks of 32
+    static_assert(BI % 4 == 0, "four waves share the BI rows");
+    extern __shared__ __attribute__((aligned(16))) uint16_t sm[];
+    uint16_t *Ws = sm;                      // [27][PPV][CoT][8]
+    uint16_t *Xs = sm + 27 * PPV * CoT * 8; // [3][PPV][PV][8]
+    const int nbk = (W + 15) / 16, nbi = (H + BI - 1) / BI;
+    const int bk = blockIdx.x % nbk, bi = (blockIdx.x / nbk) % nbi, cj = blockIdx.x / (nbk * nbi);
+    const int k0 = bk * 16, i0 = bi * BI, cb = blockIdx.y * CoT;
+    const int j0 = cj * chunk, j1 = min(j0 + chunk, D);   // the host sizes the grid so that j0 < D
+    for (int q = threadIdx.x; q < 27 * CoT * PPV; q += kConvBlock) {
+        const int piece = q % PPV, rowi = q / PPV;
+        const int t = rowi / CoT, co = rowi % CoT;
+        *reinterpret_cast<us8 *>(Ws + ((t * PPV + piece) * CoT + co) * 8) =
+            *reinterpret_cast<const us8 *>(w + ((long)t * Cout + cb + co) * Cin + 8 * piece);
+    }
+    // plane j (zeros outside the volume) into registers, registers into a slot
+    auto fetch = [&](int j, us8 (&pf)[NP]) {
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+            const int q = threadIdx.x + p * kConvBlock;
+            const int hv = q / PPV, piece = q % PPV;
+            const int hk = hv % HK, hi = hv / HK;
+            const int k = k0 - 1 + hk, i = i0 - 1 + hi;
+            us8 v = us8{0, 0, 0, 0, 0, 0, 0, 0};
+            if (q < NPIECE && j >= 0 && j < D && i >= 0 && i < H && k >= 0 && k < W)
+                v = *reinterpret_cast<const us8 *>(x + (((long)j * H + i) * W + k) * Cin + 8 * piece);
+            pf[p] = v;
+        }
+    };
+    auto stash = [&](int j, const us8 (&pf)[NP]) {
+        const int slot = ((j % 3) + 3) % 3;
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+            const int q = threadIdx.x + p * kConvBlock;
+            if (q < NPIECE)
+                *reinterpret_cast<us8 *>(Xs + ((slot * PPV + q % PPV) * PV + q / PPV) * 8) = pf[p];
+        }
+    };
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int row = lane & 15, kg = lane >> 4;
+    const int k = k0 + row;
+    float bv[CO_T][4];
+#pragma unroll
+    for (int ct = 0; ct < CO_T; ++ct)
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            bv[ct][q] = bias ? bias[cb + ct * 16 + 4 * kg + q] : 0.f;
+    us8 pf[NP];
+    fetch(j0 - 1, pf);
+    stash(j0 - 1, pf);
+    fetch(j0, pf);
+    stash(j0, pf);
+    fetch(j0 + 1, pf);
+    stash(j0 + 1, pf);
+    lds_barrier();
+    for (int j = j0; j < j1; ++j) {
+        fetch(j + 2 <= j1 ? j + 2 : -1, pf);
+        // skip values of this plane, loaded with the fetch
+        s4 sk[RTW][CO_T];
+#pragma unroll
+        for (int r = 0; r < RTW; ++r) {
+            const int i = i0 + wave * RTW + r;
+            const long v = ((long)j * H + i) * W + k;
+#pragma unroll
+            for (int ct = 0; ct < CO_T; ++ct) {
+                sk[r][ct] = s4{0, 0, 0, 0};
+                if (skip && i < H && k < W)
+                    sk[r][ct] = *reinterpret_cast<const s4 *>(skip + v * Cout + cb + ct * 16 + 4 * kg);
+            }
+        }
+        f4 acc[RTW][CO_T];
+#pragma unroll
+        for (int r = 0; r < RTW; ++r)
+#pragma unroll
+            for (int ct = 0; ct < CO_T; ++ct)
+                acc[r][ct] = f4{0.f, 0.f, 0.f, 0.f};
+        // slots of planes j - 1, j, j + 1 (j >= 0); the block loop is
+        // unrolled so that the LDS reads of later blocks are issued ahead of
+        // the MFMAs
+        const int s0 = (j + 2) % 3;
+        const uint16_t *Xpl[3] = {Xs + s0 * PPV * PV * 8, Xs + ((s0 + 1) % 3) * PPV * PV * 8,
+                                  Xs + ((s0 + 2) % 3) * PPV * PV * 8};
+#pragma unroll
+        for (int kb = 0; kb < NKB; ++kb) {
+            // this lane's tap and first channel; the tap depends on kg only
+            // when two taps share the block (Cin = 16)
+            const int f = 32 * kb + 8 * kg;
+            const bool live = f < 27 * Cin;
+            const int t = live ? f / Cin : 0, c0 = f % Cin;
+            const int dy = t / 9 - 1, dx = (t / 3) % 3 - 1, dz = t % 3 - 1;
+            const uint16_t *Xp = Xpl[dy + 1];
+            bf8v a[CO_T];
+#pragma unroll
+            for (int ct = 0; ct < CO_T; ++ct) {
+                const us8 av = live ? *reinterpret_cast<const us8 *>(Ws + ((t * PPV + c0 / 8) * CoT + ct * 16 + row) * 8)
+                                    : us8{0, 0, 0, 0, 0, 0, 0, 0};
+                a[ct] = __builtin_bit_cast(bf8v, av);
+            }
+#pragma unroll
+            for (int r = 0; r < RTW; ++r) {
+                const int hv = (wave * RTW + r + 1 + dx) * HK + (row + 1 + dz);
+                const bf8v b = __builtin_bit_cast(bf8v, *reinterpret_cast<const us8 *>(Xp + ((c0 / 8) * PV + hv) * 8));
+#pragma unroll
+                for (int ct = 0; ct < CO_T; ++ct)
+                    acc[r][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[ct], b, acc[r][ct], 0, 0, 0);
+            }
+        }
+        lds_barrier();   // every wave is done with plane j - 1's slot
+        if (j + 2 <= j1)
+            stash(j + 2, pf);
+#pragma unroll
+        for (int r = 0; r < RTW; ++r) {
+            const int i = i0 + wave * RTW + r;
+            if (i >= H || k >= W)
+                continue;
+            const long v = ((long)j * H + i) * W + k;
+#pragma unroll
+            for (int ct = 0; ct < CO_T; ++ct) {
+                const int co = cb + ct * 16 + 4 * kg;
+                float val[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    val[q] = acc[r][ct][q] + bv[ct][q];
+                    if (skip)
+                        val[q] += bf2f((uint16_t)sk[r][ct][q]);
+                    if (relu)
+                        val[q] = fmaxf(val[q], 0.f);
+                }
+                s4 o;
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    o[q] = (short)f2bf(val[q]);
+                *reinterpret_cast<s4 *>(out + v * Cout + co) = o;
+                if (out_f32)
+                    *reinterpret_cast<f4 *>(out_f32 + v * Cout + co) = f4{val[0], val[1], val[2], val[3]};
+            }
+        }
+        lds_barrier();   // plane j + 2 is in its slot
+    }
+}
+
+// LDS bytes of conv3d_roll_k<CI_T, CO_T, BI>
+__host__ __device__ constexpr size_t roll_lds(int ci_t, int co_t, int bi)
+{
+    return ((size_t)27 * co_t * 16 * ci_t * 16 + (size_t)3 * 18 * (bi + 2) * ci_t * 16) * 2;
+}
+
+// Grid of conv3d_roll_k: tiles x chunks of planes along D; the chunk count
+// minimises (rounds of resident workgroups) x (planes per chunk + 2 halo).
+template <int CI_T, int CO_T, int BI>
+static int launch_roll(int D, int H, int W, int Cout, const uint16_t *x, const uint16_t *w, const float *bias,
+                       const uint16_t *skip, uint16_t *out, float *out_f32, int relu, hipStream_t s)
+{
+    constexpr size_t lds = roll_lds(CI_T, CO_T, BI);
+    static_assert(lds <= 160 * 1024, "LDS budget");
+    static const hipError_t attr = hipFuncSetAttribute(reinterpret_cast<const void *>(&conv3d_roll_k<CI_T, CO_T, BI>),
+                                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (attr != hipSuccess) {
+        set_error("conv3d_roll LDS attribute: %s", hipGetErrorString(attr));
+        return kStatusHipError;
+    }
+    const long tiles = (long)((W + 15) / 16) * ((H + BI - 1) / BI);
+    const unsigned ycnt = (unsigned)(Cout / (16 * CO_T));
+    // resident workgroups per CU from the kernel's registers and LDS
+    static const int per_cu = [] {
+        int nb = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void *>(&conv3d_roll_k<CI_T, CO_T, BI>),
+                                                         kConvBlock, lds) != hipSuccess || nb < 1) {
+            (void)hipGetLastError();
+            nb = 1;
+        }
+        return nb;
+    }();
+    static const int cus = [] {
+        int d = 0, c = 0;
+        if (hipGetDevice(&d) != hipSuccess || hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, d) != hipSuccess ||
+            c < 1) {
+            (void)hipGetLastError();
+            c = 256;
+        }
+        return c;
+    }();
+    const long cap = (long)cus * per_cu;
+    int best_n = 1;
+    long best = -1;
+    for (int n = 1; n <= D; ++n) {
+        const int c = (D + n - 1) / n;
+        if (n > 1 && c == (D + n - 2) / (n - 1))
+            continue;
+        const long nch = (D + c - 1) / c;
+        const long cost = ((tiles * nch * ycnt + cap - 1) / cap) * (c + 2);
+        if (best < 0 || cost < best) {
+            best = cost;
+            best_n = n;
+        }
+    }
+    const int chunk = (D + best_n - 1) / best_n;
+    const long nch = (D + chunk - 1) / chunk;
+    hipLaunchKernelGGL((conv3d_roll_k<CI_T, CO_T, BI>), dim3((unsigned)(tiles * nch), ycnt), dim3(kConvBlock), lds, s,
+                       D, H, W, Cout, chunk, x, w, bias, skip, out, out_f32, relu);
+    return kStatusOk;
+}
+
 template <int CI_T, int CO_T, int BI, int BJ>
 static void launch_brick(int D, int H, int W, int Cout, const uint16_t *x, const uint16_t *w, const float *bias,
                          const uint16_t *skip, uint16_t *out, float *out_f32, int relu, hipStream_t s)
@@ -511,6 +756,70 @@ __global__ void vol_prolong_k(int D, int H, int W, int C, const uint16_t *__rest
     out[idx] = f2bf(accumulate ? bf2f(out[idx]) + s : s);
 }
 
+// The same transfers with one thread per voxel and group of 8 channels:
+// 16-byte loads and stores, 32-bit index arithmetic, and per channel the
+// sums of vol_restrict_k / vol_prolong_k in the same order.
+__global__ void vol_restrict8_k(int D, int H, int W, int C, const uint16_t *__restrict__ x, uint16_t *__restrict__ out,
+                                float scale)
+{
+    const int Dc = (D - 1) / 2 + 1, Hc = (H - 1) / 2 + 1, Wc = (W - 1) / 2 + 1, G = C / 8;
+    const int n = Dc * Hc * Wc * G;
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n)
+        return;
+    const int g = idx % G, v = idx / G;
+    const int kc = v % Wc, ic = (v / Wc) % Hc, jc = v / (Wc * Hc);
+    float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    for (int dj = -1; dj <= 1; ++dj)
+        for (int di = -1; di <= 1; ++di)
+            for (int dk = -1; dk <= 1; ++dk) {
+                const int j = 2 * jc + dj, i = 2 * ic + di, k = 2 * kc + dk;
+                if (j < 0 || j >= D || i < 0 || i >= H || k < 0 || k >= W)
+                    continue;
+                const float wgt = 1.f / (float)(1 << (abs(dj) + abs(di) + abs(dk)));
+                const us8 a = *reinterpret_cast<const us8 *>(x + ((long)(j * H + i) * W + k) * C + 8 * g);
+#pragma unroll
+                for (int q = 0; q < 8; ++q)
+                    s[q] += wgt * bf2f(a[q]);
+            }
+    us8 o;
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+        o[q] = f2bf(s[q] * scale);
+    *reinterpret_cast<us8 *>(out + (long)idx * 8) = o;
+}
+
+__global__ void vol_prolong8_k(int D, int H, int W, int C, const uint16_t *__restrict__ xc, uint16_t *__restrict__ out,
+                               float scale, int accumulate)
+{
+    const int Hc = (H - 1) / 2 + 1, Wc = (W - 1) / 2 + 1, G = C / 8;
+    const int n = D * H * W * G;
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n)
+        return;
+    const int g = idx % G, v = idx / G;
+    const int k = v % W, i = (v / W) % H, j = v / (W * H);
+    const float wgt = 1.f / (float)(1 << ((j & 1) + (i & 1) + (k & 1)));
+    float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    for (int cj = j / 2; cj <= (j + 1) / 2; ++cj)
+        for (int ci = i / 2; ci <= (i + 1) / 2; ++ci)
+            for (int ck = k / 2; ck <= (k + 1) / 2; ++ck) {
+                const us8 a = *reinterpret_cast<const us8 *>(xc + ((long)(cj * Hc + ci) * Wc + ck) * C + 8 * g);
+#pragma unroll
+                for (int q = 0; q < 8; ++q)
+                    s[q] += wgt * bf2f(a[q]);
+            }
+    uint16_t *o = out + (long)idx * 8;
+    us8 prev = accumulate ? *reinterpret_cast<const us8 *>(o) : us8{0, 0, 0, 0, 0, 0, 0, 0};
+    us8 r;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const float t = s[q] * scale;
+        r[q] = f2bf(accumulate ? bf2f(prev[q]) + t : t);
+    }
+    *reinterpret_cast<us8 *>(o) = r;
+}
+
 // Adam on FP32 master weights; also refreshes the BF16 copy used by the kernels.
 __global__ void adam_k(long n, float *__restrict__ p, const float *__restrict__ g, float *__restrict__ m,
                        float *__restrict__ v, uint16_t *__restrict__ pbf, float lr, float b1, float b2, float eps,
@@ -543,18 +852,18 @@ __global__ void f32_to_bf16_k(long n, const float *__restrict__ a, uint16_t *__r
 __global__ void net_input_k(long nv, const double *__restrict__ r, const float *__restrict__ rho_node,
                             const double *__restrict__ scal, int slot, uint16_t *__restrict__ x)
 {
-    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= nv * 16)
+    // one voxel per thread: its 16 channels as two 16-byte stores
+    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nv)
         return;
-    const long v = idx / 16;
-    const int c = (int)(idx % 16);
     const double inv_s = 1.0 / sqrt(fmax(scal[slot], 1e-300) / (double)(3 * nv));
-    float val = 0.f;
-    if (c < 3)
-        val = (float)(r[3 * v + c] * inv_s);
-    else if (c == 3 && rho_node)
-        val = rho_node[v];
-    x[idx] = f2bf(val);
+    us8 lo = us8{0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+        lo[c] = f2bf((float)(r[3 * v + c] * inv_s));
+    lo[3] = f2bf(rho_node ? rho_node[v] : 0.f);
+    *reinterpret_cast<us8 *>(x + 16 * v) = lo;
+    *reinterpret_cast<us8 *>(x + 16 * v + 8) = us8{0, 0, 0, 0, 0, 0, 0, 0};
 }
 
 // z[3v + c] = s * y[v][c] (c < 3, y has C = 16 channels), s = RMS of r
@@ -635,6 +944,26 @@ int fantom_conv3d_fwd_bf16(int D, int H, int W, int Cin, int Cout, const uint16_
     FANTOM_CHECK_ARG(Cin % 16 == 0 && Cin >= 16 && Cin <= 256, "Cin must be a multiple of 16 in [16, 256]");
     const long N = (long)D * H * W;
     hipStream_t s = as_stream(stream);
+    // planes walked along D with an LDS ring (large volumes)
+    if (N >= (1L << 16)) {
+        int st = -1;
+        if (Cin == 16 && Cout == 16)
+            st = launch_roll<1, 1, 16>(D, H, W, Cout, x, w, bias, skip, out, out_f32, relu, s);
+        else if (Cin == 16 && Cout == 32)
+            st = launch_roll<1, 2, 8>(D, H, W, Cout, x, w, bias, skip, out, out_f32, relu, s);
+        else if (Cin == 32 && Cout == 16)
+            st = launch_roll<2, 1, 8>(D, H, W, Cout, x, w, bias, skip, out, out_f32, relu, s);
+        else if (Cin == 32 && (Cout == 32 || Cout == 64))
+            st = launch_roll<2, 1, 8>(D, H, W, Cout, x, w, bias, skip, out, out_f32, relu, s);
+        else if (Cin == 64 && Cout % 16 == 0 && Cout <= 64)
+            st = launch_roll<4, 1, 8>(D, H, W, Cout, x, w, bias, skip, out, out_f32, relu, s);
+        if (st >= 0) {
+            if (st != kStatusOk)
+                return st;
+            FANTOM_LAUNCH_CHECK("conv3d_fwd");
+            return kStatusOk;
+        }
+    }
     // input bricks and weights in LDS (<= 64 KB) for Cin = 16 (the 32-channel
     // brick measured slower than the weight-staged kernel below)
     if (Cin == 16 && (Cout == 16 || Cout == 32)) {
@@ -762,7 +1091,11 @@ int fantom_vol_restrict_bf16(int D, int H, int W, int C, const uint16_t *x, uint
 {
     FANTOM_CHECK_ARG(D % 2 == 1 && H % 2 == 1 && W % 2 == 1 && x && out, "volume sizes must be odd");
     const long n = (long)((D - 1) / 2 + 1) * ((H - 1) / 2 + 1) * ((W - 1) / 2 + 1) * C;
-    hipLaunchKernelGGL(vol_restrict_k, dim3(blk(n)), dim3(256), 0, as_stream(stream), D, H, W, C, x, out, scale);
+    if (C % 8 == 0 && (long)D * H * W * C < (1L << 31))
+        hipLaunchKernelGGL(vol_restrict8_k, dim3(blk(n / 8)), dim3(256), 0, as_stream(stream), D, H, W, C, x, out,
+                           scale);
+    else
+        hipLaunchKernelGGL(vol_restrict_k, dim3(blk(n)), dim3(256), 0, as_stream(stream), D, H, W, C, x, out, scale);
     FANTOM_LAUNCH_CHECK("vol_restrict");
     return kStatusOk;
 }
@@ -772,8 +1105,12 @@ int fantom_vol_prolong_bf16(int D, int H, int W, int C, const uint16_t *xc, uint
 {
     FANTOM_CHECK_ARG(D % 2 == 1 && H % 2 == 1 && W % 2 == 1 && xc && out, "volume sizes must be odd");
     const long n = (long)D * H * W * C;
-    hipLaunchKernelGGL(vol_prolong_k, dim3(blk(n)), dim3(256), 0, as_stream(stream), D, H, W, C, xc, out, scale,
-                       accumulate);
+    if (C % 8 == 0 && n < (1L << 31))
+        hipLaunchKernelGGL(vol_prolong8_k, dim3(blk(n / 8)), dim3(256), 0, as_stream(stream), D, H, W, C, xc, out,
+                           scale, accumulate);
+    else
+        hipLaunchKernelGGL(vol_prolong_k, dim3(blk(n)), dim3(256), 0, as_stream(stream), D, H, W, C, xc, out, scale,
+                           accumulate);
     FANTOM_LAUNCH_CHECK("vol_prolong");
     return kStatusOk;
 }
@@ -801,8 +1138,7 @@ int fantom_net_input(long nv, const double *r, const float *rho_node, const doub
                      void *stream)
 {
     FANTOM_CHECK_ARG(nv > 0 && r && scal && x && slot >= 0 && slot < 16, "bad operands");
-    hipLaunchKernelGGL(net_input_k, dim3(blk(nv * 16)), dim3(256), 0, as_stream(stream), nv, r, rho_node, scal, slot,
-                       x);
+    hipLaunchKernelGGL(net_input_k, dim3(blk(nv)), dim3(256), 0, as_stream(stream), nv, r, rho_node, scal, slot, x);
     FANTOM_LAUNCH_CHECK("net_input");
     return kStatusOk;
 }

@@ -59,6 +59,41 @@ def test_conv_fwd(shape, cin, cout):
         assert _rel(_from_bf16(out).reshape(-1), ref) < 2 ** -8
 
 
+@pytest.mark.parametrize("shape,cin,cout", [((41, 45, 37), 16, 16), ((24, 200, 130), 16, 16), ((22, 70, 49), 16, 32),
+                                            ((41, 45, 37), 32, 16), ((20, 130, 70), 32, 64), ((20, 130, 70), 64, 32),
+                                            ((17, 66, 61), 64, 64)])
+def test_conv_fwd_planes(shape, cin, cout):
+    """Volumes of at least 2^16 voxels take the kernel that walks planes
+    along D (LDS ring of three planes, chunks of planes per workgroup): edge
+    tiles in W and H, one-plane and multi-plane chunks, bias, skip, ReLU and
+    the FP32 copy on and off."""
+    from pyfantom_amd import _lib
+    torch = _torch()
+    print(f"conv fwd (planes) {shape} {cin}->{cout}", flush=True)
+    rng = np.random.default_rng(1)
+    x = C.bf16_round(rng.standard_normal(shape + (cin,)))
+    w = C.bf16_round(rng.standard_normal((27, cout, cin)) / np.sqrt(27 * cin))
+    b = rng.standard_normal(cout).astype(np.float32).astype(np.float64)
+    skip = C.bf16_round(rng.standard_normal(shape + (cout,)))
+    xd, wd, sd = _bf16_dev(x), _bf16_dev(w), _bf16_dev(skip)
+    bd = torch.from_numpy(b.astype(np.float32)).cuda()
+    n = int(np.prod(shape)) * cout
+    out = torch.empty(n, dtype=torch.int16, device="cuda")
+    of = torch.full((n,), np.nan, dtype=torch.float32, device="cuda")
+    base = C.conv3d(x, w, None)
+    for relu, use_skip, use_bias, f32 in ((0, False, True, True), (1, True, True, False), (1, False, False, True)):
+        _lib.call("fantom_conv3d_fwd_bf16", *shape, cin, cout, xd.data_ptr(), wd.data_ptr(),
+                  bd.data_ptr() if use_bias else None, sd.data_ptr() if use_skip else None, out.data_ptr(),
+                  of.data_ptr() if f32 else None, relu)
+        ref = base + (b if use_bias else 0) + (skip if use_skip else 0)
+        if relu:
+            ref = np.maximum(ref, 0)
+        ref = ref.reshape(-1)
+        if f32:
+            assert _rel(of.cpu().numpy().astype(np.float64), ref) < 1e-5
+        assert _rel(_from_bf16(out).reshape(-1), ref) < 2 ** -8
+
+
 @pytest.mark.parametrize("shape,cin,cout", [((5, 7, 9), 16, 32), ((7, 5, 5), 64, 32), ((5, 5, 5), 32, 64)])
 def test_conv_backward(shape, cin, cout):
     from pyfantom_amd import _lib
@@ -94,11 +129,11 @@ def test_conv_backward(shape, cin, cout):
     assert _rel(db.cpu().numpy().astype(np.float64), dy.reshape(-1, cout).sum(0)) < 1e-5
 
 
-def test_vol_transfers():
+@pytest.mark.parametrize("shape,ch", [((9, 5, 7), 16), ((7, 11, 5), 32), ((5, 5, 9), 64)])
+def test_vol_transfers(shape, ch):
     from pyfantom_amd import _lib
     torch = _torch()
     rng = np.random.default_rng(2)
-    shape, ch = (9, 5, 7), 16
     x = C.bf16_round(rng.standard_normal(shape + (ch,)))
     cs = tuple((s - 1) // 2 + 1 for s in shape)
     xc = C.bf16_round(rng.standard_normal(cs + (ch,)))
@@ -109,3 +144,9 @@ def test_vol_transfers():
     p = torch.empty(int(np.prod(shape)) * ch, dtype=torch.int16, device="cuda")
     _lib.call("fantom_vol_prolong_bf16", *shape, ch, xcd.data_ptr(), p.data_ptr(), 1.0, 0)
     assert _rel(_from_bf16(p).reshape(-1), C.prolong_vol(xc, shape).reshape(-1)) < 2 ** -8
+    # accumulating form: out += 0.5 * P xc
+    base = C.bf16_round(rng.standard_normal(shape + (ch,)))
+    p2 = _bf16_dev(base)
+    _lib.call("fantom_vol_prolong_bf16", *shape, ch, xcd.data_ptr(), p2.data_ptr(), 0.5, 1)
+    ref = base + 0.5 * C.prolong_vol(xc, shape)
+    assert _rel(_from_bf16(p2).reshape(-1), ref.reshape(-1)) < 2 ** -7
