@@ -343,12 +343,19 @@ __global__ __launch_bounds__(kConvBlock) void conv3d_brick_k(int D, int H, int W
 // block b (one tap, 8 channels: a 16-byte LDS read for A and for B), so that
 // 16-channel layers pair two taps per instruction; entries past 27 Cin are
 // zero.  Each lane ends with four consecutive output channels of one voxel
-// and writes them as one 8-byte store (16 bytes for the FP32 copy).  LDS holds
-// both operands chunk-major, [8-channel chunk][voxel or c_out row][8], so the
-// 16 lanes of a lane group read 256 consecutive bytes: rows of 16-byte pieces
-// at a stride of 2 Cin bytes measured 1.4x slower per ds_read_b128 for every
-// stride that is not an odd multiple of 32 bytes, and 2.7x at 128 bytes.
-// Same output as conv3d_fwd_k up to the order of the FP32 sums.
+// and writes them as one 8-byte store (16 bytes for the FP32 copy).
+//
+// LDS layouts follow what ds_read_b128 was measured to read at full rate: a
+// group of 16 lanes reading one aligned 256-byte block, or 16-byte pieces at
+// a stride that is an odd multiple of 32 bytes (other strides, and blocks
+// that straddle 256 bytes, ran 1.4-2.7x slower).  Weights are
+// [tap][8-channel chunk][c_out][8]: the 16 c_out rows of a lane group are one
+// aligned block.  For BI = 16 a staged plane is [chunk][W (18)][H (18)][8]:
+// the 16 voxels along W of a B operand sit 288 bytes apart whatever the
+// tap's shift (the 16-channel layers: 149 -> 117 us at 257 x 129 x 129).  For
+// BI = 8 the plain [chunk][H][W][8] order measured faster (118 against 141 us
+// for the 64-channel layer, the same for 32), so it stays there.  Same output
+// as conv3d_fwd_k up to the order of the FP32 sums.
 typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
 typedef unsigned short us8 __attribute__((ext_vector_type(8)));
 
@@ -369,9 +376,10 @@ __global__ __launch_bounds__(kConvBlock) void conv3d_roll_k(int D, int H, int W,
     constexpr int RTW = BI / 4;                         // row tiles (values of i) per wave
     constexpr int NKB = (27 * Cin + 31) / 32;           // K blocks of 32
     static_assert(BI % 4 == 0, "four waves share the BI rows");
+    constexpr bool kWMajor = BI == 16;   // staged plane order (see above)
     extern __shared__ __attribute__((aligned(16))) uint16_t sm[];
     uint16_t *Ws = sm;                      // [27][PPV][CoT][8]
-    uint16_t *Xs = sm + 27 * PPV * CoT * 8; // [3][PPV][PV][8]
+    uint16_t *Xs = sm + 27 * PPV * CoT * 8; // [3][PPV][HK][HI][8] (kWMajor) or [3][PPV][HI][HK][8]
     const int nbk = (W + 15) / 16, nbi = (H + BI - 1) / BI;
     const int bk = blockIdx.x % nbk, bi = (blockIdx.x / nbk) % nbi, cj = blockIdx.x / (nbk * nbi);
     const int k0 = bk * 16, i0 = bi * BI, cb = blockIdx.y * CoT;
@@ -402,7 +410,9 @@ __global__ __launch_bounds__(kConvBlock) void conv3d_roll_k(int D, int H, int W,
         for (int p = 0; p < NP; ++p) {
             const int q = threadIdx.x + p * kConvBlock;
             if (q < NPIECE)
-                *reinterpret_cast<us8 *>(Xs + ((slot * PPV + q % PPV) * PV + q / PPV) * 8) = pf[p];
+                *reinterpret_cast<us8 *>(
+                    Xs + ((slot * PPV + q % PPV) * PV + (kWMajor ? (q / PPV) % HK * HI + (q / PPV) / HK : q / PPV)) * 8) =
+                    pf[p];
         }
     };
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -467,7 +477,8 @@ __global__ __launch_bounds__(kConvBlock) void conv3d_roll_k(int D, int H, int W,
             }
 #pragma unroll
             for (int r = 0; r < RTW; ++r) {
-                const int hv = (wave * RTW + r + 1 + dx) * HK + (row + 1 + dz);
+                const int hv = kWMajor ? (row + 1 + dz) * HI + (wave * RTW + r + 1 + dx)
+                                       : (wave * RTW + r + 1 + dx) * HK + (row + 1 + dz);
                 const bf8v b = __builtin_bit_cast(bf8v, *reinterpret_cast<const us8 *>(Xp + ((c0 / 8) * PV + hv) * 8));
 #pragma unroll
                 for (int ct = 0; ct < CO_T; ++ct)
