@@ -65,6 +65,7 @@ struct Gal1Op {
     // resident waves per SIMD the kernels are built for (FP64 192-224 VGPRs:
     // 2; FP32 104-120: 4)
     static constexpr int kWavesPerEU = sizeof(T) == 8 ? 2 : 4;
+    static constexpr bool staged_store(int mode) { return true; }
     static constexpr bool kScaled = false;
     static constexpr bool kDiagFromModuli = false;   // the Galerkin diagonal is read (dinv_gal1)
     __device__ __forceinline__ T inv_diag(int, T) const { return T(0); }

@@ -89,6 +89,7 @@ __device__ __forceinline__ double dpp_row(double v)
 //   static constexpr bool kScaled;  T scale(const Data &) const;
 //   static constexpr bool kDiagFromModuli;  T inv_diag(int c, T inv_sum) const;
 //   static constexpr int kWavesPerEU;
+//   static constexpr bool staged_store(int mode);
 // where load fills Data for element (ex, ey, ez) (zero modulus when !active)
 // and apply maps WHT coefficients to output coefficients (the 1/8 of the
 // inverse transform is folded into the operator's tables).  With kScaled the
@@ -99,7 +100,13 @@ __device__ __forceinline__ double dpp_row(double v)
 // elements), and the Jacobi sweeps (not the zero-guess one, whose first half
 // needs D^-1 two planes ahead) form D^-1 = inv_diag(c, 1 / sum) from the
 // moduli they already hold instead of reading a D^-1 vector.  kWavesPerEU
-// is the occupancy the register budget is held to.
+// is the occupancy the register budget is held to.  staged_store(MODE):
+// the outputs of plane L go to an LDS row buffer in layer L and are stored in
+// layer L + 1 (after its barrier) as contiguous words per tile row, 14 x 3
+// values of one x-column, instead of three 8-byte stores at a 24-byte stride
+// per node.  That cut the HBM-resident 256x128x128 K.u from 79.7 to 73.2 us
+// and the residual from 97.4 to 87.6 us; where the extra registers spill (the
+// Jacobi modes) the operator keeps the direct stores.
 template <typename T, int MODE, class Op>
 __global__ __launch_bounds__(kTThreads) __attribute__((amdgpu_waves_per_eu(Op::kWavesPerEU, 8))) void ku_tile_kernel(Grid g, Op op, const uint8_t *__restrict__ mask,
                                                           const T *__restrict__ u, T *__restrict__ y,
@@ -122,6 +129,8 @@ __global__ __launch_bounds__(kTThreads) __attribute__((amdgpu_waves_per_eu(Op::k
     __shared__ T xf[2][6][kTThreads];   // z-butterflies, exchanged with column lx - 1
     __shared__ T xr[2][3][kTThreads];   // upper-x node sums, exchanged with column lx + 1
     __shared__ T xe[2][kOwnDiag ? kTThreads : 1];   // z-pair sums of the moduli, exchanged with column lx + 1
+    constexpr bool kStaged = Op::staged_store(MODE);
+    __shared__ T ys[2][kStaged ? kTOut : 1][3 * kTOut];   // staged outputs of a plane, one row per output x-column
 
     // The hardware deals workgroups to the 8 XCDs round-robin; give each XCD
     // a contiguous range of logical tiles so that tiles sharing halo planes
@@ -144,6 +153,9 @@ __global__ __launch_bounds__(kTThreads) __attribute__((amdgpu_waves_per_eu(Op::k
         const int nb_up = tid + kTW < kTThreads ? tid + kTW : tid;   // column lx + 1 (lx = 15: any valid slot)
         const int nb_dn = tid >= kTW ? tid - kTW : tid;              // column lx - 1 (lx = 0: any valid slot)
         const int jc0 = cyk * ts.chunk;
+        const int gz0 = tz * kTOut;                 // first output node of the tile's rows
+        const int nrow = min(kTOut, g.nz - gz0);    // output nodes per row
+        const bool row_out = lx >= 1 && lx < kTW - 1 && gx < g.nx;
         const int jc1 = min(jc0 + ts.chunk, g.ny);   // output planes [jc0, jc1)
 
         // zero-guess sweeps: the node value is x0 = omega D^-1 b.  load_node
@@ -317,6 +329,13 @@ __global__ __launch_bounds__(kTThreads) __attribute__((amdgpu_waves_per_eu(Op::k
                 for (int c = 0; c < 3; ++c)
                     dgcur[c] = ((mcur >> c) & 1) ? T(1) : op.inv_diag(c, inv_sum);
             }
+            // kStaged: the outputs of plane L - 1, staged by the layer before,
+            // as contiguous words per tile row
+            if (kStaged && L - 1 >= jc0 && row_out) {
+                const long base = 3 * ((long)(L - 1) * nplane + (long)gx * g.nz + gz0);
+                for (int wi = lz; wi < 3 * nrow; wi += kTW)
+                    y[base + wi] = ys[rb ^ 1][lx - 1][wi];
+            }
             if (L >= jc0 && out_col) {
                 const long r0 = 3 * ((long)L * nplane + col);
 #pragma unroll
@@ -333,7 +352,10 @@ __global__ __launch_bounds__(kTThreads) __attribute__((amdgpu_waves_per_eu(Op::k
                         o = bcur[c] - kuv;
                     else
                         o = kuv;
-                    y[r0 + c] = o;
+                    if (kStaged)
+                        ys[rb][lx - 1][3 * (lz - 1) + c] = o;
+                    else
+                        y[r0 + c] = o;
                     if (MODE == MODE_KUDOT)
                         part += (double)ucur[c] * (double)o;
                     if (MODE == MODE_JACDOT)
@@ -371,6 +393,14 @@ __global__ __launch_bounds__(kTThreads) __attribute__((amdgpu_waves_per_eu(Op::k
 #pragma unroll
                     for (int q = 0; q < 4; ++q)
                         FpA[c][q] = FpB[c][q];
+            }
+        }
+        if (kStaged) {   // the chunk's last plane
+            __syncthreads();
+            if (jc1 - 1 >= jc0 && row_out) {
+                const long base = 3 * ((long)(jc1 - 1) * nplane + (long)gx * g.nz + gz0);
+                for (int wi = lz; wi < 3 * nrow; wi += kTW)
+                    y[base + wi] = ys[rb ^ 1][lx - 1][wi];
             }
         }
     }
