@@ -34,10 +34,11 @@ struct FineOp {
     // resident waves per SIMD the kernels are built for (FP64 152-169 VGPRs:
     // 3; FP32 76-95: 5)
     static constexpr int kWavesPerEU = sizeof(T) == 8 ? 3 : 5;
-    // staged row stores where their registers fit without spills
+    // staged row stores where their registers fit without spills (not in
+    // the sweeps that form D^-1 from the moduli)
     static constexpr bool staged_store(int mode)
     {
-        return mode == MODE_KU || mode == MODE_KUDOT || (sizeof(T) == 8 && mode == MODE_RES);
+        return mode == MODE_KU || mode == MODE_JAC || mode == MODE_RES || mode == MODE_KUDOT || mode == MODE_JAC0;
     }
     // apply leaves out the element modulus; the skeleton multiplies it in
     static constexpr bool kScaled = true;

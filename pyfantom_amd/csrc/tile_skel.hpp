@@ -256,14 +256,16 @@ __global__ __launch_bounds__(kTThreads) __attribute__((amdgpu_waves_per_eu(Op::k
             int mpf;
             load_node(L + 2 <= jc1 ? L + 2 : -1, upf, mpf, vd_pf);
             load_elem(dnx, L + 1 < jc1 ? L + 1 : -1);
-            T bnx[3] = {T(0), T(0), T(0)}, dgnx[3] = {T(0), T(0), T(0)};
-            if (kEpiB && out_col && L + 1 < jc1) {
-                const long r0 = 3 * ((long)(L + 1) * nplane + col);
+            // right-hand side (and D^-1) of plane L, used by this layer's
+            // epilogue after the barrier: loaded here rather than a layer
+            // ahead, so that they occupy registers for one layer only
+            if (kEpiB && out_col && L >= jc0) {
+                const long r0 = 3 * ((long)L * nplane + col);
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
-                    bnx[c] = bvec[r0 + c];
+                    bcur[c] = bvec[r0 + c];
                     if (kJacobi && !kOwnDiag)
-                        dgnx[c] = dinv[r0 + c];
+                        dgcur[c] = dinv[r0 + c];
                 }
             }
 
@@ -366,8 +368,6 @@ __global__ __launch_bounds__(kTThreads) __attribute__((amdgpu_waves_per_eu(Op::k
             for (int c = 0; c < 3; ++c) {
                 ucur[c] = unx[c];
                 unx[c] = upf[c];
-                bcur[c] = bnx[c];
-                dgcur[c] = dgnx[c];
             }
             mcur = mnx;
             mnx = mpf;
