@@ -35,10 +35,12 @@ struct FineOp {
     // 3; FP32 76-95: 5)
     static constexpr int kWavesPerEU = sizeof(T) == 8 ? 3 : 5;
     // staged row stores where their registers fit without spills (not in
-    // the sweeps that form D^-1 from the moduli)
+    // the FP64 sweeps that form D^-1 from the moduli and also sum b.x or start
+    // from zero, nor in any FP32 sweep that forms D^-1)
     static constexpr bool staged_store(int mode)
     {
-        return mode == MODE_KU || mode == MODE_JAC || mode == MODE_RES || mode == MODE_KUDOT || mode == MODE_JAC0;
+        return sizeof(T) == 8 ? (mode != MODE_JACDOT && mode != MODE_JAC0OWN)
+                              : (mode != MODE_JACDOT && mode != MODE_JACOWN && mode != MODE_JAC0OWN);
     }
     // apply leaves out the element modulus; the skeleton multiplies it in
     static constexpr bool kScaled = true;

@@ -331,13 +331,6 @@ __global__ __launch_bounds__(kTThreads) __attribute__((amdgpu_waves_per_eu(Op::k
                 for (int c = 0; c < 3; ++c)
                     dgcur[c] = ((mcur >> c) & 1) ? T(1) : op.inv_diag(c, inv_sum);
             }
-            // kStaged: the outputs of plane L - 1, staged by the layer before,
-            // as contiguous words per tile row
-            if (kStaged && L - 1 >= jc0 && row_out) {
-                const long base = 3 * ((long)(L - 1) * nplane + (long)gx * g.nz + gz0);
-                for (int wi = lz; wi < 3 * nrow; wi += kTW)
-                    y[base + wi] = ys[rb ^ 1][lx - 1][wi];
-            }
             if (L >= jc0 && out_col) {
                 const long r0 = 3 * ((long)L * nplane + col);
 #pragma unroll
@@ -363,6 +356,13 @@ __global__ __launch_bounds__(kTThreads) __attribute__((amdgpu_waves_per_eu(Op::k
                     if (MODE == MODE_JACDOT)
                         part += (double)bcur[c] * (double)o;
                 }
+            }
+            // kStaged: the outputs of plane L - 1, staged by the layer before,
+            // as contiguous words per tile row
+            if (kStaged && L - 1 >= jc0 && row_out) {
+                const long base = 3 * ((long)(L - 1) * nplane + (long)gx * g.nz + gz0);
+                for (int wi = lz; wi < 3 * nrow; wi += kTW)
+                    y[base + wi] = ys[rb ^ 1][lx - 1][wi];
             }
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
