@@ -190,11 +190,17 @@ int fantom_gemv_f64(int n, const double *A, const double *x, double *y, void *st
  * dot_partials: per-block partial sums of a.b into partials
  * (fantom_vec_partials(n) doubles).  finalize reduces np partials in a fixed
  * order and applies op to the device scalars (FANTOM_FIN_*; STORE writes
- * scal[slot]).  pcg_xr: x += alpha p; r -= alpha q; partials of r.r.
+ * scal[slot]; ALPHA sets alpha = r.z / p.Kp, or 0 when p.Kp is not positive
+ * and finite).  pcg_xr: x += alpha p; r -= alpha q; partials of r.r.
  * pcg_p: p = z + beta p (first != 0: p = z).  scale: x = omega*dinv*b.
  * axpby: y = a x + b y. */
 long fantom_vec_partials(long n);
 int fantom_dot_partials_f64(long n, const double *a, const double *b, double *partials, void *stream);
+/* a.b and a.c in one pass: partials[0:nb] and partials[nb:2 nb], nb =
+ * fantom_vec_partials(n) (no reference counterpart: the flexible PCG of the
+ * learned preconditioner needs r.z_new and r.z_old per iteration). */
+int fantom_dot2_partials_f64(long n, const double *a, const double *b, const double *c, double *partials,
+                             void *stream);
 int fantom_finalize_f64(const double *partials, long np, double *scal, int op, int slot, void *stream);
 int fantom_pcg_xr_f64(long n, double *x, double *r, const double *p, const double *q, const double *scal,
                       double *partials, void *stream);

@@ -55,6 +55,7 @@ SIGNATURES = {
     "fantom_gemv_f64": (_c_int, [_c_int, _vp, _vp, _vp, _vp]),
     "fantom_vec_partials": (_c_long, [_c_long]),
     "fantom_dot_partials_f64": (_c_int, [_c_long, _vp, _vp, _vp, _vp]),
+    "fantom_dot2_partials_f64": (_c_int, [_c_long, _vp, _vp, _vp, _vp, _vp]),
     "fantom_finalize_f64": (_c_int, [_vp, _c_long, _vp, _c_int, _c_int, _vp]),
     "fantom_pcg_xr_f64": (_c_int, [_c_long, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "fantom_pcg_p_f64": (_c_int, [_c_long, _vp, _vp, _vp, _c_int, _vp]),

@@ -264,6 +264,13 @@ int fantom_dot_partials_f64(long n, const double *a, const double *b, double *pa
     return launch_dot_partials(n, a, b, partials, as_stream(stream));
 }
 
+int fantom_dot2_partials_f64(long n, const double *a, const double *b, const double *c, double *partials,
+                             void *stream)
+{
+    FANTOM_CHECK_ARG(n > 0 && a && b && c && partials, "bad operands");
+    return launch_dot2_partials(n, a, b, c, partials, as_stream(stream));
+}
+
 int fantom_finalize_f64(const double *partials, long np, double *scal, int op, int slot, void *stream)
 {
     FANTOM_CHECK_ARG(np > 0 && partials && scal, "bad operands");

@@ -27,6 +27,27 @@ __global__ __launch_bounds__(kVecBlock) void dot_partials_f64(long n, const doub
         partials[blockIdx.x] = s;
 }
 
+// partials of a.b into partials[0:nb] and of a.c into partials[nb:2 nb]
+// (nb = gridDim.x): one pass over a for two dot products
+__global__ __launch_bounds__(kVecBlock) void dot2_partials_f64(long n, const double *__restrict__ a,
+                                                             const double *__restrict__ b,
+                                                             const double *__restrict__ c,
+                                                             double *__restrict__ partials)
+{
+    double s = 0.0, t = 0.0;
+    for (long i = (long)blockIdx.x * kVecBlock + threadIdx.x; i < n; i += (long)gridDim.x * kVecBlock) {
+        const double ai = a[i];
+        s += ai * b[i];
+        t += ai * c[i];
+    }
+    s = block_sum<kVecBlock>(s);
+    t = block_sum<kVecBlock>(t);
+    if (threadIdx.x == 0) {
+        partials[blockIdx.x] = s;
+        partials[gridDim.x + blockIdx.x] = t;
+    }
+}
+
 // Single block: S = sum(partials[0:np]) in fixed order, then apply op.
 __global__ __launch_bounds__(1024) void finalize_f64(const double *__restrict__ partials, long np,
                                                    double *__restrict__ scal, int op, int slot)
@@ -40,9 +61,9 @@ __global__ __launch_bounds__(1024) void finalize_f64(const double *__restrict__ 
         case FIN_STORE:
             scal[slot] = s;
             break;
-        case FIN_ALPHA:  // p.Kp -> alpha = rz / pq
+        case FIN_ALPHA:  // p.Kp -> alpha = rz / pq (0 when p.Kp is not positive and finite: no update)
             scal[S_PQ] = s;
-            scal[S_ALPHA] = scal[S_RZ] / s;
+            scal[S_ALPHA] = (s > 0.0 && isfinite(s)) ? scal[S_RZ] / s : 0.0;
             break;
         case FIN_BETA:  // new r.z -> beta = rz_new / rz_old
             scal[S_BETA] = s / scal[S_RZ];
@@ -117,6 +138,13 @@ int launch_dot_partials(long n, const double *a, const double *b, double *partia
 {
     hipLaunchKernelGGL(dot_partials_f64, dim3(vec_grid(n)), dim3(kVecBlock), 0, s, n, a, b, partials);
     FANTOM_LAUNCH_CHECK("dot_partials");
+    return kStatusOk;
+}
+
+int launch_dot2_partials(long n, const double *a, const double *b, const double *c, double *partials, hipStream_t s)
+{
+    hipLaunchKernelGGL(dot2_partials_f64, dim3(vec_grid(n)), dim3(kVecBlock), 0, s, n, a, b, c, partials);
+    FANTOM_LAUNCH_CHECK("dot2_partials");
     return kStatusOk;
 }
 

@@ -12,6 +12,7 @@ constexpr long kVecMaxBlocks = 2048;
 
 unsigned vec_grid(long n);
 int launch_dot_partials(long n, const double *a, const double *b, double *partials, hipStream_t s);
+int launch_dot2_partials(long n, const double *a, const double *b, const double *c, double *partials, hipStream_t s);
 int launch_finalize(const double *partials, long np, double *scal, int op, int slot, hipStream_t s);
 int launch_pcg_xr(long n, double *x, double *r, const double *p, const double *q, const double *scal,
                   double *partials, hipStream_t s);

@@ -346,14 +346,60 @@ def energy_loss_grad(t, e, c, kc):
 
 
 class _Vec:
-    """FP64 device vector helpers on the fantom kernels (dot, axpby, K u)."""
+    """FP64 device vector helpers on the fantom kernels (dot, axpby, K u).
+
+    The scalars of the PCG recursion live in the device array scal (slots
+    _lib.S_*, plus S_A / S_B / S_C for the two-dot reads); the methods that
+    end in a finalize leave their result there, and read() fetches several
+    slots with one host synchronisation."""
+
+    S_A, S_B, S_C = 8, 9, 10
 
     def __init__(self, kernel):
         self.k = kernel
         n = kernel.shape[0]
         dev = kernel.device
-        self.partials = torch.zeros(max(_lib.load().fantom_vec_partials(n), 4096), dtype=torch.float64, device=dev)
+        L = _lib.load()
+        self.nvp = L.fantom_vec_partials(n)
+        kid, _ = kernel.op_args()
+        self.nkp = L.fantom_ku_partials(kid, *kernel.grid)
+        self.partials = torch.zeros(max(2 * self.nvp, self.nkp, 4096), dtype=torch.float64, device=dev)
         self.scal = torch.zeros(16, dtype=torch.float64, device=dev)
+
+    def read(self, *slots):
+        """Host values of the given scal slots (one synchronisation)."""
+        v = self.scal.cpu()
+        return [float(v[s]) for s in slots]
+
+    def ku_dot(self, u, y, op=0, slot=8):
+        """y = K u and the finalize op on u.y (default: scal[slot] = u.y)."""
+        k = self.k
+        kid, k0 = k.op_args()
+        _lib.call("fantom_ku_dot_f64", kid, *k.grid, k.k0_ptr(k0), _lib.ptr(k.rho64),
+                  _lib.ptr(k.mask) if k.has_con else None, _lib.ptr(u), _lib.ptr(y), _lib.ptr(self.partials))
+        _lib.call("fantom_finalize_f64", _lib.ptr(self.partials), self.nkp, _lib.ptr(self.scal), op, slot)
+
+    def dot_to(self, a, b, op=0, slot=8):
+        """finalize op on a.b (default: scal[slot] = a.b), no host read."""
+        n = a.numel()
+        _lib.call("fantom_dot_partials_f64", n, _lib.ptr(a), _lib.ptr(b), _lib.ptr(self.partials))
+        _lib.call("fantom_finalize_f64", _lib.ptr(self.partials), self.nvp, _lib.ptr(self.scal), op, slot)
+
+    def dot2_to(self, a, b, c, op_b, slot_b, op_c, slot_c):
+        """a.b and a.c in one pass, each through its finalize op."""
+        n = a.numel()
+        _lib.call("fantom_dot2_partials_f64", n, _lib.ptr(a), _lib.ptr(b), _lib.ptr(c), _lib.ptr(self.partials))
+        P = self.partials.data_ptr()
+        _lib.call("fantom_finalize_f64", P, self.nvp, _lib.ptr(self.scal), op_b, slot_b)
+        _lib.call("fantom_finalize_f64", P + 8 * self.nvp, self.nvp, _lib.ptr(self.scal), op_c, slot_c)
+
+    def xr(self, x, r, p, q):
+        """x += alpha p, r -= alpha q with alpha = scal[S_ALPHA]; scal[S_RR] = r.r."""
+        n = x.numel()
+        _lib.call("fantom_pcg_xr_f64", n, _lib.ptr(x), _lib.ptr(r), _lib.ptr(p), _lib.ptr(q), _lib.ptr(self.scal),
+                  _lib.ptr(self.partials))
+        _lib.call("fantom_finalize_f64", _lib.ptr(self.partials), self.nvp, _lib.ptr(self.scal), _lib.FIN_STORE,
+                  _lib.S_RR)
 
     def dot(self, a, b):
         n = a.numel()
@@ -410,19 +456,21 @@ class LearnedPreconditioner:
     def __call__(self, r, z):
         if self.mode == "learned":
             self.runner.apply(r, z)
+            z.mul_(self._free)
         else:
+            # z = V(r); t2 = r - K z; c = N(t2) on the free DOFs; z += (t2.c / c.Kc) c.
+            # V(r) is zero on the constrained DOFs (identity rows, zero residual
+            # there) and so is c, hence z needs no masking.
+            v = self.vec
             self.mg.precondition_native(r, z)
-            self.vec.ku(z, self._t)
-            # t2 = r - K z, c = N(t2), z += (t2.c / c.Kc) c
-            self._t2.copy_(r)
-            self.vec.axpby(-1.0, self._t, 1.0, self._t2)
+            v.residual(r, z, self._t2)
             self.runner.apply(self._t2, self._c)
             self._c.mul_(self._free)
-            self.vec.ku(self._c, self._t)
-            ckc = self.vec.dot(self._c, self._t)
-            if math.isfinite(ckc) and ckc > 0.0:
-                self.vec.axpby(self.vec.dot(self._t2, self._c) / ckc, self._c, 1.0, z)
-        z.mul_(self._free)
+            v.ku_dot(self._c, self._t, slot=v.S_A)
+            v.dot_to(self._t2, self._c, slot=v.S_B)
+            ckc, t2c = v.read(v.S_A, v.S_B)
+            if math.isfinite(ckc) and ckc > 0.0 and math.isfinite(t2c):
+                v.axpby(t2c / ckc, self._c, 1.0, z)
         return z
 
 
@@ -468,8 +516,13 @@ def fpcg_solve(kernel, f, precond, tol=1e-8, maxiter=500, x0=None, fallback=None
         info["restarts"] += 1
         return vec.dot(r, z)
 
+    def set_rz(v):
+        # the device copy of r.z that FIN_ALPHA divides by
+        vec.scal[_lib.S_RZ] = v
+
     precond(r, z)
     rz = vec.dot(r, z)
+    set_rz(rz)
     p.copy_(z)
     it = 0
     if not ok(rz) and hist[-1] > 0.0:
@@ -478,19 +531,20 @@ def fpcg_solve(kernel, f, precond, tol=1e-8, maxiter=500, x0=None, fallback=None
             maxiter = 0
         else:
             rz = restart()
+            set_rz(rz)
     for it in range(1, maxiter + 1):
-        vec.ku(p, q)
-        pq = vec.dot(p, q)
+        # q = K p with p.q, then x += alpha p, r -= alpha q (alpha = 0 on a
+        # breakdown) and r.r, all on the device; one host read
+        vec.ku_dot(p, q, op=_lib.FIN_ALPHA)
+        vec.xr(x, r, p, q)
+        pq, rr = vec.read(_lib.S_PQ, _lib.S_RR)
         breakdown = not ok(pq)
         if breakdown:
             hist.append(hist[-1])   # one history entry per iteration keeps the stall test aligned
         else:
-            alpha = rz / pq
-            vec.axpby(alpha, p, 1.0, x)
-            vec.axpby(-alpha, q, 1.0, r)
             if record is not None:
                 record(x)
-            R = math.sqrt(max(vec.dot(r, r), 0.0)) / nb
+            R = math.sqrt(max(rr, 0.0)) / nb
             hist.append(R)
             if R < tol:
                 break
@@ -500,18 +554,22 @@ def fpcg_solve(kernel, f, precond, tol=1e-8, maxiter=500, x0=None, fallback=None
             break
         if breakdown:
             rz = restart()
+            set_rz(rz)
             continue
-        z_old.copy_(z)
+        z, z_old = z_old, z   # z_old: the previous preconditioned residual
         precond(r, z)
-        rz_new = vec.dot(r, z)
+        vec.dot2_to(r, z, z_old, _lib.FIN_STORE, vec.S_A, _lib.FIN_STORE, vec.S_B)
+        rz_new, rz_old = vec.read(vec.S_A, vec.S_B)
         if not ok(rz_new):
             if fallback is not None:
                 x = handover(x)
                 break
             rz = restart()
+            set_rz(rz)
             continue
-        beta = (rz_new - vec.dot(r, z_old)) / rz
+        beta = (rz_new - rz_old) / rz
         rz = rz_new
+        set_rz(rz)
         vec.axpby(1.0, z, beta, p)
     vec.residual(f, x, r)
     info.update(iterations=it, residual=math.sqrt(vec.dot(r, r)) / nb, history=hist)

@@ -2,16 +2,37 @@
 with a small SPD matrix and a vector backend in torch: convergence with a good
 preconditioner, restarts when the preconditioner breaks down (z = 0), and the
 hand-over to the fallback solver."""
+import math
+
 import numpy as np
 import pytest
 import torch
 
+from pyfantom_amd import _lib
 from pyfantom_amd.learned import fpcg_solve
 
 
 class _Vec:
+    """The device-vector contract of learned._Vec in torch on the CPU: the
+    scalars live in scal and the finalize ops (store, alpha = r.z / p.Kp or 0
+    on a breakdown) act as the device finalize kernel does."""
+    S_A, S_B, S_C = 8, 9, 10
+
     def __init__(self, A):
         self.A = A
+        self.scal = torch.zeros(16, dtype=torch.float64)
+
+    def _fin(self, v, op, slot):
+        if op == _lib.FIN_STORE:
+            self.scal[slot] = v
+        elif op == _lib.FIN_ALPHA:
+            self.scal[_lib.S_PQ] = v
+            self.scal[_lib.S_ALPHA] = float(self.scal[_lib.S_RZ]) / v if (v > 0.0 and math.isfinite(v)) else 0.0
+        else:
+            raise AssertionError(f"unexpected finalize op {op}")
+
+    def read(self, *slots):
+        return [float(self.scal[s]) for s in slots]
 
     def residual(self, f, x, r):
         r.copy_(f - self.A @ x)
@@ -21,6 +42,23 @@ class _Vec:
 
     def ku(self, u, y):
         y.copy_(self.A @ u)
+
+    def ku_dot(self, u, y, op=0, slot=8):
+        y.copy_(self.A @ u)
+        self._fin(float(torch.dot(u, y)), op, slot)
+
+    def dot_to(self, a, b, op=0, slot=8):
+        self._fin(float(torch.dot(a, b)), op, slot)
+
+    def dot2_to(self, a, b, c, op_b, slot_b, op_c, slot_c):
+        self._fin(float(torch.dot(a, b)), op_b, slot_b)
+        self._fin(float(torch.dot(a, c)), op_c, slot_c)
+
+    def xr(self, x, r, p, q):
+        alpha = float(self.scal[_lib.S_ALPHA])
+        x.add_(alpha * p)
+        r.sub_(alpha * q)
+        self.scal[_lib.S_RR] = float(torch.dot(r, r))
 
     def axpby(self, a, x, b, y):
         y.mul_(b).add_(a * x)
