@@ -93,6 +93,12 @@ __device__ __forceinline__ void lds_barrier()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// Fused multiply-add in the operands' own precision.  __builtin_fma is the
+// double routine: called on floats it converts all three operands to double,
+// runs the FP64 FMA and rounds back, three extra instructions per FMA.
+__device__ __forceinline__ float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 // Deterministic block sum: wave64 butterfly, then the first wave sums the
 // per-wave results in a fixed order.  Returns the sum in thread 0.
 template <int BLOCK>

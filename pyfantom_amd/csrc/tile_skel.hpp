@@ -297,7 +297,7 @@ __global__ __launch_bounds__(kTThreads) __attribute__((amdgpu_waves_per_eu(Op::k
                         const T hi = z[c][sx + 4 * sz] - z[c][sx + 2 + 4 * sz];
                         if constexpr (Op::kScaled) {
                             const T e = op.scale(dcur);
-                            A[q] = __builtin_fma(e, lo, Zu[c][q]);
+                            A[q] = fma_t(e, lo, Zu[c][q]);
                             Zu[c][q] = e * hi;
                         } else {
                             A[q] = lo + Zu[c][q];
