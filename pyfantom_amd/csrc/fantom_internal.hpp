@@ -96,7 +96,8 @@ int launch_restrict_f32(const Grid &gf, int dim, int dof, int normalize, const u
 int launch_prolong_f32(const Grid &gf, int dim, int dof, int normalize, const uint8_t *mask_f, const float *ec,
                        float *xf, int accumulate, hipStream_t s);
 int launch_coarsen_E_f32(const Grid &gf, int dim, const float *Ef, float *Ec, hipStream_t s);
-int launch_gemv_f32(int n, const double *A, const float *x, float *y, hipStream_t s);
+int launch_gemv_f32(int n, const float *A, const float *x, float *y, hipStream_t s);
+int launch_to_f32(long n, const double *a, float *b, hipStream_t s);
 int launch_diag_f32(const Grid &g, const double *K0, const float *E, const uint8_t *mask, float *d, hipStream_t s);
 int launch_coarsen_mask(const Grid &gf, int dim, const uint8_t *mf, uint8_t *mc, hipStream_t s);
 int launch_dense_assemble(const Grid &g, int dim, const double *K0, const double *E, const uint8_t *mask,

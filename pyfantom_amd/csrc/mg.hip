@@ -492,24 +492,39 @@ __global__ __launch_bounds__(256) void gj_update(int n, int k0, const double *__
     }
 }
 
-// y = A x for a dense row-major n x n matrix: one wave per row.
-template <typename T>
-__global__ __launch_bounds__(kBlock) void gemv(int n, const double *__restrict__ A, const T *__restrict__ x,
+// y = A x for a dense row-major n x n matrix, A in FP64 or FP32, sums in
+// FP64: one workgroup per row, eight partial sums per thread, so that a row
+// is read in two trips of eight loads per thread at n = 4131.  (One wave per
+// row made eight dependent trips per wave with one round of waves on the
+// chip, and read the matrix at 3.5 TB/s.)  Deterministic: fixed partial-sum
+// and block-sum order.
+template <typename TA, typename T>
+__global__ __launch_bounds__(kBlock) void gemv(int n, const TA *__restrict__ A, const T *__restrict__ x,
                                              T *__restrict__ y)
 {
-    const int row = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (row >= n)
-        return;
-    const double *Ar = A + (long)row * n;
-    double s = 0.0;
-    for (int j = lane; j < n; j += 64)
-        s += Ar[j] * (double)x[j];
+    const int row = blockIdx.x;
+    const TA *Ar = A + (long)row * n;
+    double s[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    int j = threadIdx.x;
+    for (; j + 7 * kBlock < n; j += 8 * kBlock) {
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1)
-        s += __shfl_xor(s, off, 64);
-    if (lane == 0)
-        y[row] = (T)s;
+        for (int u = 0; u < 8; ++u)
+            s[u] += (double)Ar[j + kBlock * u] * (double)x[j + kBlock * u];
+    }
+#pragma unroll
+    for (int u = 0; u < 7; ++u)
+        if (j + kBlock * u < n)
+            s[u] += (double)Ar[j + kBlock * u] * (double)x[j + kBlock * u];
+    const double t = block_sum<kBlock>(((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7])));
+    if (threadIdx.x == 0)
+        y[row] = (T)t;
+}
+
+// FP32 copy of a dense matrix (the coarsest inverse applied by the FP32 V-cycle)
+__global__ __launch_bounds__(kBlock) void to_f32_k(long n, const double *__restrict__ a, float *__restrict__ b)
+{
+    for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long)gridDim.x * kBlock)
+        b[i] = (float)a[i];
 }
 
 static unsigned nblk(long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
@@ -1028,17 +1043,22 @@ int launch_spd_inverse(int n, double *A, double *work, hipStream_t s)
 
 int launch_gemv(int n, const double *A, const double *x, double *y, hipStream_t s)
 {
-    const unsigned blocks = (unsigned)((n + (kBlock / 64) - 1) / (kBlock / 64));
-    hipLaunchKernelGGL(gemv<double>, dim3(blocks), dim3(kBlock), 0, s, n, A, x, y);
+    hipLaunchKernelGGL((gemv<double, double>), dim3((unsigned)n), dim3(kBlock), 0, s, n, A, x, y);
     FANTOM_LAUNCH_CHECK("gemv");
     return kStatusOk;
 }
 
-int launch_gemv_f32(int n, const double *A, const float *x, float *y, hipStream_t s)
+int launch_gemv_f32(int n, const float *A, const float *x, float *y, hipStream_t s)
 {
-    const unsigned blocks = (unsigned)((n + (kBlock / 64) - 1) / (kBlock / 64));
-    hipLaunchKernelGGL(gemv<float>, dim3(blocks), dim3(kBlock), 0, s, n, A, x, y);
+    hipLaunchKernelGGL((gemv<float, float>), dim3((unsigned)n), dim3(kBlock), 0, s, n, A, x, y);
     FANTOM_LAUNCH_CHECK("gemv");
+    return kStatusOk;
+}
+
+int launch_to_f32(long n, const double *a, float *b, hipStream_t s)
+{
+    hipLaunchKernelGGL(to_f32_k, dim3((unsigned)std::min<long>(nblk(n), 4096)), dim3(kBlock), 0, s, n, a, b);
+    FANTOM_LAUNCH_CHECK("to_f32");
     return kStatusOk;
 }
 

@@ -66,6 +66,7 @@ struct MxHandle {
     // FP64 PCG vectors of the FP32-preconditioned solve
     double *p64, *q64;
     double *partials, *scal, *Ainv, *gj_work;
+    float *Ainv32;   // FP32 copy of the coarsest inverse, applied by the FP32 V-cycle
     int nc;
     hipStream_t stream;
     hipGraphExec_t iter_exec, pc_exec;
@@ -419,7 +420,7 @@ static int mx_cycle(MxHandle *h, int l, const float *b, float **out, float *fina
     MxLevel &L = h->lev[l];
     if (l == h->nlev - 1) {
         float *dst = final ? final : L.x;
-        MX_TRY(launch_gemv_f32(h->nc, h->Ainv, b, dst, h->stream));
+        MX_TRY(launch_gemv_f32(h->nc, h->Ainv32, b, dst, h->stream));
         *out = dst;
         return kStatusOk;
     }
@@ -566,6 +567,7 @@ static void mx_layout(MxHandle *h, int nelx, int nely, int nelz, int n_coarse, i
     h->chs = cv.take<double>(4 * (size_t)h->nlev);
     h->nc = (int)h->lev[h->nlev - 1].n_dof;
     h->Ainv = cv.take<double>((size_t)h->nc * h->nc);
+    h->Ainv32 = cv.take<float>((size_t)h->nc * h->nc);
     h->gj_work = cv.take<double>((size_t)h->nc * h->nc + 1024);
 }
 
@@ -932,7 +934,9 @@ int fantom_mx_setup(void *handle, const double *E, const uint8_t *mask, void *st
         }
         ++h->cheb_setups;
     }
-    // the coarsest operator and its inverse stay FP64
+    // the coarsest operator and its inverse are formed in FP64; the V-cycle
+    // applies an FP32 copy of the inverse (half the bytes of its one read per
+    // cycle), with the products summed in FP64
     MxLevel &C = h->lev[h->nlev - 1];
     if (C.kind == 3)
         MX_TRY(launch_dense_from_elem(C.g, C.Ke, C.mask, h->Ainv, s));
@@ -941,6 +945,7 @@ int fantom_mx_setup(void *handle, const double *E, const uint8_t *mask, void *st
     else
         MX_TRY(launch_dense_assemble_f32E(C.g, C.K0dense_d, C.E, C.mask, h->Ainv, s));
     MX_TRY(launch_spd_inverse(h->nc, h->Ainv, h->gj_work, s));
+    MX_TRY(launch_to_f32((long)h->nc * h->nc, h->Ainv, h->Ainv32, s));
     FANTOM_LAUNCH_CHECK("mixed setup");
     return mx_join(s, as_stream(stream));
 }
