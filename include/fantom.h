@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define FANTOM_ABI_VERSION 2
+#define FANTOM_ABI_VERSION 3
 
 /* Number of double slots of the PCG device-scalar array. */
 #define FANTOM_SCALARS 16
@@ -289,19 +289,24 @@ int fantom_gal2_build_f64(int nelx, int nely, int nelz, const double *G2, const 
 /* Assembled 27-point node stencils of a Galerkin level (hex8), the form the
  * native drivers apply on levels >= 2 (the reference applies its coarse CSR
  * matrices with a generic SpMV, core/CUDA/_mgm.py:19-171).  stencil_build:
- * S (243 x n_nodes doubles, coefficient k = t*9 + 3c + d of neighbour
- * t = (dy+1)*9 + (dx+1)*3 + (dz+1), structure of arrays) from element
- * matrices Ke, with constrained rows set to identity rows and constrained
- * columns zeroed.  stencil_ku: mode 0 y = A u, 1 one Jacobi sweep
+ * S (fantom_stencil_size doubles: the operator is symmetric, so only the
+ * centre block and the 13 forward neighbour blocks are stored,
+ * FANTOM_STENCIL_COEFS = 126 coefficients k = (t - 13)*9 + 3c + d per node,
+ * neighbour t = (dy+1)*9 + (dx+1)*3 + (dz+1) >= 13, in tiles of 64 nodes,
+ * S[n/64][k][n%64]; the apply reads block t < 13 as the transpose of the
+ * neighbour's block 26 - t) from element matrices Ke, with constrained rows
+ * set to identity rows and constrained columns zeroed.  stencil_ku: mode 0 y = A u, 1 one Jacobi sweep
  * y = u + omega dinv (b - A u), 2 residual y = b - A u, 3 y = A u with the
  * per-block partial sums of u.(A u) in partials[0:stencil_partials], 4 the
  * first sweep from u = omega dinv b (u unused) (as fantom_ku_f64 / _jacobi /
  * _residual / _dot / _jacobi0; unused operands may be NULL). */
+#define FANTOM_STENCIL_COEFS 126
 int fantom_stencil_build_f64(int nelx, int nely, int nelz, const double *Ke, const uint8_t *mask, double *S,
                              void *stream);
 int fantom_stencil_ku_f64(int mode, int nelx, int nely, int nelz, const double *S, const double *u, double *y,
                           const double *b, const double *dinv, double omega, double *partials, void *stream);
 long fantom_stencil_partials(int nelx, int nely, int nelz);
+long fantom_stencil_size(int nelx, int nely, int nelz);
 
 /* ---- A9-A11: native GMG-PCG driver ------------------------------------------
  * Replaces MultiGrid._setup / _multi_grid / solve

@@ -126,6 +126,7 @@ SIGNATURES = {
     "fantom_stencil_ku_f64": (_c_int, [_c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _c_double, _vp,
                                        _vp]),
     "fantom_stencil_partials": (_c_long, [_c_int, _c_int, _c_int]),
+    "fantom_stencil_size": (_c_long, [_c_int, _c_int, _c_int]),
     "fantom_ku_general_partials": (_c_long, [_c_int, _c_int]),
     "fantom_ku_general_workspace": (_c_long, [_c_int, _c_int, _vp]),
     "fantom_ku_general_f64": (_c_int, [_c_int, _c_int, _c_int, _c_int] + [_vp] * 12),

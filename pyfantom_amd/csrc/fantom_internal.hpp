@@ -145,5 +145,6 @@ int launch_stencil_ku_f32(int mode, const Grid &g, const float *S, const float *
 int launch_dinv_stencil(const Grid &g, const double *S, double *d, hipStream_t s);
 int launch_dinv_stencil_f32(const Grid &g, const float *S, float *d, hipStream_t s);
 long stencil_partials(const Grid &g);
+long stencil_size(const Grid &g);   // coefficients (words) of a stencil
 
 }  // namespace fantom

@@ -365,7 +365,7 @@ static void layout(MgHandle *h, int nelx, int nely, int nelz, int n_coarse, int 
         if (L.kind == LK_ELEM) {
             L.Ke = cv.take<double>((size_t)L.g.n_el * 576);
             if (l < h->nlev - 1)   // the coarsest level is solved directly
-                L.S = cv.take<double>((size_t)L.g.n_nodes * 243);
+                L.S = cv.take<double>((size_t)stencil_size(L.g));
         }
     }
     h->Gtab = h->Btab = h->Gdiag = h->Ke1 = h->G2tab = nullptr;

@@ -529,7 +529,7 @@ static void mx_layout(MxHandle *h, int nelx, int nely, int nelz, int n_coarse, i
         L.ev = cv.take<float>(L.n_dof);
         L.kind = l == 0 ? 0 : (!h->galerkin ? 1 : (l == 1 ? 2 : 3));
         L.Ke = L.kind == 3 ? cv.take<double>((size_t)L.g.n_el * 576) : nullptr;
-        L.S = (L.kind == 3 && l < h->nlev - 1) ? cv.take<float>((size_t)L.g.n_nodes * 243) : nullptr;
+        L.S = (L.kind == 3 && l < h->nlev - 1) ? cv.take<float>((size_t)stencil_size(L.g)) : nullptr;
     }
     h->Gtab = h->Btab = h->Gdiag = h->Ke1 = h->G2tab = nullptr;
     h->Btab32 = nullptr;

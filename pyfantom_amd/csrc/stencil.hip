@@ -2,33 +2,46 @@
 //
 // A coarse level with explicit 24x24 element matrices is applied far faster
 // from its assembled rows than element by element: a node's three rows hold
-// 27 neighbour blocks of 3x3 (243 coefficients), against 8 x 72 element-matrix
-// entries that a node-centric gather would read.  Layout: structure of
-// arrays S[k][n] with k = t * 9 + 3 c + d, neighbour t = (dy+1)*9 + (dx+1)*3 +
-// (dz+1) (y slowest, as the node numbering), row component c, column
-// component d, so that consecutive threads (nodes) read consecutive words.
-// The Dirichlet rule (constrained rows are identity rows, constrained columns
-// are skipped) is folded into the coefficients when the stencil is built, so
-// the apply reads no mask.
+// 27 neighbour blocks of 3x3, against 8 x 72 element-matrix entries that a
+// node-centric gather would read.  The operator is symmetric, so only the
+// centre block and the 13 forward neighbour blocks (t = 13 ... 26 below) are
+// stored; the block of backward neighbour t < 13 is the transpose of the
+// forward block 26 - t of that neighbour, read at the neighbour's index as
+// its u value is.  That halves the coefficient bytes, which bound the sweep.
+// Layout: tiles of 64 nodes, each a structure of arrays over the 126
+// coefficients, S[n / 64][k][n % 64] with k = (t - 13) * 9 + 3 c + d,
+// neighbour t = (dy+1)*9 + (dx+1)*3 + (dz+1) (y slowest, as the node
+// numbering), row component c, column component d: consecutive threads
+// (nodes) read consecutive words, and a workgroup's own coefficients are
+// one contiguous 64 x 126-word block instead of 126 runs far apart.  The Dirichlet rule (constrained
+// rows are identity rows, constrained columns are skipped) is folded into the
+// coefficients when the stencil is built, so the apply reads no mask.
 #include "../../include/fantom.h"
 #include "common.hpp"
 #include "fantom_internal.hpp"
 
 namespace fantom {
 
-constexpr int kStencilK = 243;
+static_assert(FANTOM_STENCIL_COEFS == 14 * 9, "stencil: centre block and 13 forward blocks");
+constexpr int kStTile = 64;   // nodes per layout tile
 
-// one thread per (node, neighbour t): the 9 coefficients of block t, summed
-// over the (up to 8) elements around the node in a fixed order
+// word offset of coefficient k of node n
+__device__ __forceinline__ long st_at(int k, long n)
+{
+    return ((n / kStTile) * FANTOM_STENCIL_COEFS + k) * kStTile + (n % kStTile);
+}
+
+// one thread per (node, forward neighbour t): the 9 coefficients of block t,
+// summed over the (up to 8) elements around the node in a fixed order
 template <typename TS>
 __global__ void stencil_from_elem_k(Grid g, const double *__restrict__ Ke, const uint8_t *__restrict__ mask,
                                     TS *__restrict__ S)
 {
     const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= g.n_nodes * 27)
+    if (idx >= g.n_nodes * 14)
         return;
     const long n = idx % g.n_nodes;
-    const int t = (int)(idx / g.n_nodes);
+    const int t = 13 + (int)(idx / g.n_nodes);
     const int dy = t / 9 - 1, dx = (t / 3) % 3 - 1, dz = t % 3 - 1;
     const int k = (int)(n % g.nz);
     const long q = n / g.nz;
@@ -64,17 +77,19 @@ __global__ void stencil_from_elem_k(Grid g, const double *__restrict__ Ke, const
                 v = (t == 13 && c == d) ? 1.0 : 0.0;
             else if ((mm >> d) & 1)
                 v = 0.0;
-            S[(long)(t * 9 + 3 * c + d) * g.n_nodes + n] = (TS)v;
+            S[st_at((t - 13) * 9 + 3 * c + d, n)] = (TS)v;
         }
 }
 
-// y = K u from the stencil with the solver epilogues.  A block of 192
-// threads covers 64 nodes; wave p (p = 0, 1, 2) takes the 9 neighbours in
-// node plane y + p - 1 of each node, so every thread has one round of
-// independent loads (coarse levels are small and latency-bound), and wave 0
-// adds the three plane sums in a fixed order.
+// y = K u from the stencil with the solver epilogues.  A block of 576
+// threads covers 64 nodes; wave p (p = 0 ... 8) takes the 3 neighbours
+// (dy, dx) = (p / 3 - 1, p % 3 - 1), dz = -1, 0, 1 of each node, so every
+// thread has one round of 36 independent loads (coarse levels are small and
+// latency-bound: with 9 neighbours per thread the loads did not fit in
+// flight), and wave 0 adds the nine partial sums in a fixed order.
 constexpr int kStNodes = 64;
-constexpr int kStBlock = 3 * kStNodes;
+constexpr int kStWaves = 9;
+constexpr int kStBlock = kStWaves * kStNodes;
 
 template <typename T, int MODE>
 __global__ __launch_bounds__(kStBlock) void ku_stencil_k(Grid g, const T *__restrict__ S, const T *__restrict__ u,
@@ -84,7 +99,7 @@ __global__ __launch_bounds__(kStBlock) void ku_stencil_k(Grid g, const T *__rest
 {
     if (MODE == MODE_JAC || MODE == MODE_JAC0)
         omega = jacobi_weight(g, omega);
-    __shared__ T psum[3][3][kStNodes];
+    __shared__ T psum[kStWaves][3][kStNodes];
     const int ln = threadIdx.x % kStNodes, ty = threadIdx.x / kStNodes;
     const long n = (long)blockIdx.x * kStNodes + ln;
     const bool valid = n < g.n_nodes;
@@ -94,14 +109,14 @@ __global__ __launch_bounds__(kStBlock) void ku_stencil_k(Grid g, const T *__rest
     const int i = (int)(q % g.nx), j = (int)(q / g.nx);
     T acc[3] = {T(0), T(0), T(0)};
     if (valid) {
-        const int dy = ty - 1;
+        const int dy = ty / 3 - 1, dx = ty % 3 - 1;
 #pragma unroll
-        for (int tq = 0; tq < 9; ++tq) {
-            const int t = ty * 9 + tq;
-            const int dx = tq / 3 - 1, dz = tq % 3 - 1;
+        for (int tq = 0; tq < 3; ++tq) {
+            const int t = ty * 3 + tq;   // wave-uniform
+            const int dz = tq - 1;
             const int mi = i + dx, mj = j + dy, mk = k + dz;
-            // outside the grid the coefficients are 0 by construction; read
-            // the node itself so that the loads issue unconditionally
+            // outside the grid the neighbour value is 0; read the node itself
+            // so that the loads issue unconditionally
             const bool in = mi >= 0 && mi < g.nx && mj >= 0 && mj < g.ny && mk >= 0 && mk < g.nz;
             const long m = in ? ((long)mj * g.nx + mi) * g.nz + mk : nn;
             T u0, u1, u2;
@@ -114,11 +129,39 @@ __global__ __launch_bounds__(kStBlock) void ku_stencil_k(Grid g, const T *__rest
                 u1 = u[3 * m + 1];
                 u2 = u[3 * m + 2];
             }
-            const T *St = S + (long)t * 9 * g.n_nodes + nn;
+            if (!in)
+                u0 = u1 = u2 = T(0);
+            // forward block t of this node, or forward block 26 - t of the
+            // neighbour, transposed
+            const bool own = t >= 13;
+            const T *St = S + st_at((own ? t - 13 : 13 - t) * 9, own ? nn : m);
+            T s[9];
 #pragma unroll
-            for (int c = 0; c < 3; ++c)
-                acc[c] += St[(3 * c) * g.n_nodes] * u0 + St[(3 * c + 1) * g.n_nodes] * u1 +
-                          St[(3 * c + 2) * g.n_nodes] * u2;
+            for (int e = 0; e < 9; ++e)
+                s[e] = St[e * kStTile];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const T a0 = own ? s[3 * c] : s[c];
+                const T a1 = own ? s[3 * c + 1] : s[3 + c];
+                const T a2 = own ? s[3 * c + 2] : s[6 + c];
+                acc[c] += a0 * u0 + a1 * u1 + a2 * u2;
+            }
+        }
+    }
+    // the epilogue's own-node operands, loaded by wave 0 before the barrier
+    constexpr bool kNeedU = MODE == MODE_JAC || MODE == MODE_KUDOT;
+    constexpr bool kNeedB = MODE == MODE_JAC || MODE == MODE_JAC0 || MODE == MODE_RES;
+    constexpr bool kNeedD = MODE == MODE_JAC || MODE == MODE_JAC0;
+    T ur[3] = {T(0), T(0), T(0)}, br[3] = {T(0), T(0), T(0)}, dr[3] = {T(0), T(0), T(0)};
+    if (ty == 0 && valid) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            if (kNeedU)
+                ur[c] = u[3 * n + c];
+            if (kNeedB)
+                br[c] = bvec[3 * n + c];
+            if (kNeedD)
+                dr[c] = dinv[3 * n + c];
         }
     }
 #pragma unroll
@@ -129,22 +172,24 @@ __global__ __launch_bounds__(kStBlock) void ku_stencil_k(Grid g, const T *__rest
     if (ty == 0 && valid) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            const T kuv = psum[0][c][ln] + psum[1][c][ln] + psum[2][c][ln];
-            const long r = 3 * n + c;
+            T kuv = psum[0][c][ln];
+#pragma unroll
+            for (int w = 1; w < kStWaves; ++w)
+                kuv += psum[w][c][ln];
             T o;
             if (MODE == MODE_JAC)
-                o = u[r] + omega * dinv[r] * (bvec[r] - kuv);
+                o = ur[c] + omega * dr[c] * (br[c] - kuv);
             else if (MODE == MODE_JAC0) {   // x0 = omega D^-1 b, then one sweep of weight omega * w2_ratio
-                const T x0 = omega * dinv[r] * bvec[r];
-                o = x0 + (omega * (T)g.w2_ratio) * dinv[r] * (bvec[r] - kuv);
+                const T x0 = omega * dr[c] * br[c];
+                o = x0 + (omega * (T)g.w2_ratio) * dr[c] * (br[c] - kuv);
             }
             else if (MODE == MODE_RES)
-                o = bvec[r] - kuv;
+                o = br[c] - kuv;
             else
                 o = kuv;
-            y[r] = o;
+            y[3 * n + c] = o;
             if (MODE == MODE_KUDOT)
-                part += (double)u[r] * (double)o;
+                part += (double)ur[c] * (double)o;
         }
     }
     if (MODE == MODE_KUDOT) {
@@ -162,15 +207,17 @@ __global__ void dinv_stencil_k(Grid g, const T *__restrict__ S, T *__restrict__ 
     if (n >= g.n_nodes)
         return;
     for (int c = 0; c < 3; ++c)
-        d[3 * n + c] = T(1) / S[(long)(13 * 9 + 4 * c) * g.n_nodes + n];
+        d[3 * n + c] = T(1) / S[st_at(4 * c, n)];
 }
 
 long stencil_partials(const Grid &g) { return (g.n_nodes + kStNodes - 1) / kStNodes; }
 
+long stencil_size(const Grid &g) { return (g.n_nodes + kStTile - 1) / kStTile * kStTile * FANTOM_STENCIL_COEFS; }
+
 template <typename TS>
 static int stencil_build_t(const Grid &g, const double *Ke, const uint8_t *mask, TS *S, hipStream_t s)
 {
-    const long n = g.n_nodes * 27;
+    const long n = g.n_nodes * 14;
     hipLaunchKernelGGL(stencil_from_elem_k<TS>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, g, Ke, mask, S);
     FANTOM_LAUNCH_CHECK("stencil_from_elem");
     return kStatusOk;
@@ -259,6 +306,13 @@ extern "C" int fantom_stencil_ku_f64(int mode, int nelx, int nely, int nelz, con
     FANTOM_CHECK_ARG(mode != MODE_KUDOT || partials, "mode needs partials");
     return launch_stencil_ku(mode, make_grid(nelx, nely, nelz), S, u, y, b, dinv, omega, partials,
                              as_stream(stream));
+}
+
+extern "C" long fantom_stencil_size(int nelx, int nely, int nelz)
+{
+    if (!grid_ok(nelx, nely, nelz) || nelz <= 0)
+        return -1;
+    return stencil_size(make_grid(nelx, nely, nelz));
 }
 
 extern "C" long fantom_stencil_partials(int nelx, int nely, int nelz)

@@ -65,7 +65,18 @@ def test_binding_arity_and_kinds_match_header():
 
 def test_abi_version():
     from pyfantom_amd import _lib
-    assert _lib.load().fantom_abi_version() == 2
+    assert _lib.load().fantom_abi_version() == 3
+
+
+def test_stencil_size():
+    """The stencil holds 126 coefficients per node in tiles of 64 nodes (host
+    arithmetic, no GPU); bad grids return -1."""
+    from pyfantom_amd import _lib
+    lib = _lib.load()
+    for nel in ((16, 8, 8), (64, 32, 32), (1, 1, 1)):
+        nn = int(np.prod([n + 1 for n in nel]))
+        assert lib.fantom_stencil_size(*nel) == -(-nn // 64) * 64 * 126
+    assert lib.fantom_stencil_size(16, 8, 0) == -1
 
 
 def test_hex8_blocks_reconstruct(oracle):

@@ -461,7 +461,7 @@ def test_stencil_levels(oracle, nel):
     Ke = torch.empty(int(np.prod(g2)) * 576, dtype=torch.float64, device="cuda")
     _lib.call("fantom_gal_build_f64", *g2, 0, Gd_d.data_ptr(), Ed.data_ptr(), None, Ke.data_ptr())
     m2 = _dev(_mask_from_con(cons[2], 3))
-    S = torch.empty(nn2 * 243, dtype=torch.float64, device="cuda")
+    S = torch.empty(_lib.load().fantom_stencil_size(*g2), dtype=torch.float64, device="cuda")
     _lib.call("fantom_stencil_build_f64", *g2, Ke.data_ptr(), m2.data_ptr(), S.data_ptr())
     u = rng.standard_normal(nn2 * 3)
     b = rng.standard_normal(nn2 * 3)
