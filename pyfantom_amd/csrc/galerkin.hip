@@ -21,6 +21,8 @@
 // fine moduli are read.
 // Levels >= 2 store explicit 24x24 element matrices (natural corner order
 // b = bx + 2by + 4bz), built once per setup from the level below.
+#include <algorithm>
+#include <array>
 #include <cmath>
 #include <vector>
 
@@ -474,13 +476,36 @@ __global__ void dense_from_elem(Grid g, const double *__restrict__ Ke, const uin
             }
 }
 
-long gal1_partials(const Grid &g) { return (long)tile_blocks(tile_shape(g, kGal1PerCU)); }
+// Workgroups per CU the chunk choice assumes for each mode of the level-1
+// operator: the occupancy of that mode's instantiation (FP64 2, FP32 4 at
+// 116-123 VGPRs), the register budget without a device.
+template <typename T>
+static int gal1_per_cu(int mode)
+{
+    static const std::array<int, 5> pc = [] {
+        std::array<int, 5> a = {tile_occupancy<T, 0, Gal1Op<T>>(), tile_occupancy<T, 1, Gal1Op<T>>(),
+                                tile_occupancy<T, 2, Gal1Op<T>>(), tile_occupancy<T, 3, Gal1Op<T>>(),
+                                tile_occupancy<T, 4, Gal1Op<T>>()};
+        for (int &v : a)
+            if (v <= 0)
+                v = Gal1Op<T>::kWavesPerEU;   // a workgroup is one wave per SIMD
+        return a;
+    }();
+    return pc[mode];
+}
+
+// partial sums of the one mode that writes them (K.u + u.Ku), either precision
+long gal1_partials(const Grid &g)
+{
+    return (long)std::max(tile_blocks(tile_shape(g, gal1_per_cu<double>(MODE_KUDOT))),
+                          tile_blocks(tile_shape(g, gal1_per_cu<float>(MODE_KUDOT))));
+}
 
 template <typename T>
 static int gal1_launch(int mode, const Grid &g, const T *Btab, const T *Ef, const uint8_t *mask, const T *u,
                        T *y, const T *b, const T *dinv, double omega, double *partials, hipStream_t s)
 {
-    const TileShape ts = tile_shape(g, kGal1PerCU);
+    const TileShape ts = tile_shape(g, gal1_per_cu<T>(mode >= 0 && mode < 5 ? mode : 0));   // bad modes fail below
     const unsigned blocks = tile_blocks(ts);
     const T om = (T)omega;
     Gal1Op<T> op;

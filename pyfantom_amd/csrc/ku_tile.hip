@@ -13,6 +13,8 @@
 // (the three zero rows are the rigid translations).  The blocks are derived
 // on the host (fantom_hex8_blocks) and the pattern is checked before packing.
 // Gather/scatter and the shared WHT stages live in tile_skel.hpp.
+#include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdlib>
 
@@ -189,8 +191,30 @@ unsigned tile_blocks(const TileShape &ts)
     return (unsigned)((n + 7) / 8 * 8);
 }
 
-long ku_tile_partials(const Grid &g) { return (long)tile_blocks(tile_shape(g, kFineF64PerCU)); }
-long ku_tile_partials_f32(const Grid &g) { return (long)tile_blocks(tile_shape(g, kFineF32PerCU)); }
+// Workgroups per CU the chunk choice assumes for each mode of the fine
+// operator: the occupancy of that mode's instantiation (register use differs
+// by mode: FP32 84-96 VGPRs, 5-6 waves per SIMD), the register budget
+// without a device.  The two modes that write per-block partial sums share
+// one value, so that one partial count serves both.
+template <typename T>
+static int fine_per_cu(int mode)
+{
+    static const std::array<int, 8> pc = [] {
+        std::array<int, 8> a = {tile_occupancy<T, 0, FineOp<T>>(), tile_occupancy<T, 1, FineOp<T>>(),
+                                tile_occupancy<T, 2, FineOp<T>>(), tile_occupancy<T, 3, FineOp<T>>(),
+                                tile_occupancy<T, 4, FineOp<T>>(), tile_occupancy<T, 5, FineOp<T>>(),
+                                tile_occupancy<T, 6, FineOp<T>>(), tile_occupancy<T, 7, FineOp<T>>()};
+        for (int &v : a)
+            if (v <= 0)
+                v = FineOp<T>::kWavesPerEU;   // a workgroup is one wave per SIMD
+        a[MODE_KUDOT] = a[MODE_JACDOT] = std::min(a[MODE_KUDOT], a[MODE_JACDOT]);
+        return a;
+    }();
+    return pc[mode];
+}
+
+long ku_tile_partials(const Grid &g) { return (long)tile_blocks(tile_shape(g, fine_per_cu<double>(MODE_KUDOT))); }
+long ku_tile_partials_f32(const Grid &g) { return (long)tile_blocks(tile_shape(g, fine_per_cu<float>(MODE_KUDOT))); }
 
 template <typename T>
 static int launch_ku_tile_t(int mode, const Grid &g, const double *Kb, const T *E, const uint8_t *mask, const T *u,
@@ -213,7 +237,7 @@ static int launch_ku_tile_t(int mode, const Grid &g, const double *Kb, const T *
     op.E = E;
     if (mode == MODE_JAC && dinv == nullptr)
         mode = MODE_JACOWN;
-    const TileShape ts = tile_shape(g, sizeof(T) == 8 ? kFineF64PerCU : kFineF32PerCU);
+    const TileShape ts = tile_shape(g, fine_per_cu<T>(mode >= 0 && mode < 8 ? mode : 0));   // bad modes fail below
     const unsigned blocks = tile_blocks(ts);
     const T om = (T)omega;
 #define FANTOM_TILE_CASE(M)                                                                                   \

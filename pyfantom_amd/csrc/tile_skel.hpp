@@ -55,12 +55,6 @@ struct TileShape {
 TileShape tile_shape(const Grid &g, int per_cu);
 unsigned tile_blocks(const TileShape &ts);
 
-// resident workgroups per CU of the tile operators, from their register use
-// (FP64 fine 154 VGPRs -> 3 waves/SIMD; FP32 fine 76 -> 6; level-1 Galerkin
-// 190-224 -> 2); only the chunk choice depends on them
-constexpr int kFineF64PerCU = 3;
-constexpr int kFineF32PerCU = 6;
-constexpr int kGal1PerCU = 2;
 
 // DPP moves inside rows of 16 lanes; a lane whose source lies outside its row
 // reads 0 (bound_ctrl).
@@ -409,6 +403,23 @@ __global__ __launch_bounds__(kTThreads) __attribute__((amdgpu_waves_per_eu(Op::k
         if (threadIdx.x == 0)
             partials[blockIdx.x] = s;
     }
+}
+
+// Host: resident workgroups per CU of ku_tile_kernel<T, MODE, Op> from the
+// occupancy API (registers and LDS of that instantiation), 0 without a
+// device.  The chunk choice (tile_shape) depends on it: with a count above
+// what fits, the last round of workgroups runs nearly empty; below, the chip
+// is filled with fewer waves than it could hold.
+template <typename T, int MODE, class Op>
+int tile_occupancy()
+{
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void *>(&ku_tile_kernel<T, MODE, Op>),
+                                                     kTThreads, 0) != hipSuccess) {
+        (void)hipGetLastError();   // leave no error for the next launch check
+        return 0;
+    }
+    return nb;
 }
 
 }  // namespace fantom
