@@ -36,12 +36,14 @@ struct FineOp {
     // resident waves per SIMD the kernels are built for (FP64 152-169 VGPRs:
     // 3; FP32 76-95: 5)
     static constexpr int kWavesPerEU = sizeof(T) == 8 ? 3 : 5;
-    // staged row stores where their registers fit without spills (not in
-    // the FP64 sweeps that form D^-1 from the moduli and also sum b.x or start
-    // from zero, nor in any FP32 sweep that forms D^-1)
+    // staged row stores where they pay (measured per mode, 256x128x128,
+    // HBM-resident): not in the Jacobi + b.x sweeps (slower by 2-3 us: they
+    // spill 3-4 VGPRs with them), nor in the FP32 sweeps that form D^-1 from
+    // the moduli (no gain, or spills inside the layer loop).  The FP64
+    // zero-guess sweep gains 7 us with them, its spill lying outside the loop.
     static constexpr bool staged_store(int mode)
     {
-        return sizeof(T) == 8 ? (mode != MODE_JACDOT && mode != MODE_JAC0OWN)
+        return sizeof(T) == 8 ? mode != MODE_JACDOT
                               : (mode != MODE_JACDOT && mode != MODE_JACOWN && mode != MODE_JAC0OWN);
     }
     // apply leaves out the element modulus; the skeleton multiplies it in
