@@ -91,6 +91,23 @@ int fantom_ku_jacobi_f64(int kernel, int nelx, int nely, int nelz, const double 
 int fantom_ku_jacobi0_f64(int kernel, int nelx, int nely, int nelz, const double *K0, const double *E,
                           const uint8_t *mask, const double *b, const double *dinv, double omega, double *x_out,
                           void *stream);
+/* No single counterpart in the reference: its prolongation
+ * (apply_prolongation_cuda, core/CUDA/_mgm.py:220, called at
+ * solvers/CUDA/_solvers.py:541) and the first post-smoothing sweep
+ * (solvers/CUDA/_solvers.py:437-440, called at :545) in one pass, for
+ * callers that drive a V-cycle through this ABI.  x_out = v + omega D^-1 (b - K v) with v = x_in + P ec: P is the
+ * prolongation of fantom_prolong_f64 with normalize 0 (coarse node c on fine
+ * node 2c, constrained fine DOFs receive nothing), D^-1 is formed from E as
+ * in fantom_ku_jacobi_f64 with dinv NULL, and v is never stored.  Kb is the
+ * HOST block form (fantom_hex8_blocks); hex8 grids only (status 1 for 2D);
+ * x_out must not alias x_in or ec.  The sums of P may round differently from
+ * fantom_prolong_f64 in the last bit. */
+int fantom_ku_jacobi_prolong_f64(int nelx, int nely, int nelz, const double *Kb, const double *E,
+                                 const uint8_t *mask, const double *b, double omega, const double *x_in,
+                                 const double *ec, double *x_out, void *stream);
+int fantom_ku_jacobi_prolong_f32(int nelx, int nely, int nelz, const double *Kb, const float *E,
+                                 const uint8_t *mask, const float *b, double omega, const float *x_in,
+                                 const float *ec, float *x_out, void *stream);
 /* Residual fused into the operator: r = b - K x (solvers/CUDA/_solvers.py:538). */
 int fantom_ku_residual_f64(int kernel, int nelx, int nely, int nelz, const double *K0, const double *E,
                            const uint8_t *mask, const double *b, const double *x, double *r, void *stream);

@@ -33,6 +33,10 @@ SIGNATURES = {
     "fantom_ku_f32": (_c_int, [_c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
     "fantom_ku_jacobi_f64": (_c_int, [_c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _c_double, _vp,
                                       _vp, _vp]),
+    "fantom_ku_jacobi_prolong_f64": (_c_int, [_c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _c_double, _vp, _vp, _vp,
+                                              _vp]),
+    "fantom_ku_jacobi_prolong_f32": (_c_int, [_c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _c_double, _vp, _vp, _vp,
+                                              _vp]),
     "fantom_ku_jacobi0_f64": (_c_int, [_c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _c_double, _vp, _vp]),
     "fantom_ku_residual_f64": (_c_int, [_c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "fantom_ku_dot_f64": (_c_int, [_c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),

@@ -102,6 +102,30 @@ int fantom_ku_jacobi_f64(int kernel, int nelx, int nely, int nelz, const double 
     return ku_dispatch(kernel, MODE_JAC, nelx, nely, nelz, K0, E, mask, x_in, x_out, b, dinv, omega, nullptr, stream);
 }
 
+int fantom_ku_jacobi_prolong_f64(int nelx, int nely, int nelz, const double *Kb, const double *E,
+                                 const uint8_t *mask, const double *b, double omega, const double *x_in,
+                                 const double *ec, double *x_out, void *stream)
+{
+    GRID_OR_RETURN(nelx, nely, nelz);
+    FANTOM_CHECK_ARG(nelz > 0, "the prolongating sweep is implemented for hex8 grids (nelz > 0)");
+    FANTOM_CHECK_ARG(Kb && E && b && x_in && ec && x_out, "null operand");
+    FANTOM_CHECK_ARG(x_in != x_out && ec != x_out, "x_out must not alias x_in or ec");
+    return launch_ku_tile_pro(make_grid(nelx, nely, nelz), Kb, E, mask, x_in, ec, x_out, b, omega,
+                              as_stream(stream));
+}
+
+int fantom_ku_jacobi_prolong_f32(int nelx, int nely, int nelz, const double *Kb, const float *E,
+                                 const uint8_t *mask, const float *b, double omega, const float *x_in,
+                                 const float *ec, float *x_out, void *stream)
+{
+    GRID_OR_RETURN(nelx, nely, nelz);
+    FANTOM_CHECK_ARG(nelz > 0, "the prolongating sweep is implemented for hex8 grids (nelz > 0)");
+    FANTOM_CHECK_ARG(Kb && E && b && x_in && ec && x_out, "null operand");
+    FANTOM_CHECK_ARG(x_in != x_out && ec != x_out, "x_out must not alias x_in or ec");
+    return launch_ku_tile_pro_f32(make_grid(nelx, nely, nelz), Kb, E, mask, x_in, ec, x_out, b, omega,
+                                  as_stream(stream));
+}
+
 int fantom_ku_jacobi0_f64(int kernel, int nelx, int nely, int nelz, const double *K0, const double *E,
                           const uint8_t *mask, const double *b, const double *dinv, double omega, double *x_out,
                           void *stream)

@@ -23,7 +23,12 @@ enum {
     MODE_JAC0 = 4,
     MODE_JACDOT = 5,
     MODE_JACOWN = 6,
-    MODE_JAC0OWN = 7
+    MODE_JAC0OWN = 7,
+    // MODE_JACOWN on u + P e (hex8 tile kernels only): the first
+    // post-smoothing sweep of the fine level with the prolongation of the
+    // coarse correction e (the transfers' trilinear P, normalize 0) formed
+    // as the node values load, so that u + P e is never stored
+    MODE_JACPRO = 8
 };
 
 // Adaptive Jacobi weight of the reference's MultiGrid.solve
@@ -82,6 +87,11 @@ int launch_ku_tile(int mode, const Grid &g, const double *Kb, const double *E, c
 int launch_ku_tile_f32(int mode, const Grid &g, const double *Kb, const float *E, const uint8_t *mask,
                        const float *u, float *y, const float *b, const float *dinv, double omega, double *partials,
                        hipStream_t s);
+// MODE_JACPRO: y = one Jacobi sweep (D^-1 from the moduli) on u + P ec
+int launch_ku_tile_pro(const Grid &g, const double *Kb, const double *E, const uint8_t *mask, const double *u,
+                       const double *ec, double *y, const double *b, double omega, hipStream_t s);
+int launch_ku_tile_pro_f32(const Grid &g, const double *Kb, const float *E, const uint8_t *mask, const float *u,
+                           const float *ec, float *y, const float *b, double omega, hipStream_t s);
 long ku_tile_partials(const Grid &g);       // FP64 operator
 long ku_tile_partials_f32(const Grid &g);   // FP32 operator
 

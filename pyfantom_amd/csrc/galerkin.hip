@@ -67,6 +67,7 @@ struct Gal1Op {
     // resident waves per SIMD the kernels are built for (FP64 192-224 VGPRs:
     // 2; FP32 104-120: 4)
     static constexpr int kWavesPerEU = sizeof(T) == 8 ? 2 : 4;
+    static constexpr int waves_for(int) { return kWavesPerEU; }
     static constexpr bool staged_store(int mode) { return true; }
     static constexpr bool kScaled = false;
     static constexpr bool kDiagFromModuli = false;   // the Galerkin diagonal is read (dinv_gal1)
@@ -486,9 +487,9 @@ static int gal1_per_cu(int mode)
         std::array<int, 5> a = {tile_occupancy<T, 0, Gal1Op<T>>(), tile_occupancy<T, 1, Gal1Op<T>>(),
                                 tile_occupancy<T, 2, Gal1Op<T>>(), tile_occupancy<T, 3, Gal1Op<T>>(),
                                 tile_occupancy<T, 4, Gal1Op<T>>()};
-        for (int &v : a)
-            if (v <= 0)
-                v = Gal1Op<T>::kWavesPerEU;   // a workgroup is one wave per SIMD
+        for (int m = 0; m < (int)a.size(); ++m)
+            if (a[m] <= 0)
+                a[m] = Gal1Op<T>::waves_for(m);   // a workgroup is one wave per SIMD
         return a;
     }();
     return pc[mode];
@@ -515,7 +516,7 @@ static int gal1_launch(int mode, const Grid &g, const T *Btab, const T *Ef, cons
 #define GAL_CASE(M)                                                                                           \
     case M:                                                                                                   \
         hipLaunchKernelGGL((ku_tile_kernel<T, M, Gal1Op<T>>), dim3(blocks), dim3(kTThreads), 0, s, g, op, mask, u, \
-                           y, b, dinv, om, partials, ts);                                                     \
+                           y, b, dinv, om, partials, ts, nullptr);                                            \
         break;
     switch (mode) {
         GAL_CASE(MODE_KU)

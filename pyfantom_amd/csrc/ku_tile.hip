@@ -36,6 +36,10 @@ struct FineOp {
     // resident waves per SIMD the kernels are built for (FP64 152-169 VGPRs:
     // 3; FP32 76-95: 5)
     static constexpr int kWavesPerEU = sizeof(T) == 8 ? 3 : 5;
+    // the prolongating sweep at the budget above spills 10-19 values inside
+    // its layer loop; one wave fewer and no spills measured 127 / 67 us
+    // against 180 / 94 us (FP64 / FP32, 256x128x128)
+    static constexpr int waves_for(int mode) { return mode == MODE_JACPRO ? kWavesPerEU - 1 : kWavesPerEU; }
     // staged row stores where they pay (measured per mode, 256x128x128,
     // HBM-resident): not in the Jacobi + b.x sweeps (slower by 2-3 us: they
     // spill 3-4 VGPRs with them), nor in the FP32 sweeps that form D^-1 from
@@ -43,8 +47,9 @@ struct FineOp {
     // zero-guess sweep gains 7 us with them, its spill lying outside the loop.
     static constexpr bool staged_store(int mode)
     {
-        return sizeof(T) == 8 ? mode != MODE_JACDOT
-                              : (mode != MODE_JACDOT && mode != MODE_JACOWN && mode != MODE_JAC0OWN);
+        return sizeof(T) == 8 ? (mode != MODE_JACDOT && mode != MODE_JACPRO)
+                              : (mode != MODE_JACDOT && mode != MODE_JACOWN && mode != MODE_JAC0OWN &&
+                                 mode != MODE_JACPRO);
     }
     // apply leaves out the element modulus; the skeleton multiplies it in
     static constexpr bool kScaled = true;
@@ -201,14 +206,15 @@ unsigned tile_blocks(const TileShape &ts)
 template <typename T>
 static int fine_per_cu(int mode)
 {
-    static const std::array<int, 8> pc = [] {
-        std::array<int, 8> a = {tile_occupancy<T, 0, FineOp<T>>(), tile_occupancy<T, 1, FineOp<T>>(),
+    static const std::array<int, 9> pc = [] {
+        std::array<int, 9> a = {tile_occupancy<T, 0, FineOp<T>>(), tile_occupancy<T, 1, FineOp<T>>(),
                                 tile_occupancy<T, 2, FineOp<T>>(), tile_occupancy<T, 3, FineOp<T>>(),
                                 tile_occupancy<T, 4, FineOp<T>>(), tile_occupancy<T, 5, FineOp<T>>(),
-                                tile_occupancy<T, 6, FineOp<T>>(), tile_occupancy<T, 7, FineOp<T>>()};
-        for (int &v : a)
-            if (v <= 0)
-                v = FineOp<T>::kWavesPerEU;   // a workgroup is one wave per SIMD
+                                tile_occupancy<T, 6, FineOp<T>>(), tile_occupancy<T, 7, FineOp<T>>(),
+                                tile_occupancy<T, 8, FineOp<T>>()};
+        for (int m = 0; m < (int)a.size(); ++m)
+            if (a[m] <= 0)
+                a[m] = FineOp<T>::waves_for(m);   // a workgroup is one wave per SIMD
         a[MODE_KUDOT] = a[MODE_JACDOT] = std::min(a[MODE_KUDOT], a[MODE_JACDOT]);
         return a;
     }();
@@ -218,9 +224,12 @@ static int fine_per_cu(int mode)
 long ku_tile_partials(const Grid &g) { return (long)tile_blocks(tile_shape(g, fine_per_cu<double>(MODE_KUDOT))); }
 long ku_tile_partials_f32(const Grid &g) { return (long)tile_blocks(tile_shape(g, fine_per_cu<float>(MODE_KUDOT))); }
 
+// ec: the coarse correction of MODE_JACPRO (coarse node c on fine node 2c),
+// unused by the other modes
 template <typename T>
 static int launch_ku_tile_t(int mode, const Grid &g, const double *Kb, const T *E, const uint8_t *mask, const T *u,
-                            T *y, const T *b, const T *dinv, double omega, double *partials, hipStream_t s)
+                            T *y, const T *b, const T *dinv, double omega, double *partials, hipStream_t s,
+                            const T *ec = nullptr)
 {
     double packed[kPackedBlocks];
     const int st = pack_blocks(Kb, packed);
@@ -239,13 +248,13 @@ static int launch_ku_tile_t(int mode, const Grid &g, const double *Kb, const T *
     op.E = E;
     if (mode == MODE_JAC && dinv == nullptr)
         mode = MODE_JACOWN;
-    const TileShape ts = tile_shape(g, fine_per_cu<T>(mode >= 0 && mode < 8 ? mode : 0));   // bad modes fail below
+    const TileShape ts = tile_shape(g, fine_per_cu<T>(mode >= 0 && mode <= MODE_JACPRO ? mode : 0));   // bad modes fail below
     const unsigned blocks = tile_blocks(ts);
     const T om = (T)omega;
 #define FANTOM_TILE_CASE(M)                                                                                   \
     case M:                                                                                                   \
         hipLaunchKernelGGL((ku_tile_kernel<T, M, FineOp<T>>), dim3(blocks), dim3(kTThreads), 0, s, g, op, mask, u, \
-                           y, b, dinv, om, partials, ts);                                                     \
+                           y, b, dinv, om, partials, ts, ec);                                                 \
         break;
     switch (mode) {
         FANTOM_TILE_CASE(MODE_KU)
@@ -256,6 +265,7 @@ static int launch_ku_tile_t(int mode, const Grid &g, const double *Kb, const T *
         FANTOM_TILE_CASE(MODE_JACDOT)
         FANTOM_TILE_CASE(MODE_JACOWN)
         FANTOM_TILE_CASE(MODE_JAC0OWN)
+        FANTOM_TILE_CASE(MODE_JACPRO)
     default:
         set_error("unknown matvec mode %d", mode);
         return kStatusBadArg;
@@ -277,6 +287,18 @@ int launch_ku_tile_f32(int mode, const Grid &g, const double *Kb, const float *E
                        hipStream_t s)
 {
     return launch_ku_tile_t<float>(mode, g, Kb, E, mask, u, y, b, dinv, omega, partials, s);
+}
+
+int launch_ku_tile_pro(const Grid &g, const double *Kb, const double *E, const uint8_t *mask, const double *u,
+                       const double *ec, double *y, const double *b, double omega, hipStream_t s)
+{
+    return launch_ku_tile_t<double>(MODE_JACPRO, g, Kb, E, mask, u, y, b, nullptr, omega, nullptr, s, ec);
+}
+
+int launch_ku_tile_pro_f32(const Grid &g, const double *Kb, const float *E, const uint8_t *mask, const float *u,
+                           const float *ec, float *y, const float *b, double omega, hipStream_t s)
+{
+    return launch_ku_tile_t<float>(MODE_JACPRO, g, Kb, E, mask, u, y, b, nullptr, omega, nullptr, s, ec);
 }
 
 }  // namespace fantom

@@ -220,8 +220,10 @@ static bool lev_has_jac0(const MgHandle *h, int l)
 // (optional) receives the last sweep instead of the ping-pong buffer.  With
 // rz (level 0 on the tiled 3D kernel only) that last sweep also leaves the
 // block partials of b.final in h->partials and *rz is set.
+// pro (optional): a coarse correction to prolongate onto x first, which the
+// first sweep (not the last) does as it loads x (lev_pro_ok)
 static int mg_smooth(MgHandle *h, int l, const double *b, double **px, bool from_zero, double *final = nullptr,
-                     bool *rz = nullptr)
+                     bool *rz = nullptr, const double *pro = nullptr)
 {
     MgLevel &L = h->lev[l];
     double *x = *px;
@@ -242,7 +244,9 @@ static int mg_smooth(MgHandle *h, int l, const double *b, double **px, bool from
         }
         const bool last = final && s == h->n_smooth - 1;
         double *dst = last ? final : other;
-        if (last && rz && l == 0 && h->kernel == FANTOM_KU_TILE && L.dim == 3) {
+        if (pro && s == 0) {
+            MG_TRY(launch_ku_tile_pro(L.g, L.Kb, L.E, L.mask, x, pro, dst, b, w(0), h->stream));
+        } else if (last && rz && l == 0 && h->kernel == FANTOM_KU_TILE && L.dim == 3) {
             MG_TRY(lev_ku(h, l, MODE_JACDOT, x, dst, b, L.dinv, w(s), h->partials));
             *rz = true;
         } else {
@@ -273,8 +277,13 @@ static int mg_cycle(MgHandle *h, int l, const double *b, double **out, double *f
     MG_TRY(launch_restrict(L.g, L.dim, L.dof, 0, C.mask, L.r, C.b, h->stream));
     double *e = nullptr;
     MG_TRY(mg_cycle(h, l + 1, C.b, &e));
-    MG_TRY(launch_prolong(L.g, L.dim, L.dof, 0, L.mask, e, x, 1, h->stream));
-    MG_TRY(mg_smooth(h, l, b, &x, false, final, rz));
+    // on the hex8 tile level the first post-smoothing sweep prolongates e
+    // as it loads x (x + P e is never stored); it must not be the last sweep,
+    // which may be the one that also sums r.z
+    const bool fuse = l == 0 && h->kernel == FANTOM_KU_TILE && L.dim == 3 && h->n_smooth >= 2;
+    if (!fuse)
+        MG_TRY(launch_prolong(L.g, L.dim, L.dof, 0, L.mask, e, x, 1, h->stream));
+    MG_TRY(mg_smooth(h, l, b, &x, false, final, rz, fuse ? e : nullptr));
     *out = x;
     return kStatusOk;
 }

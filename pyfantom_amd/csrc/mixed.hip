@@ -377,8 +377,10 @@ static int mx_ku(MxHandle *h, int l, int mode, const float *u, float *y, const f
 // n_smooth sweeps on x (*px); final (optional) receives the last sweep.
 // rz (optional, fine level): the last sweep also leaves the partial sums of
 // b.x (the PCG's r.z, b being the FP32 residual) and *rz is set.
+// pro (optional): a coarse correction to prolongate onto x first, which the
+// first sweep (not the last) does as it loads x
 static int mx_smooth(MxHandle *h, int l, const float *b, float **px, bool from_zero, float *final = nullptr,
-                     bool *rz = nullptr)
+                     bool *rz = nullptr, const float *pro = nullptr)
 {
     MxLevel &L = h->lev[l];
     float *x = *px;
@@ -401,7 +403,9 @@ static int mx_smooth(MxHandle *h, int l, const float *b, float **px, bool from_z
         }
         const bool last = final && s == h->n_smooth - 1;
         float *dst = last ? final : other;
-        if (last && rz && l == 0 && !(h->op_kind & 1)) {
+        if (pro && s == 0) {
+            MX_TRY(launch_ku_tile_pro_f32(L.g, L.Kb, L.E, L.mask, x, pro, dst, b, w(0), h->stream));
+        } else if (last && rz && l == 0 && !(h->op_kind & 1)) {
             MX_TRY(mx_ku(h, l, MODE_JACDOT, x, dst, b, L.dinv, h->partials, w(s)));
             *rz = true;
         } else {
@@ -431,8 +435,12 @@ static int mx_cycle(MxHandle *h, int l, const float *b, float **out, float *fina
     MX_TRY(launch_restrict_f32(L.g, 3, 3, 0, C.mask, L.r, C.b, h->stream));
     float *e = nullptr;
     MX_TRY(mx_cycle(h, l + 1, C.b, &e));
-    MX_TRY(launch_prolong_f32(L.g, 3, 3, 0, L.mask, e, x, 1, h->stream));
-    MX_TRY(mx_smooth(h, l, b, &x, false, final, rz));
+    // on the FP32 tile level the first post-smoothing sweep prolongates e as
+    // it loads x (x + P e is never stored); not on the matrix-core sweeps
+    const bool fuse = l == 0 && !(h->op_kind & 1) && h->n_smooth >= 2;
+    if (!fuse)
+        MX_TRY(launch_prolong_f32(L.g, 3, 3, 0, L.mask, e, x, 1, h->stream));
+    MX_TRY(mx_smooth(h, l, b, &x, false, final, rz, fuse ? e : nullptr));
     *out = x;
     return kStatusOk;
 }

@@ -82,7 +82,7 @@ __device__ __forceinline__ double dpp_row(double v)
 //   void apply(const Data &, const T (&w)[3][8], T (&z)[3][8]) const;
 //   static constexpr bool kScaled;  T scale(const Data &) const;
 //   static constexpr bool kDiagFromModuli;  T inv_diag(int c, T inv_sum) const;
-//   static constexpr int kWavesPerEU;
+//   static constexpr int kWavesPerEU;  static constexpr int waves_for(int mode);
 //   static constexpr bool staged_store(int mode);
 // where load fills Data for element (ex, ey, ez) (zero modulus when !active)
 // and apply maps WHT coefficients to output coefficients (the 1/8 of the
@@ -94,7 +94,9 @@ __device__ __forceinline__ double dpp_row(double v)
 // elements), and the Jacobi sweeps (not the zero-guess one, whose first half
 // needs D^-1 two planes ahead) form D^-1 = inv_diag(c, 1 / sum) from the
 // moduli they already hold instead of reading a D^-1 vector.  kWavesPerEU
-// is the occupancy the register budget is held to.  staged_store(MODE):
+// is the occupancy the register budget is held to, waves_for(MODE) the one a
+// mode is built for (lower where the budget would spill inside the layer
+// loop: one wave fewer measured faster than spilling).  staged_store(MODE):
 // the outputs of plane L go to an LDS row buffer in layer L and are stored in
 // layer L + 1 (after its barrier) as contiguous words per tile row, 14 x 3
 // values of one x-column, instead of three 8-byte stores at a 24-byte stride
@@ -102,15 +104,19 @@ __device__ __forceinline__ double dpp_row(double v)
 // and the residual from 97.4 to 87.6 us; where the extra registers spill (the
 // Jacobi modes) the operator keeps the direct stores.
 template <typename T, int MODE, class Op>
-__global__ __launch_bounds__(kTThreads) __attribute__((amdgpu_waves_per_eu(Op::kWavesPerEU, 8))) void ku_tile_kernel(Grid g, Op op, const uint8_t *__restrict__ mask,
+__global__ __launch_bounds__(kTThreads) __attribute__((amdgpu_waves_per_eu(Op::waves_for(MODE), 8))) void ku_tile_kernel(Grid g, Op op, const uint8_t *__restrict__ mask,
                                                           const T *__restrict__ u, T *__restrict__ y,
                                                           const T *__restrict__ bvec, const T *__restrict__ dinv,
-                                                          T omega, double *__restrict__ partials, TileShape ts)
+                                                          T omega, double *__restrict__ partials, TileShape ts,
+                                                          const T *__restrict__ ec)
 {
     constexpr bool kZeroGuess = MODE == MODE_JAC0 || MODE == MODE_JAC0OWN;
-    constexpr bool kJacobi = kZeroGuess || MODE == MODE_JAC || MODE == MODE_JACOWN || MODE == MODE_JACDOT;
-    constexpr bool kOwnDiag =
-        Op::kDiagFromModuli && (MODE == MODE_JACOWN || MODE == MODE_JACDOT || MODE == MODE_JAC0OWN);
+    constexpr bool kPro = MODE == MODE_JACPRO;   // node values u + P ec
+    constexpr bool kJacobi =
+        kZeroGuess || MODE == MODE_JAC || MODE == MODE_JACOWN || MODE == MODE_JACDOT || kPro;
+    constexpr bool kOwnDiag = Op::kDiagFromModuli && (MODE == MODE_JACOWN || MODE == MODE_JACDOT ||
+                                                      MODE == MODE_JAC0OWN || kPro);
+    static_assert(!kPro || Op::kDiagFromModuli, "the prolongating sweep forms D^-1 from the moduli");
     // right-hand side (and D^-1 unless formed here) read for the epilogue
     constexpr bool kEpiB = (kJacobi || MODE == MODE_RES) && MODE != MODE_JAC0OWN;
     if (kJacobi)
@@ -125,6 +131,11 @@ __global__ __launch_bounds__(kTThreads) __attribute__((amdgpu_waves_per_eu(Op::k
     __shared__ T xe[2][kOwnDiag ? kTThreads : 1];   // z-pair sums of the moduli, exchanged with column lx + 1
     constexpr bool kStaged = Op::staged_store(MODE);
     __shared__ T ys[2][kStaged ? kTOut : 1][3 * kTOut];   // staged outputs of a plane, one row per output x-column
+    // kPro: the coarse correction of a fine node plane, combined in y (the
+    // mean of the two coarse planes around an odd fine plane), over the 9 x 9
+    // coarse nodes under the tile, 3 components; three planes in a ring
+    constexpr int kProNodes = kTOut / 2 + 2;
+    __shared__ T cs[kPro ? 3 : 1][kPro ? kTThreads : 1];   // 243 of the 256 slots used
 
     // The hardware deals workgroups to the 8 XCDs round-robin; give each XCD
     // a contiguous range of logical tiles so that tiles sharing halo planes
@@ -216,7 +227,52 @@ __global__ __launch_bounds__(kTThreads) __attribute__((amdgpu_waves_per_eu(Op::k
         auto load_elem = [&](typename Op::Data &d, int L) {
             op.load(d, g, el_col && L >= 0 && L < g.nely, gx, L, gz);
         };
-
+        // kPro.  Coarse node c sits on fine node 2c; the tile's fine columns
+        // tx*14 - 1 ... tx*14 + 14 lie over coarse columns cx0 ... cx0 + 8
+        // (cz0 ... cz0 + 8 in z).  Staging thread tid < 243 holds component
+        // tid % 3 of coarse column (tid / 3) / 9, row (tid / 3) % 9; its
+        // offset within a coarse plane is fixed, -1 outside the grid.  Every
+        // access below is in range for every thread, so that none of it
+        // branches.
+        const int nxc = g.nelx / 2 + 1, nyc = g.nely / 2 + 1, nzc = g.nelz / 2 + 1;
+        const int cx0 = tx * (kTOut / 2) - 1, cz0 = tz * (kTOut / 2) - 1;
+        [[maybe_unused]] int pro_off = -1;
+        if constexpr (kPro) {
+            const int cc = tid / 3, ic = cx0 + cc / kProNodes, kc = cz0 + cc % kProNodes;
+            if (tid < kProNodes * kProNodes * 3 && ic >= 0 && ic < nxc && kc >= 0 && kc < nzc)
+                pro_off = 3 * (ic * nzc + kc) + tid % 3;
+        }
+        // pro_fetch: this thread's share of fine plane j's coarse values (the
+        // mean of coarse planes j / 2 and j / 2 + 1 for odd j), loaded now and
+        // written to LDS by pro_put later
+        auto pro_fetch = [&](int j) -> T {
+            const bool ok = pro_off >= 0 && j >= 0 && j < g.ny;
+            const int jc = ok ? j >> 1 : 0;
+            const bool two = ok && (j & 1) && jc + 1 < nyc;
+            const long plane = 3L * nxc * nzc;
+            const long i = ok ? jc * plane + pro_off : 0;
+            const T a = ec[i], b = ec[two ? i + plane : i];
+            return ok ? (two ? T(0.5) * (a + b) : a) : T(0);
+        };
+        auto pro_put = [&](int buf, T v) { cs[buf][tid] = v; };   // slots >= 243: unused
+        // v += (P e) at this thread's node of plane j, from ring buffer buf
+        // (bilinear in x and z over the staged plane; constrained DOFs get 0)
+        auto pro_add = [&](int j, T (&v)[3], int m, int buf) {
+            const bool ok = col_in && j >= 0 && j < g.ny;
+            const int ic0 = gx >> 1, kc0 = gz >> 1;
+            const bool two_x = (gx & 1) && ic0 + 1 < nxc, two_z = (gz & 1) && kc0 + 1 < nzc;
+            const T wx1 = two_x ? T(0.5) : T(0), wx0 = T(1) - wx1;
+            const T wz1 = two_z ? T(0.5) : T(0), wz0 = T(1) - wz1;
+            const int o00 = 3 * ((ic0 - cx0) * kProNodes + (kc0 - cz0));
+            const int o01 = o00 + (two_z ? 3 : 0), o10 = o00 + (two_x ? 3 * kProNodes : 0);
+            const int o11 = o10 + (two_z ? 3 : 0);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const T a = wz0 * cs[buf][o00 + c] + wz1 * cs[buf][o01 + c];
+                const T b = wz0 * cs[buf][o10 + c] + wz1 * cs[buf][o11 + c];
+                v[c] += (ok && !((m >> c) & 1)) ? wx0 * a + wx1 * b : T(0);
+            }
+        };
         T FpA[3][4], FpB[3][4], ucur[3], unx[3];
         int mcur, mnx;
         typename Op::Data dcur, dnx;
@@ -224,6 +280,16 @@ __global__ __launch_bounds__(kTThreads) __attribute__((amdgpu_waves_per_eu(Op::k
         load_node(jc0, unx, mnx, vd_nx);
         form_x0(ucur, vd_cur);
         form_x0(unx, vd_nx);
+        if constexpr (kPro) {
+            // ring buffer of plane P: (P - (jc0 - 1)) % 3
+            const T c0 = pro_fetch(jc0 - 1), c1 = pro_fetch(jc0), c2 = pro_fetch(jc0 + 1);
+            pro_put(0, c0);
+            pro_put(1, c1);
+            pro_put(2, c2);
+            __syncthreads();
+            pro_add(jc0 - 1, ucur, mcur, 0);
+            pro_add(jc0, unx, mnx, 1);
+        }
         load_elem(dcur, jc0 - 1);
         face_write(ucur, mcur, 0);
         __syncthreads();
@@ -250,6 +316,14 @@ __global__ __launch_bounds__(kTThreads) __attribute__((amdgpu_waves_per_eu(Op::k
             int mpf;
             load_node(L + 2 <= jc1 ? L + 2 : -1, upf, mpf, vd_pf);
             load_elem(dnx, L + 1 < jc1 ? L + 1 : -1);
+            // kPro: plane L + 3's coarse values go to ring buffer r % 3 before
+            // this layer's barrier; plane L + 2's were staged by the layer before
+            [[maybe_unused]] int pro_r = 0;
+            [[maybe_unused]] T pro_next = T(0);
+            if constexpr (kPro) {
+                pro_r = L - (jc0 - 1);
+                pro_next = pro_fetch(L + 3 <= jc1 ? L + 3 : -1);
+            }
             // right-hand side (and D^-1) of plane L, used by this layer's
             // epilogue after the barrier: loaded here rather than a layer
             // ahead, so that they occupy registers for one layer only
@@ -309,6 +383,10 @@ __global__ __launch_bounds__(kTThreads) __attribute__((amdgpu_waves_per_eu(Op::k
                 xr[rb][c][tid] = r1;
             }
             form_x0(upf, vd_pf);
+            if constexpr (kPro) {
+                pro_add(L + 2 <= jc1 ? L + 2 : -1, upf, mpf, (pro_r + 2) % 3);
+                pro_put(pro_r % 3, pro_next);
+            }
             face_write(upf, mpf, pb ^ 1);   // plane L + 2, read by the next layer
             if constexpr (kOwnDiag) {
                 const T e = op.scale(dcur);

@@ -100,10 +100,12 @@ def test_mma_device_control_repeatable():
             one = (ctypes.c_double * 1)(1e5)
             z1 = (ctypes.c_double * 1)(0.0)
             f1 = (ctypes.c_double * 1)(0.0)
-            _lib.call("fantom_mma_update_f64", 16, 1, 1, t.data_ptr(), None, None, 0.0, 1.0,
-                      (-t).contiguous().data_ptr(), f1, (t / 16).contiguous().data_ptr(), lo.data_ptr(),
-                      up.data_ptr(), 0.2, 1.0, z1, one, z1, 1e-7, 100, xx.data_ptr(), w2.data_ptr(),
-                      w2.numel() * 8, None)
+            # operands held in names: a temporary freed right after data_ptr()
+            # can hand its block to the next allocation before the launch
+            g0, g1 = (-t).contiguous(), (t / 16).contiguous()
+            _lib.call("fantom_mma_update_f64", 16, 1, 1, t.data_ptr(), None, None, 0.0, 1.0, g0.data_ptr(), f1,
+                      g1.data_ptr(), lo.data_ptr(), up.data_ptr(), 0.2, 1.0, z1, one, z1, 1e-7, 100, xx.data_ptr(),
+                      w2.data_ptr(), w2.numel() * 8, None)
             torch.cuda.synchronize()
     print(f"  steps {[o[1] for o in out]}", flush=True)
     for xo, so in out[1:]:

@@ -491,6 +491,45 @@ def test_stencil_levels(oracle, nel):
     assert abs(float(scal[5].item()) - u @ ref) <= 1e-12 * abs(u @ ref)
 
 
+@pytest.mark.parametrize("nel", SIZES_3D + [(64, 32, 40)])
+def test_jacobi_prolong_sweep(oracle, nel):
+    """The first post-smoothing sweep with the prolongation formed as x
+    loads: v = x + P ec (the oracle's prolongation, normalize 0, nothing on
+    constrained DOFs), then v + omega D^-1 (b - A v) with D^-1 = 1 / diag(A),
+    against the oracle's assembled A.  Odd element counts give fine nodes
+    with a missing parent; (64, 32, 40) runs several tiles and y chunks."""
+    from pyfantom_amd import _lib
+    from pyfantom_amd.stiffness import hex8_blocks
+    torch = _torch()
+    print(f"jacobi prolong {nel}", flush=True)
+    K0, el, E, x, con = _case(oracle, nel, bc="random", seed=11)
+    n = len(x)
+    rng = np.random.default_rng(12)
+    b = rng.standard_normal(n)
+    nc = int(np.prod([m // 2 + 1 for m in nel])) * 3
+    ec = rng.standard_normal(nc)
+    A = oracle.assemble(K0, el, E, n, con)
+    pe = oracle.prolong(ec, nel, 3, normalize=False)
+    pe[con] = 0.0
+    v = x + pe
+    ref = v + 0.6 * (b - A @ v) / A.diagonal()
+    kb = hex8_blocks(K0)
+    # device operands held in names: a temporary freed right after
+    # data_ptr() can hand its block to the next allocation before the launch
+    md = _dev(_mask_from_con(con, 3))
+    Ed, bd, xd, ecd = _dev(E), _dev(b), _dev(x), _dev(ec)
+    y = torch.empty(n, dtype=torch.float64, device="cuda")
+    _lib.call("fantom_ku_jacobi_prolong_f64", *nel, kb.ctypes.data, Ed.data_ptr(), md.data_ptr(), bd.data_ptr(),
+              0.6, xd.data_ptr(), ecd.data_ptr(), y.data_ptr())
+    assert _rel(_np(y), ref) <= 1e-13
+    f32 = torch.float32
+    E32, b32, x32, ec32 = _dev(E, f32), _dev(b, f32), _dev(x, f32), _dev(ec, f32)
+    y32 = torch.empty(n, dtype=f32, device="cuda")
+    _lib.call("fantom_ku_jacobi_prolong_f32", *nel, kb.ctypes.data, E32.data_ptr(), md.data_ptr(), b32.data_ptr(),
+              0.6, x32.data_ptr(), ec32.data_ptr(), y32.data_ptr())
+    assert _rel(_np(y32).astype(np.float64), ref) <= 2e-5
+
+
 @pytest.mark.parametrize("nel", [(8, 4, 4), (30, 14, 45)])
 def test_jacobi0_matches_scale_then_sweep(oracle, nel):
     """The fused first sweep (x0 = omega D^-1 b formed on load) equals the
