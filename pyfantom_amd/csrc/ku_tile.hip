@@ -36,20 +36,24 @@ struct FineOp {
     // resident waves per SIMD the kernels are built for (FP64 152-169 VGPRs:
     // 3; FP32 76-95: 5)
     static constexpr int kWavesPerEU = sizeof(T) == 8 ? 3 : 5;
-    // the prolongating sweep at the budget above spills 10-19 values inside
-    // its layer loop; one wave fewer and no spills measured 127 / 67 us
-    // against 180 / 94 us (FP64 / FP32, 256x128x128)
-    static constexpr int waves_for(int mode) { return mode == MODE_JACPRO ? kWavesPerEU - 1 : kWavesPerEU; }
+    // one wave fewer where the budget above would spill inside the layer
+    // loop: the prolongating sweep (10-19 spilled values; 127 / 67 us
+    // against 180 / 94 us, FP64 / FP32, 256x128x128) and the FP32 zero-guess
+    // sweep with staged stores (60.3 us against 64.7 unstaged at five waves).
+    // Not the sweeps that sum partials: K.u + p.Kp shares their chunk shape.
+    static constexpr int waves_for(int mode)
+    {
+        return (mode == MODE_JACPRO || (sizeof(T) == 4 && mode == MODE_JAC0OWN)) ? kWavesPerEU - 1 : kWavesPerEU;
+    }
     // staged row stores where they pay (measured per mode, 256x128x128,
-    // HBM-resident): not in the Jacobi + b.x sweeps (slower by 2-3 us: they
-    // spill 3-4 VGPRs with them), nor in the FP32 sweeps that form D^-1 from
-    // the moduli (no gain, or spills inside the layer loop).  The FP64
-    // zero-guess sweep gains 7 us with them, its spill lying outside the loop.
+    // HBM-resident): not in the Jacobi + b.x sweeps (slower by 2-3 us with
+    // them, and by 5-22 us at one wave fewer without spills), nor in the FP32
+    // Jacobi sweep (no gain), nor in the prolongating sweep (registers).  The
+    // FP64 zero-guess sweep gains 7 us with them (its spill lies outside the
+    // loop), the FP32 one 4 us at one wave fewer.
     static constexpr bool staged_store(int mode)
     {
-        return sizeof(T) == 8 ? (mode != MODE_JACDOT && mode != MODE_JACPRO)
-                              : (mode != MODE_JACDOT && mode != MODE_JACOWN && mode != MODE_JAC0OWN &&
-                                 mode != MODE_JACPRO);
+        return mode != MODE_JACDOT && mode != MODE_JACPRO && (sizeof(T) == 8 || mode != MODE_JACOWN);
     }
     // apply leaves out the element modulus; the skeleton multiplies it in
     static constexpr bool kScaled = true;
