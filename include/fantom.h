@@ -288,6 +288,16 @@ int fantom_mma_update_f64(long n, int m, int iter, const double *x, const double
 int fantom_galerkin_tables(const double *K0, double *G, double *Btab, double *Gdiag, double *check);
 int fantom_gal1_ku_f64(int nelx, int nely, int nelz, const double *Btab, const double *Efine, const uint8_t *mask,
                        const double *u, double *y, void *stream);
+/* No counterpart in the reference (its level-1 operator is an assembled CSR
+ * product, core/CUDA/_mgm.py:19-171): for cubic elements the 279 nonzero
+ * entries of Btab take 20 distinct values, the same classes for every
+ * Poisson ratio.  gal1_classes (host arrays): *is_cubic = 1 and cls[20] the
+ * class values if Btab has that structure, else 0.  gal1_ku_cls: gal1_ku
+ * with the operator's coefficients passed as those 20 values (cls is a HOST
+ * array, read at launch) instead of read from a device table. */
+int fantom_gal1_classes(const double *Btab, double *cls, int *is_cubic);
+int fantom_gal1_ku_cls_f64(int nelx, int nely, int nelz, const double *cls, const double *Efine, const uint8_t *mask,
+                           const double *u, double *y, void *stream);
 int fantom_gal_build_f64(int nelx, int nely, int nelz, int src_kind, const double *G, const double *Efine,
                          const double *Kfine, double *Kc, void *stream);
 int fantom_elem_ku_f64(int nelx, int nely, int nelz, const double *Ke, const uint8_t *mask, const double *u,

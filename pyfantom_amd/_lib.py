@@ -121,6 +121,8 @@ SIGNATURES = {
     "fantom_node_density": (_c_int, [_c_int, _c_int, _c_int, _vp, _vp, _vp]),
     "fantom_galerkin_tables": (_c_int, [_vp, _vp, _vp, _vp, _vp]),
     "fantom_gal1_ku_f64": (_c_int, [_c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "fantom_gal1_classes": (_c_int, [_vp, _vp, _vp]),
+    "fantom_gal1_ku_cls_f64": (_c_int, [_c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
     "fantom_gal_build_f64": (_c_int, [_c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp]),
     "fantom_elem_ku_f64": (_c_int, [_c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp]),
     "fantom_elem_from_fine_f64": (_c_int, [_c_int, _c_int, _c_int, _vp, _vp, _vp, _vp]),

@@ -118,10 +118,17 @@ int launch_spd_inverse(int n, double *A, double *work, hipStream_t s);
 int launch_gemv(int n, const double *A, const double *x, double *y, hipStream_t s);
 
 // galerkin.hip
+// cls (optional, host): the 20 class values of a cubic element's table
+// (gal1_class_values); the operator then takes them as kernel arguments
 int launch_gal1(int mode, const Grid &g, const double *Btab, const double *Ef, const uint8_t *mask, const double *u,
-                double *y, const double *b, const double *dinv, double omega, double *partials, hipStream_t s);
+                double *y, const double *b, const double *dinv, double omega, double *partials, hipStream_t s,
+                const double *cls = nullptr);
 int launch_gal1_f32(int mode, const Grid &g, const float *Btab, const float *Ef, const uint8_t *mask, const float *u,
-                    float *y, const float *b, const float *dinv, double omega, double *partials, hipStream_t s);
+                    float *y, const float *b, const float *dinv, double omega, double *partials, hipStream_t s,
+                    const double *cls = nullptr);
+// the class values of a level-1 table B (576 entries) if it has the cubic
+// element's structure
+bool gal1_class_values(const double *B, double *cls);
 long gal1_partials(const Grid &g);
 int launch_elem(int mode, const Grid &g, const double *Ke, const uint8_t *mask, const double *u, double *y,
                 const double *b, const double *dinv, double omega, double *partials, hipStream_t s);

@@ -41,6 +41,45 @@ constexpr unsigned short kGalNZ[64] = {
     0x03f, 0x036, 0x02d, 0x024, 0x01b, 0x017, 0x039, 0x035, 0x038, 0x134, 0x028, 0x13c, 0x059, 0x155, 0x079, 0x17d,
     0x007, 0x006, 0x125, 0x127, 0x093, 0x097, 0x1b1, 0x1b7, 0x000, 0x116, 0x129, 0x13f, 0x0d1, 0x1d7, 0x1f9, 0x1ff};
 
+// Classes of equal entries of the level-1 table for cubic elements: entry
+// (s * 8 + s') * 9 + 3 d + d' holds class value kGalClass[...] (-1: zero).
+// The partition into 20 classes comes from the symmetry of the cube under
+// axis permutations and is the same for every Poisson ratio (generic ratios
+// give 20 distinct values, special ones fewer); gal1_class_values checks a
+// given table against it on the host, and other boxes (71 distinct values
+// for 1 x 1.5 x 0.7) keep the table path.  With the classes the operator
+// takes its 20 coefficients as kernel arguments, held in scalar registers,
+// instead of reloading the 279 table entries through the scalar cache on
+// every layer (the loads issued a few instructions before each use left the
+// two waves per SIMD waiting on them).
+constexpr int kGalClasses = 20;
+constexpr signed char kGalClass[576] = {
+    0, 1, 1, 1, 0, 1, 1, 1, 0, -1, 2, 2, -1, 3, 2, -1, 2, 3, 3, -1, 2, 2, -1, 2,
+    2, -1, 3, -1, -1, 4, -1, -1, 4, -1, -1, 5, 3, 2, -1, 2, 3, -1, 2, 2, -1, -1, 4, -1,
+    -1, 5, -1, -1, 4, -1, 5, -1, -1, 4, -1, -1, 4, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1,
+    -1, -1, -1, 2, 3, 2, 2, 2, 3, -1, -1, -1, -1, 6, 7, -1, 7, 6, -1, -1, -1, 8, -1, 4,
+    4, -1, 5, -1, -1, -1, 9, 9, 10, -1, -1, 11, -1, -1, -1, 4, 5, -1, 8, 4, -1, -1, -1, -1,
+    -1, 11, -1, 9, 10, 9, -1, -1, -1, 12, -1, -1, 12, -1, -1, -1, -1, -1, 13, 13, -1, 13, -1, 13,
+    3, 2, 2, -1, -1, -1, 2, 2, 3, -1, 8, 4, -1, -1, -1, -1, 4, 5, 6, -1, 7, -1, -1, -1,
+    7, -1, 6, 9, 9, 10, -1, -1, -1, -1, -1, 11, 5, 4, -1, -1, -1, -1, 4, 8, -1, -1, 12, -1,
+    -1, -1, -1, -1, 12, -1, 11, -1, -1, -1, -1, -1, 10, 9, 9, 13, 13, -1, -1, -1, -1, -1, 13, 13,
+    -1, -1, -1, -1, -1, -1, 4, 4, 5, -1, 9, -1, -1, 9, -1, -1, 10, 11, 9, -1, -1, 9, -1, -1,
+    10, -1, 11, 14, 14, -1, 14, 14, -1, -1, -1, 15, -1, -1, -1, -1, -1, -1, 12, 12, -1, -1, 13, -1,
+    -1, 13, -1, 13, 16, 13, 13, -1, -1, 13, -1, -1, 16, 13, 13, 9, 9, -1, 9, 9, -1, 17, 17, 18,
+    3, 2, 2, 2, 3, 2, -1, -1, -1, -1, 4, 8, -1, 5, 4, -1, -1, -1, 5, -1, 4, 4, -1, 8,
+    -1, -1, -1, -1, -1, 12, -1, -1, 12, -1, -1, -1, 6, 7, -1, 7, 6, -1, -1, -1, -1, 9, 10, 9,
+    -1, 11, -1, -1, -1, -1, 11, -1, -1, 10, 9, 9, -1, -1, -1, 13, -1, 13, -1, 13, 13, -1, -1, -1,
+    -1, -1, -1, 4, 5, 4, -1, -1, -1, -1, -1, 9, -1, 11, 10, -1, -1, 9, -1, -1, -1, 12, -1, 12,
+    -1, -1, -1, -1, -1, 13, 13, 13, 16, -1, -1, 13, 9, -1, -1, 10, 11, -1, 9, -1, -1, 14, -1, 14,
+    -1, 15, -1, 14, -1, 14, 13, -1, -1, 16, 13, 13, 13, -1, -1, 9, -1, 9, 17, 18, 17, 9, -1, 9,
+    5, 4, 4, -1, -1, -1, -1, -1, -1, -1, 12, 12, -1, -1, -1, -1, -1, -1, 11, -1, 10, -1, -1, 9,
+    -1, -1, 9, 13, 13, 16, -1, -1, 13, -1, -1, 13, 11, 10, -1, -1, 9, -1, -1, 9, -1, 13, 16, 13,
+    -1, 13, -1, -1, 13, -1, 15, -1, -1, -1, 14, 14, -1, 14, 14, 18, 17, 17, -1, 9, 9, -1, 9, 9,
+    -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, 13, 13, -1, 13, -1, -1, -1, 13, 13, -1, -1, 13, -1, 13,
+    -1, -1, 13, 9, 9, 17, 9, 9, 17, -1, -1, 18, 13, -1, -1, -1, 13, -1, 13, 13, -1, 9, 17, 9,
+    -1, 18, -1, 9, 17, 9, 18, -1, -1, 17, 9, 9, 17, 9, 9, 19, 18, 18, 18, 19, 18, 18, 18, 19,
+};
+
 template <typename T>
 __device__ __forceinline__ void gwht8(T *w)
 {
@@ -62,7 +101,7 @@ __device__ __forceinline__ void gwht8(T *w)
 // the operator's precision, block (s, s') at Btab[(s*8 + s')*9 + 3*d + d']
 // (scaled by 1/8); uniform
 // reads, served by the scalar cache.
-template <typename T>
+template <typename T, bool kCls = false>
 struct Gal1Op {
     // resident waves per SIMD the kernels are built for (FP64 192-224 VGPRs:
     // 2; FP32 104-120: 4)
@@ -75,6 +114,7 @@ struct Gal1Op {
     const T *Btab;
     const T *Ef;
     int zero;   // 0, set by the host
+    T cls[kCls ? kGalClasses : 1];   // kCls: the class values (kGalClass)
     struct Data {
         T e[8];   // child moduli, child bits cx + 2 cy + 4 cz
         int k;    // ey & zero
@@ -114,11 +154,15 @@ struct Gal1Op {
                 for (int dr = 0; dr < 3; ++dr) {
                     if (((nz >> (3 * dr)) & 7) == 0)
                         continue;
-                    T t = T(0);
+                        T t = T(0);
 #pragma unroll
                     for (int dc = 0; dc < 3; ++dc)
-                        if ((nz >> (3 * dr + dc)) & 1)
-                            t += Bs[3 * dr + dc] * h[dc];
+                        if ((nz >> (3 * dr + dc)) & 1) {
+                            if constexpr (kCls)
+                                t += cls[kGalClass[(s * 8 + s2) * 9 + 3 * dr + dc]] * h[dc];
+                            else
+                                t += Bs[3 * dr + dc] * h[dc];
+                        }
                     a[dr] += eh[s ^ s2] * t;
                 }
             }
@@ -480,42 +524,77 @@ __global__ void dense_from_elem(Grid g, const double *__restrict__ Ke, const uin
 // Workgroups per CU the chunk choice assumes for each mode of the level-1
 // operator: the occupancy of that mode's instantiation (FP64 2, FP32 4 at
 // 116-123 VGPRs), the register budget without a device.
-template <typename T>
+template <typename T, bool kCls>
 static int gal1_per_cu(int mode)
 {
+    using Op = Gal1Op<T, kCls>;
     static const std::array<int, 5> pc = [] {
-        std::array<int, 5> a = {tile_occupancy<T, 0, Gal1Op<T>>(), tile_occupancy<T, 1, Gal1Op<T>>(),
-                                tile_occupancy<T, 2, Gal1Op<T>>(), tile_occupancy<T, 3, Gal1Op<T>>(),
-                                tile_occupancy<T, 4, Gal1Op<T>>()};
+        std::array<int, 5> a = {tile_occupancy<T, 0, Op>(), tile_occupancy<T, 1, Op>(), tile_occupancy<T, 2, Op>(),
+                                tile_occupancy<T, 3, Op>(), tile_occupancy<T, 4, Op>()};
         for (int m = 0; m < (int)a.size(); ++m)
             if (a[m] <= 0)
-                a[m] = Gal1Op<T>::waves_for(m);   // a workgroup is one wave per SIMD
+                a[m] = Op::waves_for(m);   // a workgroup is one wave per SIMD
         return a;
     }();
     return pc[mode];
 }
 
-// partial sums of the one mode that writes them (K.u + u.Ku), either precision
+// partial sums of the one mode that writes them (K.u + u.Ku), any variant
 long gal1_partials(const Grid &g)
 {
-    return (long)std::max(tile_blocks(tile_shape(g, gal1_per_cu<double>(MODE_KUDOT))),
-                          tile_blocks(tile_shape(g, gal1_per_cu<float>(MODE_KUDOT))));
+    unsigned n = 0;
+    for (int v : {gal1_per_cu<double, false>(MODE_KUDOT), gal1_per_cu<float, false>(MODE_KUDOT),
+                  gal1_per_cu<double, true>(MODE_KUDOT), gal1_per_cu<float, true>(MODE_KUDOT)})
+        n = std::max(n, tile_blocks(tile_shape(g, v)));
+    return (long)n;
 }
 
-template <typename T>
-static int gal1_launch(int mode, const Grid &g, const T *Btab, const T *Ef, const uint8_t *mask, const T *u,
-                       T *y, const T *b, const T *dinv, double omega, double *partials, hipStream_t s)
+bool gal1_class_values(const double *B, double *cls)
 {
-    const TileShape ts = tile_shape(g, gal1_per_cu<T>(mode >= 0 && mode < 5 ? mode : 0));   // bad modes fail below
+    double mx = 0.0;
+    for (int i = 0; i < 576; ++i)
+        mx = std::fmax(mx, std::fabs(B[i]));
+    if (!(mx > 0.0))
+        return false;
+    bool seen[kGalClasses] = {};
+    for (int i = 0; i < 576; ++i) {
+        const int c = kGalClass[i];
+        if (c < 0) {
+            if (std::fabs(B[i]) > 1e-12 * mx)
+                return false;
+            continue;
+        }
+        if (!seen[c]) {
+            cls[c] = B[i];
+            seen[c] = true;
+        } else if (std::fabs(B[i] - cls[c]) > 1e-12 * mx) {
+            return false;
+        }
+    }
+    for (bool v : seen)
+        if (!v)
+            return false;
+    return true;
+}
+
+template <typename T, bool kCls>
+static int gal1_launch(int mode, const Grid &g, const T *Btab, const T *Ef, const uint8_t *mask, const T *u,
+                       T *y, const T *b, const T *dinv, double omega, double *partials, hipStream_t s,
+                       const double *cls)
+{
+    const TileShape ts = tile_shape(g, gal1_per_cu<T, kCls>(mode >= 0 && mode < 5 ? mode : 0));   // bad modes fail below
     const unsigned blocks = tile_blocks(ts);
     const T om = (T)omega;
-    Gal1Op<T> op;
+    Gal1Op<T, kCls> op;
     op.Btab = Btab;
     op.Ef = Ef;
     op.zero = 0;
+    if constexpr (kCls)
+        for (int c = 0; c < kGalClasses; ++c)
+            op.cls[c] = (T)cls[c];
 #define GAL_CASE(M)                                                                                           \
     case M:                                                                                                   \
-        hipLaunchKernelGGL((ku_tile_kernel<T, M, Gal1Op<T>>), dim3(blocks), dim3(kTThreads), 0, s, g, op, mask, u, \
+        hipLaunchKernelGGL((ku_tile_kernel<T, M, Gal1Op<T, kCls>>), dim3(blocks), dim3(kTThreads), 0, s, g, op, mask, u, \
                            y, b, dinv, om, partials, ts, nullptr);                                            \
         break;
     switch (mode) {
@@ -534,15 +613,19 @@ static int gal1_launch(int mode, const Grid &g, const T *Btab, const T *Ef, cons
 }
 
 int launch_gal1(int mode, const Grid &g, const double *Btab, const double *Ef, const uint8_t *mask, const double *u,
-                double *y, const double *b, const double *dinv, double omega, double *partials, hipStream_t s)
+                double *y, const double *b, const double *dinv, double omega, double *partials, hipStream_t s,
+                const double *cls)
 {
-    return gal1_launch<double>(mode, g, Btab, Ef, mask, u, y, b, dinv, omega, partials, s);
+    return cls ? gal1_launch<double, true>(mode, g, Btab, Ef, mask, u, y, b, dinv, omega, partials, s, cls)
+               : gal1_launch<double, false>(mode, g, Btab, Ef, mask, u, y, b, dinv, omega, partials, s, nullptr);
 }
 
 int launch_gal1_f32(int mode, const Grid &g, const float *Btab, const float *Ef, const uint8_t *mask, const float *u,
-                    float *y, const float *b, const float *dinv, double omega, double *partials, hipStream_t s)
+                    float *y, const float *b, const float *dinv, double omega, double *partials, hipStream_t s,
+                    const double *cls)
 {
-    return gal1_launch<float>(mode, g, Btab, Ef, mask, u, y, b, dinv, omega, partials, s);
+    return cls ? gal1_launch<float, true>(mode, g, Btab, Ef, mask, u, y, b, dinv, omega, partials, s, cls)
+               : gal1_launch<float, false>(mode, g, Btab, Ef, mask, u, y, b, dinv, omega, partials, s, nullptr);
 }
 
 template <typename T>
@@ -833,6 +916,22 @@ extern "C" int fantom_gal1_ku_f64(int nelx, int nely, int nelz, const double *Bt
     FANTOM_CHECK_ARG(grid_ok(nelx, nely, nelz) && nelz > 0 && Btab && Efine && u && y && u != y, "bad operands");
     return launch_gal1(MODE_KU, make_grid(nelx, nely, nelz), Btab, Efine, mask, u, y, nullptr, nullptr, 0.0, nullptr,
                        as_stream(stream));
+}
+
+extern "C" int fantom_gal1_classes(const double *B, double *cls, int *is_cubic)
+{
+    FANTOM_CHECK_ARG(B && cls && is_cubic, "null operand");
+    *is_cubic = gal1_class_values(B, cls) ? 1 : 0;
+    return kStatusOk;
+}
+
+extern "C" int fantom_gal1_ku_cls_f64(int nelx, int nely, int nelz, const double *cls, const double *Efine,
+                                      const uint8_t *mask, const double *u, double *y, void *stream)
+{
+    FANTOM_CHECK_ARG(grid_ok(nelx, nely, nelz) && nelz > 0, "bad 3D grid");
+    FANTOM_CHECK_ARG(cls && Efine && u && y && u != y, "bad operands");
+    return launch_gal1(MODE_KU, make_grid(nelx, nely, nelz), nullptr, Efine, mask, u, y, nullptr, nullptr, 0.0,
+                       nullptr, as_stream(stream), cls);
 }
 
 extern "C" int fantom_gal_build_f64(int nelx, int nely, int nelz, int src_kind, const double *G, const double *Efine,

@@ -60,6 +60,8 @@ enum { LK_FINE = 0, LK_REDISC = 1, LK_GAL1 = 2, LK_ELEM = 3 };
 struct MgHandle {
     int dim, dof, nlev, n_smooth, kernel, galerkin;
     double *Gtab, *Btab, *Gdiag;     // device Galerkin tables (hex8)
+    double gal_cls[20];              // class values of Btab (cubic elements), if gal_cls_ok
+    bool gal_cls_ok = false;
     double *G2tab;                   // 64 x 576 level-2 tables (levels >= 2 built from the fine moduli)
     double *Ke1;                     // level-1 element matrices when level 1 is the coarsest
     double omega;
@@ -183,7 +185,8 @@ static int lev_ku(MgHandle *h, int l, int mode, const double *u, double *y, cons
 {
     MgLevel &L = h->lev[l];
     if (L.kind == LK_GAL1)
-        return launch_gal1(mode, L.g, h->Btab, h->lev[0].E, L.mask, u, y, b, dinv, omega, partials, h->stream);
+        return launch_gal1(mode, L.g, h->Btab, h->lev[0].E, L.mask, u, y, b, dinv, omega, partials, h->stream,
+                           h->gal_cls_ok ? h->gal_cls : nullptr);
     if (L.kind == LK_ELEM)
         return L.S ? launch_stencil_ku(mode, L.g, L.S, u, y, b, dinv, omega, partials, h->stream)
                    : launch_elem(mode, L.g, L.Ke, L.mask, u, y, b, dinv, omega, partials, h->stream);
@@ -512,6 +515,7 @@ int fantom_mg_create(int nelx, int nely, int nelz, int n_coarse, int n_smooth, d
             e = hipMemcpyAsync(h->G2tab, tG2.data(), sizeof(double) * tG2.size(), hipMemcpyHostToDevice, h->stream);
         if (e == hipSuccess)
             e = hipMemcpyAsync(h->Btab, tB.data(), sizeof(double) * tB.size(), hipMemcpyHostToDevice, h->stream);
+        h->gal_cls_ok = gal1_class_values(tB.data(), h->gal_cls);
         if (e == hipSuccess)
             e = hipMemcpyAsync(h->Gdiag, tD.data(), sizeof(double) * tD.size(), hipMemcpyHostToDevice, h->stream);
     }

@@ -52,6 +52,8 @@ struct MxHandle {
     double *Gtab, *Btab, *Gdiag, *Ke1;
     double *G2tab;   // 64 x 576 level-2 tables
     float *Btab32;   // the level-1 table in the FP32 operator's precision
+    double gal_cls[20];   // class values of Btab (cubic elements), if gal_cls_ok
+    bool gal_cls_ok = false;
     double omega;
     OmegaSchedule sched;   // adaptive weight of the FP64-PCG scheme
     std::vector<MxLevel> lev;
@@ -364,7 +366,8 @@ static int mx_ku(MxHandle *h, int l, int mode, const float *u, float *y, const f
                                    h->stream, split & 1);
     }
     if (L.kind == 2)
-        return launch_gal1_f32(mode, L.g, h->Btab32, h->lev[0].E, L.mask, u, y, b, dinv, om, partials, h->stream);
+        return launch_gal1_f32(mode, L.g, h->Btab32, h->lev[0].E, L.mask, u, y, b, dinv, om, partials, h->stream,
+                               h->gal_cls_ok ? h->gal_cls : nullptr);
     if (L.kind == 3)
         return L.S ? launch_stencil_ku_f32(mode, L.g, L.S, u, y, b, dinv, om, partials, h->stream)
                    : launch_elem_f32(mode, L.g, L.Ke, L.mask, u, y, b, dinv, om, partials, h->stream);
@@ -693,6 +696,7 @@ int fantom_mx_create(int nelx, int nely, int nelz, int n_coarse, int n_smooth, d
         e = hipMemcpyAsync(h->G2tab, tG2.data(), sizeof(double) * tG2.size(), hipMemcpyHostToDevice, h->stream);
     if (h->galerkin && e == hipSuccess)
         e = hipMemcpyAsync(h->Btab, tB.data(), sizeof(double) * tB.size(), hipMemcpyHostToDevice, h->stream);
+    h->gal_cls_ok = h->galerkin && gal1_class_values(tB.data(), h->gal_cls);
     const std::vector<float> tB32(tB.begin(), tB.end());
     if (h->galerkin && e == hipSuccess)
         e = hipMemcpyAsync(h->Btab32, tB32.data(), sizeof(float) * tB32.size(), hipMemcpyHostToDevice, h->stream);

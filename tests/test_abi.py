@@ -68,6 +68,24 @@ def test_abi_version():
     assert _lib.load().fantom_abi_version() == 3
 
 
+def test_gal1_classes(oracle):
+    """The level-1 table of a cubic element has 20 classes of equal entries
+    for any Poisson ratio (host code, no GPU); a non-cubic box does not."""
+    from pyfantom_amd import _lib
+    lib = _lib.load()
+    for nu, h, want in ((0.3, (1, 1, 1), 1), (0.4471, (1, 1, 1), 1), (0.3, (1, 1.5, 0.7), 0)):
+        K0 = np.ascontiguousarray(oracle.k0_exact(3, nu, *h))
+        G, B, Gd = np.zeros(8 * 576), np.zeros(576), np.zeros(8 * 24)
+        _lib.check(lib.fantom_galerkin_tables(K0.ctypes.data, G.ctypes.data, B.ctypes.data, Gd.ctypes.data, None))
+        cls, cubic = np.zeros(20), np.zeros(1, np.int32)
+        _lib.check(lib.fantom_gal1_classes(B.ctypes.data, cls.ctypes.data, cubic.ctypes.data))
+        assert cubic[0] == want
+        if want:
+            nz = B[np.abs(B) > 1e-12 * np.abs(B).max()]
+            assert len(nz) == 279
+            assert np.abs(np.abs(nz)[:, None] - np.abs(cls)[None, :]).min(axis=1).max() <= 1e-12 * np.abs(B).max()
+
+
 def test_stencil_size():
     """The stencil holds 126 coefficients per node in tiles of 64 nodes (host
     arithmetic, no GPU); bad grids return -1."""

@@ -395,6 +395,14 @@ def test_galerkin_levels(oracle, nel):
     y1 = torch.empty_like(u1d)
     _lib.call("fantom_gal1_ku_f64", *g1, Bd.data_ptr(), Ed.data_ptr(), m1.data_ptr(), u1d.data_ptr(), y1.data_ptr())
     assert _rel(_np(y1), apply(1, u1)) <= 1e-13
+    # the same operator with its 20 class values as kernel arguments (cubic elements)
+    cls, cubic = np.zeros(20), np.zeros(1, np.int32)
+    _lib.check(_lib.load().fantom_gal1_classes(B.ctypes.data, cls.ctypes.data, cubic.ctypes.data))
+    assert cubic[0] == 1
+    y1c = torch.empty_like(u1d)
+    _lib.call("fantom_gal1_ku_cls_f64", *g1, cls.ctypes.data, Ed.data_ptr(), m1.data_ptr(), u1d.data_ptr(),
+              y1c.data_ptr())
+    assert _rel(_np(y1c), apply(1, u1)) <= 1e-13
     # level 2 (explicit element matrices built from the fine moduli)
     g2 = nels[2]
     n2 = int(np.prod([n + 1 for n in g2])) * 3
